@@ -1,0 +1,627 @@
+// Host side of the wedg C ABI (include/wedg.h): problem upload with
+// de-duplicated coefficient tables, device-resident workspaces, and the RHS /
+// RK4 / advance entry points that launch the per-degree kernels.
+#include "../../include/wedg.h"
+#include "wedg_kernels.h"
+#include "wedg_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace wedg;
+
+#define HIPOK(x)                                   \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return WEDG_ERR_HIP; \
+    } while (0)
+
+namespace {
+
+template <class T>
+T* dalloc(size_t n, hipError_t* err) {
+    void* p = nullptr;
+    *err = hipMalloc(&p, n * sizeof(T) + (n == 0 ? 8 : 0));
+    return static_cast<T*>(p);
+}
+
+uint64_t fnv1a(const double* v, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(v);
+    for (size_t i = 0; i < n * sizeof(double); ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// Classes of identical per-element records (bitwise). rec(e, buf) fills the
+// record of element e; returns the class per element and the unique records.
+template <class F>
+void dedup(long E, size_t len, F rec, std::vector<int>& cls, std::vector<double>& uniq) {
+    std::unordered_map<uint64_t, std::vector<int>> map;
+    std::vector<double> buf(len);
+    cls.assign(E, 0);
+    uniq.clear();
+    int ncls = 0;
+    for (long e = 0; e < E; ++e) {
+        rec(e, buf.data());
+        uint64_t h = fnv1a(buf.data(), len);
+        auto& cands = map[h];
+        int found = -1;
+        for (int c : cands)
+            if (std::memcmp(&uniq[(size_t)c * len], buf.data(), len * sizeof(double)) == 0) { found = c; break; }
+        if (found < 0) {
+            found = ncls++;
+            uniq.insert(uniq.end(), buf.begin(), buf.end());
+            cands.push_back(found);
+        }
+        cls[e] = found;
+    }
+}
+
+bool all_equal(const double* p, size_t n) {
+    for (size_t i = 1; i < n; ++i)
+        if (std::memcmp(&p[i], &p[0], sizeof(double)) != 0) return false;
+    return true;
+}
+
+}  // namespace
+
+struct wedg_problem {
+    int N = 0, E = 0, nu = 0, nv = 0, nh = 0, qu = 0;
+    double omega = 0, theta = 0, tau = 0, lu = 0, lv = 0, mass_tol = 1e-14;
+    int mms_on = 0, src_on = 0, one_sided = 0;
+    double src_omega = 0, src_Ts = 0;
+    wedg_side bc[4];
+    int uni_ca = 0;
+    int ncls_ca = 0, ncls_b = 0;
+    std::vector<void*> allocs;
+    double* tab = nullptr;
+    int* cls_ca = nullptr;
+    double *Cw = nullptr, *Aw = nullptr, *Ainv = nullptr;
+    int* cls_b = nullptr;
+    double* beta = nullptr;
+    FaceArr f[F_COUNT];
+    double* bcd[8] = {};
+    double* proj[5] = {};   // PiSP, PiBSP2, PiCdiv, PiDdiv, src
+    int has_proj[5] = {};
+    size_t device_bytes = 0;
+
+    template <class T>
+    T* upload(const T* h, size_t n, hipError_t* err) {
+        T* d = dalloc<T>(n, err);
+        if (*err != hipSuccess) return nullptr;
+        allocs.push_back(d);
+        device_bytes += n * sizeof(T);
+        *err = hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+        return d;
+    }
+};
+
+struct wedg_workspace {
+    wedg_problem* p = nullptr;
+    int cheb_global = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    double *yu = nullptr, *yv = nullptr, *su = nullptr, *sv = nullptr;
+    double *au = nullptr, *av = nullptr, *ou = nullptr, *ov = nullptr;
+    double* tr = nullptr;
+    double *Rb = nullptr, *Bb = nullptr;
+    DevErr* err = nullptr;
+    DevErr err_init{};       // host source of the reset copy (kept alive with the workspace)
+    int* cnt = nullptr;
+    int serial = 0;
+    long step_base = 0;
+    wedg_error_info last{};
+    // timing
+    int timing = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    double t_ms[4] = {0, 0, 0, 0};
+    long t_n[4] = {0, 0, 0, 0};
+    size_t nU() const { return (size_t)p->E * p->nu * p->nu; }
+    size_t nV() const { return (size_t)p->E * p->nv * p->nv; }
+};
+
+static hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s) {
+    switch (p->qu) {
+        case 1: return launch_q1(a, p->nv, which, s, p->uni_ca);
+        case 2: return launch_q2(a, p->nv, which, s, p->uni_ca);
+        case 3: return launch_q3(a, p->nv, which, s, p->uni_ca);
+        case 4: return launch_q4(a, p->nv, which, s, p->uni_ca);
+        case 5: return launch_q5(a, p->nv, which, s, p->uni_ca);
+        case 6: return launch_q6(a, p->nv, which, s, p->uni_ca);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** out) {
+    if (!pre || !out) return WEDG_ERR_ARGUMENT;
+    *out = nullptr;
+    const int N = pre->N, nu = pre->qu1, nv = pre->qv1, nh = pre->q_high;
+    if (N < 1 || nu < 2 || nu > 7 || (nv != nu && nv != nu - 1)) return WEDG_ERR_ARGUMENT;
+    if (pre->mass_solver != 0) return WEDG_ERR_ARGUMENT;   // Chebyshev only on the device
+    const int qv = nv - 1;
+    int nh_expect = (3 * qv + 1) / 2 + 1;
+    if (nh_expect < nu) nh_expect = nu;
+    if (nh != nh_expect) return WEDG_ERR_DIMENSION;
+    auto* p = new wedg_problem();
+    p->N = N; p->E = N * N; p->nu = nu; p->nv = nv; p->nh = nh; p->qu = nu - 1;
+    p->omega = pre->omega; p->theta = pre->theta; p->tau = pre->tau;
+    p->lu = pre->lambda_u; p->lv = pre->lambda_v; p->mass_tol = pre->mass_tol;
+    p->mms_on = pre->mms_on; p->src_on = pre->src_on; p->one_sided = pre->one_sided;
+    p->src_omega = pre->src_omega; p->src_Ts = pre->src_Ts;
+    for (int s = 0; s < 4; ++s) p->bc[s] = pre->bc[s];
+    const long E = p->E;
+    const int nq = nu;
+    hipError_t he = hipSuccess;
+#define UP(dst, src, n)                                  \
+    do {                                                 \
+        dst = p->upload(src, n, &he);                    \
+        if (he != hipSuccess) { wedg_problem_destroy(p); return WEDG_ERR_HIP; } \
+    } while (0)
+
+    // 1D tables: P_u, S_u, w, Ebl, Ebr, E1bl, E1br, P_h_v, W2d_h (compile-time offsets)
+    {
+        std::vector<double> t;
+        t.insert(t.end(), pre->P_u, pre->P_u + nq * nu);
+        t.insert(t.end(), pre->S_u, pre->S_u + nq * nu);
+        t.insert(t.end(), pre->w, pre->w + nq);
+        t.insert(t.end(), pre->Ebl_u, pre->Ebl_u + nu);
+        t.insert(t.end(), pre->Ebr_u, pre->Ebr_u + nu);
+        t.insert(t.end(), pre->E1bl_u, pre->E1bl_u + nu);
+        t.insert(t.end(), pre->E1br_u, pre->E1br_u + nu);
+        t.insert(t.end(), pre->P_h_v, pre->P_h_v + nh * nv);
+        t.insert(t.end(), pre->W2d_h, pre->W2d_h + nh * nh);
+        UP(p->tab, t.data(), t.size());
+    }
+    // (Cw, Aw, Ainv) per element -> classes
+    {
+        const int npk = (nu * nu * (nu * nu + 1)) / 2;
+        const size_t len = 2 * nq * nq + npk;
+        std::vector<int> cls;
+        std::vector<double> uniq;
+        dedup(E, len, [&](long e, double* b) {
+            for (int ix = 0; ix < nq; ++ix)
+                for (int iy = 0; iy < nq; ++iy) {
+                    b[iy + nq * ix] = pre->Cw[iy + (long)nq * (e + E * ix)];
+                    b[nq * nq + iy + nq * ix] = pre->Aw[iy + (long)nq * (e + E * ix)];
+                }
+            std::memcpy(b + 2 * nq * nq, pre->Ainv_u + (long)npk * e, npk * sizeof(double));
+        }, cls, uniq);
+        const int nc = (int)(uniq.size() / len);
+        p->ncls_ca = nc;
+        p->uni_ca = nc == 1;
+        const int n = nu * nu;
+        std::vector<double> cw((size_t)nc * nq * nq), aw((size_t)nc * nq * nq), ai((size_t)nc * n * n);
+        for (int c = 0; c < nc; ++c) {
+            std::memcpy(&cw[(size_t)c * nq * nq], &uniq[(size_t)c * len], nq * nq * sizeof(double));
+            std::memcpy(&aw[(size_t)c * nq * nq], &uniq[(size_t)c * len + nq * nq], nq * nq * sizeof(double));
+            // unpack the upper triangle (column jj at jj (jj + 1) / 2) into the full symmetric matrix
+            const double* pk = &uniq[(size_t)c * len + 2 * nq * nq];
+            double* full = &ai[(size_t)c * n * n];
+            for (int jj = 0; jj < n; ++jj)
+                for (int i = 0; i <= jj; ++i) {
+                    const double v = pk[(jj * (jj + 1)) / 2 + i];
+                    full[i + n * jj] = v;
+                    full[jj + n * i] = v;
+                }
+        }
+        UP(p->Cw, cw.data(), cw.size());
+        UP(p->Aw, aw.data(), aw.size());
+        UP(p->Ainv, ai.data(), ai.size());
+        UP(p->cls_ca, cls.data(), cls.size());
+    }
+    // beta at the high-order points per element -> classes
+    {
+        const size_t len = (size_t)nh * nh;
+        std::vector<int> cls;
+        std::vector<double> uniq;
+        dedup(E, len, [&](long e, double* b) {
+            for (int q = 0; q < nh; ++q)
+                for (int pp = 0; pp < nh; ++pp) b[pp + nh * q] = pre->beta_hq[pp + (long)nh * (e + E * q)];
+        }, cls, uniq);
+        p->ncls_b = (int)(uniq.size() / len);
+        UP(p->beta, uniq.data(), uniq.size());
+        UP(p->cls_b, cls.data(), cls.size());
+    }
+    // face arrays: [i][face] with the face index fastest, or one constant
+    {
+        const double* src[F_COUNT] = {pre->hc2, pre->haf, pre->hcf, pre->hbf, pre->vc2, pre->vaf, pre->vcf,
+                                      pre->vbf, pre->hc2m, pre->hafm, pre->vc2m, pre->vafm, pre->hJC, pre->vJC};
+        const size_t nf = (size_t)N * (N + 1);
+        for (int a = 0; a < F_COUNT; ++a) {
+            const double* s = src[a];
+            const size_t n = nf * nq;
+            if (all_equal(s, n)) {
+                double v = s[0];
+                UP(p->f[a].p, &v, 1);
+                p->f[a].uni = 1;
+            } else {
+                std::vector<double> t(n);
+                for (size_t face = 0; face < nf; ++face)
+                    for (int i = 0; i < nq; ++i) t[(size_t)i * nf + face] = s[i + (size_t)nq * face];
+                UP(p->f[a].p, t.data(), n);
+                p->f[a].uni = 0;
+            }
+        }
+    }
+    // boundary data (NQ, N), unchanged
+    {
+        const double* src[8] = {pre->bcB_sp, pre->bcB_spy, pre->bcT_sp, pre->bcT_spy,
+                                pre->bcL_sp, pre->bcL_spx, pre->bcR_sp, pre->bcR_spx};
+        for (int a = 0; a < 8; ++a) UP(p->bcd[a], src[a], (size_t)nq * N);
+    }
+    // projections used by the mode, element-fastest [m][E]; the others enter the
+    // reference's sum with an exactly zero factor (src/rhs.jl:344-348)
+    {
+        const double* src[5] = {pre->PiSP, pre->PiBSP2, pre->PiCdiv, pre->PiDdiv, pre->src_projE};
+        int use[5] = {0, 0, 0, 0, 0};
+        if (pre->mms_on) { use[0] = use[1] = use[2] = use[3] = 1; }
+        else use[2] = 1;                 // (1 - ct) with ct = 0
+        if (pre->src_on) use[4] = 1;
+        const size_t n2 = (size_t)nv * nv;
+        for (int a = 0; a < 5; ++a) {
+            if (!use[a]) continue;
+            std::vector<double> t(n2 * E);
+            bool zero = true;
+            for (long e = 0; e < E; ++e)
+                for (size_t m = 0; m < n2; ++m) {
+                    double v = src[a][m + n2 * e];
+                    t[m * E + e] = v;
+                    if (v != 0.0) zero = false;
+                }
+            if (zero && !(pre->mms_on && a < 4)) continue;
+            UP(p->proj[a], t.data(), t.size());
+            p->has_proj[a] = 1;
+        }
+    }
+#undef UP
+    *out = p;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_problem_destroy(wedg_problem* p) {
+    if (!p) return WEDG_OK;
+    for (void* a : p->allocs) (void)hipFree(a);
+    delete p;
+    return WEDG_OK;
+}
+
+static wedg_status reset_err(wedg_workspace* ws) {
+    DevErr& h = ws->err_init;
+    h.bkey = INT_MAX; h.mfail = INT_MAX; h.failed = 0; h.nonfinite = INT_MAX; h.maxcheb = 0;
+    HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, wedg_workspace** out) {
+    if (!p || !out) return WEDG_ERR_ARGUMENT;
+    if (cheb_policy != WEDG_CHEB_GLOBAL && cheb_policy != WEDG_CHEB_ELEMENT) return WEDG_ERR_ARGUMENT;
+    auto* ws = new wedg_workspace();
+    ws->p = p;
+    ws->cheb_global = cheb_policy == WEDG_CHEB_GLOBAL;
+    hipError_t he = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { delete ws; return WEDG_ERR_HIP; }
+    auto A = [&](size_t n) -> double* {
+        double* d = dalloc<double>(n, &he);
+        if (he == hipSuccess) { ws->allocs.push_back(d); (void)hipMemset(d, 0, n * sizeof(double)); }
+        return d;
+    };
+    const size_t nu = ws->nU(), nv = ws->nV();
+    ws->yu = A(nu); ws->yv = A(nv); ws->su = A(nu); ws->sv = A(nv);
+    ws->au = A(nu); ws->av = A(nv); ws->ou = A(nu); ws->ov = A(nv);
+    ws->tr = A((size_t)12 * p->nu * p->E);
+    ws->Rb = A((size_t)p->nv * p->nv * p->E);
+    ws->Bb = A((size_t)p->nu * p->nu * p->E);
+    if (he == hipSuccess) {
+        void* e = nullptr;
+        he = hipMalloc(&e, sizeof(DevErr) + 16);
+        if (he == hipSuccess) { ws->allocs.push_back(e); ws->err = (DevErr*)e; }
+    }
+    if (he == hipSuccess) {
+        void* c = nullptr;
+        he = hipMalloc(&c, 4 * sizeof(int));
+        if (he == hipSuccess) {
+            ws->allocs.push_back(c);
+            ws->cnt = (int*)c;
+            int init[4] = {1, 1, 0, 0};
+            he = hipMemcpy(c, init, sizeof(init), hipMemcpyHostToDevice);
+        }
+    }
+    if (he != hipSuccess) { wedg_workspace_destroy(ws); return WEDG_ERR_HIP; }
+    if (reset_err(ws) != WEDG_OK || hipStreamSynchronize(ws->stream) != hipSuccess) {
+        wedg_workspace_destroy(ws);
+        return WEDG_ERR_HIP;
+    }
+    *out = ws;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
+    if (!ws) return WEDG_OK;
+    if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+    for (auto& u : ws->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    for (auto& e : ws->ev_pool) (void)hipEventDestroy(e);
+    for (void* a : ws->allocs) (void)hipFree(a);
+    if (ws->stream) (void)hipStreamDestroy(ws->stream);
+    delete ws;
+    return WEDG_OK;
+}
+
+static KArgs base_args(wedg_workspace* ws) {
+    wedg_problem* p = ws->p;
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.N = p->N; a.E = p->E;
+    a.cheb_global = ws->cheb_global;
+    a.one_sided = p->one_sided;
+    const double h = 1.0 / p->N;
+    a.two_h = 2.0 / h; a.four_hh = 4.0 / (h * h);
+    a.theta = p->theta; a.tau = p->tau; a.lu = p->lu; a.lv = p->lv; a.omega = p->omega;
+    a.cheb_L = std::acosh(1.0 / p->mass_tol);
+    for (int s = 0; s < 4; ++s) {
+        a.kind[s] = p->bc[s].kind; a.gam[s] = p->bc[s].gamma_abc; a.eta[s] = p->bc[s].eta_abc;
+        a.sig[s] = p->bc[s].sigma; a.kb[s] = p->bc[s].kb; a.kap[s] = p->bc[s].kappa;
+    }
+    a.tab = p->tab;
+    a.cls_ca = p->cls_ca; a.Cw = p->Cw; a.Aw = p->Aw; a.Ainv = p->Ainv;
+    a.cls_b = p->cls_b; a.beta = p->beta;
+    for (int i = 0; i < F_COUNT; ++i) a.f[i] = p->f[i];
+    a.fstride = p->N * (p->N + 1);
+    for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
+    a.tr = ws->tr; a.err = ws->err; a.cnt = ws->cnt;
+    a.Rbuf = ws->Rb; a.Bbuf = ws->Bb;
+    return a;
+}
+
+// Time factors of the stage at time t and the projection coefficients, in the
+// order of the reference's sum (src/rhs.jl:344-348, 509-511).
+static void stage_time(const wedg_problem* p, double t, KArgs& a) {
+    a.ct = p->mms_on ? std::cos(p->omega * t) : 0.0;
+    a.st = p->mms_on ? std::sin(p->omega * t) : 0.0;
+    double srct = 0.0;
+    if (p->src_on && t < p->src_Ts) {
+        const double sv = std::sin(M_PI * t / p->src_Ts);
+        srct = std::sin(p->src_omega * t) * (sv * sv);
+    }
+    a.srct = srct;
+    const double om = p->omega, ct = a.ct, st = a.st;
+    const double coef[5] = {-(om * om) * ct, -2.0 * (om * om * om) * st * ct, 1.0 - ct, om * st, srct};
+    a.nproj = 0;
+    for (int i = 0; i < 5; ++i)
+        if (p->has_proj[i]) {
+            a.pc[a.nproj] = coef[i];
+            a.proj[a.nproj] = p->proj[i];
+            ++a.nproj;
+        }
+}
+
+static hipEvent_t ev_get(wedg_workspace* ws) {
+    if (!ws->ev_pool.empty()) {
+        hipEvent_t e = ws->ev_pool.back();
+        ws->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+static wedg_status run_kernel(wedg_workspace* ws, const KArgs& a, int which) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ws->timing) {
+        e0 = ev_get(ws);
+        e1 = ev_get(ws);
+        HIPOK(hipEventRecord(e0, ws->stream));
+    }
+    HIPOK(launch(a, ws->p, which, ws->stream));
+    if (ws->timing) {
+        HIPOK(hipEventRecord(e1, ws->stream));
+        ws->ev_used.push_back({which, {e0, e1}});
+    }
+    return WEDG_OK;
+}
+
+// One stage: k_traces on Y_s, then k_element writing the stage output.
+static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, const double* yv,
+                             const double* y0u, const double* y0v, double* outu, double* outv,
+                             double t, double c, long step) {
+    KArgs a = base_args(ws);
+    a.stage = stage;
+    a.serial = ++ws->serial;
+    if (ws->serial > 0x3fffffff) ws->serial = 1;
+    a.slot = a.serial & 1;
+    a.step = (int)step;
+    a.cstage = c;
+    stage_time(ws->p, t, a);
+    a.yu = yu; a.yv = yv; a.y0u = y0u; a.y0v = y0v;
+    a.accu = ws->au; a.accv = ws->av; a.outu = outu; a.outv = outv;
+    for (int which = 0; which < 4; ++which) {
+        wedg_status st = run_kernel(ws, a, which);
+        if (st != WEDG_OK) return st;
+    }
+    return WEDG_OK;
+}
+
+static wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step) {
+    wedg_status s;
+    const double hd = 0.5 * dt;
+    if ((s = run_stage(ws, 1, ws->yu, ws->yv, ws->yu, ws->yv, ws->su, ws->sv, t, hd, step))) return s;
+    if ((s = run_stage(ws, 2, ws->su, ws->sv, ws->yu, ws->yv, ws->su, ws->sv, t + 0.5 * dt, hd, step))) return s;
+    if ((s = run_stage(ws, 3, ws->su, ws->sv, ws->yu, ws->yv, ws->su, ws->sv, t + 0.5 * dt, dt, step))) return s;
+    return run_stage(ws, 4, ws->su, ws->sv, ws->yu, ws->yv, ws->yu, ws->yv, t + dt, dt / 6.0, step);
+}
+
+// Reads the failure record and maps it to a status (boundary before mass,
+// src/rhs.jl:530-531).
+static wedg_status collect(wedg_workspace* ws, int nonfinite_is_error) {
+    HIPOK(hipStreamSynchronize(ws->stream));
+    if (ws->timing) {
+        for (auto& u : ws->ev_used) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, u.second.first, u.second.second) == hipSuccess) {
+                ws->t_ms[u.first] += ms;
+                ws->t_n[u.first] += 1;
+            }
+            ws->ev_pool.push_back(u.second.first);
+            ws->ev_pool.push_back(u.second.second);
+        }
+        ws->ev_used.clear();
+    }
+    DevErr h;
+    HIPOK(hipMemcpy(&h, ws->err, sizeof(h), hipMemcpyDeviceToHost));
+    wedg_error_info& L = ws->last;
+    std::memset(&L, 0, sizeof(L));
+    L.cheb_steps = h.maxcheb;
+    const int nq = ws->p->nu, N = ws->p->N;
+    if (h.bkey != INT_MAX) {
+        L.status = (h.bkey % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
+        L.key = h.bkey;
+        L.side = (h.bkey - 1) / (2 * nq * N);
+        return (wedg_status)L.status;
+    }
+    if (h.mfail != INT_MAX) {
+        L.status = WEDG_ERR_MASS;
+        L.element = h.mfail;
+        return WEDG_ERR_MASS;
+    }
+    if (h.nonfinite != INT_MAX) {
+        L.step = h.nonfinite;
+        if (nonfinite_is_error) {
+            L.status = WEDG_ERR_NONFINITE;
+            return WEDG_ERR_NONFINITE;
+        }
+    }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_rhs_device(wedg_workspace* ws, const double* u, const double* v, double t,
+                                       double* ut, double* vt) {
+    if (!ws || !u || !v || !ut || !vt) return WEDG_ERR_ARGUMENT;
+    wedg_status s;
+    if ((s = reset_err(ws))) return s;
+    return run_stage(ws, 0, u, v, u, v, ut, vt, t, 0.0, 0);
+}
+
+extern "C" wedg_status wedg_sync(wedg_workspace* ws) {
+    if (!ws) return WEDG_ERR_ARGUMENT;
+    return collect(ws, 0);
+}
+
+extern "C" wedg_status wedg_rhs(wedg_workspace* ws, const double* u, const double* v, double t, int N,
+                                double h, double* ut, double* vt) {
+    if (!ws || !u || !v || !ut || !vt) return WEDG_ERR_ARGUMENT;
+    if (N != ws->p->N || h != 1.0 / N) return WEDG_ERR_DIMENSION;
+    HIPOK(hipMemcpyAsync(ws->su, u, ws->nU() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    HIPOK(hipMemcpyAsync(ws->sv, v, ws->nV() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    wedg_status s;
+    if ((s = reset_err(ws))) return s;
+    if ((s = run_stage(ws, 0, ws->su, ws->sv, ws->su, ws->sv, ws->ou, ws->ov, t, 0.0, 0))) return s;
+    if ((s = collect(ws, 0))) return s;
+    HIPOK(hipMemcpy(ut, ws->ou, ws->nU() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(vt, ws->ov, ws->nV() * sizeof(double), hipMemcpyDeviceToHost));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_set_state(wedg_workspace* ws, const double* u, const double* v) {
+    if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
+    HIPOK(hipMemcpyAsync(ws->yu, u, ws->nU() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    HIPOK(hipMemcpyAsync(ws->yv, v, ws->nV() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_get_state(wedg_workspace* ws, double* u, double* v) {
+    if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
+    HIPOK(hipStreamSynchronize(ws->stream));
+    HIPOK(hipMemcpy(u, ws->yu, ws->nU() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(v, ws->yv, ws->nV() * sizeof(double), hipMemcpyDeviceToHost));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_state_device(wedg_workspace* ws, double** u, double** v) {
+    if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
+    *u = ws->yu;
+    *v = ws->yv;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double dt, long nsteps,
+                                          int stop_on_nonfinite, double* t_end) {
+    if (!ws || nsteps < 0) return WEDG_ERR_ARGUMENT;
+    (void)stop_on_nonfinite;
+    wedg_status s;
+    if ((s = reset_err(ws))) return s;
+    double t = t0;
+    for (long it = 0; it < nsteps; ++it) {
+        if ((s = rk4_device(ws, t, dt, it + 1))) return s;
+        t += dt;
+    }
+    if (t_end) *t_end = t;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_advance(wedg_workspace* ws, double t0, double dt, long nsteps, int stop_on_nonfinite,
+                                    double* t_end) {
+    wedg_status s = wedg_advance_async(ws, t0, dt, nsteps, stop_on_nonfinite, t_end);
+    if (s) return s;
+    return collect(ws, stop_on_nonfinite);
+}
+
+extern "C" wedg_status wedg_rk4_step(wedg_workspace* ws, double* u, double* v, double t, double dt, int N, double h,
+                                     int* finite) {
+    if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
+    if (N != ws->p->N || h != 1.0 / N) return WEDG_ERR_DIMENSION;
+    wedg_status s;
+    if ((s = wedg_set_state(ws, u, v))) return s;
+    if ((s = reset_err(ws))) return s;
+    if ((s = rk4_device(ws, t, dt, 1))) return s;
+    s = collect(ws, 0);
+    if (finite) *finite = ws->last.step == 0;
+    if (s) return s;
+    return wedg_get_state(ws, u, v);
+}
+
+extern "C" wedg_status wedg_last_error(const wedg_workspace* ws, wedg_error_info* info) {
+    if (!ws || !info) return WEDG_ERR_ARGUMENT;
+    *info = ws->last;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_timing_enable(wedg_workspace* ws, int on) {
+    if (!ws) return WEDG_ERR_ARGUMENT;
+    ws->timing = on;
+    for (int i = 0; i < 4; ++i) { ws->t_ms[i] = 0; ws->t_n[i] = 0; }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* launches) {
+    if (!ws || family < 0 || family > 3) return WEDG_ERR_ARGUMENT;
+    if (ms) *ms = ws->t_ms[family];
+    if (launches) *launches = ws->t_n[family];
+    return WEDG_OK;
+}
+
+extern "C" const char* wedg_version(void) { return "wedg 0.1 gfx950 fp64"; }
+
+extern "C" wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs) {
+    int dev = 0;
+    HIPOK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPOK(hipGetDeviceProperties(&prop, dev));
+    if (name && len) {
+        std::strncpy(name, prop.gcnArchName, len - 1);
+        name[len - 1] = 0;
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_gbs) *hbm_gbs = 2.0 * prop.memoryClockRate * 1e3 * (prop.memoryBusWidth / 8) / 1e9;
+    return WEDG_OK;
+}

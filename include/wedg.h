@@ -1,0 +1,155 @@
+/* wedg — MI355X (gfx950) right-hand side and RK4 time stepping of the EDG
+ * discretization of the Westervelt equation, behind a plain C ABI.
+ *
+ * The boundary replaces the reference's Julia functions of the hot path
+ * (EDG-Westervelt/EDG_Westervelt, WesterveltEDG.jl):
+ *   precompute outputs   -> wedg_problem_create      (src/solver.jl:428-694, 716-770: `pre`)
+ *   Workspace(pre, N)    -> wedg_workspace_create    (src/rhs.jl:184-220)
+ *   edg_rhs!(ws,u,v,t,…) -> wedg_rhs / wedg_rhs_device (src/rhs.jl:490-494)
+ *   rk4_step!(u,v,t,dt,…)-> wedg_rk4_step           (src/solver.jl:289-297)
+ *   the loop of solve    -> wedg_advance            (src/solver.jl:213-216, 937-940)
+ * Arrays are the reference's column-major Julia arrays, byte for byte: a Julia
+ * caller passes pointer(pre.Cw), pointer(u), … unchanged. The state u is
+ * (N, N, q_u+1, q_u+1) and v is (N, N, q_v+1, q_v+1), element e = j + N k
+ * fastest. Everything is FP64.
+ *
+ * Conventions: every function returns a wedg_status; no C++ exception crosses
+ * the ABI. One workspace owns one HIP stream and is not re-entrant; different
+ * workspaces may be used from different threads. Host-pointer entry points
+ * copy in, run and copy out synchronously; *_device entry points take device
+ * pointers and are asynchronous on the workspace stream.
+ */
+#ifndef WEDG_H
+#define WEDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum wedg_status {
+    WEDG_OK = 0,
+    WEDG_ERR_BOUNDARY_TRACE = 1,  /* 1 - k beta v_h <= 0 at a face point (src/rhs.jl:114) */
+    WEDG_ERR_BOUNDARY_DATA = 2,   /* 1 - k beta g <= 0 for the boundary data (src/rhs.jl:115) */
+    WEDG_ERR_MASS = 3,            /* mass matrix not positive definite (src/rhs.jl:916) */
+    WEDG_ERR_DIMENSION = 4,       /* sizes do not match (DimensionMismatch, src/rhs.jl:500-503) */
+    WEDG_ERR_NONFINITE = 5,       /* non-finite state after a step (src/solver.jl:318, 940) */
+    WEDG_ERR_HIP = 6,             /* HIP runtime error */
+    WEDG_ERR_ARGUMENT = 7,        /* invalid argument or unsupported option */
+} wedg_status;
+
+/* Boundary kinds, per side (src/solver.jl:37). */
+enum { WEDG_ABSORBING_2BETA = 0, WEDG_ABSORBING = 1, WEDG_DIRICHLET = 2, WEDG_WALL = 3 };
+
+/* Parameters of one side (src/solver.jl:39-45). */
+typedef struct wedg_side {
+    int kind;
+    double gamma_abc, eta_abc, sigma, kb, kappa;
+} wedg_side;
+
+/* The tables of `precompute` (the fields of the reference's `pre` with the same
+ * names and layouts), plus its scalars. Pointers are host memory; they are read
+ * during wedg_problem_create only. The v-space tables (P_v, S_v, E*_v) are the
+ * leading q_v+1 columns of the u-space ones and are not passed. */
+typedef struct wedg_pre {
+    int N, qu1, qv1, q_high;
+    double omega;                              /* 2 pi sqrt(kx^2 + ky^2) */
+    double theta, tau, lambda_u, lambda_v;     /* flux parameters (src/solver.jl:16-21) */
+    int mms_on, src_on, one_sided;
+    int mass_solver;                           /* 0 = Chebyshev (the only device solver) */
+    double mass_tol;                           /* 1e-14 */
+    double src_omega, src_Ts;                  /* GaussianSource omega, Ts (src/solver.jl:80-86) */
+    wedg_side bc[4];                           /* bot, top, lft, rgt */
+    /* 1D tables */
+    const double *w, *P_u, *S_u, *Ebl_u, *Ebr_u, *E1bl_u, *E1br_u, *P_h_v, *W2d_h;
+    /* per element */
+    const double *Cw, *Aw;                     /* (q_u+1, E, q_u+1) */
+    const double *Ainv_u;                      /* (n_u^2 (n_u^2+1)/2, E), packed upper triangle */
+    const double *beta_hq;                     /* (q_high, E, q_high) */
+    /* per face point: horizontal (q_u+1, N+1, N), vertical (q_u+1, N, N+1) */
+    const double *hc2, *haf, *hcf, *hbf, *vc2, *vaf, *vcf, *vbf;
+    const double *hc2m, *hafm, *vc2m, *vafm, *hJC, *vJC;
+    /* boundary data (q_u+1, N) */
+    const double *bcB_sp, *bcB_spy, *bcT_sp, *bcT_spy, *bcL_sp, *bcL_spx, *bcR_sp, *bcR_spx;
+    /* projections (q_v+1, q_v+1, N, N) */
+    const double *PiSP, *PiBSP2, *PiCdiv, *PiDdiv, *src_projE;
+} wedg_pre;
+
+typedef struct wedg_problem wedg_problem;
+typedef struct wedg_workspace wedg_workspace;
+
+/* Details of the last failure of a workspace, for the reference's messages. */
+typedef struct wedg_error_info {
+    int status;         /* wedg_status */
+    int side;           /* 0..3 for boundary errors */
+    long key;           /* _boundary_key (src/rhs.jl:732) */
+    long element;       /* first failing element, linear and 1-based (src/rhs.jl:916) */
+    long step;          /* step (1-based, within the last advance) of a failure */
+    int cheb_steps;     /* largest Chebyshev step count of the last evaluation */
+} wedg_error_info;
+
+/* Chebyshev step-count policies. */
+enum {
+    WEDG_CHEB_GLOBAL = 0,   /* every element takes the maximum count over all elements (the reference) */
+    WEDG_CHEB_ELEMENT = 1,  /* every element takes its own a-priori count, same 1e-14 bound */
+};
+
+/* Uploads the tables of one problem to the current device; replaces `pre`. */
+wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** out);
+wedg_status wedg_problem_destroy(wedg_problem* p);
+
+/* Device-resident work arrays and state of one problem (src/rhs.jl:184-220).
+ * cheb_policy is WEDG_CHEB_GLOBAL or WEDG_CHEB_ELEMENT. */
+wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, wedg_workspace** out);
+wedg_status wedg_workspace_destroy(wedg_workspace* ws);
+
+/* edg_rhs!: (u_t, v_t) at time t, host arrays (src/rhs.jl:490-550). N, h are
+ * checked against the problem (WEDG_ERR_DIMENSION). */
+wedg_status wedg_rhs(wedg_workspace* ws, const double* u, const double* v, double t,
+                     int N, double h, double* ut, double* vt);
+
+/* The same on device arrays of the current device, asynchronous on the
+ * workspace stream; errors of the evaluation are reported by wedg_sync. */
+wedg_status wedg_rhs_device(wedg_workspace* ws, const double* u, const double* v, double t,
+                            double* ut, double* vt);
+
+/* rk4_step!: one classic RK4 step in place on host arrays (src/solver.jl:289-321).
+ * *finite receives all(ws.fin). */
+wedg_status wedg_rk4_step(wedg_workspace* ws, double* u, double* v, double t, double dt,
+                          int N, double h, int* finite);
+
+/* Device-resident state of the workspace: set, read back, and advance it by
+ * nsteps RK4 steps from t0 with t += dt after every step (src/solver.jl:213-216).
+ * stop_on_nonfinite = 1 ends the run at the first non-finite step, as the
+ * blow-up check of source mode (src/solver.jl:940). */
+wedg_status wedg_set_state(wedg_workspace* ws, const double* u, const double* v);
+wedg_status wedg_get_state(wedg_workspace* ws, double* u, double* v);
+wedg_status wedg_advance(wedg_workspace* ws, double t0, double dt, long nsteps,
+                         int stop_on_nonfinite, double* t_end);
+/* The same without waiting: the caller calls wedg_sync before reading results. */
+wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double dt, long nsteps,
+                               int stop_on_nonfinite, double* t_end);
+wedg_status wedg_sync(wedg_workspace* ws);
+
+/* Device pointers of the workspace state (u, v) for callers that own the data flow. */
+wedg_status wedg_state_device(wedg_workspace* ws, double** u, double** v);
+
+wedg_status wedg_last_error(const wedg_workspace* ws, wedg_error_info* info);
+
+/* Kernel timing with HIP events on the workspace stream over the next run(s):
+ * enable, then read the accumulated per-kernel-family milliseconds and counts.
+ * family: 0 = traces, 1 = volume and source, 2 = faces, lifts, u-solve and the u update,
+ * 3 = mass solve and the v update. */
+wedg_status wedg_timing_enable(wedg_workspace* ws, int on);
+wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* launches);
+
+/* Library and device facts: compiled architecture, device name, CU count. */
+const char* wedg_version(void);
+wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

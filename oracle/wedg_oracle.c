@@ -667,3 +667,12 @@ int orc_mass_cg(const orc_pre* p, const double* V, const double* R, double* X, i
     }
     return 0;
 }
+
+/* Pointwise boundary flux of one side (src/rhs.jl:100-103), exposed for the
+ * closed-form checks of test/runtests.jl:295-317. */
+void orc_side_flux(const orc_pre* p, int s, double c2f, double af, double cf, double bf, double v_fp,
+                   double gun, double gvn, double gu_ex, double gv_ex, double v_ex, double* F1, double* dv) {
+    side_flux(p, s, c2f, af, cf, bf, v_fp, gun, gvn, gu_ex, gv_ex, v_ex, F1, dv);
+}
+
+int orc_weight_check(double kb, double bf, double v_fp, double v_ex) { return weight_check(kb, bf, v_fp, v_ex); }

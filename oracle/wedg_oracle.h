@@ -58,6 +58,11 @@ int orc_mass_cheb(const orc_pre* p, const double* V, const double* R, double* X,
 int orc_mass_cg(const orc_pre* p, const double* V, const double* R, double* X,
                 int nthreads, orc_status* st);
 
+/* Pointwise boundary flux of side s (0..3) with the parameters in p->kind/gam/… */
+void orc_side_flux(const orc_pre* p, int s, double c2f, double af, double cf, double bf, double v_fp,
+                   double gun, double gvn, double gu_ex, double gv_ex, double v_ex, double* F1, double* dv);
+int orc_weight_check(double kb, double bf, double v_fp, double v_ex);
+
 #ifdef __cplusplus
 }
 #endif

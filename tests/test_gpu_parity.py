@@ -1,0 +1,214 @@
+"""Device parity: the HIP path (libwedg.so through the C ABI) against the CPU
+oracle on the same seeded inputs, over the reference's own test matrix
+(test/runtests.jl:168-293, 340-357, 548-599, 702-732).
+
+Tolerance: 1e-12 relative per right-hand side (max |a - b| / max |a|, the
+reference's rhs_rel), as the reference's own batched-vs-reference tests.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import edg_westervelt_amd as w
+from edg_westervelt_amd import precompute
+from oracle import batched as ob
+from tests.cases import MIXED, MIXED2, quadrant_coefficients, rand_state
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def both(pre, N, u, v, t, cheb="global"):
+    a = ob.rhs(ob.OraclePre(pre, N), u, v, t, cheb_mode=0 if cheb == "global" else 1)
+    b = w.edg_rhs(u, v, t, N, 1.0 / N, pre, cheb=cheb)
+    return a, b
+
+
+@pytest.mark.parametrize("mode", ["mms", "initial_value", "source"])
+@pytest.mark.parametrize("boundary", ["absorbing_2beta", "absorbing", "dirichlet", "wall"])
+def test_rhs_matrix(mode, boundary):
+    """test/runtests.jl:189-202, 266-278: every flux, pairing, q in {1,3,4}, N in {1,2,4}."""
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for q in (1, 3, 4):
+        for N in (1, 2, 4):
+            for flux in ("upwind", "central", "alternating"):
+                for mixed in (True, False):
+                    say(f"case rhs mode={mode} boundary={boundary} q={q} N={N} flux={flux} mixed={mixed}")
+                    pre, _ = precompute(q, N, flux=flux, mixed=mixed, mode=mode, boundary=boundary,
+                                        sigma=0.3, sigma_dirichlet=0.6, kappa=0.7, phase=(0.4, 0.3))
+                    u, v = rand_state(rng, N, pre)
+                    a, b = both(pre, N, u, v, 0.037)
+                    worst = max(worst, rel(a[0], b[0]), rel(a[1], b[1]))
+    assert worst < TOL
+
+
+@pytest.mark.parametrize("boundary", [MIXED, MIXED2])
+@pytest.mark.parametrize("mode", ["mms", "source"])
+def test_rhs_custom_flux_mixed_sides(boundary, mode):
+    """lambda_u != lambda_v (test/runtests.jl:281-293) with per-side kinds."""
+    rng = np.random.default_rng(3)
+    fp = dict(theta=0.3, tau=0.7, lambda_u=0.4, lambda_v=0.9)
+    worst = 0.0
+    for q in (1, 3, 6):
+        for N in (2, 4, 7):
+            for mixed in (True, False):
+                say(f"case custom flux mode={mode} q={q} N={N} mixed={mixed}")
+                pre, _ = precompute(q, N, flux_params=fp, mixed=mixed, mode=mode, boundary=boundary,
+                                    sigma=0.3, sigma_dirichlet=0.6, kappa=0.7, phase=(0.4, 0.3))
+                u, v = rand_state(rng, N, pre)
+                a, b = both(pre, N, u, v, 0.037)
+                worst = max(worst, rel(a[0], b[0]), rel(a[1], b[1]))
+    assert worst < TOL
+
+
+@pytest.mark.parametrize("mode", ["initial_value", "source"])
+def test_rhs_one_sided(mode):
+    """Jumps of c, alpha, beta on x = 1/2, y = 1/2 (test/runtests.jl:716-732)."""
+    rng = np.random.default_rng(6)
+    co = quadrant_coefficients(0.5, 0.5, smooth=True)
+    worst = 0.0
+    for q in (1, 3, 5):
+        for N in (2, 4, 8):
+            for boundary in ("absorbing_2beta", "dirichlet", "wall"):
+                for fp in (dict(theta=1.0, tau=1.0, lambda_u=1.0, lambda_v=1.0),
+                           dict(theta=0.5, tau=0.0, lambda_u=0.0, lambda_v=0.0),
+                           dict(theta=0.3, tau=0.7, lambda_u=0.4, lambda_v=0.9)):
+                    say(f"case one-sided mode={mode} q={q} N={N} boundary={boundary} fp={fp}")
+                    pre, _ = precompute(q, N, flux_params=fp, mode=mode, boundary=boundary, coefficients=co,
+                                        sigma=0.3, sigma_dirichlet=0.6, kappa=0.7, phase=(0.4, 0.3))
+                    assert pre.one_sided
+                    u, v = rand_state(rng, N, pre)
+                    a, b = both(pre, N, u, v, 0.037)
+                    worst = max(worst, rel(a[0], b[0]), rel(a[1], b[1]))
+    assert worst < TOL
+
+
+def test_rhs_larger_meshes_and_element_counts():
+    """q_u = 5 and 6 on larger meshes, both Chebyshev count policies."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for (q, N, mixed) in ((5, 16, False), (6, 12, True), (2, 33, False), (4, 20, True)):
+        pre, _ = precompute(q, N, mixed=mixed, mode="source",
+                            coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0))
+        u, v = rand_state(rng, N, pre, su=1e-2, sv=0.05)
+        for cheb in ("global", "element"):
+            say(f"case large q={q} N={N} mixed={mixed} cheb={cheb}")
+            a, b = both(pre, N, u, v, 0.21, cheb=cheb)
+            worst = max(worst, rel(a[0], b[0]), rel(a[1], b[1]))
+    assert worst < TOL
+
+
+def test_initialization_known_answer():
+    """Zero state: u_t = 0 and v_t = L2 projection of -w^2 Phi (test/runtests.jl:168-184)."""
+    for boundary in ("absorbing_2beta", "dirichlet", "wall", MIXED):
+        for flux in ("upwind", "central", "alternating"):
+            for mixed in (True, False):
+                say(f"case init boundary={boundary} flux={flux} mixed={mixed}")
+                N = 4
+                pre, _ = precompute(3, N, flux=flux, mixed=mixed, boundary=boundary, sigma=0.3, kappa=0.5,
+                                    phase=(0.4, 0.3))
+                u = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+                v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+                ut, vt = w.edg_rhs(u, v, 0.0, N, 1.0 / N, pre)
+                assert np.abs(ut).max() < 1e-12
+                M = pre.Phi2d_h_v.T @ (pre.W2d_h.ravel(order="F")[:, None] * pre.Phi2d_h_v)
+                r = s = 0.0
+                for j in range(N):
+                    for k in range(N):
+                        b = pre.P_h_v.T @ (pre.W2d_h * (-pre.omega ** 2 * pre.mms_SP[j, k])) @ pre.P_h_v
+                        proj = np.linalg.solve(M, b.ravel(order="F")).reshape(pre.qv1, pre.qv1, order="F")
+                        r = max(r, np.abs(vt[j, k] - proj).max())
+                        s = max(s, np.abs(proj).max())
+                assert r / s < 1e-12
+
+
+def test_rk4_steps_match_oracle():
+    """RK4 stage algebra and the final combination (src/solver.jl:289-321)."""
+    rng = np.random.default_rng(5)
+    for (q, N, mode) in ((3, 7, "mms"), (2, 5, "source"), (4, 6, "initial_value")):
+        say(f"case rk4 q={q} N={N} mode={mode}")
+        pre, _ = precompute(q, N, mode=mode, boundary=MIXED, sigma=0.3, sigma_dirichlet=0.6, kappa=0.7,
+                            phase=(0.4, 0.3))
+        u, v = rand_state(rng, N, pre, su=1e-2, sv=0.05)
+        op = ob.OraclePre(pre, N)
+        uo, vo, _, _ = ob.rk4_steps(op, u, v, 0.01, 1e-4, 1)
+        ws = w.Workspace(pre, N)
+        ug = np.array(u, order="F", copy=True)
+        vg = np.array(v, order="F", copy=True)
+        w.rk4_step_(ug, vg, 0.01, 1e-4, N, 1.0 / N, pre, ws)
+        assert all(ws.fin)
+        assert rel(uo - u, ug - u) < 1e-11 and rel(vo - v, vg - v) < 1e-11
+        assert rel(uo, ug) < 1e-13 and rel(vo, vg) < 1e-13
+
+
+def test_solve_cfg1_golden():
+    """solve(3, 10, 0.1; flux=:upwind, mixed=true) against data/convergence/results.csv:364."""
+    say("case solve cfg1")
+    e = w.solve(3, 10, 0.1, flux="upwind", mixed=True)
+    gold = dict(erru=1.0717765588e-04, errv=4.5772517217e-03, errgradu=7.1299988239e-03, errE=5.9884034343e-03)
+    for k, g in gold.items():
+        assert abs(getattr(e, k) / g - 1) < 1e-9, (k, getattr(e, k), g)
+
+
+def test_solve_regression_n6():
+    """test/runtests.jl:207-217 (rtol 1e-9), both boundary weights."""
+    say("case solve regression N=6")
+    e = w.solve(3, 6, 0.1, flux="upwind", mixed=True)
+    for k, g in dict(erru=5.9847841944e-04, errv=1.9998237454e-02, errgradu=2.4571643407e-02,
+                     errE=2.2400357448e-02).items():
+        assert abs(getattr(e, k) / g - 1) < 1e-9
+    e = w.solve(3, 6, 0.1, flux="upwind", mixed=True, boundary="absorbing")
+    for k, g in dict(erru=5.9847813824e-04, errv=1.9998249758e-02, errgradu=2.4571634466e-02,
+                     errE=2.2400354434e-02).items():
+        assert abs(getattr(e, k) / g - 1) < 1e-9
+
+
+def test_errors_in_serial_order():
+    """Boundary weight before mass matrix; first failure in serial order
+    (test/runtests.jl:548-577)."""
+    N = 8
+    h = 1.0 / N
+    co = w.replace_coefficients(w.paper_coefficients(), alpha=1e-6, beta=1.0)
+    bd = dict(bot="absorbing_2beta", top="absorbing", lft="absorbing", rgt="wall")
+    pre, _ = precompute(3, N, coefficients=co, boundary=bd)
+    u = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+    v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+    v[0, N - 1, 0, 0] = 2.5
+    v[3, 0, 0, 0] = 2.5
+    wv = np.zeros_like(v)
+    wv[3, 5, 0, 0] = 2.5
+    wv[4, 1, 0, 0] = 2.5
+    for cheb in ("global", "element"):
+        say(f"case errors cheb={cheb}")
+        with pytest.raises(w.WedgError) as ei:
+            w.edg_rhs(u, v, 0.0, N, h, pre, cheb=cheb)
+        assert str(ei.value) == "boundary weight 1 - 2βv_h ≤ 0 at a face point (absorbing_2beta side)"
+        with pytest.raises(w.WedgError) as ei:
+            w.edg_rhs(u, wv, 0.0, N, h, pre, cheb=cheb)
+        assert str(ei.value) == "mass matrix not positive definite on element 13: 1 - 2βv_h ≤ 0"
+    pre, _ = precompute(3, N, coefficients=co, phase=(0.4, 0.3),
+                        boundary=dict(bot="wall", top="wall", lft="absorbing", rgt="absorbing_2beta"))
+    with pytest.raises(w.WedgError) as ei:
+        w.edg_rhs(np.zeros_like(u), np.zeros_like(v), 0.07, N, h, pre)
+    assert str(ei.value) == "boundary weight 1 - βg ≤ 0 for the boundary data g (absorbing side)"
+
+
+def test_dimension_mismatch():
+    """Sizes are refused before any kernel (test/runtests.jl:588-595)."""
+    pre, _ = precompute(2, 8)
+    pre9, _ = precompute(2, 9)
+    with pytest.raises(w.DimensionMismatch):
+        w.edg_rhs(np.zeros((8, 8, 3, 3), order="F"), np.zeros((8, 7, 2, 2), order="F"), 0.0, 8, 1 / 8, pre)
+    with pytest.raises(w.DimensionMismatch):
+        w.edg_rhs_(w.Workspace(pre9, 9), np.zeros((8, 8, 3, 3), order="F"), np.zeros((8, 8, 2, 2), order="F"),
+                   0.0, 8, 1 / 8, pre)
+    with pytest.raises(w.DimensionMismatch):
+        w.edg_rhs(np.zeros((8, 8, 3, 3), order="F"), np.zeros((8, 8, 2, 2), order="F"), 0.0, 8, 1 / 8, pre9)
