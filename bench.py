@@ -1,0 +1,201 @@
+"""Throughput of the device right-hand side + RK4 on the large single instance.
+
+Workload (one MI355X): q_u = q_v = 5 on a uniform 256 x 256 mesh (65,536
+elements, 36 modes per field per element, 4,718,592 DOF), mode :initial_value
+with the Gaussian u0 = 0.05 exp(-|x - (1/2,1/2)|^2 / (2 0.05^2)), constant
+coefficients c = 1, alpha = 1e-4, beta = 2, upwind flux, absorbing_2beta on all
+sides, Chebyshev mass solve to 1e-14, FP64 RK4 with dt = h / (10 q_u^2)
+(32,000 steps to T = 0.5). A step is one RK4 step (four right-hand sides) of
+the whole mesh; the state stays in HBM for the timed region.
+
+Prints one JSON line: DOF-stages/s, the roofline of the dominant kernel from
+HIP events on the workspace stream, and the CPU oracle (C/OpenMP restatement
+of the reference's algorithm) timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6        # MI355X FP64 (vector = matrix), TFLOP/s
+METRIC = "FP64 DOF·stages/s for edg_rhs!+RK4"
+FAMILIES = ("traces", "volume", "faces+u-solve", "mass")
+
+
+def gaussian_profile(s=0.05, amp=0.05):
+    def u0(x, y):
+        return amp * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / (2 * s * s))
+
+    return dict(u0=u0,
+                u0_x=lambda x, y: -(x - 0.5) / s ** 2 * u0(x, y),
+                u0_y=lambda x, y: -(y - 0.5) / s ** 2 * u0(x, y),
+                u0_xx=lambda x, y: ((x - 0.5) ** 2 / s ** 4 - 1 / s ** 2) * u0(x, y),
+                u0_yy=lambda x, y: ((y - 0.5) ** 2 / s ** 4 - 1 / s ** 2) * u0(x, y))
+
+
+def build_problem(q, N):
+    import edg_westervelt_amd as w
+    pre, geo = w.precompute(q, N, mixed=False, flux="upwind", boundary="absorbing_2beta",
+                            mode="initial_value", initial_profile=gaussian_profile(),
+                            coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0))
+    return pre, geo
+
+
+def flops_per_element_stage(nu, nv, nh, nc):
+    """Algorithmic FP64 flops of one element and stage (traces, volume, u-solve,
+    lifts, Chebyshev set-up, nc - 1 Chebyshev steps)."""
+    nq = nu
+    base = (2 * (4 * (nu * nu + nq * nu) + 8 * (nv * nv + nq * nv))
+            + 2 * (2 * (nq * nu * nu + nq * nq * nu) + 2 * (nq * nv * nv + nq * nq * nv)
+                   + 2 * (nv * nq * nq + nv * nv * nq) + 2 * (nu * nq * nq + nu * nu * nq))
+            + 2 * nu ** 4 + 4 * (3 * nq * (2 * nv + nu) + 3 * (2 * nv * nv + nu * nu))
+            + 2 * (nh * nv * nv + nh * nh * nv))
+    step = 4 * (nh * nv * nv + nh * nh * nv) + nh * nh + 6 * nv * nv
+    mass = 2 * (nh * nv * nv + nh * nh * nv) + (nc - 1) * step
+    return base + (nc - 1) * step, mass
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed RK4 steps")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--full", action="store_true", help="time all 32,000 steps to T = 0.5")
+    ap.add_argument("--cheb", choices=("global", "element"), default="global")
+    ap.add_argument("--cpu-steps", type=int, default=16, help="RK4 steps of the CPU oracle sample (0: skip)")
+    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
+    ap.add_argument("--q", type=int, default=5)
+    ap.add_argument("--N", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import _lib
+    from edg_westervelt_amd.solver import time_step
+
+    q, N, T = args.q, args.N, 0.5
+    t_pre = time.perf_counter()
+    pre, geo = build_problem(q, N)
+    t_pre = time.perf_counter() - t_pre
+    dt, nsteps_full = time_step(q, N, T)
+    steps = nsteps_full if args.full else args.steps
+    E = N * N
+    nu, nv, nh = pre.qu1, pre.qv1, pre.q_high
+    dof = E * (nu * nu + nv * nv)
+
+    ws = w.Workspace(pre, N, cheb=args.cheb)
+    h = ws.dev.h
+    L = _lib.lib
+    u = np.zeros((N, N, nu, nu), order="F")
+    v = np.zeros((N, N, nv, nv), order="F")
+    ws.dev.check(L.wedg_set_state(h, u.ctypes.data_as(C.POINTER(C.c_double)), v.ctypes.data_as(C.POINTER(C.c_double))))
+    tend = C.c_double(0.0)
+    print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    ws.dev.check(L.wedg_advance(h, 0.0, dt, args.warmup, 0, C.byref(tend)))
+    t0 = tend.value
+    # the CPU sample starts from this state, so both time the same window
+    uw = np.zeros_like(u)
+    vw = np.zeros_like(v)
+    ws.dev.check(L.wedg_get_state(h, uw.ctypes.data_as(C.POINTER(C.c_double)), vw.ctypes.data_as(C.POINTER(C.c_double))))
+
+    # timed region: the state is already in HBM; synchronized on both sides
+    print(f"# timing {steps} steps", file=sys.stderr, flush=True)
+    ws.dev.check(L.wedg_sync(h))
+    c0 = time.perf_counter()
+    ws.dev.check(L.wedg_advance_async(h, t0, dt, steps, 0, C.byref(tend)))
+    ws.dev.check(L.wedg_sync(h))
+    c1 = time.perf_counter()
+    elapsed = c1 - c0
+    mean_nc = C.c_double(0.0)
+    nst = C.c_long(0)
+    L.wedg_cheb_stats(h, C.byref(mean_nc), C.byref(nst))
+    info = ws.dev.last_error()
+
+    # per-kernel durations: HIP events around every launch on the workspace stream
+    tsteps = min(50, max(5, steps // 20))
+    L.wedg_timing_enable(h, 1)
+    ws.dev.check(L.wedg_advance(h, tend.value, dt, tsteps, 0, C.byref(tend)))
+    fam = {}
+    for i, name in enumerate(FAMILIES):
+        ms = C.c_double(0.0)
+        n = C.c_long(0)
+        L.wedg_timing_read(h, i, C.byref(ms), C.byref(n))
+        fam[name] = dict(ms_total=ms.value, launches=n.value, us_avg=1e3 * ms.value / max(1, n.value))
+    L.wedg_timing_enable(h, 0)
+    L.wedg_cheb_stats(h, C.byref(mean_nc), C.byref(nst))
+    nc_timing = mean_nc.value
+
+    # final state is finite (the run is meaningful)
+    ws.dev.check(L.wedg_get_state(h, u.ctypes.data_as(C.POINTER(C.c_double)), v.ctypes.data_as(C.POINTER(C.c_double))))
+    finite = bool(np.isfinite(u).all() and np.isfinite(v).all())
+
+    stages = 4 * steps
+    value = dof * stages / elapsed
+    ms_per_step = 1e3 * elapsed / steps
+
+    # roofline of the dominant kernel (the Chebyshev mass solve)
+    f_total, f_mass = flops_per_element_stage(nu, nv, nh, nc_timing)
+    dom = max(fam, key=lambda k: fam[k]["us_avg"])
+    dom_s = fam[dom]["us_avg"] * 1e-6
+    if dom == "mass":
+        achieved = E * f_mass / dom_s / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=FP64_PEAK_TFS, unit="TFLOP/s",
+                    frac=achieved / FP64_PEAK_TFS)
+    else:
+        roof = dict(bound="mfma", achieved=None, peak=FP64_PEAK_TFS, unit="TFLOP/s", frac=None)
+    roof["kernel"] = dom
+    roof["traffic"] = None
+    if args.traffic and os.path.exists(args.traffic):
+        roof["traffic"] = json.load(open(args.traffic)).get(dom)
+    stage_s = sum(f["us_avg"] for f in fam.values()) * 1e-6
+    bytes_step = 8 * 17 * dof
+    whole = dict(flops_per_stage=E * f_total, tflops=E * f_total / stage_s / 1e12,
+                 fp64_frac=E * f_total / stage_s / 1e12 / FP64_PEAK_TFS,
+                 hbm_gbs_model=bytes_step / 4 / stage_s / 1e9,
+                 hbm_frac_model=bytes_step / 4 / stage_s / 1e9 / HBM_PEAK_GBS,
+                 mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
+
+    # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
+    # problem, a bounded sample of RK4 steps from the same initial state
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import batched as ob
+        ob.build() if not os.path.exists(ob._SO) else None
+        threads = min(16, os.cpu_count() or 1)
+        op = ob.OraclePre(pre, N)
+        print(f"# cpu oracle {args.cpu_steps} steps on {threads} threads", file=sys.stderr, flush=True)
+        c0 = time.perf_counter()
+        ob.rk4_steps(op, uw, vw, t0, dt, args.cpu_steps, cheb_mode=0 if args.cheb == "global" else 1,
+                     nthreads=threads)
+        cs = time.perf_counter() - c0
+        cpu = dict(value=dof * 4 * args.cpu_steps / cs, unit="DOF·stages/s", cores=threads, kind="port",
+                   sample=f"{args.cpu_steps} RK4 steps of the same instance from the state after the warmup "
+                          f"({cs:.1f} s, C/OpenMP oracle)")
+
+    out = dict(metric=METRIC, value=value, unit="DOF·stages/s", n_gpus=1, steps=steps, warmup=args.warmup,
+               ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic",
+               config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, c=1 alpha=1e-4 beta=2, "
+                                    f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
+                           elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, full_run_steps=nsteps_full,
+                           precompute_s=round(t_pre, 2)),
+               roofline=roof, whole_step=whole, kernels_us=fam, cpu_baseline=cpu, final_state_finite=finite)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

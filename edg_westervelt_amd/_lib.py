@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwedg.so")
+LIB_PATH = os.environ.get("WEDG_LIBRARY") or os.path.join(_HERE, "libwedg.so")
 
 KINDS = {"absorbing_2beta": 0, "absorbing": 1, "dirichlet": 2, "wall": 3}
 SIDES = ("bot", "top", "lft", "rgt")
@@ -75,6 +75,7 @@ def _load():
         "wedg_timing_enable": [vp, C.c_int],
         "wedg_timing_read": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "wedg_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+        "wedg_cheb_stats": [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -94,13 +95,25 @@ struct wedg_problem {
     int has_proj[5] = {};
     size_t device_bytes = 0;
 
+    // Host data reaches the device only through this pinned buffer: copies
+    // straight from pageable memory would make the runtime register (pin) the
+    // caller's memory on the fly, which is not available on every system.
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
+
     template <class T>
     T* upload(const T* h, size_t n, hipError_t* err) {
         T* d = dalloc<T>(n, err);
         if (*err != hipSuccess) return nullptr;
         allocs.push_back(d);
         device_bytes += n * sizeof(T);
-        *err = hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+        size_t done = 0, total = n * sizeof(T);
+        while (done < total && *err == hipSuccess) {
+            const size_t chunk = std::min(stage_bytes, total - done);
+            std::memcpy(stage, reinterpret_cast<const char*>(h) + done, chunk);
+            *err = hipMemcpy(reinterpret_cast<char*>(d) + done, stage, chunk, hipMemcpyHostToDevice);
+            done += chunk;
+        }
         return d;
     }
 };
@@ -115,10 +128,13 @@ struct wedg_workspace {
     double* tr = nullptr;
     double *Rb = nullptr, *Bb = nullptr;
     DevErr* err = nullptr;
-    DevErr err_init{};       // host source of the reset copy (kept alive with the workspace)
+    DevErr* herr = nullptr;  // pinned host copy of the failure record
+    double *hu = nullptr, *hv = nullptr;   // pinned staging of the state (host <-> device)
     int* cnt = nullptr;
     int serial = 0;
     long step_base = 0;
+    long stages = 0;                 // stages run since the last reset
+    unsigned long long cheb_wave_sum = 0;
     wedg_error_info last{};
     // timing
     int timing = 0;
@@ -153,6 +169,11 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
     if (nh_expect < nu) nh_expect = nu;
     if (nh != nh_expect) return WEDG_ERR_DIMENSION;
     auto* p = new wedg_problem();
+    p->stage_bytes = (size_t)4 << 20;
+    if (hipHostMalloc(&p->stage, p->stage_bytes, hipHostMallocDefault) != hipSuccess) {
+        delete p;
+        return WEDG_ERR_HIP;
+    }
     p->N = N; p->E = N * N; p->nu = nu; p->nv = nv; p->nh = nh; p->qu = nu - 1;
     p->omega = pre->omega; p->theta = pre->theta; p->tau = pre->tau;
     p->lu = pre->lambda_u; p->lv = pre->lambda_v; p->mass_tol = pre->mass_tol;
@@ -284,6 +305,8 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         }
     }
 #undef UP
+    (void)hipHostFree(p->stage);
+    p->stage = nullptr;
     *out = p;
     return WEDG_OK;
 }
@@ -291,14 +314,34 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
 extern "C" wedg_status wedg_problem_destroy(wedg_problem* p) {
     if (!p) return WEDG_OK;
     for (void* a : p->allocs) (void)hipFree(a);
+    if (p->stage) (void)hipHostFree(p->stage);
     delete p;
     return WEDG_OK;
 }
 
 static wedg_status reset_err(wedg_workspace* ws) {
-    DevErr& h = ws->err_init;
+    // the pinned record may still be the source of a copy in flight
+    HIPOK(hipStreamSynchronize(ws->stream));
+    DevErr& h = *ws->herr;
     h.bkey = INT_MAX; h.mfail = INT_MAX; h.failed = 0; h.nonfinite = INT_MAX; h.maxcheb = 0;
+    h.cheb_wave_sum = 0;
+    ws->stages = 0;
     HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
+    return WEDG_OK;
+}
+
+// host <-> device copies of user arrays through the pinned staging buffers
+static wedg_status stage_in(wedg_workspace* ws, double* d, double* pinned, const double* h, size_t n) {
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(pinned, h, n * sizeof(double));
+    HIPOK(hipMemcpyAsync(d, pinned, n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    return WEDG_OK;
+}
+
+static wedg_status stage_out(wedg_workspace* ws, double* h, double* pinned, const double* d, size_t n) {
+    HIPOK(hipMemcpyAsync(pinned, d, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(h, pinned, n * sizeof(double));
     return WEDG_OK;
 }
 
@@ -326,16 +369,23 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
         he = hipMalloc(&e, sizeof(DevErr) + 16);
         if (he == hipSuccess) { ws->allocs.push_back(e); ws->err = (DevErr*)e; }
     }
+    if (he == hipSuccess) he = hipHostMalloc((void**)&ws->herr, sizeof(DevErr), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipHostMalloc((void**)&ws->hu, nu * sizeof(double), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipHostMalloc((void**)&ws->hv, nv * sizeof(double), hipHostMallocDefault);
     if (he == hipSuccess) {
         void* c = nullptr;
         he = hipMalloc(&c, 4 * sizeof(int));
         if (he == hipSuccess) {
             ws->allocs.push_back(c);
             ws->cnt = (int*)c;
-            int init[4] = {1, 1, 0, 0};
-            he = hipMemcpy(c, init, sizeof(init), hipMemcpyHostToDevice);
+            // the counts start at 1 (src/rhs.jl:895); written by a fill, no host copy
+            he = hipMemset(c, 0, 4 * sizeof(int));
+            if (he == hipSuccess) he = hipMemset(c, 1, 1);
+            if (he == hipSuccess) he = hipMemset((char*)c + 4, 1, 1);
         }
     }
+    // the fills above run on the null stream; finish them before the workspace stream is used
+    if (he == hipSuccess) he = hipDeviceSynchronize();
     if (he != hipSuccess) { wedg_workspace_destroy(ws); return WEDG_ERR_HIP; }
     if (reset_err(ws) != WEDG_OK || hipStreamSynchronize(ws->stream) != hipSuccess) {
         wedg_workspace_destroy(ws);
@@ -351,6 +401,9 @@ extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     for (auto& u : ws->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto& e : ws->ev_pool) (void)hipEventDestroy(e);
     for (void* a : ws->allocs) (void)hipFree(a);
+    if (ws->herr) (void)hipHostFree(ws->herr);
+    if (ws->hu) (void)hipHostFree(ws->hu);
+    if (ws->hv) (void)hipHostFree(ws->hv);
     if (ws->stream) (void)hipStreamDestroy(ws->stream);
     delete ws;
     return WEDG_OK;
@@ -444,6 +497,7 @@ static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, co
     stage_time(ws->p, t, a);
     a.yu = yu; a.yv = yv; a.y0u = y0u; a.y0v = y0v;
     a.accu = ws->au; a.accv = ws->av; a.outu = outu; a.outv = outv;
+    ++ws->stages;
     for (int which = 0; which < 4; ++which) {
         wedg_status st = run_kernel(ws, a, which);
         if (st != WEDG_OK) return st;
@@ -476,11 +530,13 @@ static wedg_status collect(wedg_workspace* ws, int nonfinite_is_error) {
         }
         ws->ev_used.clear();
     }
-    DevErr h;
-    HIPOK(hipMemcpy(&h, ws->err, sizeof(h), hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpyAsync(ws->herr, ws->err, sizeof(DevErr), hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    const DevErr h = *ws->herr;
     wedg_error_info& L = ws->last;
     std::memset(&L, 0, sizeof(L));
     L.cheb_steps = h.maxcheb;
+    ws->cheb_wave_sum = h.cheb_wave_sum;
     const int nq = ws->p->nu, N = ws->p->N;
     if (h.bkey != INT_MAX) {
         L.status = (h.bkey % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
@@ -520,31 +576,30 @@ extern "C" wedg_status wedg_rhs(wedg_workspace* ws, const double* u, const doubl
                                 double h, double* ut, double* vt) {
     if (!ws || !u || !v || !ut || !vt) return WEDG_ERR_ARGUMENT;
     if (N != ws->p->N || h != 1.0 / N) return WEDG_ERR_DIMENSION;
-    HIPOK(hipMemcpyAsync(ws->su, u, ws->nU() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
-    HIPOK(hipMemcpyAsync(ws->sv, v, ws->nV() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
     wedg_status s;
+    if ((s = stage_in(ws, ws->su, ws->hu, u, ws->nU()))) return s;
+    if ((s = stage_in(ws, ws->sv, ws->hv, v, ws->nV()))) return s;
     if ((s = reset_err(ws))) return s;
     if ((s = run_stage(ws, 0, ws->su, ws->sv, ws->su, ws->sv, ws->ou, ws->ov, t, 0.0, 0))) return s;
     if ((s = collect(ws, 0))) return s;
-    HIPOK(hipMemcpy(ut, ws->ou, ws->nU() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPOK(hipMemcpy(vt, ws->ov, ws->nV() * sizeof(double), hipMemcpyDeviceToHost));
-    return WEDG_OK;
+    if ((s = stage_out(ws, ut, ws->hu, ws->ou, ws->nU()))) return s;
+    return stage_out(ws, vt, ws->hv, ws->ov, ws->nV());
 }
 
 extern "C" wedg_status wedg_set_state(wedg_workspace* ws, const double* u, const double* v) {
     if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
-    HIPOK(hipMemcpyAsync(ws->yu, u, ws->nU() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
-    HIPOK(hipMemcpyAsync(ws->yv, v, ws->nV() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    wedg_status s;
+    if ((s = stage_in(ws, ws->yu, ws->hu, u, ws->nU()))) return s;
+    if ((s = stage_in(ws, ws->yv, ws->hv, v, ws->nV()))) return s;
     HIPOK(hipStreamSynchronize(ws->stream));
     return WEDG_OK;
 }
 
 extern "C" wedg_status wedg_get_state(wedg_workspace* ws, double* u, double* v) {
     if (!ws || !u || !v) return WEDG_ERR_ARGUMENT;
-    HIPOK(hipStreamSynchronize(ws->stream));
-    HIPOK(hipMemcpy(u, ws->yu, ws->nU() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPOK(hipMemcpy(v, ws->yv, ws->nV() * sizeof(double), hipMemcpyDeviceToHost));
-    return WEDG_OK;
+    wedg_status s;
+    if ((s = stage_out(ws, u, ws->hu, ws->yu, ws->nU()))) return s;
+    return stage_out(ws, v, ws->hv, ws->yv, ws->nV());
 }
 
 extern "C" wedg_status wedg_state_device(wedg_workspace* ws, double** u, double** v) {
@@ -607,6 +662,15 @@ extern "C" wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* 
     if (!ws || family < 0 || family > 3) return WEDG_ERR_ARGUMENT;
     if (ms) *ms = ws->t_ms[family];
     if (launches) *launches = ws->t_n[family];
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_steps, long* stages) {
+    if (!ws) return WEDG_ERR_ARGUMENT;
+    const long waves = (ws->p->E + 63) / 64;
+    if (stages) *stages = ws->stages;
+    if (mean_steps)
+        *mean_steps = ws->stages > 0 ? (double)ws->cheb_wave_sum / ((double)waves * ws->stages) : 0.0;
     return WEDG_OK;
 }
 

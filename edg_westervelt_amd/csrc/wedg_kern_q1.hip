@@ -8,16 +8,18 @@ template <int NU, int NV>
 static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int uni) {
     const int grid = (a.E + kBlock - 1) / kBlock;
     switch (which) {
-        case 0: k_traces<NU, NV><<<grid, kBlock, 0, s>>>(a); break;
+        case 0: k_traces<NU, NV><<<grid, kBlock, 0, s>>>(a, a.tab, a.beta); break;
         case 1:
-            if (uni) k_volume<NU, NV, true><<<grid, kBlock, 0, s>>>(a, a.Rbuf, a.Bbuf);
-            else k_volume<NU, NV, false><<<grid, kBlock, 0, s>>>(a, a.Rbuf, a.Bbuf);
+            if (uni) k_volume<NU, NV, true><<<grid, kBlock, 0, s>>>(a, a.tab, a.Cw, a.Aw, a.Rbuf, a.Bbuf);
+            else k_volume<NU, NV, false><<<grid, kBlock, 0, s>>>(a, a.tab, a.Cw, a.Aw, a.Rbuf, a.Bbuf);
             break;
         case 2:
-            if (uni) k_faces<NU, NV, true><<<grid, kBlock, 0, s>>>(a, a.Rbuf, a.Bbuf);
-            else k_faces<NU, NV, false><<<grid, kBlock, 0, s>>>(a, a.Rbuf, a.Bbuf);
+            if (uni) k_faces<NU, NV, true><<<grid, kBlock, 0, s>>>(a, a.tab, a.Ainv, a.Rbuf, a.Bbuf);
+            else k_faces<NU, NV, false><<<grid, kBlock, 0, s>>>(a, a.tab, a.Ainv, a.Rbuf, a.Bbuf);
             break;
-        case 3: k_mass<NU, NV><<<grid, kBlock, 0, s>>>(a, a.Rbuf); break;
+        case 3:
+            k_mass<NU, NV><<<(a.E + kMassBlock - 1) / kMassBlock, kMassBlock, 0, s>>>(a, a.tab, a.beta, a.Rbuf);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -29,7 +29,9 @@
 
 namespace wedg {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;      // lanes per block of the element kernels
+constexpr int kMassBlock = 64;   // one wave per block for the mass solve (per-lane LDS)
+constexpr int kMassLdsDoubles = 80;   // Chebyshev weights per lane kept in LDS: 4 waves fill 160 KB
 
 template <int NU_, int NV_>
 struct Dims {
@@ -65,6 +67,7 @@ struct DevErr {
     int nonfinite;     // first RK4 step with a non-finite entry (INT_MAX = none)
     int maxcheb;       // largest Chebyshev count taken
     int pad[3];
+    unsigned long long cheb_wave_sum;   // sum over waves and stages of the count each wave ran
 };
 
 enum { F_HC2 = 0, F_HAF, F_HCF, F_HBF, F_VC2, F_VAF, F_VCF, F_VBF,
@@ -102,8 +105,24 @@ struct KArgs {
     int* cnt;                               // Chebyshev counts, two slots
 };
 
-__device__ __forceinline__ double fv(const FaceArr& a, int i, int idx, int stride) {
-    return a.uni ? a.p[0] : a.p[(long)i * stride + idx];
+// value of a face array at flat index fi = i * (N (N + 1)) + face
+__device__ __forceinline__ double fv(const FaceArr& a, long fi) { return a.p[a.uni ? 0 : fi]; }
+
+
+// The pointer, opaque to the optimizer: loads through it are issued where they
+// are used instead of being hoisted out of a loop and kept alive (the tables
+// are small and read from the scalar cache at every use).
+template <class T>
+__device__ __forceinline__ const T* fresh(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(p));
+#endif
+    return p;
+}
+
+__device__ __forceinline__ bool skip_stage(const KArgs& a) {
+    const int fl = a.err->failed;
+    return fl != 0 && fl < a.serial;
 }
 
 // ---------------------------------------------------------------------------
@@ -145,68 +164,83 @@ __device__ __forceinline__ void trace4(const double* __restrict__ T, const doubl
     }
 }
 
-// G = (2 beta) o (P_h V P_h^T) and the interval data of the Chebyshev
-// iteration (src/rhs.jl:889-907). Returns the a-priori step count, or 0 when
-// 1 - 2 beta v_q <= 0 somewhere. G is left without the weights W.
+// One row p of g = (2 beta) o (P_h V P_h^T) at the high-order points.
 template <class D>
-__device__ __forceinline__ int cheb_setup(const double* __restrict__ T, const double (&V)[D::NV * D::NV],
-                                          const double* __restrict__ beta, double L,
-                                          double (&G)[D::NH * D::NH], double& th, double& de, double& s1) {
+__device__ __forceinline__ void g_row(const double* __restrict__ T, const double (&V)[D::NV * D::NV],
+                                      const double* __restrict__ beta, int p, double (&g)[D::NH]) {
     constexpr int NV = D::NV, NH = D::NH;
-    double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
+    double tp[NV];
 #pragma unroll
-    for (int p = 0; p < NH; ++p) {
-        double tp[NV];
+    for (int b = 0; b < NV; ++b) {
+        double s = 0.0;
 #pragma unroll
-        for (int b = 0; b < NV; ++b) {
-            double s = 0.0;
-#pragma unroll
-            for (int a = 0; a < NV; ++a) s = fma(T[D::oPh + p + NH * a], V[a + NV * b], s);
-            tp[b] = s;
-        }
-#pragma unroll
-        for (int q = 0; q < NH; ++q) {
-            double s = 0.0;
-#pragma unroll
-            for (int b = 0; b < NV; ++b) s = fma(tp[b], T[D::oPh + q + NH * b], s);
-            const double g = s * (2.0 * beta[p + NH * q]);
-            G[p + NH * q] = g;
-            gmin = fmin(gmin, g);
-            gmax = fmax(gmax, g);
-        }
+        for (int a = 0; a < NV; ++a) s = fma(T[D::oPh + p + NH * a], V[a + NV * b], s);
+        tp[b] = s;
     }
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NV; ++b) s = fma(tp[b], T[D::oPh + q + NH * b], s);
+        g[q] = s * (2.0 * beta[p + NH * q]);
+    }
+}
+
+// Interval data of the Chebyshev iteration from the extremes of g
+// (src/rhs.jl:896-904): the a-priori step count, or 0 when 1 - 2 beta v_q <= 0
+// somewhere (or a value is not a number).
+__device__ __forceinline__ int cheb_interval(double gmin, double gmax, bool nan, double L, double& th,
+                                             double& de, double& s1) {
     const double amin = 1.0 - gmax, amax = 1.0 - gmin;
-    if (!(amin > 0.0)) return 0;
+    if (nan || !(amin > 0.0)) return 0;
     th = 0.5 * (amax + amin);
     de = fmax(0.5 * (amax - amin), 1e-15 * th);
     s1 = th / de;
-    int n = (int)ceil(L / acosh(s1));
+    const int n = (int)ceil(L / acosh(s1));
     return n < 1 ? 1 : n;
 }
 
 // ---------------------------------------------------------------------------
+// Traces of every element, and with the global policy the largest Chebyshev
+// step count (src/rhs.jl:889-918) reduced over the grid.
 template <int NU, int NV>
-__global__ __launch_bounds__(kBlock) void k_traces(const KArgs a) {
+__global__ __launch_bounds__(kBlock) void k_traces(const KArgs a, const double* __restrict__ T0,
+                                                   const double* __restrict__ beta0) {
     using D = Dims<NU, NV>;
+    constexpr int NQ = D::NQ, NH = D::NH;
     if (a.err->failed != 0) return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     const int E = a.E;
     int n_e = 1;
     if (e < E) {
-        const double* __restrict__ T = a.tab;
-        double U[NU * NU], V[NV * NV];
+        {
+            double U[NU * NU];
 #pragma unroll
-        for (int m = 0; m < NU * NU; ++m) U[m] = a.yu[(long)m * E + e];
+            for (int m = 0; m < NU * NU; ++m) U[m] = a.yu[(long)m * E + e];
+            trace4<NQ, NU, D::oP, D::oE1l, D::oE1r>(fresh(T0), U, a.tr, 0, e, E);
+        }
+        double V[NV * NV];
 #pragma unroll
         for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-        trace4<D::NQ, NU, D::oP, D::oE1l, D::oE1r>(T, U, a.tr, 0, e, E);
-        trace4<D::NQ, NV, D::oP, D::oEl, D::oEr>(T, V, a.tr, 4, e, E);
-        trace4<D::NQ, NV, D::oP, D::oE1l, D::oE1r>(T, V, a.tr, 8, e, E);
+        trace4<NQ, NV, D::oP, D::oEl, D::oEr>(fresh(T0), V, a.tr, 4, e, E);
+        trace4<NQ, NV, D::oP, D::oE1l, D::oE1r>(fresh(T0), V, a.tr, 8, e, E);
         if (a.cheb_global) {
-            const double* beta = a.beta + (long)a.cls_b[e] * D::NH * D::NH;
-            double G[D::NH * D::NH];
+            const double* beta = beta0 + (long)a.cls_b[e] * NH * NH;
+            double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
+            bool nan = false;
+#pragma unroll 1
+            for (int p = 0; p < NH; ++p) {
+                double g[NH];
+                g_row<D>(fresh(T0), V, beta, p, g);
+#pragma unroll
+                for (int q = 0; q < NH; ++q) {
+                    gmin = fmin(gmin, g[q]);
+                    gmax = fmax(gmax, g[q]);
+                    nan = nan || (g[q] != g[q]);
+                }
+            }
             double th, de, s1;
-            n_e = cheb_setup<D>(T, V, beta, a.cheb_L, G, th, de, s1);
+            n_e = cheb_interval(gmin, gmax, nan, a.cheb_L, th, de, s1);
             if (n_e == 0) {
                 atomicMin(&a.err->mfail, e + 1);
                 n_e = 1;
@@ -214,321 +248,186 @@ __global__ __launch_bounds__(kBlock) void k_traces(const KArgs a) {
         }
     }
     if (a.cheb_global) {
-        // wave maximum, then one atomic per wave
         for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
         if ((threadIdx.x & 63) == 0) atomicMax(&a.cnt[a.slot], n_e);
     }
 }
 
 // ---------------------------------------------------------------------------
-// interior face slots of one element side. plus = 1: the element is the "+"
-// element of the face (its B or L side), else the "-" element (T or R side).
-// Own traces (du, v, dv) and the neighbour's facing traces are given.
-template <int NQ>
-__device__ __forceinline__ void interior_slots(const KArgs& a, bool plus, int fidx, bool hz,
-                                               const double (&odu)[NQ], const double (&ov)[NQ],
-                                               const double (&odv)[NQ], const double (&ndu)[NQ],
-                                               const double (&nv)[NQ], const double (&ndv)[NQ],
-                                               double (&A)[NQ], double (&Dd)[NQ], double (&Ff)[NQ]) {
+// Slot values (A, D, F) of one face point of an interior face, for the element
+// on the "+" side (plus) or the "-" side (src/rhs.jl:620-700). vm, vp, ... are
+// the traces of the "-" and "+" elements at the point; fi indexes the face
+// arrays at the point.
+__device__ __forceinline__ void interior_point(const KArgs& a, bool plus, bool hz, long fi, double vm, double vp,
+                                               double dum, double dup, double dvm, double dvp, double& A,
+                                               double& Dd, double& Ff) {
     const double th = a.theta, tau = a.tau, lu = a.lu, lv = a.lv, two_h = a.two_h;
-    const int st = a.fstride;
     const FaceArr& fc2 = a.f[hz ? F_HC2 : F_VC2];
     const FaceArr& faf = a.f[hz ? F_HAF : F_VAF];
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        // "-" / "+" traces of this face point
-        const double vm = plus ? nv[i] : ov[i], vp = plus ? ov[i] : nv[i];
-        const double dum = plus ? ndu[i] : odu[i], dup = plus ? odu[i] : ndu[i];
-        const double dvm = plus ? ndv[i] : odv[i], dvp = plus ? odv[i] : ndv[i];
-        if (a.one_sided) {
-            const double c2m = fv(a.f[hz ? F_HC2M : F_VC2M], i, fidx, st), c2p = fv(fc2, i, fidx, st);
-            const double afm = fv(a.f[hz ? F_HAFM : F_VAFM], i, fidx, st), afp = fv(faf, i, fidx, st);
-            const double K = fv(a.f[hz ? F_HJC : F_VJC], i, fidx, st);
-            const double J = two_h * (c2m * dum - c2p * dup + afm * dvm - afp * dvp) + K;
-            const double G = two_h * ((1 - th) * (c2m * dum + afm * dvm) + th * (c2p * dup + afp * dvp));
-            const double pen = (0.5 * (c2m + c2p)) * lu + (0.5 * (afm + afp)) * lv;
-            const double s = vm - vp;
-            if (plus) {
-                const double dvsp = th * s - tau * J;
-                A[i] = pen * s - G - (1 - th) * K;
-                Dd[i] = -afp * dvsp;
-                Ff[i] = c2p * dvsp;
-            } else {
-                const double dvsm = (1 - th) * (vp - vm) - tau * J;
-                A[i] = G - pen * s - th * K;
-                Dd[i] = afm * dvsm;
-                Ff[i] = -c2m * dvsm;
-            }
+    if (a.one_sided) {
+        const double c2m = fv(a.f[hz ? F_HC2M : F_VC2M], fi), c2p = fv(fc2, fi);
+        const double afm = fv(a.f[hz ? F_HAFM : F_VAFM], fi), afp = fv(faf, fi);
+        const double K = fv(a.f[hz ? F_HJC : F_VJC], fi);
+        const double J = two_h * (c2m * dum - c2p * dup + afm * dvm - afp * dvp) + K;
+        const double G = two_h * ((1 - th) * (c2m * dum + afm * dvm) + th * (c2p * dup + afp * dvp));
+        const double pen = (0.5 * (c2m + c2p)) * lu + (0.5 * (afm + afp)) * lv;
+        const double s = vm - vp;
+        if (plus) {
+            const double dvsp = th * s - tau * J;
+            A = pen * s - G - (1 - th) * K;
+            Dd = -afp * dvsp;
+            Ff = c2p * dvsp;
         } else {
-            const double c2 = fv(fc2, i, fidx, st), af = fv(faf, i, fidx, st);
-            const double jmp = vm - vp;
-            const double gst = two_h * (c2 * (th * dup + (1 - th) * dum) + af * (th * dvp + (1 - th) * dvm));
-            const double tj = tau * (c2 * (two_h * (dum - dup)) + af * (two_h * (dvm - dvp)));
-            if (plus) {
-                A[i] = (c2 * lu + af * lv) * jmp - gst;
-                Dd[i] = -af * (th * jmp - tj);
-                Ff[i] = c2 * (th * jmp - tj);
-            } else {
-                A[i] = gst - (c2 * lu + af * lv) * jmp;
-                Dd[i] = af * ((1 - th) * (vp - vm) - tj);
-                Ff[i] = -c2 * ((1 - th) * (vp - vm) - tj);
-            }
+            const double dvsm = (1 - th) * (vp - vm) - tau * J;
+            A = G - pen * s - th * K;
+            Dd = afm * dvsm;
+            Ff = -c2m * dvsm;
         }
-    }
-}
-
-// boundary face slots of one element side s (src/rhs.jl:738-765); returns the
-// smallest failing key of the side (0 if none).
-template <int NQ>
-__device__ __forceinline__ int boundary_slots(const KArgs& a, int s, int fidx, int m,
-                                              const double (&du)[NQ], const double (&vt)[NQ],
-                                              const double (&dv)[NQ], double (&A)[NQ], double (&Dd)[NQ],
-                                              double (&Ff)[NQ]) {
-    const bool hz = s < 2;
-    const int st = a.fstride;
-    const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
-    const double* sp = a.bcd[2 * s];
-    const double* spn = a.bcd[2 * s + 1];
-    const int kind = a.kind[s];
-    const double two_h = a.two_h, om = a.omega, ct = a.ct, stt = a.st;
-    int key = 0;
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        const double c2f = fv(a.f[hz ? F_HC2 : F_VC2], i, fidx, st);
-        const double af = fv(a.f[hz ? F_HAF : F_VAF], i, fidx, st);
-        const double cf = fv(a.f[hz ? F_HCF : F_VCF], i, fidx, st);
-        const double bf = fv(a.f[hz ? F_HBF : F_VBF], i, fidx, st);
-        const double v_fp = vt[i];
-        const double gun = sgn * (two_h * du[i]);
-        const double gvn = sgn * (two_h * dv[i]);
-        const double spv = sp[i + NQ * m], spnv = spn[i + NQ * m];
-        const double gu_ex = sgn * (ct * spnv - spnv);
-        const double gv_ex = sgn * (-om * stt * spnv);
-        const double v_ex = -om * stt * spv;
-        double F1, dvv;
-        if (kind == 3) {                                   // wall (src/rhs.jl:91-96)
-            const double g2 = c2f * gu_ex + af * gv_ex;
-            const double rho_w = c2f * gun + af * gvn - g2;
-            F1 = g2;
-            dvv = -(a.kap[s] / cf) * rho_w;
-        } else {
-            const double kb = a.kb[s];
-            const int check = 1.0 - kb * bf * v_fp <= 0.0 ? 1 : (1.0 - kb * bf * v_ex <= 0.0 ? 2 : 0);
-            if (check) {
-                const int k2 = 2 * (s * NQ * a.N + i + m * NQ) + check;
-                key = key == 0 ? k2 : min(key, k2);
-                A[i] = 0.0; Dd[i] = 0.0; Ff[i] = 0.0;
-                continue;
-            }
-            const double g = a.gam[s], et = a.eta[s], sg = a.sig[s];   // src/rhs.jl:74-85
-            const double rho_num = et * (c2f * gun + af * gvn) + g * cf * sqrt(1.0 - kb * bf * v_fp) * v_fp;
-            const double rho_ex = et * (c2f * gu_ex + af * gv_ex) + g * cf * sqrt(1.0 - kb * bf * v_ex) * v_ex;
-            const double rho_net = rho_num - rho_ex;
-            const double sqf = sqrt(1.0 - kb * bf * v_fp);
-            F1 = c2f * gun + af * gvn - (et + sg * g) * rho_net;
-            dvv = -(g - sg * et) / (cf * sqf) * rho_net;
-        }
-        A[i] = F1;
-        Dd[i] = sgn * (af * dvv);
-        Ff[i] = -sgn * (c2f * dvv);
-    }
-    return key;
-}
-
-// lift of one side's slots (src/rhs.jl:433-470)
-template <class D>
-__device__ __forceinline__ void lift_side(const double* __restrict__ T, int s, double two_h, double four_hh,
-                                          const double (&A)[D::NQ], const double (&Dd)[D::NQ],
-                                          const double (&Ff)[D::NQ], double (&Rv)[D::NV * D::NV],
-                                          double (&Fu)[D::NU * D::NU]) {
-    constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV;
-    const int oE = (s == 0 || s == 2) ? D::oEl : D::oEr;
-    const int oE1 = (s == 0 || s == 2) ? D::oE1l : D::oE1r;
-    double YA[NV], YD[NV], YF[NU];
-#pragma unroll
-    for (int b = 0; b < NV; ++b) {
-        double sA = 0.0, sD = 0.0;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            sA = fma(T[D::oP + i + NQ * b], T[D::oW + i] * A[i], sA);
-            sD = fma(T[D::oP + i + NQ * b], T[D::oW + i] * Dd[i], sD);
-        }
-        YA[b] = sA; YD[b] = sD;
-    }
-#pragma unroll
-    for (int b = 0; b < NU; ++b) {
-        double sF = 0.0;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) sF = fma(T[D::oP + i + NQ * b], T[D::oW + i] * Ff[i], sF);
-        YF[b] = sF;
-    }
-    if (s < 2) {
-#pragma unroll
-        for (int b = 0; b < NV; ++b)
-#pragma unroll
-            for (int aa = 0; aa < NV; ++aa) {
-                double r = Rv[aa + NV * b];
-                r = r + (two_h * T[oE + aa]) * YA[b];
-                r = r + (four_hh * T[oE1 + aa]) * YD[b];
-                Rv[aa + NV * b] = r;
-            }
-#pragma unroll
-        for (int b = 0; b < NU; ++b)
-#pragma unroll
-            for (int aa = 0; aa < NU; ++aa) Fu[aa + NU * b] = Fu[aa + NU * b] + (1.0 * T[oE1 + aa]) * YF[b];
     } else {
-#pragma unroll
-        for (int b = 0; b < NV; ++b)
-#pragma unroll
-            for (int aa = 0; aa < NV; ++aa) {
-                double r = Rv[aa + NV * b];
-                r = r + (two_h * YA[aa]) * T[oE + b];
-                r = r + (four_hh * YD[aa]) * T[oE1 + b];
-                Rv[aa + NV * b] = r;
-            }
-#pragma unroll
-        for (int b = 0; b < NU; ++b)
-#pragma unroll
-            for (int aa = 0; aa < NU; ++aa) Fu[aa + NU * b] = Fu[aa + NU * b] + (1.0 * YF[aa]) * T[oE1 + b];
+        const double c2 = fv(fc2, fi), af = fv(faf, fi);
+        const double jmp = vm - vp;
+        const double gst = two_h * (c2 * (th * dup + (1 - th) * dum) + af * (th * dvp + (1 - th) * dvm));
+        const double tj = tau * (c2 * (two_h * (dum - dup)) + af * (two_h * (dvm - dvp)));
+        if (plus) {
+            A = (c2 * lu + af * lv) * jmp - gst;
+            Dd = -af * (th * jmp - tj);
+            Ff = c2 * (th * jmp - tj);
+        } else {
+            A = gst - (c2 * lu + af * lv) * jmp;
+            Dd = af * ((1 - th) * (vp - vm) - tj);
+            Ff = -c2 * ((1 - th) * (vp - vm) - tj);
+        }
     }
 }
 
-template <int NQ>
-__device__ __forceinline__ void load_tr(const double* __restrict__ tr, int arr, int e, int E, double (&o)[NQ]) {
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) o[i] = tr[(long)(arr * NQ + i) * E + e];
-}
-
-__device__ __forceinline__ bool skip_stage(const KArgs& a) {
-    const int fl = a.err->failed;
-    return fl != 0 && fl < a.serial;
+// Slot values of one point i of a physical boundary side s (src/rhs.jl:738-765);
+// returns the point's failing _boundary_key (src/rhs.jl:732) or 0.
+__device__ __forceinline__ int boundary_point(const KArgs& a, int s, long fi, int i, int m, int nq, double du,
+                                              double vt, double dv, double& A, double& Dd, double& Ff) {
+    const bool hz = s < 2;
+    const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
+    const double c2f = fv(a.f[hz ? F_HC2 : F_VC2], fi);
+    const double af = fv(a.f[hz ? F_HAF : F_VAF], fi);
+    const double cf = fv(a.f[hz ? F_HCF : F_VCF], fi);
+    const double bf = fv(a.f[hz ? F_HBF : F_VBF], fi);
+    const double two_h = a.two_h, om = a.omega, ct = a.ct, stt = a.st;
+    const double gun = sgn * (two_h * du);
+    const double gvn = sgn * (two_h * dv);
+    const double spv = a.bcd[2 * s][i + nq * m], spnv = a.bcd[2 * s + 1][i + nq * m];
+    const double gu_ex = sgn * (ct * spnv - spnv);
+    const double gv_ex = sgn * (-om * stt * spnv);
+    const double v_ex = -om * stt * spv;
+    double F1, dvv;
+    if (a.kind[s] == 3) {                                  // wall (src/rhs.jl:91-96)
+        const double g2 = c2f * gu_ex + af * gv_ex;
+        const double rho_w = c2f * gun + af * gvn - g2;
+        F1 = g2;
+        dvv = -(a.kap[s] / cf) * rho_w;
+    } else {
+        const double kb = a.kb[s];
+        const int check = 1.0 - kb * bf * vt <= 0.0 ? 1 : (1.0 - kb * bf * v_ex <= 0.0 ? 2 : 0);
+        if (check) {
+            A = 0.0; Dd = 0.0; Ff = 0.0;
+            return 2 * (s * nq * a.N + i + m * nq) + check;
+        }
+        const double g = a.gam[s], et = a.eta[s], sg = a.sig[s];   // src/rhs.jl:74-85
+        const double rho_num = et * (c2f * gun + af * gvn) + g * cf * sqrt(1.0 - kb * bf * vt) * vt;
+        const double rho_ex = et * (c2f * gu_ex + af * gv_ex) + g * cf * sqrt(1.0 - kb * bf * v_ex) * v_ex;
+        const double rho_net = rho_num - rho_ex;
+        const double sqf = sqrt(1.0 - kb * bf * vt);
+        F1 = c2f * gun + af * gvn - (et + sg * g) * rho_net;
+        dvv = -(g - sg * et) / (cf * sqf) * rho_net;
+    }
+    A = F1;
+    Dd = sgn * (af * dvv);
+    Ff = -sgn * (c2f * dvv);
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
 // Volume terms (src/rhs.jl:317-333) scaled by -4/h^2 plus the source
 // (src/rhs.jl:340-350) into R, and VB = (c^2 grad phi_u, grad v_h) into B.
-// The point values are formed one point row p at a time and contracted back
-// immediately (rank-one updates), so only the modal arrays stay live.
+// The point values are formed one point row p at a time and contracted back at
+// once (rank-one updates), so only the modal arrays stay live.
+template <class D, int n, bool UNIC>
+__device__ __forceinline__ void vol_pass(const double* __restrict__ T0, const double* __restrict__ W0,
+                                         const double* __restrict__ Wv, const double (&X)[n * n],
+                                         int nout, double* __restrict__ out) {
+    // out (nout x nout) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S over the points
+    constexpr int NQ = D::NQ;
+#pragma unroll 1
+    for (int p = 0; p < NQ; ++p) {
+        const double* __restrict__ T = fresh(T0);
+        const double* __restrict__ W = UNIC ? fresh(W0) : Wv;
+        double t1[n], t2[n];
+#pragma unroll
+        for (int b = 0; b < n; ++b) {
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < n; ++aa) {
+                s1 = fma(T[D::oS + p + NQ * aa], X[aa + n * b], s1);
+                s2 = fma(T[D::oP + p + NQ * aa], X[aa + n * b], s2);
+            }
+            t1[b] = s1; t2[b] = s2;
+        }
+        double z1[D::NU], z2[D::NU];
+#pragma unroll
+        for (int b = 0; b < D::NU; ++b) { z1[b] = 0.0; z2[b] = 0.0; }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double g1 = 0.0, g2 = 0.0;
+#pragma unroll
+            for (int b = 0; b < n; ++b) {
+                g1 = fma(t1[b], T[D::oP + q + NQ * b], g1);
+                g2 = fma(t2[b], T[D::oS + q + NQ * b], g2);
+            }
+            const double w = W[p + NQ * q];
+            const double h1 = w * g1, h2 = w * g2;
+#pragma unroll
+            for (int b = 0; b < D::NU; ++b) {
+                if (b < nout) {
+                    z1[b] = fma(h1, T[D::oP + q + NQ * b], z1[b]);
+                    z2[b] = fma(h2, T[D::oS + q + NQ * b], z2[b]);
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < D::NU; ++b)
+#pragma unroll
+            for (int aa = 0; aa < D::NU; ++aa)
+                if (b < nout && aa < nout)
+                    out[aa + nout * b] = fma(T[D::oS + p + NQ * aa], z1[b],
+                                             fma(T[D::oP + p + NQ * aa], z2[b], out[aa + nout * b]));
+    }
+}
+
 template <int NU, int NV, bool UNIC>
-__global__ __launch_bounds__(kBlock) void k_volume(const KArgs a, double* __restrict__ R,
-                                                   double* __restrict__ B) {
+__global__ __launch_bounds__(kBlock) void k_volume(const KArgs a, const double* __restrict__ T0,
+                                                   const double* __restrict__ Cw0, const double* __restrict__ Aw0,
+                                                   double* __restrict__ R, double* __restrict__ B) {
     using D = Dims<NU, NV>;
     constexpr int NQ = D::NQ;
     if (skip_stage(a)) return;
     const int E = a.E;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= E) return;
-    const double* __restrict__ T = a.tab;
     const long cc = UNIC ? 0 : (long)a.cls_ca[e];
-    const double* __restrict__ Cw = a.Cw + cc * NQ * NQ;
-    const double* __restrict__ Aw = a.Aw + cc * NQ * NQ;
+    const double* __restrict__ Cw = Cw0 + cc * NQ * NQ;
+    const double* __restrict__ Aw = Aw0 + cc * NQ * NQ;
     double Rv[NV * NV];
 #pragma unroll
     for (int m = 0; m < NV * NV; ++m) Rv[m] = 0.0;
-    // (c^2 grad phi_v, grad u_h)
-    {
+    {   // (c^2 grad phi_v, grad u_h)
         double U[NU * NU];
 #pragma unroll
         for (int m = 0; m < NU * NU; ++m) U[m] = a.yu[(long)m * E + e];
-#pragma unroll
-        for (int p = 0; p < NQ; ++p) {
-            double t1[NU], t2[NU];
-#pragma unroll
-            for (int b = 0; b < NU; ++b) {
-                double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-                for (int aa = 0; aa < NU; ++aa) {
-                    s1 = fma(T[D::oS + p + NQ * aa], U[aa + NU * b], s1);
-                    s2 = fma(T[D::oP + p + NQ * aa], U[aa + NU * b], s2);
-                }
-                t1[b] = s1; t2[b] = s2;
-            }
-            double z1[NV], z2[NV];
-#pragma unroll
-            for (int b = 0; b < NV; ++b) { z1[b] = 0.0; z2[b] = 0.0; }
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                double g1 = 0.0, g2 = 0.0;
-#pragma unroll
-                for (int b = 0; b < NU; ++b) {
-                    g1 = fma(t1[b], T[D::oP + q + NQ * b], g1);
-                    g2 = fma(t2[b], T[D::oS + q + NQ * b], g2);
-                }
-                const double cw = Cw[p + NQ * q];
-                const double h1 = cw * g1, h2 = cw * g2;
-#pragma unroll
-                for (int b = 0; b < NV; ++b) {
-                    z1[b] = fma(h1, T[D::oP + q + NQ * b], z1[b]);
-                    z2[b] = fma(h2, T[D::oS + q + NQ * b], z2[b]);
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < NV; ++b)
-#pragma unroll
-                for (int aa = 0; aa < NV; ++aa)
-                    Rv[aa + NV * b] = fma(T[D::oS + p + NQ * aa], z1[b], fma(T[D::oP + p + NQ * aa], z2[b], Rv[aa + NV * b]));
-        }
+        vol_pass<D, NU, UNIC>(T0, Cw0, Cw, U, NV, Rv);
     }
-    // (alpha grad phi_v, grad v_h) into Rv and (c^2 grad phi_u, grad v_h) into VB
-    {
-        double V[NV * NV], VB[NU * NU];
+    double V[NV * NV];
 #pragma unroll
-        for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-#pragma unroll
-        for (int m = 0; m < NU * NU; ++m) VB[m] = 0.0;
-#pragma unroll
-        for (int p = 0; p < NQ; ++p) {
-            double t1[NV], t2[NV];
-#pragma unroll
-            for (int b = 0; b < NV; ++b) {
-                double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-                for (int aa = 0; aa < NV; ++aa) {
-                    s1 = fma(T[D::oS + p + NQ * aa], V[aa + NV * b], s1);
-                    s2 = fma(T[D::oP + p + NQ * aa], V[aa + NV * b], s2);
-                }
-                t1[b] = s1; t2[b] = s2;
-            }
-            double z1[NV], z2[NV], y1[NU], y2[NU];
-#pragma unroll
-            for (int b = 0; b < NV; ++b) { z1[b] = 0.0; z2[b] = 0.0; }
-#pragma unroll
-            for (int b = 0; b < NU; ++b) { y1[b] = 0.0; y2[b] = 0.0; }
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                double g1 = 0.0, g2 = 0.0;
-#pragma unroll
-                for (int b = 0; b < NV; ++b) {
-                    g1 = fma(t1[b], T[D::oP + q + NQ * b], g1);
-                    g2 = fma(t2[b], T[D::oS + q + NQ * b], g2);
-                }
-                const double aw = Aw[p + NQ * q], cw = Cw[p + NQ * q];
-                const double h1 = aw * g1, h2 = aw * g2, c1 = cw * g1, c2 = cw * g2;
-#pragma unroll
-                for (int b = 0; b < NV; ++b) {
-                    z1[b] = fma(h1, T[D::oP + q + NQ * b], z1[b]);
-                    z2[b] = fma(h2, T[D::oS + q + NQ * b], z2[b]);
-                }
-#pragma unroll
-                for (int b = 0; b < NU; ++b) {
-                    y1[b] = fma(c1, T[D::oP + q + NQ * b], y1[b]);
-                    y2[b] = fma(c2, T[D::oS + q + NQ * b], y2[b]);
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < NV; ++b)
-#pragma unroll
-                for (int aa = 0; aa < NV; ++aa)
-                    Rv[aa + NV * b] = fma(T[D::oS + p + NQ * aa], z1[b], fma(T[D::oP + p + NQ * aa], z2[b], Rv[aa + NV * b]));
-#pragma unroll
-            for (int b = 0; b < NU; ++b)
-#pragma unroll
-                for (int aa = 0; aa < NU; ++aa)
-                    VB[aa + NU * b] = fma(T[D::oS + p + NQ * aa], y1[b], fma(T[D::oP + p + NQ * aa], y2[b], VB[aa + NU * b]));
-        }
-#pragma unroll
-        for (int m = 0; m < NU * NU; ++m) B[(long)m * E + e] = VB[m];
-    }
+    for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
+    // (alpha grad phi_v, grad v_h)
+    vol_pass<D, NV, UNIC>(T0, Aw0, Aw, V, NV, Rv);
     // scale and source, in the order of the reference's sum
 #pragma unroll
     for (int m = 0; m < NV * NV; ++m) {
@@ -536,15 +435,69 @@ __global__ __launch_bounds__(kBlock) void k_volume(const KArgs a, double* __rest
         for (int t = 0; t < a.nproj; ++t) r = r + a.pc[t] * a.proj[t][(long)m * E + e];
         R[(long)m * E + e] = r;
     }
+    // (c^2 grad phi_u, grad v_h)
+    double VB[NU * NU];
+#pragma unroll
+    for (int m = 0; m < NU * NU; ++m) VB[m] = 0.0;
+    vol_pass<D, NV, UNIC>(T0, Cw0, Cw, V, NU, VB);
+#pragma unroll
+    for (int m = 0; m < NU * NU; ++m) B[(long)m * E + e] = VB[m];
 }
 
 // ---------------------------------------------------------------------------
 // Face slots of the element's four sides from its own and its neighbours'
 // traces, lifted into R and Fu (src/rhs.jl:421-473, 589-776); then the
-// u-equation with the packed inverse (src/rhs.jl:361-382) and the u part of
-// the RK4 stage update (src/solver.jl:300-321).
+// u-equation (src/rhs.jl:361-382) and the u part of the RK4 stage update
+// (src/solver.jl:300-321).
+template <class D>
+struct SideGeom {
+    int fidx, nb, m;
+    bool bnd, hz, plus;
+};
+
+template <class D>
+__device__ __forceinline__ SideGeom<D> side_geom(int s, int e, int j, int k, int N) {
+    SideGeom<D> g;
+    g.hz = s < 2;
+    g.plus = (s == 0 || s == 2);
+    if (s == 0)      { g.fidx = j + (N + 1) * k;     g.nb = e - 1; g.bnd = j == 0;     g.m = k; }
+    else if (s == 1) { g.fidx = j + 1 + (N + 1) * k; g.nb = e + 1; g.bnd = j == N - 1; g.m = k; }
+    else if (s == 2) { g.fidx = j + N * k;           g.nb = e - N; g.bnd = k == 0;     g.m = j; }
+    else             { g.fidx = j + N * (k + 1);     g.nb = e + N; g.bnd = k == N - 1; g.m = j; }
+    if (g.bnd) g.nb = e;    // not read on the boundary
+    return g;
+}
+
+// Slot values (A, D, F) of point i of side s of element e and its failing
+// boundary key (0 if none).
+template <class D>
+__device__ __forceinline__ int slot_point(const KArgs& a, const SideGeom<D>& g, int s, int i, int e, double& A,
+                                          double& Dd, double& Ff) {
+    constexpr int NQ = D::NQ;
+    const int E = a.E;
+    const double odu = a.tr[(long)((0 + s) * NQ + i) * E + e];
+    const double ov = a.tr[(long)((4 + s) * NQ + i) * E + e];
+    const double odv = a.tr[(long)((8 + s) * NQ + i) * E + e];
+    const long fi = (long)i * a.fstride + g.fidx;
+    if (g.bnd) return boundary_point(a, s, fi, i, g.m, NQ, odu, ov, odv, A, Dd, Ff);
+    const int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
+    const double ndu = a.tr[(long)((0 + ns) * NQ + i) * E + g.nb];
+    const double nvv = a.tr[(long)((4 + ns) * NQ + i) * E + g.nb];
+    const double ndv = a.tr[(long)((8 + ns) * NQ + i) * E + g.nb];
+    if (g.plus) interior_point(a, true, g.hz, fi, nvv, ov, ndu, odu, ndv, odv, A, Dd, Ff);
+    else interior_point(a, false, g.hz, fi, ov, nvv, odu, ndu, odv, ndv, A, Dd, Ff);
+    return 0;
+}
+
+// Face slots of the element's four sides from its own and its neighbours'
+// traces, lifted into R (src/rhs.jl:421-473, 589-776); then the same slots
+// lifted into Fu, the u-equation (src/rhs.jl:361-382) and the u part of the
+// RK4 stage update (src/solver.jl:300-321). The two lifts are separate passes
+// over the sides so that R and Fu are never live together; the slot values,
+// a few flops per point, are formed in both.
 template <int NU, int NV, bool UNIC>
-__global__ __launch_bounds__(kBlock) void k_faces(const KArgs a, double* __restrict__ R,
+__global__ __launch_bounds__(kBlock) void k_faces(const KArgs a, const double* __restrict__ T0,
+                                                  const double* __restrict__ Ai0, double* __restrict__ R,
                                                   const double* __restrict__ B) {
     using D = Dims<NU, NV>;
     constexpr int NQ = D::NQ;
@@ -553,68 +506,107 @@ __global__ __launch_bounds__(kBlock) void k_faces(const KArgs a, double* __restr
     const int mfail_prev = a.cheb_global ? a.err->mfail : 0x7fffffff;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= E) return;
-    const double* __restrict__ T = a.tab;
     const int j = e % N, k = e / N;
-    double Rv[NV * NV], Fu[NU * NU];
-#pragma unroll
-    for (int m = 0; m < NV * NV; ++m) Rv[m] = R[(long)m * E + e];
-#pragma unroll
-    for (int m = 0; m < NU * NU; ++m) Fu[m] = 0.0;
     int bkey = 0;
+    {   // lifts of A and D into the v-equation (src/rhs.jl:449-463)
+        double Rv[NV * NV];
+#pragma unroll
+        for (int m = 0; m < NV * NV; ++m) Rv[m] = R[(long)m * E + e];
 #pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-        const bool hz = s < 2;
-        const bool plus = (s == 0 || s == 2);
-        double odu[NQ], ov[NQ], odv[NQ];
-        load_tr<NQ>(a.tr, 0 + s, e, E, odu);
-        load_tr<NQ>(a.tr, 4 + s, e, E, ov);
-        load_tr<NQ>(a.tr, 8 + s, e, E, odv);
-        int fidx, nb, m;
-        bool bnd;
-        if (s == 0)      { fidx = j + (N + 1) * k;     nb = e - 1; bnd = j == 0;     m = k; }
-        else if (s == 1) { fidx = j + 1 + (N + 1) * k; nb = e + 1; bnd = j == N - 1; m = k; }
-        else if (s == 2) { fidx = j + N * k;           nb = e - N; bnd = k == 0;     m = j; }
-        else             { fidx = j + N * (k + 1);     nb = e + N; bnd = k == N - 1; m = j; }
-        double A[NQ], Dd[NQ], Ff[NQ];
-        if (bnd) {
-            const int k2 = boundary_slots<NQ>(a, s, fidx, m, odu, ov, odv, A, Dd, Ff);
-            if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
-        } else {
-            const int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
-            double ndu[NQ], nvv[NQ], ndv[NQ];
-            load_tr<NQ>(a.tr, 0 + ns, nb, E, ndu);
-            load_tr<NQ>(a.tr, 4 + ns, nb, E, nvv);
-            load_tr<NQ>(a.tr, 8 + ns, nb, E, ndv);
-            interior_slots<NQ>(a, plus, fidx, hz, odu, ov, odv, ndu, nvv, ndv, A, Dd, Ff);
+        for (int s = 0; s < 4; ++s) {
+            const SideGeom<D> g = side_geom<D>(s, e, j, k, N);
+            double YA[NV], YD[NV];
+#pragma unroll
+            for (int b = 0; b < NV; ++b) { YA[b] = 0.0; YD[b] = 0.0; }
+#pragma unroll 1
+            for (int i = 0; i < NQ; ++i) {
+                const double* __restrict__ T = fresh(T0);
+                double A, Dd, Ff;
+                const int k2 = slot_point<D>(a, g, s, i, e, A, Dd, Ff);
+                if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
+                // Y_Z[b] += P[i, b] (w[i] Z[i]), summed in point order (src/rhs.jl:434-441)
+                const double wA = T[D::oW + i] * A, wD = T[D::oW + i] * Dd;
+#pragma unroll
+                for (int b = 0; b < NV; ++b) {
+                    YA[b] = fma(T[D::oP + i + NQ * b], wA, YA[b]);
+                    YD[b] = fma(T[D::oP + i + NQ * b], wD, YD[b]);
+                }
+            }
+            const double* __restrict__ T = fresh(T0);
+            const int oE = g.plus ? D::oEl : D::oEr;
+            const int oE1 = g.plus ? D::oE1l : D::oE1r;
+#pragma unroll
+            for (int b = 0; b < NV; ++b)
+#pragma unroll
+                for (int aa = 0; aa < NV; ++aa) {
+                    double r = Rv[aa + NV * b];
+                    if (g.hz) {
+                        r = r + (a.two_h * T[oE + aa]) * YA[b];
+                        r = r + (a.four_hh * T[oE1 + aa]) * YD[b];
+                    } else {
+                        r = r + (a.two_h * YA[aa]) * T[oE + b];
+                        r = r + (a.four_hh * YD[aa]) * T[oE1 + b];
+                    }
+                    Rv[aa + NV * b] = r;
+                }
         }
-        lift_side<D>(T, s, a.two_h, a.four_hh, A, Dd, Ff, Rv, Fu);
+#pragma unroll
+        for (int m = 0; m < NV * NV; ++m) R[(long)m * E + e] = Rv[m];
     }
     if (bkey) {
         atomicMin(&a.err->bkey, bkey);
         a.err->failed = a.serial;
     }
     if (mfail_prev != 0x7fffffff) a.err->failed = a.serial;
+
+    constexpr int n = NU * NU;
+    double b[n];   // b = VB - Fu, then b[1] += V[1] (src/rhs.jl:366-370)
+    {
+        double Fu[n];
 #pragma unroll
-    for (int m = 0; m < NV * NV; ++m) R[(long)m * E + e] = Rv[m];
+        for (int m = 0; m < n; ++m) Fu[m] = 0.0;
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+            const SideGeom<D> g = side_geom<D>(s, e, j, k, N);
+            double YF[NU];
+#pragma unroll
+            for (int bb = 0; bb < NU; ++bb) YF[bb] = 0.0;
+#pragma unroll 1
+            for (int i = 0; i < NQ; ++i) {
+                const double* __restrict__ T = fresh(T0);
+                double A, Dd, Ff;
+                slot_point<D>(a, g, s, i, e, A, Dd, Ff);
+                const double wF = T[D::oW + i] * Ff;
+#pragma unroll
+                for (int bb = 0; bb < NU; ++bb) YF[bb] = fma(T[D::oP + i + NQ * bb], wF, YF[bb]);
+            }
+            const double* __restrict__ T = fresh(T0);
+            const int oE1 = g.plus ? D::oE1l : D::oE1r;
+#pragma unroll
+            for (int bb = 0; bb < NU; ++bb)
+#pragma unroll
+                for (int aa = 0; aa < NU; ++aa) {
+                    if (g.hz) Fu[aa + NU * bb] = Fu[aa + NU * bb] + (1.0 * T[oE1 + aa]) * YF[bb];
+                    else Fu[aa + NU * bb] = Fu[aa + NU * bb] + (1.0 * YF[aa]) * T[oE1 + bb];
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) b[i] = B[(long)i * E + e] - Fu[i];
+    }
+    b[0] += a.yv[e];
 
     // u-equation u_t = A_e^-1 b (src/rhs.jl:361-382). The inverse of the class is
     // stored unpacked; it is symmetric, so row i is the contiguous column i and
     // each output is finished (and written) before the next one starts.
-    constexpr int n = NU * NU;
     const long cc = UNIC ? 0 : (long)a.cls_ca[e];
-    const double* __restrict__ Ai = a.Ainv + cc * (n * n);
-    double b[n];
-#pragma unroll
-    for (int i = 0; i < n; ++i) b[i] = B[(long)i * E + e] - Fu[i];
-    b[0] += a.yv[e];
-    // RK4 stage update, u part (src/solver.jl:300-321)
     const double c = a.cstage;
     bool fin = true;
-#pragma unroll
+#pragma unroll 1
     for (int mm = 0; mm < n; ++mm) {
+        const double* __restrict__ Ar = (UNIC ? fresh(Ai0) : Ai0 + cc * (n * n)) + (long)n * mm;
         double kv = 0.0;
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj) kv = fma(Ai[jj + n * mm], b[jj], kv);
+        for (int jj = 0; jj < n; ++jj) kv = fma(Ar[jj], b[jj], kv);
         const long idx = (long)mm * E + e;
         if (a.stage == 0) {
             a.outu[idx] = kv;
@@ -634,28 +626,94 @@ __global__ __launch_bounds__(kBlock) void k_faces(const KArgs a, double* __restr
 }
 
 // ---------------------------------------------------------------------------
+// One point row p of K d = P_h^T (G o (P_h d P_h^T)) P_h accumulated into r.
+template <class D>
+__device__ __forceinline__ void kd_row(const double* __restrict__ T, int p, const double (&d)[D::NV * D::NV],
+                                       const double (&Gp)[D::NH], double (&r)[D::NV * D::NV]) {
+    constexpr int NV = D::NV, NH = D::NH;
+    double tp[NV], zp[NV];
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        double s = 0.0;
+#pragma unroll
+        for (int aa = 0; aa < NV; ++aa) s = fma(T[D::oPh + p + NH * aa], d[aa + NV * b], s);
+        tp[b] = s;
+        zp[b] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+        double y = 0.0;
+#pragma unroll
+        for (int b = 0; b < NV; ++b) y = fma(tp[b], T[D::oPh + q + NH * b], y);
+        y *= Gp[q];
+#pragma unroll
+        for (int b = 0; b < NV; ++b) zp[b] = fma(y, T[D::oPh + q + NH * b], zp[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < NV; ++b)
+#pragma unroll
+        for (int aa = 0; aa < NV; ++aa) r[aa + NV * b] = fma(T[D::oPh + p + NH * aa], zp[b], r[aa + NV * b]);
+}
+
 // The mass system M(v_h) X = R by Chebyshev iteration (src/rhs.jl:886-939)
-// and the v part of the RK4 stage update.
+// and the v part of the RK4 stage update. One wave per block: the first rows
+// of the weights G = 2 beta v_q W of every lane live in LDS ([entry][lane],
+// conflict-free), the remaining rows in registers.
 template <int NU, int NV>
-__global__ __launch_bounds__(kBlock) void k_mass(const KArgs a, const double* __restrict__ R) {
+__global__ __launch_bounds__(kMassBlock) void k_mass(const KArgs a, const double* __restrict__ T0,
+                                                     const double* __restrict__ beta0,
+                                                     const double* __restrict__ R) {
     using D = Dims<NU, NV>;
     constexpr int NH = D::NH;
+    constexpr int RL0 = kMassLdsDoubles / NH;
+    constexpr int RL = RL0 < NH ? RL0 : NH;          // rows of G in LDS
+    constexpr int RT = NH - RL;                      // rows of G in registers
+    __shared__ double Gs[RL * NH * kMassBlock];
     if (skip_stage(a)) return;
     const int E = a.E;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[a.slot ^ 1] = 1;   // the next stage's count
-    const int e = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int e = blockIdx.x * kMassBlock + lane;
     int ncheb = 0;
     if (e < E) {
-        const double* __restrict__ T = a.tab;
-        double G[NH * NH];
+        double Gt[RT > 0 ? RT * NH : 1];
         double th = 1.0, de = 1.0, s1 = 1.0;
         int n_e;
         {
             double V[NV * NV];
 #pragma unroll
             for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-            const double* beta = a.beta + (long)a.cls_b[e] * NH * NH;
-            n_e = cheb_setup<D>(T, V, beta, a.cheb_L, G, th, de, s1);
+            const double* beta = beta0 + (long)a.cls_b[e] * NH * NH;
+            double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
+            bool nan = false;
+#pragma unroll 1
+            for (int p = 0; p < RL; ++p) {
+                const double* __restrict__ T = fresh(T0);
+                double g[NH];
+                g_row<D>(T, V, beta, p, g);
+#pragma unroll
+                for (int q = 0; q < NH; ++q) {
+                    gmin = fmin(gmin, g[q]);
+                    gmax = fmax(gmax, g[q]);
+                    nan = nan || (g[q] != g[q]);
+                    Gs[(p * NH + q) * kMassBlock + lane] = g[q] * T[D::oWh + p + NH * q];
+                }
+            }
+#pragma unroll
+            for (int pt = 0; pt < RT; ++pt) {
+                const double* __restrict__ T = fresh(T0);
+                const int p = RL + pt;
+                double g[NH];
+                g_row<D>(T, V, beta, p, g);
+#pragma unroll
+                for (int q = 0; q < NH; ++q) {
+                    gmin = fmin(gmin, g[q]);
+                    gmax = fmax(gmax, g[q]);
+                    nan = nan || (g[q] != g[q]);
+                    Gt[pt * NH + q] = g[q] * T[D::oWh + p + NH * q];
+                }
+            }
+            n_e = cheb_interval(gmin, gmax, nan, a.cheb_L, th, de, s1);
         }
         double X[NV * NV];
 #pragma unroll
@@ -666,8 +724,6 @@ __global__ __launch_bounds__(kBlock) void k_mass(const KArgs a, const double* __
                 a.err->failed = a.serial;
             }
         } else {
-#pragma unroll
-            for (int m = 0; m < NH * NH; ++m) G[m] *= T[D::oWh + m];
             const int n = a.cheb_global ? a.cnt[a.slot] : n_e;
             ncheb = n;
             double r[NV * NV], d[NV * NV];
@@ -678,34 +734,22 @@ __global__ __launch_bounds__(kBlock) void k_mass(const KArgs a, const double* __
 #pragma unroll
                 for (int m = 0; m < NV * NV; ++m) X[m] += d[m];
                 if (step == n) break;
-                // r <- (r - d) + K d,  K d = P_h^T (G o (P_h d P_h^T)) P_h
+                // r <- (r - d) + K d
 #pragma unroll
                 for (int m = 0; m < NV * NV; ++m) r[m] = r[m] - d[m];
+#pragma unroll 1
+                for (int p = 0; p < RL; ++p) {
+                    double Gp[NH];
 #pragma unroll
-                for (int p = 0; p < NH; ++p) {
-                    double tp[NV], zp[NV];
+                    for (int q = 0; q < NH; ++q) Gp[q] = Gs[(p * NH + q) * kMassBlock + lane];
+                    kd_row<D>(fresh(T0), p, d, Gp, r);
+                }
 #pragma unroll
-                    for (int b = 0; b < NV; ++b) {
-                        double s = 0.0;
+                for (int pt = 0; pt < RT; ++pt) {
+                    double Gp[NH];
 #pragma unroll
-                        for (int aa = 0; aa < NV; ++aa) s = fma(T[D::oPh + p + NH * aa], d[aa + NV * b], s);
-                        tp[b] = s;
-                        zp[b] = 0.0;
-                    }
-#pragma unroll
-                    for (int q = 0; q < NH; ++q) {
-                        double y = 0.0;
-#pragma unroll
-                        for (int b = 0; b < NV; ++b) y = fma(tp[b], T[D::oPh + q + NH * b], y);
-                        y *= G[p + NH * q];
-#pragma unroll
-                        for (int b = 0; b < NV; ++b) zp[b] = fma(y, T[D::oPh + q + NH * b], zp[b]);
-                    }
-#pragma unroll
-                    for (int b = 0; b < NV; ++b)
-#pragma unroll
-                        for (int aa = 0; aa < NV; ++aa)
-                            r[aa + NV * b] = fma(T[D::oPh + p + NH * aa], zp[b], r[aa + NV * b]);
+                    for (int q = 0; q < NH; ++q) Gp[q] = Gt[pt * NH + q];
+                    kd_row<D>(fresh(T0), RL + pt, d, Gp, r);
                 }
                 const double rhon = 1.0 / (2.0 * s1 - rho);
                 const double c1 = rhon * rho, c2 = 2.0 * rhon / de;
@@ -738,7 +782,10 @@ __global__ __launch_bounds__(kBlock) void k_mass(const KArgs a, const double* __
         if (!fin) atomicMin(&a.err->nonfinite, a.step);
     }
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
-    if ((threadIdx.x & 63) == 0 && ncheb > 0) atomicMax(&a.err->maxcheb, ncheb);
+    if (lane == 0 && ncheb > 0) {
+        atomicMax(&a.err->maxcheb, ncheb);
+        atomicAdd(&a.err->cheb_wave_sum, (unsigned long long)ncheb);
+    }
 }
 
 }  // namespace wedg

@@ -145,6 +145,10 @@ wedg_status wedg_last_error(const wedg_workspace* ws, wedg_error_info* info);
 wedg_status wedg_timing_enable(wedg_workspace* ws, int on);
 wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* launches);
 
+/* Chebyshev steps of the last synchronized call: the mean over stages and
+ * waves of the count each wave of 64 elements ran, and the number of stages. */
+wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_steps, long* stages);
+
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);
 wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);

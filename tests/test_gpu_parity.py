@@ -98,7 +98,7 @@ def test_rhs_larger_meshes_and_element_counts():
     for (q, N, mixed) in ((5, 16, False), (6, 12, True), (2, 33, False), (4, 20, True)):
         pre, _ = precompute(q, N, mixed=mixed, mode="source",
                             coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0))
-        u, v = rand_state(rng, N, pre, su=1e-2, sv=0.05)
+        u, v = rand_state(rng, N, pre, su=1e-2, sv=0.005)
         for cheb in ("global", "element"):
             say(f"case large q={q} N={N} mixed={mixed} cheb={cheb}")
             a, b = both(pre, N, u, v, 0.21, cheb=cheb)
