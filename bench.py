@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--probe-copy", type=float, default=0.0,
+                    help="only run a plain copy kernel of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this build",
@@ -86,6 +88,16 @@ def main():
     import edg_westervelt_amd as w
     from edg_westervelt_amd import _lib
     from edg_westervelt_amd.solver import time_step
+
+    if args.probe_copy > 0:
+        n = int(args.probe_copy * (1 << 20) / 8)
+        ms = C.c_double(0.0)
+        st = _lib.lib.wedg_probe_copy(n, 10, C.byref(ms))
+        if st != 0:
+            raise SystemExit(f"copy probe failed ({st})")
+        print(json.dumps(dict(copy_probe_bytes_each_way=8 * n, launches=11, ms_avg=ms.value,
+                              gbs=2 * 8 * n / ms.value / 1e6)))
+        return
 
     q, N, T = args.q, args.N, 0.5
     t_pre = time.perf_counter()

@@ -76,6 +76,7 @@ def _load():
         "wedg_timing_read": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "wedg_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)],
         "wedg_cheb_stats": [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)],
+        "wedg_probe_copy": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

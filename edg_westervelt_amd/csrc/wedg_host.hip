@@ -674,6 +674,38 @@ extern "C" wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_st
     return WEDG_OK;
 }
 
+namespace {
+__global__ void k_copy_probe(const double* __restrict__ src, double* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" wedg_status wedg_probe_copy(size_t n, int repeats, double* ms) {
+    double *a = nullptr, *b = nullptr;
+    HIPOK(hipMalloc(&a, n * sizeof(double)));
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) { (void)hipFree(a); return WEDG_ERR_HIP; }
+    hipEvent_t e0, e1;
+    wedg_status st = WEDG_OK;
+    if (hipMemset(a, 0, n * sizeof(double)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        st = WEDG_ERR_HIP;
+    } else {
+        k_copy_probe<<<1024, 256>>>(a, b, n);
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < repeats; ++r) k_copy_probe<<<1024, 256>>>(a, b, n);
+        (void)hipEventRecord(e1, nullptr);
+        float t = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) st = WEDG_ERR_HIP;
+        if (ms) *ms = repeats > 0 ? t / repeats : 0.0;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return st;
+}
+
 extern "C" const char* wedg_version(void) { return "wedg 0.1 gfx950 fp64"; }
 
 extern "C" wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs) {

@@ -149,6 +149,11 @@ wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* l
  * waves of the count each wave of 64 elements ran, and the number of stages. */
 wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_steps, long* stages);
 
+/* Measurement utility (no counterpart in the reference): a plain copy kernel of
+ * n doubles run `repeats` times on the current device, for calibrating the
+ * byte counters of a profiler; *ms receives the average time per copy. */
+wedg_status wedg_probe_copy(size_t n, int repeats, double* ms);
+
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);
 wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);
