@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6        # MI355X FP64 (vector = matrix), TFLOP/s
 METRIC = "FP64 DOF·stages/s for edg_rhs!+RK4"
-FAMILIES = ("traces", "volume", "faces+u-solve", "mass")
+FAMILIES = ("traces", "element")   # k_traces; k_element (volume, faces, u-solve, mass, RK4 update)
 
 
 def gaussian_profile(s=0.05, amp=0.05):
@@ -53,17 +53,18 @@ def build_problem(q, N):
 
 
 def flops_per_element_stage(nu, nv, nh, nc):
-    """Algorithmic FP64 flops of one element and stage (traces, volume, u-solve,
-    lifts, Chebyshev set-up, nc - 1 Chebyshev steps)."""
+    """Algorithmic FP64 flops of one element and stage: traces, volume, u-solve,
+    lifts, Chebyshev set-up and nc - 1 Chebyshev steps (the plain sum-factorized
+    counts of the reference's algorithm). Returns (total, traces part)."""
     nq = nu
-    base = (2 * (4 * (nu * nu + nq * nu) + 8 * (nv * nv + nq * nv))
+    traces = 2 * (4 * (nu * nu + nq * nu) + 8 * (nv * nv + nq * nv))
+    base = (traces
             + 2 * (2 * (nq * nu * nu + nq * nq * nu) + 2 * (nq * nv * nv + nq * nq * nv)
                    + 2 * (nv * nq * nq + nv * nv * nq) + 2 * (nu * nq * nq + nu * nu * nq))
             + 2 * nu ** 4 + 4 * (3 * nq * (2 * nv + nu) + 3 * (2 * nv * nv + nu * nu))
             + 2 * (nh * nv * nv + nh * nh * nv))
     step = 4 * (nh * nv * nv + nh * nh * nv) + nh * nh + 6 * nv * nv
-    mass = 2 * (nh * nv * nv + nh * nh * nv) + (nc - 1) * step
-    return base + (nc - 1) * step, mass
+    return base + (nc - 1) * step, traces
 
 
 def main():
@@ -137,10 +138,11 @@ def main():
     L.wedg_cheb_stats(h, C.byref(mean_nc), C.byref(nst))
     info = ws.dev.last_error()
 
-    # per-kernel durations: HIP events around every launch on the workspace stream
-    tsteps = min(50, max(5, steps // 20))
+    # per-kernel durations: HIP events around every launch on the workspace
+    # stream, over the same steps from the same state as the timed region
+    ws.dev.check(L.wedg_set_state(h, uw.ctypes.data_as(C.POINTER(C.c_double)), vw.ctypes.data_as(C.POINTER(C.c_double))))
     L.wedg_timing_enable(h, 1)
-    ws.dev.check(L.wedg_advance(h, tend.value, dt, tsteps, 0, C.byref(tend)))
+    ws.dev.check(L.wedg_advance(h, t0, dt, steps, 0, C.byref(tend)))
     fam = {}
     for i, name in enumerate(FAMILIES):
         ms = C.c_double(0.0)
@@ -159,20 +161,25 @@ def main():
     value = dof * stages / elapsed
     ms_per_step = 1e3 * elapsed / steps
 
-    # roofline of the dominant kernel (the Chebyshev mass solve)
-    f_total, f_mass = flops_per_element_stage(nu, nv, nh, nc_timing)
-    dom = max(fam, key=lambda k: fam[k]["us_avg"])
-    dom_s = fam[dom]["us_avg"] * 1e-6
-    if dom == "mass":
-        achieved = E * f_mass / dom_s / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=FP64_PEAK_TFS, unit="TFLOP/s",
-                    frac=achieved / FP64_PEAK_TFS)
-    else:
-        roof = dict(bound="mfma", achieved=None, peak=FP64_PEAK_TFS, unit="TFLOP/s", frac=None)
-    roof["kernel"] = dom
-    roof["traffic"] = None
-    if args.traffic and os.path.exists(args.traffic):
-        roof["traffic"] = json.load(open(args.traffic)).get(dom)
+    # rooflines: k_element is FP64-bound (its algorithmic flops per element and
+    # stage over its average duration), k_traces HBM-bound (72 + 72 words per
+    # element and stage at q = 5: the state read, the traces written)
+    f_total, f_traces = flops_per_element_stage(nu, nv, nh, nc_timing)
+    el_s = fam["element"]["us_avg"] * 1e-6
+    achieved = E * (f_total - f_traces) / el_s / 1e12
+    roof = dict(bound="mfma", achieved=achieved, peak=FP64_PEAK_TFS, unit="TFLOP/s",
+                frac=achieved / FP64_PEAK_TFS, traffic=None, kernel="k_element",
+                flops_per_launch=E * (f_total - f_traces), us_avg=fam["element"]["us_avg"])
+    tr_bytes = 8 * E * (nu * nu + nv * nv + 12 * nu)
+    tr_s = fam["traces"]["us_avg"] * 1e-6
+    roof_traces = dict(bound="hbm", achieved=tr_bytes / tr_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                       frac=tr_bytes / tr_s / 1e9 / HBM_PEAK_GBS, kernel="k_traces",
+                       bytes_per_launch=tr_bytes, us_avg=fam["traces"]["us_avg"])
+    traffic_file = args.traffic or os.path.join(ROOT, "profiles", f"traffic_q{q}_n{N}.json")
+    if os.path.exists(traffic_file):
+        tj = json.load(open(traffic_file))
+        roof["traffic"] = tj.get("k_element")
+        roof_traces["traffic"] = tj.get("k_traces")
     stage_s = sum(f["us_avg"] for f in fam.values()) * 1e-6
     bytes_step = 8 * 17 * dof
     whole = dict(flops_per_stage=E * f_total, tflops=E * f_total / stage_s / 1e12,
@@ -205,7 +212,8 @@ def main():
                                     f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
                            elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, full_run_steps=nsteps_full,
                            precompute_s=round(t_pre, 2)),
-               roofline=roof, whole_step=whole, kernels_us=fam, cpu_baseline=cpu, final_state_finite=finite)
+               roofline=roof, roofline_traces=roof_traces, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
+               final_state_finite=finite)
     print(json.dumps(out))
 
 

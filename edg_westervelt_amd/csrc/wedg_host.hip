@@ -67,6 +67,90 @@ void dedup(long E, size_t len, F rec, std::vector<int>& cls, std::vector<double>
     }
 }
 
+// Eigen-decomposition of a small symmetric matrix A (n x n, column-major) by
+// cyclic Jacobi rotations: A = V diag(lam) V^T, eigenvalues ascending.
+void sym_eigen(std::vector<double> A, int n, std::vector<double>& lam, std::vector<double>& V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += A[i + (size_t)n * j] * A[i + (size_t)n * j];
+                if (i != j) off += A[i + (size_t)n * j] * A[i + (size_t)n * j];
+            }
+        if (off <= 1e-34 * tot) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p + (size_t)n * q];
+                if (apq == 0.0) continue;
+                const double app = A[p + (size_t)n * p], aqq = A[q + (size_t)n * q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {   // A <- J^T A J
+                    const double akp = A[k + (size_t)n * p], akq = A[k + (size_t)n * q];
+                    A[k + (size_t)n * p] = c * akp - sn * akq;
+                    A[k + (size_t)n * q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[p + (size_t)n * k], aqk = A[q + (size_t)n * k];
+                    A[p + (size_t)n * k] = c * apk - sn * aqk;
+                    A[q + (size_t)n * k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
+                    V[k + (size_t)n * p] = c * vkp - sn * vkq;
+                    V[k + (size_t)n * q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    lam.resize(n);
+    for (int i = 0; i < n; ++i) lam[i] = A[i + (size_t)n * i];
+    // ascending order
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (lam[j] < lam[i]) {
+                std::swap(lam[i], lam[j]);
+                for (int k = 0; k < n; ++k) std::swap(V[k + (size_t)n * i], V[k + (size_t)n * j]);
+            }
+}
+
+// Spectral factors of the u-stiffness for constant c^2 on an element
+// (src/solver.jl:509-536): with K = S^T diag(w) S the 1D stiffness (its modes
+// split by parity, mode 0 its null vector) K = Q diag(lam) Q^T, and the pinned
+// stiffness is A = c^2 (I (x) K + K (x) I) + e0 e0^T. Q[a + nu i] keeps each
+// eigenvector in a slot of the parity of its modes.
+void stiffness_spectrum(const double* S, const double* w, int nu, std::vector<double>& Q,
+                        std::vector<double>& lam) {
+    const int nq = nu;
+    std::vector<double> K((size_t)nu * nu, 0.0);
+    for (int a = 0; a < nu; ++a)
+        for (int b = 0; b < nu; ++b) {
+            double s = 0.0;
+            for (int p = 0; p < nq; ++p) s += w[p] * S[p + nq * a] * S[p + nq * b];
+            K[a + (size_t)nu * b] = s;
+        }
+    Q.assign((size_t)nu * nu, 0.0);
+    lam.assign(nu, 0.0);
+    Q[0] = 1.0;   // the constant mode: S e0 = 0
+    for (int par = 0; par < 2; ++par) {
+        std::vector<int> idx;
+        for (int a = (par == 0 ? 2 : 1); a < nu; a += 2) idx.push_back(a);
+        const int m = (int)idx.size();
+        if (m == 0) continue;
+        std::vector<double> B((size_t)m * m), l, V;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) B[i + (size_t)m * j] = 0.5 * (K[idx[i] + (size_t)nu * idx[j]] + K[idx[j] + (size_t)nu * idx[i]]);
+        sym_eigen(B, m, l, V);
+        for (int c = 0; c < m; ++c) {
+            const int slot = idx[c];
+            lam[slot] = l[c];
+            for (int i = 0; i < m; ++i) Q[idx[i] + (size_t)nu * slot] = V[i + (size_t)m * c];
+        }
+    }
+}
+
 bool all_equal(const double* p, size_t n) {
     for (size_t i = 1; i < n; ++i)
         if (std::memcmp(&p[i], &p[0], sizeof(double)) != 0) return false;
@@ -81,7 +165,7 @@ struct wedg_problem {
     int mms_on = 0, src_on = 0, one_sided = 0;
     double src_omega = 0, src_Ts = 0;
     wedg_side bc[4];
-    int uni_ca = 0;
+    int mode = MODE_GENERAL;   // kernel specialisation (wedg_kernels.h)
     int ncls_ca = 0, ncls_b = 0;
     std::vector<void*> allocs;
     double* tab = nullptr;
@@ -89,6 +173,7 @@ struct wedg_problem {
     double *Cw = nullptr, *Aw = nullptr, *Ainv = nullptr;
     int* cls_b = nullptr;
     double* beta = nullptr;
+    std::vector<double> lam, Q;   // spectrum of the 1D stiffness
     FaceArr f[F_COUNT];
     double* bcd[8] = {};
     double* proj[5] = {};   // PiSP, PiBSP2, PiCdiv, PiDdiv, src
@@ -126,7 +211,6 @@ struct wedg_workspace {
     double *yu = nullptr, *yv = nullptr, *su = nullptr, *sv = nullptr;
     double *au = nullptr, *av = nullptr, *ou = nullptr, *ov = nullptr;
     double* tr = nullptr;
-    double *Rb = nullptr, *Bb = nullptr;
     DevErr* err = nullptr;
     DevErr* herr = nullptr;  // pinned host copy of the failure record
     double *hu = nullptr, *hv = nullptr;   // pinned staging of the state (host <-> device)
@@ -148,12 +232,12 @@ struct wedg_workspace {
 
 static hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s) {
     switch (p->qu) {
-        case 1: return launch_q1(a, p->nv, which, s, p->uni_ca);
-        case 2: return launch_q2(a, p->nv, which, s, p->uni_ca);
-        case 3: return launch_q3(a, p->nv, which, s, p->uni_ca);
-        case 4: return launch_q4(a, p->nv, which, s, p->uni_ca);
-        case 5: return launch_q5(a, p->nv, which, s, p->uni_ca);
-        case 6: return launch_q6(a, p->nv, which, s, p->uni_ca);
+        case 1: return launch_q1(a, p->nv, which, s, p->mode);
+        case 2: return launch_q2(a, p->nv, which, s, p->mode);
+        case 3: return launch_q3(a, p->nv, which, s, p->mode);
+        case 4: return launch_q4(a, p->nv, which, s, p->mode);
+        case 5: return launch_q5(a, p->nv, which, s, p->mode);
+        case 6: return launch_q6(a, p->nv, which, s, p->mode);
         default: return hipErrorInvalidValue;
     }
 }
@@ -201,7 +285,12 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         t.insert(t.end(), pre->E1br_u, pre->E1br_u + nu);
         t.insert(t.end(), pre->P_h_v, pre->P_h_v + nh * nv);
         t.insert(t.end(), pre->W2d_h, pre->W2d_h + nh * nh);
+        std::vector<double> Q, lam;
+        stiffness_spectrum(pre->S_u, pre->w, nu, Q, lam);
+        t.insert(t.end(), Q.begin(), Q.end());
         UP(p->tab, t.data(), t.size());
+        p->lam = lam;
+        p->Q = Q;
     }
     // (Cw, Aw, Ainv) per element -> classes
     {
@@ -219,9 +308,10 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         }, cls, uniq);
         const int nc = (int)(uniq.size() / len);
         p->ncls_ca = nc;
-        p->uni_ca = nc == 1;
         const int n = nu * nu;
         std::vector<double> cw((size_t)nc * nq * nq), aw((size_t)nc * nq * nq), ai((size_t)nc * n * n);
+        std::vector<double> dfac((size_t)nc * n);
+        bool separable = true;
         for (int c = 0; c < nc; ++c) {
             std::memcpy(&cw[(size_t)c * nq * nq], &uniq[(size_t)c * len], nq * nq * sizeof(double));
             std::memcpy(&aw[(size_t)c * nq * nq], &uniq[(size_t)c * len + nq * nq], nq * nq * sizeof(double));
@@ -234,11 +324,51 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
                     full[i + n * jj] = v;
                     full[jj + n * i] = v;
                 }
+            // c^2 constant on the element: cw = c^2 w (x) w; then the inverse has
+            // the spectral form (Q (x) Q) diag(D) (Q (x) Q)^T, checked against the
+            // stored inverse entry by entry
+            if (!separable) continue;
+            const double* cwc = &cw[(size_t)c * nq * nq];
+            double c2 = 0.0;
+            for (int ix = 0; ix < nq; ++ix)
+                for (int iy = 0; iy < nq; ++iy) c2 += cwc[iy + nq * ix] / (pre->w[iy] * pre->w[ix]);
+            c2 /= nq * nq;
+            for (int ix = 0; ix < nq && separable; ++ix)
+                for (int iy = 0; iy < nq; ++iy)
+                    if (!(std::fabs(cwc[iy + nq * ix] - c2 * pre->w[iy] * pre->w[ix]) <= 1e-13 * std::fabs(cwc[iy + nq * ix])) ||
+                        !(c2 > 0.0)) {
+                        separable = false;
+                        break;
+                    }
+            if (!separable) continue;
+            double* D = &dfac[(size_t)c * n];
+            for (int jj = 0; jj < nu; ++jj)
+                for (int i = 0; i < nu; ++i)
+                    D[i + nu * jj] = 1.0 / (c2 * (p->lam[i] + p->lam[jj]) + (i == 0 && jj == 0 ? 1.0 : 0.0));
+            double amax = 0.0, dmax = 0.0;
+            for (int col = 0; col < n && separable; ++col) {
+                const int a2 = col % nu, b2 = col / nu;
+                for (int row = 0; row < n; ++row) {
+                    const int a1 = row % nu, b1 = row / nu;
+                    double s = 0.0;
+                    for (int jj = 0; jj < nu; ++jj)
+                        for (int i = 0; i < nu; ++i)
+                            s += p->Q[a1 + nu * i] * p->Q[b1 + nu * jj] * D[i + nu * jj] * p->Q[a2 + nu * i] *
+                                 p->Q[b2 + nu * jj];
+                    const double ref = full[row + n * col];
+                    amax = std::max(amax, std::fabs(ref));
+                    dmax = std::max(dmax, std::fabs(s - ref));
+                }
+            }
+            if (!(dmax <= 1e-10 * amax)) separable = false;
         }
+        const bool fd = separable;
         UP(p->Cw, cw.data(), cw.size());
         UP(p->Aw, aw.data(), aw.size());
-        UP(p->Ainv, ai.data(), ai.size());
+        if (fd) UP(p->Ainv, dfac.data(), dfac.size());
+        else UP(p->Ainv, ai.data(), ai.size());
         UP(p->cls_ca, cls.data(), cls.size());
+        p->mode = fd ? MODE_SEPARABLE : MODE_GENERAL;
     }
     // beta at the high-order points per element -> classes
     {
@@ -273,6 +403,9 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
                 p->f[a].uni = 0;
             }
         }
+        bool faces_uni = !pre->one_sided;
+        for (int a = 0; a < F_COUNT; ++a) faces_uni = faces_uni && p->f[a].uni;
+        if (p->mode == MODE_SEPARABLE && p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) p->mode = MODE_UNIFORM;
     }
     // boundary data (NQ, N), unchanged
     {
@@ -362,8 +495,6 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     ws->yu = A(nu); ws->yv = A(nv); ws->su = A(nu); ws->sv = A(nv);
     ws->au = A(nu); ws->av = A(nv); ws->ou = A(nu); ws->ov = A(nv);
     ws->tr = A((size_t)12 * p->nu * p->E);
-    ws->Rb = A((size_t)p->nv * p->nv * p->E);
-    ws->Bb = A((size_t)p->nu * p->nu * p->E);
     if (he == hipSuccess) {
         void* e = nullptr;
         he = hipMalloc(&e, sizeof(DevErr) + 16);
@@ -431,7 +562,6 @@ static KArgs base_args(wedg_workspace* ws) {
     a.fstride = p->N * (p->N + 1);
     for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
     a.tr = ws->tr; a.err = ws->err; a.cnt = ws->cnt;
-    a.Rbuf = ws->Rb; a.Bbuf = ws->Bb;
     return a;
 }
 
@@ -498,7 +628,7 @@ static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, co
     a.yu = yu; a.yv = yv; a.y0u = y0u; a.y0v = y0v;
     a.accu = ws->au; a.accv = ws->av; a.outu = outu; a.outv = outv;
     ++ws->stages;
-    for (int which = 0; which < 4; ++which) {
+    for (int which = 0; which < 2; ++which) {
         wedg_status st = run_kernel(ws, a, which);
         if (st != WEDG_OK) return st;
     }

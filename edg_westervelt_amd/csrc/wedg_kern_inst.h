@@ -1,0 +1,26 @@
+// The kernel launches of one (n_u, n_v) pair, included by the per-degree
+// instantiation files wedg_kern_q*.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "wedg_kernels.h"
+
+namespace wedg {
+
+// The launches of one (n_u, n_v) pair; used by the instantiation files only.
+template <int NU, int NV>
+static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) {
+    const int grid = (a.E + kWave - 1) / kWave;
+    if (which == 0) {
+        if (mode == MODE_UNIFORM) k_traces<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
+        else k_traces<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
+    } else if (which == 1) {
+        if (mode == MODE_UNIFORM) k_element<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
+        else if (mode == MODE_SEPARABLE) k_element<NU, NV, MODE_SEPARABLE><<<grid, kWave, 0, s>>>(a);
+        else k_element<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace wedg

@@ -2,23 +2,30 @@
 //
 // One lane owns one element e = j + N k: it reads the element's modal
 // coefficients straight from the state layout (N, N, n, n) — consecutive lanes
-// read consecutive elements, so every load and store is coalesced — and keeps
-// the element's small matrices in registers. The 1D Legendre tables live in a
-// read-only buffer addressed only through compile-time offsets, so they are
-// fetched with scalar loads and feed the FMAs as SGPR operands.
+// read consecutive elements, so every load and store is one coalesced wave
+// access — and keeps the element's small matrices in registers. The 1D
+// Legendre tables live in a read-only buffer read through the constant
+// address space, so they arrive by scalar loads and feed the FMAs as SGPR
+// operands.
+//
+// The tables are values (or derivatives) of Legendre polynomials at Gauss
+// points, which are mirrored about the centre: T[n-1-p][a] = (-1)^a T[p][a]
+// for values and -(-1)^a T[p][a] for derivatives. Every contraction is split
+// into its even and odd modes (rows_sym / cols_sym), which halves its FMAs; the
+// results differ from the plain sums by rounding only.
 //
 // Two kernels per right-hand side:
 //   k_traces   the traces of every element on its four sides (reference
 //              _traces!/_trace4!, src/rhs.jl:384-410, 573-580) into a trace
 //              buffer, and with the global step-count policy the Chebyshev
 //              intervals (src/rhs.jl:889-918) reduced to one count;
-//   k_element  each element's own face slots from its traces and its
-//              neighbours' (interior fluxes src/rhs.jl:620-700, boundary
-//              fluxes src/rhs.jl:65-128, 738-765), the volume terms
-//              (src/rhs.jl:317-333), the source (src/rhs.jl:340-350), the lifts
+//   k_element  the volume terms (src/rhs.jl:317-333) and source
+//              (src/rhs.jl:340-350), each element's own face slots from its
+//              traces and its neighbours' (interior fluxes src/rhs.jl:620-700,
+//              boundary fluxes src/rhs.jl:65-128, 738-765), the lifts
 //              (src/rhs.jl:421-473), the u-equation (src/rhs.jl:361-382), the
 //              Chebyshev mass solve (src/rhs.jl:889-939) and the RK4 stage
-//              update (src/solver.jl:300-321) in its epilogue.
+//              update (src/solver.jl:300-321).
 // Every face slot is computed by the element that owns it, so no slot has two
 // writers and no floating-point atomics exist; results do not depend on the
 // launch shape.
@@ -29,9 +36,15 @@
 
 namespace wedg {
 
-constexpr int kBlock = 256;      // lanes per block of the element kernels
-constexpr int kMassBlock = 64;   // one wave per block for the mass solve (per-lane LDS)
+constexpr int kWave = 64;             // lanes per block: one wave
 constexpr int kMassLdsDoubles = 80;   // Chebyshev weights per lane kept in LDS: 4 waves fill 160 KB
+
+// Coefficient layouts the element kernel is specialised for.
+enum {
+    MODE_GENERAL = 0,     // per-element classes, stored inverse of the u-stiffness
+    MODE_SEPARABLE = 1,   // per-element classes, c^2 constant on each element: spectral u-solve
+    MODE_UNIFORM = 2,     // one class, one beta record, every face array one value; spectral u-solve
+};
 
 template <int NU_, int NV_>
 struct Dims {
@@ -39,7 +52,10 @@ struct Dims {
     static constexpr int QU = NU_ - 1, QV = NV_ - 1;
     static constexpr int NH0 = (3 * QV + 1) / 2 + 1;                 // ceil(3 q_v / 2) + 1
     static constexpr int NH = NH0 > NU_ ? NH0 : NU_;                  // src/solver.jl:483
-    static constexpr int NPK = (NU * NU * (NU * NU + 1)) / 2;
+    static constexpr int NP = NH / 2;                                 // mirrored pairs of high-order rows
+    static constexpr int PL0 = kMassLdsDoubles / (2 * NH);
+    static constexpr int PL = PL0 < NP ? PL0 : NP;                    // pairs of G rows in LDS
+    static constexpr int PR = NP - PL;                                // pairs of G rows in registers
     // offsets in the table buffer
     static constexpr int oP = 0;                    // P_u  (NQ x NU), column-major
     static constexpr int oS = oP + NQ * NU;         // S_u  (NQ x NU)
@@ -50,7 +66,8 @@ struct Dims {
     static constexpr int oE1r = oE1l + NU;          // E1br
     static constexpr int oPh = oE1r + NU;           // P_h_v (NH x NV)
     static constexpr int oWh = oPh + NH * NV;       // W2d_h (NH x NH)
-    static constexpr int nTab = oWh + NH * NH;
+    static constexpr int oQ = oWh + NH * NH;        // eigenvectors of the 1D stiffness (NU x NU)
+    static constexpr int nTab = oQ + NU * NU;
 };
 
 struct FaceArr {
@@ -83,11 +100,12 @@ struct KArgs {
     double pc[5];                   // coefficients of the projections
     double gam[4], eta[4], sig[4], kb[4], kap[4];
     const double* tab;
-    const int* cls_ca;              // class of (Cw, Aw, Ainv) per element, null = one class
+    const int* cls_ca;              // class of (Cw, Aw, u-solve record) per element
     const double* Cw;               // per class (NQ*NQ), iy + NQ ix
     const double* Aw;
-    const double* Ainv;             // per class (NU^2 x NU^2), symmetric, unpacked
-    const int* cls_b;               // class of beta_hq per element (all 0 for one class)
+    const double* Ainv;             // per class: the inverse (NU^2 x NU^2, symmetric) or, for the
+                                    // spectral u-solve, its eigenvalue factors D (NU x NU)
+    const int* cls_b;               // class of beta_hq per element
     const double* beta;             // per class (NH*NH), p + NH q
     FaceArr f[F_COUNT];
     int fstride;                    // N (N + 1): stride of the point index of face arrays
@@ -99,91 +117,163 @@ struct KArgs {
     double* accu; double* accv;             // RK4 accumulator
     double* outu; double* outv;             // Y_{s+1}, y (last stage) or (u_t, v_t)
     double* tr;                             // traces [arr][i][E]
-    double* Rbuf;                           // right-hand side of the mass systems [m][E]
-    double* Bbuf;                           // (c^2 grad phi_u, grad v_h) [m][E]
     DevErr* err;
     int* cnt;                               // Chebyshev counts, two slots
 };
 
-// value of a face array at flat index fi = i * (N (N + 1)) + face
-__device__ __forceinline__ double fv(const FaceArr& a, long fi) { return a.p[a.uni ? 0 : fi]; }
-
-
-// The pointer, opaque to the optimizer: loads through it are issued where they
-// are used instead of being hoisted out of a loop and kept alive (the tables
-// are small and read from the scalar cache at every use).
-template <class T>
-__device__ __forceinline__ const T* fresh(const T* p) {
+// ---------------------------------------------------------------------------
+// Table access.
 #if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" : "+s"(p));
+#define WEDG_CONST_AS __attribute__((address_space(4)))
+#else
+#define WEDG_CONST_AS
 #endif
-    return p;
+typedef const WEDG_CONST_AS double cdbl;
+
+// The table pointer in the constant address space, opaque to the optimizer:
+// loads through it are scalar and are issued where they are used instead of
+// being hoisted out of a loop and kept alive for the whole kernel.
+__device__ __forceinline__ cdbl* ctab(const double* p) {
+    cdbl* q = (cdbl*)p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(q));
+#endif
+    return q;
 }
 
-__device__ __forceinline__ bool skip_stage(const KArgs& a) {
+// value of a face array at flat index fi = i * (N (N + 1)) + face
+template <bool UNI, class FA>
+__device__ __forceinline__ double fv(const FA& a, long fi) {
+    if (UNI) return ((cdbl*)a.p)[0];
+    return a.uni ? ((cdbl*)a.p)[0] : a.p[fi];
+}
+
+// The kernel's own arguments read through the kernarg segment pointer in the
+// constant address space, opaque to the optimizer like ctab(): fields are
+// loaded (scalar) where they are used instead of being kept in SGPRs across
+// the whole kernel. Every kernel takes one KArgs by value, at offset 0.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const WEDG_CONST_AS KArgs CKArgs;
+__device__ __forceinline__ CKArgs* kargs() {
+    CKArgs* p = (CKArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+#else
+typedef const KArgs CKArgs;
+extern const KArgs* wedg_emu_kargs;   // set by a host build that runs the kernels on the CPU
+__device__ __forceinline__ CKArgs* kargs() { return wedg_emu_kargs; }
+#endif
+
+template <class KA>
+__device__ __forceinline__ bool skip_stage(const KA& a) {
     const int fl = a.err->failed;
     return fl != 0 && fl < a.serial;
 }
 
+// First element of the block. Blocks are dealt to the 8 XCDs in turn; this
+// gives each XCD a contiguous band of elements, so the neighbours across k
+// live in the same L2 except at the band edges.
+__device__ __forceinline__ int block_first_element() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    return lb * kWave;
+}
+
 // ---------------------------------------------------------------------------
-// traces of one field on the four sides (src/rhs.jl:388-410): for side B
-//   o[i] = sum_b P[i, b] sum_a El[a] X[a, b]
-// and L/R the same on the transpose; sums in order, fused like muladd.
-template <int NQ, int N, int oP, int oEl, int oEr>
-__device__ __forceinline__ void trace4(const double* __restrict__ T, const double (&X)[N * N],
-                                       double* __restrict__ tr, int arr0, int e, int E) {
-    double oB[NQ], oT[NQ], oL[NQ], oR[NQ];
+// Mirrored contractions. T is column-major with NR rows (points) and at least
+// n columns (modes); ODD marks a derivative table.
+
+// rows p and NR-1-p (p < NR/2) of sum_a T[r, a] x(a)
+template <int NR, int n, bool ODD, class X>
+__device__ __forceinline__ void row_pair(cdbl* T, int p, X x, double& yp, double& yq) {
+    double ev = 0.0, od = 0.0;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) { oB[i] = 0.0; oT[i] = 0.0; oL[i] = 0.0; oR[i] = 0.0; }
+    for (int a = 0; a < n; a += 2) ev = fma(T[p + NR * a], x(a), ev);
 #pragma unroll
-    for (int b = 0; b < N; ++b) {
-        double sB = 0.0, sT = 0.0, sL = 0.0, sR = 0.0;
+    for (int a = 1; a < n; a += 2) od = fma(T[p + NR * a], x(a), od);
+    yp = ev + od;
+    yq = ODD ? od - ev : ev - od;
+}
+
+// the middle row (NR odd): only the modes of one parity are non-zero there
+template <int NR, int n, bool ODD, class X>
+__device__ __forceinline__ double row_mid(cdbl* T, X x) {
+    constexpr int m = NR / 2;
+    double s = 0.0;
 #pragma unroll
-        for (int a = 0; a < N; ++a) {
-            const double x = X[a + N * b], xt = X[b + N * a];
-            sB = fma(T[oEl + a], x, sB);
-            sT = fma(T[oEr + a], x, sT);
-            sL = fma(T[oEl + a], xt, sL);
-            sR = fma(T[oEr + a], xt, sR);
-        }
+    for (int a = ODD ? 1 : 0; a < n; a += 2) s = fma(T[m + NR * a], x(a), s);
+    return s;
+}
+
+// y[r] = sum_a T[r, a] x(a) for every row r
+template <int NR, int n, bool ODD, class X>
+__device__ __forceinline__ void rows_sym(cdbl* T, X x, double (&y)[NR]) {
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const double pp = T[oP + i + NQ * b];
-            oB[i] = fma(pp, sB, oB[i]);
-            oT[i] = fma(pp, sT, oT[i]);
-            oL[i] = fma(pp, sL, oL[i]);
-            oR[i] = fma(pp, sR, oR[i]);
-        }
+    for (int p = 0; p < NR / 2; ++p) row_pair<NR, n, ODD>(T, p, x, y[p], y[NR - 1 - p]);
+    if (NR & 1) y[NR / 2] = row_mid<NR, n, ODD>(T, x);
+}
+
+// z[a] = sum_r T[r, a] y[r] for the first n columns
+template <int NR, int n, bool ODD>
+__device__ __forceinline__ void cols_sym(cdbl* T, const double (&y)[NR], double (&z)[n]) {
+    constexpr int H = NR / 2;
+    double s[H > 0 ? H : 1], d[H > 0 ? H : 1];
+#pragma unroll
+    for (int p = 0; p < H; ++p) {
+        s[p] = y[p] + y[NR - 1 - p];
+        d[p] = y[p] - y[NR - 1 - p];
     }
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        tr[(long)((arr0 + 0) * NQ + i) * E + e] = oB[i];
-        tr[(long)((arr0 + 1) * NQ + i) * E + e] = oT[i];
-        tr[(long)((arr0 + 2) * NQ + i) * E + e] = oL[i];
-        tr[(long)((arr0 + 3) * NQ + i) * E + e] = oR[i];
+    for (int a = 0; a < n; ++a) {
+        const bool same = ((a & 1) == 0) != ODD;   // T[NR-1-p, a] == T[p, a]
+        double acc = 0.0;
+#pragma unroll
+        for (int p = 0; p < H; ++p) acc = fma(T[p + NR * a], same ? s[p] : d[p], acc);
+        if ((NR & 1) && same) acc = fma(T[H + NR * a], y[H], acc);
+        z[a] = acc;
     }
 }
 
-// One row p of g = (2 beta) o (P_h V P_h^T) at the high-order points.
-template <class D>
-__device__ __forceinline__ void g_row(const double* __restrict__ T, const double (&V)[D::NV * D::NV],
-                                      const double* __restrict__ beta, int p, double (&g)[D::NH]) {
-    constexpr int NV = D::NV, NH = D::NH;
-    double tp[NV];
+// ---------------------------------------------------------------------------
+// traces of one field X (n x n) on the four sides (src/rhs.jl:388-410): for
+// side B  o[i] = sum_b P[i, b] sum_a El[a] X[a, b], T with Er, L/R the same on
+// the transpose. El = (-1)^a Er for values, -(-1)^a Er for derivatives (DER).
+template <class D, int n, bool DER>
+__device__ __forceinline__ void trace4(cdbl* T, int oEr, const double (&X)[n * n], double* __restrict__ tr,
+                                       int arr0, int e, int E) {
+    constexpr int NQ = D::NQ;
+    double sB[n], sT[n], sL[n], sR[n];
 #pragma unroll
-    for (int b = 0; b < NV; ++b) {
-        double s = 0.0;
+    for (int b = 0; b < n; ++b) {
+        double ev = 0.0, od = 0.0, evt = 0.0, odt = 0.0;
 #pragma unroll
-        for (int a = 0; a < NV; ++a) s = fma(T[D::oPh + p + NH * a], V[a + NV * b], s);
-        tp[b] = s;
+        for (int a = 0; a < n; ++a) {
+            const double er = T[oEr + a];
+            if (a & 1) {
+                od = fma(er, X[a + n * b], od);
+                odt = fma(er, X[b + n * a], odt);
+            } else {
+                ev = fma(er, X[a + n * b], ev);
+                evt = fma(er, X[b + n * a], evt);
+            }
+        }
+        sT[b] = ev + od;
+        sB[b] = DER ? od - ev : ev - od;
+        sR[b] = evt + odt;
+        sL[b] = DER ? odt - evt : evt - odt;
     }
+    cdbl* P = T + D::oP;
+    auto emit = [&](const double (&sv)[n], int sd) {
+        double o[NQ];
+        rows_sym<NQ, n, false>(P, [&](int b) { return sv[b]; }, o);
 #pragma unroll
-    for (int q = 0; q < NH; ++q) {
-        double s = 0.0;
-#pragma unroll
-        for (int b = 0; b < NV; ++b) s = fma(tp[b], T[D::oPh + q + NH * b], s);
-        g[q] = s * (2.0 * beta[p + NH * q]);
-    }
+        for (int i = 0; i < NQ; ++i) tr[(long)((arr0 + sd) * NQ + i) * E + e] = o[i];
+    };
+    emit(sB, 0);
+    emit(sT, 1);
+    emit(sL, 2);
+    emit(sR, 3);
 }
 
 // Interval data of the Chebyshev iteration from the extremes of g
@@ -200,56 +290,92 @@ __device__ __forceinline__ int cheb_interval(double gmin, double gmax, bool nan,
     return n < 1 ? 1 : n;
 }
 
+// Rows p and NH-1-p of v_q = P_h V P_h^T (the values of v_h at the high-order
+// points); the middle row when p == NH/2 and NH is odd (then yq is unused).
+template <class D>
+__device__ __forceinline__ void vq_rows(cdbl* Ph, int p, bool mid, const double (&V)[D::NV * D::NV],
+                                        double (&yp)[D::NH], double (&yq)[D::NH]) {
+    constexpr int NV = D::NV, NH = D::NH;
+    double tp[NV], tq[NV];
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        if (mid) {
+            tp[b] = row_mid<NH, NV, false>(Ph, [&](int a) { return V[a + NV * b]; });
+            tq[b] = 0.0;
+        } else {
+            row_pair<NH, NV, false>(Ph, p, [&](int a) { return V[a + NV * b]; }, tp[b], tq[b]);
+        }
+    }
+    rows_sym<NH, NV, false>(Ph, [&](int b) { return tp[b]; }, yp);
+    if (!mid) rows_sym<NH, NV, false>(Ph, [&](int b) { return tq[b]; }, yq);
+}
+
 // ---------------------------------------------------------------------------
 // Traces of every element, and with the global policy the largest Chebyshev
 // step count (src/rhs.jl:889-918) reduced over the grid.
-template <int NU, int NV>
-__global__ __launch_bounds__(kBlock) void k_traces(const KArgs a, const double* __restrict__ T0,
-                                                   const double* __restrict__ beta0) {
+template <int NU, int NV, int MODE>
+__global__ __launch_bounds__(kWave) void k_traces(const KArgs args) {
+    (void)args;
     using D = Dims<NU, NV>;
-    constexpr int NQ = D::NQ, NH = D::NH;
-    if (a.err->failed != 0) return;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    const int E = a.E;
+    constexpr int NH = D::NH;
+    constexpr bool UNI = MODE == MODE_UNIFORM;
+    if (kargs()->err->failed != 0) return;
+    const int e = block_first_element() + threadIdx.x;
+    const int E = kargs()->E;
+    const bool global = kargs()->cheb_global != 0;
     int n_e = 1;
     if (e < E) {
         {
+            const double* yu = kargs()->yu;
             double U[NU * NU];
 #pragma unroll
-            for (int m = 0; m < NU * NU; ++m) U[m] = a.yu[(long)m * E + e];
-            trace4<NQ, NU, D::oP, D::oE1l, D::oE1r>(fresh(T0), U, a.tr, 0, e, E);
+            for (int m = 0; m < NU * NU; ++m) U[m] = yu[(long)m * E + e];
+            trace4<D, NU, true>(ctab(kargs()->tab), D::oE1r, U, kargs()->tr, 0, e, E);
         }
+        const double* yv = kargs()->yv;
         double V[NV * NV];
 #pragma unroll
-        for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-        trace4<NQ, NV, D::oP, D::oEl, D::oEr>(fresh(T0), V, a.tr, 4, e, E);
-        trace4<NQ, NV, D::oP, D::oE1l, D::oE1r>(fresh(T0), V, a.tr, 8, e, E);
-        if (a.cheb_global) {
-            const double* beta = beta0 + (long)a.cls_b[e] * NH * NH;
+        for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
+        trace4<D, NV, false>(ctab(kargs()->tab), D::oEr, V, kargs()->tr, 4, e, E);
+        trace4<D, NV, true>(ctab(kargs()->tab), D::oE1r, V, kargs()->tr, 8, e, E);
+        if (global) {
+            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
             double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
             bool nan = false;
 #pragma unroll 1
-            for (int p = 0; p < NH; ++p) {
-                double g[NH];
-                g_row<D>(fresh(T0), V, beta, p, g);
+            for (int p = 0; p < (NH + 1) / 2; ++p) {
+                cdbl* Ph = ctab(kargs()->tab) + D::oPh;
+                cdbl* bu = ctab(kargs()->beta);
+                const bool mid = (NH & 1) && p == NH / 2;
+                double yp[NH], yq[NH];
+                vq_rows<D>(Ph, p, mid, V, yp, yq);
 #pragma unroll
                 for (int q = 0; q < NH; ++q) {
-                    gmin = fmin(gmin, g[q]);
-                    gmax = fmax(gmax, g[q]);
-                    nan = nan || (g[q] != g[q]);
+                    const double bp = UNI ? bu[p + NH * q] : betav[p + NH * q];
+                    const double g = yp[q] * (2.0 * bp);
+                    gmin = fmin(gmin, g);
+                    gmax = fmax(gmax, g);
+                    nan = nan || (g != g);
+                    if (!mid) {
+                        const double bq = UNI ? bu[NH - 1 - p + NH * q] : betav[NH - 1 - p + NH * q];
+                        const double g2 = yq[q] * (2.0 * bq);
+                        gmin = fmin(gmin, g2);
+                        gmax = fmax(gmax, g2);
+                        nan = nan || (g2 != g2);
+                    }
                 }
             }
             double th, de, s1;
-            n_e = cheb_interval(gmin, gmax, nan, a.cheb_L, th, de, s1);
+            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
             if (n_e == 0) {
-                atomicMin(&a.err->mfail, e + 1);
+                atomicMin(&kargs()->err->mfail, e + 1);
                 n_e = 1;
             }
         }
     }
-    if (a.cheb_global) {
+    if (global) {
         for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(&a.cnt[a.slot], n_e);
+        if ((threadIdx.x & 63) == 0) atomicMax(&kargs()->cnt[kargs()->slot], n_e);
     }
 }
 
@@ -258,16 +384,17 @@ __global__ __launch_bounds__(kBlock) void k_traces(const KArgs a, const double* 
 // on the "+" side (plus) or the "-" side (src/rhs.jl:620-700). vm, vp, ... are
 // the traces of the "-" and "+" elements at the point; fi indexes the face
 // arrays at the point.
-__device__ __forceinline__ void interior_point(const KArgs& a, bool plus, bool hz, long fi, double vm, double vp,
+template <bool UNI, class KA>
+__device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, long fi, double vm, double vp,
                                                double dum, double dup, double dvm, double dvp, double& A,
                                                double& Dd, double& Ff) {
     const double th = a.theta, tau = a.tau, lu = a.lu, lv = a.lv, two_h = a.two_h;
-    const FaceArr& fc2 = a.f[hz ? F_HC2 : F_VC2];
-    const FaceArr& faf = a.f[hz ? F_HAF : F_VAF];
-    if (a.one_sided) {
-        const double c2m = fv(a.f[hz ? F_HC2M : F_VC2M], fi), c2p = fv(fc2, fi);
-        const double afm = fv(a.f[hz ? F_HAFM : F_VAFM], fi), afp = fv(faf, fi);
-        const double K = fv(a.f[hz ? F_HJC : F_VJC], fi);
+    const auto& fc2 = a.f[hz ? F_HC2 : F_VC2];
+    const auto& faf = a.f[hz ? F_HAF : F_VAF];
+    if (!UNI && a.one_sided) {
+        const double c2m = fv<UNI>(a.f[hz ? F_HC2M : F_VC2M], fi), c2p = fv<UNI>(fc2, fi);
+        const double afm = fv<UNI>(a.f[hz ? F_HAFM : F_VAFM], fi), afp = fv<UNI>(faf, fi);
+        const double K = fv<UNI>(a.f[hz ? F_HJC : F_VJC], fi);
         const double J = two_h * (c2m * dum - c2p * dup + afm * dvm - afp * dvp) + K;
         const double G = two_h * ((1 - th) * (c2m * dum + afm * dvm) + th * (c2p * dup + afp * dvp));
         const double pen = (0.5 * (c2m + c2p)) * lu + (0.5 * (afm + afp)) * lv;
@@ -284,7 +411,7 @@ __device__ __forceinline__ void interior_point(const KArgs& a, bool plus, bool h
             Ff = -c2m * dvsm;
         }
     } else {
-        const double c2 = fv(fc2, fi), af = fv(faf, fi);
+        const double c2 = fv<UNI>(fc2, fi), af = fv<UNI>(faf, fi);
         const double jmp = vm - vp;
         const double gst = two_h * (c2 * (th * dup + (1 - th) * dum) + af * (th * dvp + (1 - th) * dvm));
         const double tj = tau * (c2 * (two_h * (dum - dup)) + af * (two_h * (dvm - dvp)));
@@ -302,14 +429,15 @@ __device__ __forceinline__ void interior_point(const KArgs& a, bool plus, bool h
 
 // Slot values of one point i of a physical boundary side s (src/rhs.jl:738-765);
 // returns the point's failing _boundary_key (src/rhs.jl:732) or 0.
-__device__ __forceinline__ int boundary_point(const KArgs& a, int s, long fi, int i, int m, int nq, double du,
+template <bool UNI, class KA>
+__device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i, int m, int nq, double du,
                                               double vt, double dv, double& A, double& Dd, double& Ff) {
     const bool hz = s < 2;
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
-    const double c2f = fv(a.f[hz ? F_HC2 : F_VC2], fi);
-    const double af = fv(a.f[hz ? F_HAF : F_VAF], fi);
-    const double cf = fv(a.f[hz ? F_HCF : F_VCF], fi);
-    const double bf = fv(a.f[hz ? F_HBF : F_VBF], fi);
+    const double c2f = fv<UNI>(a.f[hz ? F_HC2 : F_VC2], fi);
+    const double af = fv<UNI>(a.f[hz ? F_HAF : F_VAF], fi);
+    const double cf = fv<UNI>(a.f[hz ? F_HCF : F_VCF], fi);
+    const double bf = fv<UNI>(a.f[hz ? F_HBF : F_VBF], fi);
     const double two_h = a.two_h, om = a.omega, ct = a.ct, stt = a.st;
     const double gun = sgn * (two_h * du);
     const double gvn = sgn * (two_h * dv);
@@ -345,119 +473,87 @@ __device__ __forceinline__ int boundary_point(const KArgs& a, int s, long fi, in
 }
 
 // ---------------------------------------------------------------------------
-// Volume terms (src/rhs.jl:317-333) scaled by -4/h^2 plus the source
-// (src/rhs.jl:340-350) into R, and VB = (c^2 grad phi_u, grad v_h) into B.
-// The point values are formed one point row p at a time and contracted back at
-// once (rank-one updates), so only the modal arrays stay live.
-template <class D, int n, bool UNIC>
-__device__ __forceinline__ void vol_pass(const double* __restrict__ T0, const double* __restrict__ W0,
-                                         const double* __restrict__ Wv, const double (&X)[n * n],
-                                         int nout, double* __restrict__ out) {
-    // out (nout x nout) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S over the points
+// Volume terms (src/rhs.jl:317-333):
+//   out (NO x NO) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S
+// for the modal X (n x n) and point weights W(p, q). The point values of the
+// gradient are formed one mirrored pair of point rows at a time and contracted
+// back at once, so only the modal arrays stay live.
+template <class D, int n, int NO, class WF>
+__device__ __forceinline__ void vol_row(cdbl* P, cdbl* S, const double (&t1)[n], const double (&t2)[n], WF W,
+                                        int r, double (&z1)[NO], double (&z2)[NO]) {
+    constexpr int NQ = D::NQ;
+    double g1[NQ], g2[NQ];
+    rows_sym<NQ, n, false>(P, [&](int b) { return t1[b]; }, g1);   // (S X P^T)[r, q]
+    rows_sym<NQ, n, true>(S, [&](int b) { return t2[b]; }, g2);    // (P X S^T)[r, q]
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double w = W(r, q);
+        g1[q] = w * g1[q];
+        g2[q] = w * g2[q];
+    }
+    cols_sym<NQ, NO, false>(P, g1, z1);
+    cols_sym<NQ, NO, true>(S, g2, z2);
+}
+
+template <class D, int n, int NO, class WF>
+__device__ __forceinline__ void vol_pass(const double* tab, const double (&X)[n * n], WF W, double (&out)[NO * NO]) {
     constexpr int NQ = D::NQ;
 #pragma unroll 1
-    for (int p = 0; p < NQ; ++p) {
-        const double* __restrict__ T = fresh(T0);
-        const double* __restrict__ W = UNIC ? fresh(W0) : Wv;
+    for (int p = 0; p < NQ / 2; ++p) {
+        cdbl* T = ctab(tab);
+        cdbl* P = T + D::oP;
+        cdbl* S = T + D::oS;
+        double t1p[n], t1q[n], t2p[n], t2q[n];
+#pragma unroll
+        for (int b = 0; b < n; ++b) {
+            row_pair<NQ, n, true>(S, p, [&](int aa) { return X[aa + n * b]; }, t1p[b], t1q[b]);
+            row_pair<NQ, n, false>(P, p, [&](int aa) { return X[aa + n * b]; }, t2p[b], t2q[b]);
+        }
+        double z1p[NO], z2p[NO], z1q[NO], z2q[NO];
+        vol_row<D, n, NO>(P, S, t1p, t2p, W, p, z1p, z2p);
+        vol_row<D, n, NO>(P, S, t1q, t2q, W, NQ - 1 - p, z1q, z2q);
+        // S[q', a] = -(-1)^a S[p, a], P[q', a] = (-1)^a P[p, a]
+#pragma unroll
+        for (int b = 0; b < NO; ++b) {
+            const double s1e = z1p[b] - z1q[b], s1o = z1p[b] + z1q[b];
+            const double s2e = z2p[b] + z2q[b], s2o = z2p[b] - z2q[b];
+#pragma unroll
+            for (int aa = 0; aa < NO; ++aa)
+                out[aa + NO * b] = fma(S[p + NQ * aa], (aa & 1) ? s1o : s1e,
+                                       fma(P[p + NQ * aa], (aa & 1) ? s2o : s2e, out[aa + NO * b]));
+        }
+    }
+    if (NQ & 1) {
+        constexpr int m = NQ / 2;
+        cdbl* T = ctab(tab);
+        cdbl* P = T + D::oP;
+        cdbl* S = T + D::oS;
         double t1[n], t2[n];
 #pragma unroll
         for (int b = 0; b < n; ++b) {
-            double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-            for (int aa = 0; aa < n; ++aa) {
-                s1 = fma(T[D::oS + p + NQ * aa], X[aa + n * b], s1);
-                s2 = fma(T[D::oP + p + NQ * aa], X[aa + n * b], s2);
-            }
-            t1[b] = s1; t2[b] = s2;
+            t1[b] = row_mid<NQ, n, true>(S, [&](int aa) { return X[aa + n * b]; });
+            t2[b] = row_mid<NQ, n, false>(P, [&](int aa) { return X[aa + n * b]; });
         }
-        double z1[D::NU], z2[D::NU];
+        double z1[NO], z2[NO];
+        vol_row<D, n, NO>(P, S, t1, t2, W, m, z1, z2);
 #pragma unroll
-        for (int b = 0; b < D::NU; ++b) { z1[b] = 0.0; z2[b] = 0.0; }
+        for (int b = 0; b < NO; ++b)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double g1 = 0.0, g2 = 0.0;
-#pragma unroll
-            for (int b = 0; b < n; ++b) {
-                g1 = fma(t1[b], T[D::oP + q + NQ * b], g1);
-                g2 = fma(t2[b], T[D::oS + q + NQ * b], g2);
-            }
-            const double w = W[p + NQ * q];
-            const double h1 = w * g1, h2 = w * g2;
-#pragma unroll
-            for (int b = 0; b < D::NU; ++b) {
-                if (b < nout) {
-                    z1[b] = fma(h1, T[D::oP + q + NQ * b], z1[b]);
-                    z2[b] = fma(h2, T[D::oS + q + NQ * b], z2[b]);
-                }
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < D::NU; ++b)
-#pragma unroll
-            for (int aa = 0; aa < D::NU; ++aa)
-                if (b < nout && aa < nout)
-                    out[aa + nout * b] = fma(T[D::oS + p + NQ * aa], z1[b],
-                                             fma(T[D::oP + p + NQ * aa], z2[b], out[aa + nout * b]));
+            for (int aa = 0; aa < NO; ++aa)
+                out[aa + NO * b] = (aa & 1) ? fma(S[m + NQ * aa], z1[b], out[aa + NO * b])
+                                            : fma(P[m + NQ * aa], z2[b], out[aa + NO * b]);
     }
-}
-
-template <int NU, int NV, bool UNIC>
-__global__ __launch_bounds__(kBlock) void k_volume(const KArgs a, const double* __restrict__ T0,
-                                                   const double* __restrict__ Cw0, const double* __restrict__ Aw0,
-                                                   double* __restrict__ R, double* __restrict__ B) {
-    using D = Dims<NU, NV>;
-    constexpr int NQ = D::NQ;
-    if (skip_stage(a)) return;
-    const int E = a.E;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= E) return;
-    const long cc = UNIC ? 0 : (long)a.cls_ca[e];
-    const double* __restrict__ Cw = Cw0 + cc * NQ * NQ;
-    const double* __restrict__ Aw = Aw0 + cc * NQ * NQ;
-    double Rv[NV * NV];
-#pragma unroll
-    for (int m = 0; m < NV * NV; ++m) Rv[m] = 0.0;
-    {   // (c^2 grad phi_v, grad u_h)
-        double U[NU * NU];
-#pragma unroll
-        for (int m = 0; m < NU * NU; ++m) U[m] = a.yu[(long)m * E + e];
-        vol_pass<D, NU, UNIC>(T0, Cw0, Cw, U, NV, Rv);
-    }
-    double V[NV * NV];
-#pragma unroll
-    for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-    // (alpha grad phi_v, grad v_h)
-    vol_pass<D, NV, UNIC>(T0, Aw0, Aw, V, NV, Rv);
-    // scale and source, in the order of the reference's sum
-#pragma unroll
-    for (int m = 0; m < NV * NV; ++m) {
-        double r = (-a.four_hh) * Rv[m];
-        for (int t = 0; t < a.nproj; ++t) r = r + a.pc[t] * a.proj[t][(long)m * E + e];
-        R[(long)m * E + e] = r;
-    }
-    // (c^2 grad phi_u, grad v_h)
-    double VB[NU * NU];
-#pragma unroll
-    for (int m = 0; m < NU * NU; ++m) VB[m] = 0.0;
-    vol_pass<D, NV, UNIC>(T0, Cw0, Cw, V, NU, VB);
-#pragma unroll
-    for (int m = 0; m < NU * NU; ++m) B[(long)m * E + e] = VB[m];
 }
 
 // ---------------------------------------------------------------------------
-// Face slots of the element's four sides from its own and its neighbours'
-// traces, lifted into R and Fu (src/rhs.jl:421-473, 589-776); then the
-// u-equation (src/rhs.jl:361-382) and the u part of the RK4 stage update
-// (src/solver.jl:300-321).
-template <class D>
+// Geometry of side s of element e = j + N k.
 struct SideGeom {
     int fidx, nb, m;
     bool bnd, hz, plus;
 };
 
-template <class D>
-__device__ __forceinline__ SideGeom<D> side_geom(int s, int e, int j, int k, int N) {
-    SideGeom<D> g;
+__device__ __forceinline__ SideGeom side_geom(int s, int e, int j, int k, int N) {
+    SideGeom g;
     g.hz = s < 2;
     g.plus = (s == 0 || s == 2);
     if (s == 0)      { g.fidx = j + (N + 1) * k;     g.nb = e - 1; g.bnd = j == 0;     g.m = k; }
@@ -468,323 +564,430 @@ __device__ __forceinline__ SideGeom<D> side_geom(int s, int e, int j, int k, int
     return g;
 }
 
-// Slot values (A, D, F) of point i of side s of element e and its failing
-// boundary key (0 if none).
+// ---------------------------------------------------------------------------
+// One mirrored pair of point rows (p, NH-1-p) of
+//   K d = P_h^T (G o (P_h d P_h^T)) P_h
+// accumulated into r; Gp, Gq are the weight rows p and NH-1-p.
 template <class D>
-__device__ __forceinline__ int slot_point(const KArgs& a, const SideGeom<D>& g, int s, int i, int e, double& A,
-                                          double& Dd, double& Ff) {
-    constexpr int NQ = D::NQ;
-    const int E = a.E;
-    const double odu = a.tr[(long)((0 + s) * NQ + i) * E + e];
-    const double ov = a.tr[(long)((4 + s) * NQ + i) * E + e];
-    const double odv = a.tr[(long)((8 + s) * NQ + i) * E + e];
-    const long fi = (long)i * a.fstride + g.fidx;
-    if (g.bnd) return boundary_point(a, s, fi, i, g.m, NQ, odu, ov, odv, A, Dd, Ff);
-    const int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
-    const double ndu = a.tr[(long)((0 + ns) * NQ + i) * E + g.nb];
-    const double nvv = a.tr[(long)((4 + ns) * NQ + i) * E + g.nb];
-    const double ndv = a.tr[(long)((8 + ns) * NQ + i) * E + g.nb];
-    if (g.plus) interior_point(a, true, g.hz, fi, nvv, ov, ndu, odu, ndv, odv, A, Dd, Ff);
-    else interior_point(a, false, g.hz, fi, ov, nvv, odu, ndu, odv, ndv, A, Dd, Ff);
-    return 0;
+__device__ __forceinline__ void kd_pair(cdbl* Ph, int p, const double (&d)[D::NV * D::NV],
+                                        const double (&Gp)[D::NH], const double (&Gq)[D::NH],
+                                        double (&r)[D::NV * D::NV]) {
+    constexpr int NV = D::NV, NH = D::NH;
+    double tp[NV], tq[NV];
+#pragma unroll
+    for (int b = 0; b < NV; ++b)
+        row_pair<NH, NV, false>(Ph, p, [&](int aa) { return d[aa + NV * b]; }, tp[b], tq[b]);
+    double zp[NV], zq[NV];
+    {
+        double y[NH];
+        rows_sym<NH, NV, false>(Ph, [&](int b) { return tp[b]; }, y);
+#pragma unroll
+        for (int q = 0; q < NH; ++q) y[q] *= Gp[q];
+        cols_sym<NH, NV, false>(Ph, y, zp);
+    }
+    {
+        double y[NH];
+        rows_sym<NH, NV, false>(Ph, [&](int b) { return tq[b]; }, y);
+#pragma unroll
+        for (int q = 0; q < NH; ++q) y[q] *= Gq[q];
+        cols_sym<NH, NV, false>(Ph, y, zq);
+    }
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        const double zs = zp[b] + zq[b], zd = zp[b] - zq[b];
+#pragma unroll
+        for (int aa = 0; aa < NV; ++aa) r[aa + NV * b] = fma(Ph[p + NH * aa], (aa & 1) ? zd : zs, r[aa + NV * b]);
+    }
 }
 
-// Face slots of the element's four sides from its own and its neighbours'
-// traces, lifted into R (src/rhs.jl:421-473, 589-776); then the same slots
-// lifted into Fu, the u-equation (src/rhs.jl:361-382) and the u part of the
-// RK4 stage update (src/solver.jl:300-321). The two lifts are separate passes
-// over the sides so that R and Fu are never live together; the slot values,
-// a few flops per point, are formed in both.
-template <int NU, int NV, bool UNIC>
-__global__ __launch_bounds__(kBlock) void k_faces(const KArgs a, const double* __restrict__ T0,
-                                                  const double* __restrict__ Ai0, double* __restrict__ R,
-                                                  const double* __restrict__ B) {
-    using D = Dims<NU, NV>;
-    constexpr int NQ = D::NQ;
-    if (skip_stage(a)) return;
-    const int E = a.E, N = a.N;
-    const int mfail_prev = a.cheb_global ? a.err->mfail : 0x7fffffff;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= E) return;
-    const int j = e % N, k = e / N;
-    int bkey = 0;
-    {   // lifts of A and D into the v-equation (src/rhs.jl:449-463)
-        double Rv[NV * NV];
+// the middle point row (NH odd)
+template <class D>
+__device__ __forceinline__ void kd_mid(cdbl* Ph, const double (&d)[D::NV * D::NV], const double (&Gm)[D::NH],
+                                       double (&r)[D::NV * D::NV]) {
+    constexpr int NV = D::NV, NH = D::NH, m = NH / 2;
+    double tm[NV];
 #pragma unroll
-        for (int m = 0; m < NV * NV; ++m) Rv[m] = R[(long)m * E + e];
+    for (int b = 0; b < NV; ++b) tm[b] = row_mid<NH, NV, false>(Ph, [&](int aa) { return d[aa + NV * b]; });
+    double y[NH], z[NV];
+    rows_sym<NH, NV, false>(Ph, [&](int b) { return tm[b]; }, y);
+#pragma unroll
+    for (int q = 0; q < NH; ++q) y[q] *= Gm[q];
+    cols_sym<NH, NV, false>(Ph, y, z);
+#pragma unroll
+    for (int b = 0; b < NV; ++b)
+#pragma unroll
+        for (int aa = 0; aa < NV; aa += 2) r[aa + NV * b] = fma(Ph[m + NH * aa], z[b], r[aa + NV * b]);
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stage update of one field (src/solver.jl:300-321) from the stage's
+// derivative k of the element: stage 0 writes k itself (a plain right-hand
+// side), stage 1 starts the accumulator, stages 2-3 add 2 k, stage 4 forms the
+// new y. Loads are issued in chunks before the arithmetic that needs them.
+template <int n>
+__device__ __forceinline__ bool rk_update(int stage, double c, const double (&kv)[n], const double* y0, double* acc,
+                                          double* out, int E, int e) {
+    bool fin = true;
+    if (stage == 0) {
+#pragma unroll
+        for (int m = 0; m < n; ++m) out[(long)m * E + e] = kv[m];
+    } else if (stage == 1) {
+        constexpr int C = 12;
+#pragma unroll
+        for (int m0 = 0; m0 < n; m0 += C) {
+            double y[C];
+#pragma unroll
+            for (int m = 0; m < C; ++m)
+                if (m0 + m < n) y[m] = y0[(long)(m0 + m) * E + e];
+#pragma unroll
+            for (int m = 0; m < C; ++m)
+                if (m0 + m < n) {
+                    out[(long)(m0 + m) * E + e] = y[m] + c * kv[m0 + m];
+                    acc[(long)(m0 + m) * E + e] = kv[m0 + m];
+                }
+        }
+    } else {
+        constexpr int C = 9;
+#pragma unroll
+        for (int m0 = 0; m0 < n; m0 += C) {
+            double y[C], ac[C];
+#pragma unroll
+            for (int m = 0; m < C; ++m)
+                if (m0 + m < n) {
+                    y[m] = y0[(long)(m0 + m) * E + e];
+                    ac[m] = acc[(long)(m0 + m) * E + e];
+                }
+#pragma unroll
+            for (int m = 0; m < C; ++m)
+                if (m0 + m < n) {
+                    const double k = kv[m0 + m];
+                    if (stage < 4) {
+                        out[(long)(m0 + m) * E + e] = y[m] + c * k;
+                        acc[(long)(m0 + m) * E + e] = ac[m] + 2.0 * k;
+                    } else {
+                        const double yn = y[m] + c * (ac[m] + k);
+                        out[(long)(m0 + m) * E + e] = yn;
+                        fin = fin && isfinite(yn);
+                    }
+                }
+        }
+    }
+    return fin;
+}
+
+// The element kernel: volume terms, source, face slots and lifts, u-equation,
+// mass solve and the RK4 stage update of one element per lane. One wave per
+// block: the Chebyshev weights G = 2 beta v_q W of the first mirrored row
+// pairs live in LDS ([entry][lane], conflict-free), the rest in registers.
+template <int NU, int NV, int MODE>
+__global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
+    (void)args;
+    using D = Dims<NU, NV>;
+    constexpr int NQ = D::NQ, NH = D::NH, NP = D::NP, PL = D::PL, PR = D::PR;
+    constexpr bool UNI = MODE == MODE_UNIFORM;
+    constexpr bool FD = MODE != MODE_GENERAL;
+    constexpr int nU = NU * NU, nV = NV * NV;
+    __shared__ double Gs[(PL > 0 ? 2 * PL * NH : 1) * kWave];
+    if (skip_stage(*kargs())) return;
+    const int E = kargs()->E, N = kargs()->N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) kargs()->cnt[kargs()->slot ^ 1] = 1;   // the next stage's count
+    const int mfail_prev = kargs()->cheb_global ? kargs()->err->mfail : 0x7fffffff;
+    const int lane = threadIdx.x;
+    const int e = block_first_element() + lane;
+    int ncheb = 0;
+    if (e < E) {
+        const int j = e % N, k = e / N;
+        const long cc = UNI ? 0 : (long)kargs()->cls_ca[e];
+        auto Wc = [&](int p, int q) -> double {
+            return UNI ? ctab(kargs()->Cw)[p + NQ * q] : kargs()->Cw[cc * NQ * NQ + p + NQ * q];
+        };
+        auto Wa = [&](int p, int q) -> double {
+            return UNI ? ctab(kargs()->Aw)[p + NQ * q] : kargs()->Aw[cc * NQ * NQ + p + NQ * q];
+        };
+
+        // ---- volume terms: Rv (v-test) and VB (u-test), src/rhs.jl:317-333
+        double Rv[nV], bv[nU];
+#pragma unroll
+        for (int m = 0; m < nV; ++m) Rv[m] = 0.0;
+#pragma unroll
+        for (int m = 0; m < nU; ++m) bv[m] = 0.0;
+        double v0;
+        {
+            const double* yv = kargs()->yv;
+            double V[nV];
+#pragma unroll
+            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
+            v0 = V[0];
+            vol_pass<D, NV, NU>(kargs()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
+            vol_pass<D, NV, NV>(kargs()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
+        }
+        {
+            const double* yu = kargs()->yu;
+            double U[nU];
+#pragma unroll
+            for (int m = 0; m < nU; ++m) U[m] = yu[(long)m * E + e];
+            vol_pass<D, NU, NV>(kargs()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
+        }
+        // scale and source, in the order of the reference's sum (src/rhs.jl:340-350)
+        {
+            const double s = -kargs()->four_hh;
+#pragma unroll
+            for (int m = 0; m < nV; ++m) Rv[m] = s * Rv[m];
+            const int np = kargs()->nproj;
+            for (int t = 0; t < np; ++t) {
+                const double pc = kargs()->pc[t];
+                const double* pr = kargs()->proj[t];
+                double pv[nV];
+#pragma unroll
+                for (int m = 0; m < nV; ++m) pv[m] = pr[(long)m * E + e];
+#pragma unroll
+                for (int m = 0; m < nV; ++m) Rv[m] = Rv[m] + pc * pv[m];
+            }
+        }
+
+        // ---- face slots of the four sides, lifted into Rv (A, D) and b (F)
+        // (src/rhs.jl:421-473, 589-776)
+        int bkey = 0;
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
-            const SideGeom<D> g = side_geom<D>(s, e, j, k, N);
-            double YA[NV], YD[NV];
+            CKArgs& A = *kargs();
+            const SideGeom g = side_geom(s, e, j, k, N);
+            const int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
+            const double* tr = A.tr;
+            double odu[NQ], ov[NQ], odv[NQ], ndu[NQ], nvv[NQ], ndv[NQ];
 #pragma unroll
-            for (int b = 0; b < NV; ++b) { YA[b] = 0.0; YD[b] = 0.0; }
-#pragma unroll 1
             for (int i = 0; i < NQ; ++i) {
-                const double* __restrict__ T = fresh(T0);
-                double A, Dd, Ff;
-                const int k2 = slot_point<D>(a, g, s, i, e, A, Dd, Ff);
-                if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
-                // Y_Z[b] += P[i, b] (w[i] Z[i]), summed in point order (src/rhs.jl:434-441)
-                const double wA = T[D::oW + i] * A, wD = T[D::oW + i] * Dd;
-#pragma unroll
-                for (int b = 0; b < NV; ++b) {
-                    YA[b] = fma(T[D::oP + i + NQ * b], wA, YA[b]);
-                    YD[b] = fma(T[D::oP + i + NQ * b], wD, YD[b]);
-                }
+                odu[i] = tr[(long)((0 + s) * NQ + i) * E + e];
+                ov[i] = tr[(long)((4 + s) * NQ + i) * E + e];
+                odv[i] = tr[(long)((8 + s) * NQ + i) * E + e];
+                ndu[i] = tr[(long)((0 + ns) * NQ + i) * E + g.nb];
+                nvv[i] = tr[(long)((4 + ns) * NQ + i) * E + g.nb];
+                ndv[i] = tr[(long)((8 + ns) * NQ + i) * E + g.nb];
             }
-            const double* __restrict__ T = fresh(T0);
+            double wA[NQ], wD[NQ], wF[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const long fi = (long)i * A.fstride + g.fidx;
+                double Av, Dd, Ff;
+                if (g.bnd) {
+                    const int k2 = boundary_point<UNI>(A, s, fi, i, g.m, NQ, odu[i], ov[i], odv[i], Av, Dd, Ff);
+                    if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
+                } else if (g.plus) {
+                    interior_point<UNI>(A, true, g.hz, fi, nvv[i], ov[i], ndu[i], odu[i], ndv[i], odv[i], Av, Dd, Ff);
+                } else {
+                    interior_point<UNI>(A, false, g.hz, fi, ov[i], nvv[i], odu[i], ndu[i], odv[i], ndv[i], Av, Dd, Ff);
+                }
+                const double wi = ctab(A.tab)[D::oW + i];
+                wA[i] = wi * Av;
+                wD[i] = wi * Dd;
+                wF[i] = wi * Ff;
+            }
+            // Y_Z[b] = sum_i P[i, b] w[i] Z[i] (src/rhs.jl:434-441)
+            cdbl* T = ctab(A.tab);
+            double YA[NV], YD[NV], YF[NU];
+            cols_sym<NQ, NV, false>(T + D::oP, wA, YA);
+            cols_sym<NQ, NV, false>(T + D::oP, wD, YD);
+            cols_sym<NQ, NU, false>(T + D::oP, wF, YF);
             const int oE = g.plus ? D::oEl : D::oEr;
             const int oE1 = g.plus ? D::oE1l : D::oE1r;
+            const double two_h = A.two_h, four_hh = A.four_hh;
 #pragma unroll
             for (int b = 0; b < NV; ++b)
 #pragma unroll
                 for (int aa = 0; aa < NV; ++aa) {
                     double r = Rv[aa + NV * b];
                     if (g.hz) {
-                        r = r + (a.two_h * T[oE + aa]) * YA[b];
-                        r = r + (a.four_hh * T[oE1 + aa]) * YD[b];
+                        r = r + (two_h * T[oE + aa]) * YA[b];
+                        r = r + (four_hh * T[oE1 + aa]) * YD[b];
                     } else {
-                        r = r + (a.two_h * YA[aa]) * T[oE + b];
-                        r = r + (a.four_hh * YD[aa]) * T[oE1 + b];
+                        r = r + (two_h * YA[aa]) * T[oE + b];
+                        r = r + (four_hh * YD[aa]) * T[oE1 + b];
                     }
                     Rv[aa + NV * b] = r;
                 }
+            // b = VB - Fu, Fu = sum over sides of E1 (x) Y_F (src/rhs.jl:464-473)
+#pragma unroll
+            for (int b = 0; b < NU; ++b)
+#pragma unroll
+                for (int aa = 0; aa < NU; ++aa)
+                    bv[aa + NU * b] -= g.hz ? T[oE1 + aa] * YF[b] : YF[aa] * T[oE1 + b];
         }
-#pragma unroll
-        for (int m = 0; m < NV * NV; ++m) R[(long)m * E + e] = Rv[m];
-    }
-    if (bkey) {
-        atomicMin(&a.err->bkey, bkey);
-        a.err->failed = a.serial;
-    }
-    if (mfail_prev != 0x7fffffff) a.err->failed = a.serial;
+        if (bkey) {
+            atomicMin(&kargs()->err->bkey, bkey);
+            kargs()->err->failed = kargs()->serial;
+        }
+        if (mfail_prev != 0x7fffffff) kargs()->err->failed = kargs()->serial;
+        bv[0] += v0;   // (src/rhs.jl:366-370)
 
-    constexpr int n = NU * NU;
-    double b[n];   // b = VB - Fu, then b[1] += V[1] (src/rhs.jl:366-370)
-    {
-        double Fu[n];
+        // ---- u-equation u_t = A_e^-1 b (src/rhs.jl:361-382) and the u part of
+        // the RK4 stage update (src/solver.jl:300-321)
+        double ku[nU];
+        if (FD) {
+            // A_e = c^2 (I (x) K + K (x) I) + e0 e0^T with K = Q diag(lambda) Q^T:
+            // A_e^-1 b = (Q (x) Q) (Dc o ((Q (x) Q)^T b)); Q pairs modes of one parity.
+            cdbl* Q = ctab(kargs()->tab) + D::oQ;
+            double Y1[nU];
 #pragma unroll
-        for (int m = 0; m < n; ++m) Fu[m] = 0.0;
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s) {
-            const SideGeom<D> g = side_geom<D>(s, e, j, k, N);
-            double YF[NU];
+            for (int b = 0; b < NU; ++b)
 #pragma unroll
-            for (int bb = 0; bb < NU; ++bb) YF[bb] = 0.0;
-#pragma unroll 1
-            for (int i = 0; i < NQ; ++i) {
-                const double* __restrict__ T = fresh(T0);
-                double A, Dd, Ff;
-                slot_point<D>(a, g, s, i, e, A, Dd, Ff);
-                const double wF = T[D::oW + i] * Ff;
+                for (int i = 0; i < NU; ++i) {
+                    double s = 0.0;
 #pragma unroll
-                for (int bb = 0; bb < NU; ++bb) YF[bb] = fma(T[D::oP + i + NQ * bb], wF, YF[bb]);
-            }
-            const double* __restrict__ T = fresh(T0);
-            const int oE1 = g.plus ? D::oE1l : D::oE1r;
+                    for (int aa = (i & 1); aa < NU; aa += 2) s = fma(Q[aa + NU * i], bv[aa + NU * b], s);
+                    Y1[i + NU * b] = s;
+                }
+            const double* Dv = kargs()->Ainv + cc * nU;
+            cdbl* Du = ctab(kargs()->Ainv);
 #pragma unroll
-            for (int bb = 0; bb < NU; ++bb)
+            for (int i = 0; i < NU; ++i)
+#pragma unroll
+                for (int jj = 0; jj < NU; ++jj) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int b = (jj & 1); b < NU; b += 2) s = fma(Y1[i + NU * b], Q[b + NU * jj], s);
+                    bv[i + NU * jj] = s * (UNI ? Du[i + NU * jj] : Dv[i + NU * jj]);
+                }
+#pragma unroll
+            for (int jj = 0; jj < NU; ++jj)
 #pragma unroll
                 for (int aa = 0; aa < NU; ++aa) {
-                    if (g.hz) Fu[aa + NU * bb] = Fu[aa + NU * bb] + (1.0 * T[oE1 + aa]) * YF[bb];
-                    else Fu[aa + NU * bb] = Fu[aa + NU * bb] + (1.0 * YF[aa]) * T[oE1 + bb];
+                    double s = 0.0;
+#pragma unroll
+                    for (int i = (aa & 1); i < NU; i += 2) s = fma(Q[aa + NU * i], bv[i + NU * jj], s);
+                    Y1[aa + NU * jj] = s;
                 }
-        }
 #pragma unroll
-        for (int i = 0; i < n; ++i) b[i] = B[(long)i * E + e] - Fu[i];
-    }
-    b[0] += a.yv[e];
-
-    // u-equation u_t = A_e^-1 b (src/rhs.jl:361-382). The inverse of the class is
-    // stored unpacked; it is symmetric, so row i is the contiguous column i and
-    // each output is finished (and written) before the next one starts.
-    const long cc = UNIC ? 0 : (long)a.cls_ca[e];
-    const double c = a.cstage;
-    bool fin = true;
-#pragma unroll 1
-    for (int mm = 0; mm < n; ++mm) {
-        const double* __restrict__ Ar = (UNIC ? fresh(Ai0) : Ai0 + cc * (n * n)) + (long)n * mm;
-        double kv = 0.0;
+            for (int b = 0; b < NU; ++b)
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj) kv = fma(Ar[jj], b[jj], kv);
-        const long idx = (long)mm * E + e;
-        if (a.stage == 0) {
-            a.outu[idx] = kv;
-        } else if (a.stage == 1) {
-            a.outu[idx] = a.y0u[idx] + c * kv;
-            a.accu[idx] = kv;
-        } else if (a.stage < 4) {
-            a.outu[idx] = a.y0u[idx] + c * kv;
-            a.accu[idx] = a.accu[idx] + 2.0 * kv;
+                for (int aa = 0; aa < NU; ++aa) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int jj = (b & 1); jj < NU; jj += 2) s = fma(Y1[aa + NU * jj], Q[b + NU * jj], s);
+                    ku[aa + NU * b] = s;
+                }
         } else {
-            const double y = a.y0u[idx] + c * (a.accu[idx] + kv);
-            a.outu[idx] = y;
-            fin = fin && isfinite(y);
+            // the stored inverse of the class, symmetric: row mm is the contiguous column mm
+            const double* Ai = kargs()->Ainv + cc * (nU * nU);
+#pragma unroll
+            for (int mm = 0; mm < nU; ++mm) {
+                const double* Ar = Ai + (long)nU * mm;
+                double kv = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < nU; ++jj) kv = fma(Ar[jj], bv[jj], kv);
+                ku[mm] = kv;
+            }
         }
-    }
-    if (!fin) atomicMin(&a.err->nonfinite, a.step);
-}
+        {
+            CKArgs& A = *kargs();
+            if (!rk_update<nU>(A.stage, A.cstage, ku, A.y0u, A.accu, A.outu, E, e))
+                atomicMin(&kargs()->err->nonfinite, kargs()->step);
+        }
 
-// ---------------------------------------------------------------------------
-// One point row p of K d = P_h^T (G o (P_h d P_h^T)) P_h accumulated into r.
-template <class D>
-__device__ __forceinline__ void kd_row(const double* __restrict__ T, int p, const double (&d)[D::NV * D::NV],
-                                       const double (&Gp)[D::NH], double (&r)[D::NV * D::NV]) {
-    constexpr int NV = D::NV, NH = D::NH;
-    double tp[NV], zp[NV];
-#pragma unroll
-    for (int b = 0; b < NV; ++b) {
-        double s = 0.0;
-#pragma unroll
-        for (int aa = 0; aa < NV; ++aa) s = fma(T[D::oPh + p + NH * aa], d[aa + NV * b], s);
-        tp[b] = s;
-        zp[b] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < NH; ++q) {
-        double y = 0.0;
-#pragma unroll
-        for (int b = 0; b < NV; ++b) y = fma(tp[b], T[D::oPh + q + NH * b], y);
-        y *= Gp[q];
-#pragma unroll
-        for (int b = 0; b < NV; ++b) zp[b] = fma(y, T[D::oPh + q + NH * b], zp[b]);
-    }
-#pragma unroll
-    for (int b = 0; b < NV; ++b)
-#pragma unroll
-        for (int aa = 0; aa < NV; ++aa) r[aa + NV * b] = fma(T[D::oPh + p + NH * aa], zp[b], r[aa + NV * b]);
-}
-
-// The mass system M(v_h) X = R by Chebyshev iteration (src/rhs.jl:886-939)
-// and the v part of the RK4 stage update. One wave per block: the first rows
-// of the weights G = 2 beta v_q W of every lane live in LDS ([entry][lane],
-// conflict-free), the remaining rows in registers.
-template <int NU, int NV>
-__global__ __launch_bounds__(kMassBlock) void k_mass(const KArgs a, const double* __restrict__ T0,
-                                                     const double* __restrict__ beta0,
-                                                     const double* __restrict__ R) {
-    using D = Dims<NU, NV>;
-    constexpr int NH = D::NH;
-    constexpr int RL0 = kMassLdsDoubles / NH;
-    constexpr int RL = RL0 < NH ? RL0 : NH;          // rows of G in LDS
-    constexpr int RT = NH - RL;                      // rows of G in registers
-    __shared__ double Gs[RL * NH * kMassBlock];
-    if (skip_stage(a)) return;
-    const int E = a.E;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[a.slot ^ 1] = 1;   // the next stage's count
-    const int lane = threadIdx.x;
-    const int e = blockIdx.x * kMassBlock + lane;
-    int ncheb = 0;
-    if (e < E) {
-        double Gt[RT > 0 ? RT * NH : 1];
+        // ---- the mass system M(v_h) X = Rv by Chebyshev iteration
+        // (src/rhs.jl:886-939)
+        double Gt[PR > 0 ? 2 * PR * NH : 1];
+        double Gm[NH];
         double th = 1.0, de = 1.0, s1 = 1.0;
         int n_e;
         {
-            double V[NV * NV];
+            const double* yv = kargs()->yv;
+            double V[nV];
 #pragma unroll
-            for (int m = 0; m < NV * NV; ++m) V[m] = a.yv[(long)m * E + e];
-            const double* beta = beta0 + (long)a.cls_b[e] * NH * NH;
+            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
+            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
             double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
             bool nan = false;
-#pragma unroll 1
-            for (int p = 0; p < RL; ++p) {
-                const double* __restrict__ T = fresh(T0);
-                double g[NH];
-                g_row<D>(T, V, beta, p, g);
+#pragma unroll
+            for (int p = 0; p < (NH + 1) / 2; ++p) {
+                cdbl* T = ctab(kargs()->tab);
+                cdbl* bu = ctab(kargs()->beta);
+                const bool mid = (NH & 1) && p == NH / 2;
+                double yp[NH], yq[NH];
+                vq_rows<D>(T + D::oPh, p, mid, V, yp, yq);
 #pragma unroll
                 for (int q = 0; q < NH; ++q) {
-                    gmin = fmin(gmin, g[q]);
-                    gmax = fmax(gmax, g[q]);
-                    nan = nan || (g[q] != g[q]);
-                    Gs[(p * NH + q) * kMassBlock + lane] = g[q] * T[D::oWh + p + NH * q];
+                    const double gp = yp[q] * (2.0 * (UNI ? bu[p + NH * q] : betav[p + NH * q]));
+                    gmin = fmin(gmin, gp);
+                    gmax = fmax(gmax, gp);
+                    nan = nan || (gp != gp);
+                    const double Gpv = gp * T[D::oWh + p + NH * q];
+                    if (mid) {
+                        Gm[q] = Gpv;
+                    } else {
+                        const int pq = NH - 1 - p;
+                        const double gq = yq[q] * (2.0 * (UNI ? bu[pq + NH * q] : betav[pq + NH * q]));
+                        gmin = fmin(gmin, gq);
+                        gmax = fmax(gmax, gq);
+                        nan = nan || (gq != gq);
+                        const double Gqv = gq * T[D::oWh + pq + NH * q];
+                        if (p < PL) {
+                            Gs[((2 * p) * NH + q) * kWave + lane] = Gpv;
+                            Gs[((2 * p + 1) * NH + q) * kWave + lane] = Gqv;
+                        } else {
+                            Gt[(2 * (p - PL)) * NH + q] = Gpv;
+                            Gt[(2 * (p - PL) + 1) * NH + q] = Gqv;
+                        }
+                    }
                 }
             }
-#pragma unroll
-            for (int pt = 0; pt < RT; ++pt) {
-                const double* __restrict__ T = fresh(T0);
-                const int p = RL + pt;
-                double g[NH];
-                g_row<D>(T, V, beta, p, g);
-#pragma unroll
-                for (int q = 0; q < NH; ++q) {
-                    gmin = fmin(gmin, g[q]);
-                    gmax = fmax(gmax, g[q]);
-                    nan = nan || (g[q] != g[q]);
-                    Gt[pt * NH + q] = g[q] * T[D::oWh + p + NH * q];
-                }
-            }
-            n_e = cheb_interval(gmin, gmax, nan, a.cheb_L, th, de, s1);
+            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
         }
-        double X[NV * NV];
+        double X[nV];
 #pragma unroll
-        for (int m = 0; m < NV * NV; ++m) X[m] = 0.0;
+        for (int m = 0; m < nV; ++m) X[m] = 0.0;
         if (n_e == 0) {
-            if (!a.cheb_global) {
-                atomicMin(&a.err->mfail, e + 1);
-                a.err->failed = a.serial;
+            if (!kargs()->cheb_global) {
+                atomicMin(&kargs()->err->mfail, e + 1);
+                kargs()->err->failed = kargs()->serial;
             }
         } else {
-            const int n = a.cheb_global ? a.cnt[a.slot] : n_e;
+            const int n = kargs()->cheb_global ? kargs()->cnt[kargs()->slot] : n_e;
             ncheb = n;
-            double r[NV * NV], d[NV * NV];
+            double r[nV], d[nV];
 #pragma unroll
-            for (int m = 0; m < NV * NV; ++m) { r[m] = R[(long)m * E + e]; d[m] = r[m] / th; }
+            for (int m = 0; m < nV; ++m) { r[m] = Rv[m]; d[m] = r[m] / th; }
             double rho = 1.0 / s1;
             for (int step = 1; step <= n; ++step) {
 #pragma unroll
-                for (int m = 0; m < NV * NV; ++m) X[m] += d[m];
+                for (int m = 0; m < nV; ++m) X[m] += d[m];
                 if (step == n) break;
-                // r <- (r - d) + K d
+                // r <- r - (d - K d)
 #pragma unroll
-                for (int m = 0; m < NV * NV; ++m) r[m] = r[m] - d[m];
-#pragma unroll 1
-                for (int p = 0; p < RL; ++p) {
-                    double Gp[NH];
+                for (int m = 0; m < nV; ++m) r[m] = r[m] - d[m];
+                cdbl* Ph = ctab(kargs()->tab) + D::oPh;
 #pragma unroll
-                    for (int q = 0; q < NH; ++q) Gp[q] = Gs[(p * NH + q) * kMassBlock + lane];
-                    kd_row<D>(fresh(T0), p, d, Gp, r);
+                for (int p = 0; p < NP; ++p) {
+                    double Gp[NH], Gq[NH];
+#pragma unroll
+                    for (int q = 0; q < NH; ++q) {
+                        if (p < PL) {
+                            Gp[q] = Gs[((2 * p) * NH + q) * kWave + lane];
+                            Gq[q] = Gs[((2 * p + 1) * NH + q) * kWave + lane];
+                        } else {
+                            Gp[q] = Gt[(2 * (p - PL)) * NH + q];
+                            Gq[q] = Gt[(2 * (p - PL) + 1) * NH + q];
+                        }
+                    }
+                    kd_pair<D>(Ph, p, d, Gp, Gq, r);
                 }
-#pragma unroll
-                for (int pt = 0; pt < RT; ++pt) {
-                    double Gp[NH];
-#pragma unroll
-                    for (int q = 0; q < NH; ++q) Gp[q] = Gt[pt * NH + q];
-                    kd_row<D>(fresh(T0), RL + pt, d, Gp, r);
-                }
+                if (NH & 1) kd_mid<D>(Ph, d, Gm, r);
                 const double rhon = 1.0 / (2.0 * s1 - rho);
                 const double c1 = rhon * rho, c2 = 2.0 * rhon / de;
 #pragma unroll
-                for (int m = 0; m < NV * NV; ++m) d[m] = c1 * d[m] + c2 * r[m];
+                for (int m = 0; m < nV; ++m) d[m] = c1 * d[m] + c2 * r[m];
                 rho = rhon;
             }
         }
-        // RK4 stage update, v part
-        const double c = a.cstage;
-        bool fin = true;
-#pragma unroll
-        for (int m = 0; m < NV * NV; ++m) {
-            const long idx = (long)m * E + e;
-            const double kv = X[m];
-            if (a.stage == 0) {
-                a.outv[idx] = kv;
-            } else if (a.stage == 1) {
-                a.outv[idx] = a.y0v[idx] + c * kv;
-                a.accv[idx] = kv;
-            } else if (a.stage < 4) {
-                a.outv[idx] = a.y0v[idx] + c * kv;
-                a.accv[idx] = a.accv[idx] + 2.0 * kv;
-            } else {
-                const double y = a.y0v[idx] + c * (a.accv[idx] + kv);
-                a.outv[idx] = y;
-                fin = fin && isfinite(y);
-            }
+        {
+            CKArgs& A = *kargs();
+            if (!rk_update<nV>(A.stage, A.cstage, X, A.y0v, A.accv, A.outv, E, e))
+                atomicMin(&kargs()->err->nonfinite, kargs()->step);
         }
-        if (!fin) atomicMin(&a.err->nonfinite, a.step);
     }
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
     if (lane == 0 && ncheb > 0) {
-        atomicMax(&a.err->maxcheb, ncheb);
-        atomicAdd(&a.err->cheb_wave_sum, (unsigned long long)ncheb);
+        atomicMax(&kargs()->err->maxcheb, ncheb);
+        atomicAdd(&kargs()->err->cheb_wave_sum, (unsigned long long)ncheb);
     }
 }
 

@@ -4,11 +4,13 @@
 #include "wedg_kernels.h"
 
 namespace wedg {
-// which: 0 = k_traces, 1 = k_volume, 2 = k_faces, 3 = k_mass; uni: one coefficient class for all elements
-hipError_t launch_q1(const KArgs& a, int nv, int which, hipStream_t s, int uni);
-hipError_t launch_q2(const KArgs& a, int nv, int which, hipStream_t s, int uni);
-hipError_t launch_q3(const KArgs& a, int nv, int which, hipStream_t s, int uni);
-hipError_t launch_q4(const KArgs& a, int nv, int which, hipStream_t s, int uni);
-hipError_t launch_q5(const KArgs& a, int nv, int which, hipStream_t s, int uni);
-hipError_t launch_q6(const KArgs& a, int nv, int which, hipStream_t s, int uni);
+
+// which: 0 = k_traces, 1 = k_element; mode: MODE_GENERAL, MODE_SEPARABLE or MODE_UNIFORM
+hipError_t launch_q1(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_q2(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_q3(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_q4(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_q5(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_q6(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+
 }  // namespace wedg

@@ -140,8 +140,8 @@ wedg_status wedg_last_error(const wedg_workspace* ws, wedg_error_info* info);
 
 /* Kernel timing with HIP events on the workspace stream over the next run(s):
  * enable, then read the accumulated per-kernel-family milliseconds and counts.
- * family: 0 = traces, 1 = volume and source, 2 = faces, lifts, u-solve and the u update,
- * 3 = mass solve and the v update. */
+ * family: 0 = the traces kernel, 1 = the element kernel (volume terms, source, faces,
+ * lifts, u-solve, mass solve and the RK4 update); 2 and 3 are not used. */
 wedg_status wedg_timing_enable(wedg_workspace* ws, int on);
 wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* launches);
 
