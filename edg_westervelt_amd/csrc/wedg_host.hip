@@ -175,6 +175,7 @@ struct wedg_problem {
     double* beta = nullptr;
     std::vector<double> lam, Q;   // spectrum of the 1D stiffness
     FaceArr f[F_COUNT];
+    double fu[F_COUNT] = {};   // values of the face arrays that are one constant
     double* bcd[8] = {};
     double* proj[5] = {};   // PiSP, PiBSP2, PiCdiv, PiDdiv, src
     int has_proj[5] = {};
@@ -211,6 +212,7 @@ struct wedg_workspace {
     double *yu = nullptr, *yv = nullptr, *su = nullptr, *sv = nullptr;
     double *au = nullptr, *av = nullptr, *ou = nullptr, *ov = nullptr;
     double* tr = nullptr;
+    double* bsl = nullptr;   // boundary slots [side][i][A, D, F][N]
     DevErr* err = nullptr;
     DevErr* herr = nullptr;  // pinned host copy of the failure record
     double *hu = nullptr, *hv = nullptr;   // pinned staging of the state (host <-> device)
@@ -288,6 +290,12 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         std::vector<double> Q, lam;
         stiffness_spectrum(pre->S_u, pre->w, nu, Q, lam);
         t.insert(t.end(), Q.begin(), Q.end());
+        // lift rows scaled as in the reference's products (2/h) E[a], (4/h^2) E1[a] (src/rhs.jl:449-463)
+        const double h = 1.0 / N, two_h = 2.0 / h, four_hh = 4.0 / (h * h);
+        for (int a = 0; a < nu; ++a) t.push_back(two_h * pre->Ebl_u[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(two_h * pre->Ebr_u[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(four_hh * pre->E1bl_u[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(four_hh * pre->E1br_u[a]);
         UP(p->tab, t.data(), t.size());
         p->lam = lam;
         p->Q = Q;
@@ -395,6 +403,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
                 double v = s[0];
                 UP(p->f[a].p, &v, 1);
                 p->f[a].uni = 1;
+                p->fu[a] = v;
             } else {
                 std::vector<double> t(n);
                 for (size_t face = 0; face < nf; ++face)
@@ -495,6 +504,7 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     ws->yu = A(nu); ws->yv = A(nv); ws->su = A(nu); ws->sv = A(nv);
     ws->au = A(nu); ws->av = A(nv); ws->ou = A(nu); ws->ov = A(nv);
     ws->tr = A((size_t)12 * p->nu * p->E);
+    ws->bsl = A((size_t)4 * p->nu * 3 * p->N);
     if (he == hipSuccess) {
         void* e = nullptr;
         he = hipMalloc(&e, sizeof(DevErr) + 16);
@@ -558,10 +568,10 @@ static KArgs base_args(wedg_workspace* ws) {
     a.tab = p->tab;
     a.cls_ca = p->cls_ca; a.Cw = p->Cw; a.Aw = p->Aw; a.Ainv = p->Ainv;
     a.cls_b = p->cls_b; a.beta = p->beta;
-    for (int i = 0; i < F_COUNT; ++i) a.f[i] = p->f[i];
+    for (int i = 0; i < F_COUNT; ++i) { a.f[i] = p->f[i]; a.fu[i] = p->fu[i]; }
     a.fstride = p->N * (p->N + 1);
     for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
-    a.tr = ws->tr; a.err = ws->err; a.cnt = ws->cnt;
+    a.tr = ws->tr; a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt;
     return a;
 }
 

@@ -56,6 +56,9 @@ struct Dims {
     static constexpr int PL0 = kMassLdsDoubles / (2 * NH);
     static constexpr int PL = PL0 < NP ? PL0 : NP;                    // pairs of G rows in LDS
     static constexpr int PR = NP - PL;                                // pairs of G rows in registers
+    static constexpr int NYL = 4 * (2 * NV_ + NU_);                   // lift vectors of the four sides
+    static constexpr int NLDS0 = 2 * PL > 0 ? 2 * PL * NH : 1;
+    static constexpr int NLDS = NLDS0 > NYL ? NLDS0 : NYL;            // LDS doubles per lane
     // offsets in the table buffer
     static constexpr int oP = 0;                    // P_u  (NQ x NU), column-major
     static constexpr int oS = oP + NQ * NU;         // S_u  (NQ x NU)
@@ -67,7 +70,11 @@ struct Dims {
     static constexpr int oPh = oE1r + NU;           // P_h_v (NH x NV)
     static constexpr int oWh = oPh + NH * NV;       // W2d_h (NH x NH)
     static constexpr int oQ = oWh + NH * NH;        // eigenvectors of the 1D stiffness (NU x NU)
-    static constexpr int nTab = oQ + NU * NU;
+    static constexpr int oTEl = oQ + NU * NU;       // (2/h) Ebl, (2/h) Ebr, (4/h^2) E1bl, (4/h^2) E1br (NU each)
+    static constexpr int oTEr = oTEl + NU;
+    static constexpr int oFE1l = oTEr + NU;
+    static constexpr int oFE1r = oFE1l + NU;
+    static constexpr int nTab = oFE1r + NU;
 };
 
 struct FaceArr {
@@ -108,6 +115,7 @@ struct KArgs {
     const int* cls_b;               // class of beta_hq per element
     const double* beta;             // per class (NH*NH), p + NH q
     FaceArr f[F_COUNT];
+    double fu[F_COUNT];             // the value of each face array that is one constant
     int fstride;                    // N (N + 1): stride of the point index of face arrays
     int pad1;
     const double* bcd[8];           // boundary data (NQ, N): sp, spn for bot, top, lft, rgt
@@ -117,6 +125,7 @@ struct KArgs {
     double* accu; double* accv;             // RK4 accumulator
     double* outu; double* outv;             // Y_{s+1}, y (last stage) or (u_t, v_t)
     double* tr;                             // traces [arr][i][E]
+    double* bsl;                            // boundary slots A, D, F [side][i][3][N]
     DevErr* err;
     int* cnt;                               // Chebyshev counts, two slots
 };
@@ -141,12 +150,6 @@ __device__ __forceinline__ cdbl* ctab(const double* p) {
     return q;
 }
 
-// value of a face array at flat index fi = i * (N (N + 1)) + face
-template <bool UNI, class FA>
-__device__ __forceinline__ double fv(const FA& a, long fi) {
-    if (UNI) return ((cdbl*)a.p)[0];
-    return a.uni ? ((cdbl*)a.p)[0] : a.p[fi];
-}
 
 // The kernel's own arguments read through the kernarg segment pointer in the
 // constant address space, opaque to the optimizer like ctab(): fields are
@@ -164,6 +167,32 @@ typedef const KArgs CKArgs;
 extern const KArgs* wedg_emu_kargs;   // set by a host build that runs the kernels on the CPU
 __device__ __forceinline__ CKArgs* kargs() { return wedg_emu_kargs; }
 #endif
+
+// A per-lane pointer, opaque to the optimizer: loads through it are not
+// hoisted above this point (and kept in registers until their use).
+__device__ __forceinline__ const double* vfresh(const double* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(p));
+#endif
+    return p;
+}
+
+// value of face array x at flat index fi = i * (N (N + 1)) + face
+template <bool UNI, class KA>
+__device__ __forceinline__ double fval(const KA& a, int x, long fi) {
+    if (UNI) return a.fu[x];
+    return a.f[x].uni ? a.fu[x] : a.f[x].p[fi];
+}
+
+// A phase boundary inside a kernel: neither the optimizer nor the scheduler
+// moves loads or arithmetic of a later phase into an earlier one, which
+// would hold the later phase's operands in registers through the earlier one.
+__device__ __forceinline__ void phase_barrier() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
 
 template <class KA>
 __device__ __forceinline__ bool skip_stage(const KA& a) {
@@ -311,75 +340,6 @@ __device__ __forceinline__ void vq_rows(cdbl* Ph, int p, bool mid, const double 
 }
 
 // ---------------------------------------------------------------------------
-// Traces of every element, and with the global policy the largest Chebyshev
-// step count (src/rhs.jl:889-918) reduced over the grid.
-template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_traces(const KArgs args) {
-    (void)args;
-    using D = Dims<NU, NV>;
-    constexpr int NH = D::NH;
-    constexpr bool UNI = MODE == MODE_UNIFORM;
-    if (kargs()->err->failed != 0) return;
-    const int e = block_first_element() + threadIdx.x;
-    const int E = kargs()->E;
-    const bool global = kargs()->cheb_global != 0;
-    int n_e = 1;
-    if (e < E) {
-        {
-            const double* yu = kargs()->yu;
-            double U[NU * NU];
-#pragma unroll
-            for (int m = 0; m < NU * NU; ++m) U[m] = yu[(long)m * E + e];
-            trace4<D, NU, true>(ctab(kargs()->tab), D::oE1r, U, kargs()->tr, 0, e, E);
-        }
-        const double* yv = kargs()->yv;
-        double V[NV * NV];
-#pragma unroll
-        for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
-        trace4<D, NV, false>(ctab(kargs()->tab), D::oEr, V, kargs()->tr, 4, e, E);
-        trace4<D, NV, true>(ctab(kargs()->tab), D::oE1r, V, kargs()->tr, 8, e, E);
-        if (global) {
-            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
-            double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
-            bool nan = false;
-#pragma unroll 1
-            for (int p = 0; p < (NH + 1) / 2; ++p) {
-                cdbl* Ph = ctab(kargs()->tab) + D::oPh;
-                cdbl* bu = ctab(kargs()->beta);
-                const bool mid = (NH & 1) && p == NH / 2;
-                double yp[NH], yq[NH];
-                vq_rows<D>(Ph, p, mid, V, yp, yq);
-#pragma unroll
-                for (int q = 0; q < NH; ++q) {
-                    const double bp = UNI ? bu[p + NH * q] : betav[p + NH * q];
-                    const double g = yp[q] * (2.0 * bp);
-                    gmin = fmin(gmin, g);
-                    gmax = fmax(gmax, g);
-                    nan = nan || (g != g);
-                    if (!mid) {
-                        const double bq = UNI ? bu[NH - 1 - p + NH * q] : betav[NH - 1 - p + NH * q];
-                        const double g2 = yq[q] * (2.0 * bq);
-                        gmin = fmin(gmin, g2);
-                        gmax = fmax(gmax, g2);
-                        nan = nan || (g2 != g2);
-                    }
-                }
-            }
-            double th, de, s1;
-            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
-            if (n_e == 0) {
-                atomicMin(&kargs()->err->mfail, e + 1);
-                n_e = 1;
-            }
-        }
-    }
-    if (global) {
-        for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(&kargs()->cnt[kargs()->slot], n_e);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Slot values (A, D, F) of one face point of an interior face, for the element
 // on the "+" side (plus) or the "-" side (src/rhs.jl:620-700). vm, vp, ... are
 // the traces of the "-" and "+" elements at the point; fi indexes the face
@@ -389,12 +349,11 @@ __device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, 
                                                double dum, double dup, double dvm, double dvp, double& A,
                                                double& Dd, double& Ff) {
     const double th = a.theta, tau = a.tau, lu = a.lu, lv = a.lv, two_h = a.two_h;
-    const auto& fc2 = a.f[hz ? F_HC2 : F_VC2];
-    const auto& faf = a.f[hz ? F_HAF : F_VAF];
+    const int fc2 = hz ? F_HC2 : F_VC2, faf = hz ? F_HAF : F_VAF;
     if (!UNI && a.one_sided) {
-        const double c2m = fv<UNI>(a.f[hz ? F_HC2M : F_VC2M], fi), c2p = fv<UNI>(fc2, fi);
-        const double afm = fv<UNI>(a.f[hz ? F_HAFM : F_VAFM], fi), afp = fv<UNI>(faf, fi);
-        const double K = fv<UNI>(a.f[hz ? F_HJC : F_VJC], fi);
+        const double c2m = fval<UNI>(a, hz ? F_HC2M : F_VC2M, fi), c2p = fval<UNI>(a, fc2, fi);
+        const double afm = fval<UNI>(a, hz ? F_HAFM : F_VAFM, fi), afp = fval<UNI>(a, faf, fi);
+        const double K = fval<UNI>(a, hz ? F_HJC : F_VJC, fi);
         const double J = two_h * (c2m * dum - c2p * dup + afm * dvm - afp * dvp) + K;
         const double G = two_h * ((1 - th) * (c2m * dum + afm * dvm) + th * (c2p * dup + afp * dvp));
         const double pen = (0.5 * (c2m + c2p)) * lu + (0.5 * (afm + afp)) * lv;
@@ -411,7 +370,7 @@ __device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, 
             Ff = -c2m * dvsm;
         }
     } else {
-        const double c2 = fv<UNI>(fc2, fi), af = fv<UNI>(faf, fi);
+        const double c2 = fval<UNI>(a, fc2, fi), af = fval<UNI>(a, faf, fi);
         const double jmp = vm - vp;
         const double gst = two_h * (c2 * (th * dup + (1 - th) * dum) + af * (th * dvp + (1 - th) * dvm));
         const double tj = tau * (c2 * (two_h * (dum - dup)) + af * (two_h * (dvm - dvp)));
@@ -434,10 +393,10 @@ __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i
                                               double vt, double dv, double& A, double& Dd, double& Ff) {
     const bool hz = s < 2;
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
-    const double c2f = fv<UNI>(a.f[hz ? F_HC2 : F_VC2], fi);
-    const double af = fv<UNI>(a.f[hz ? F_HAF : F_VAF], fi);
-    const double cf = fv<UNI>(a.f[hz ? F_HCF : F_VCF], fi);
-    const double bf = fv<UNI>(a.f[hz ? F_HBF : F_VBF], fi);
+    const double c2f = fval<UNI>(a, hz ? F_HC2 : F_VC2, fi);
+    const double af = fval<UNI>(a, hz ? F_HAF : F_VAF, fi);
+    const double cf = fval<UNI>(a, hz ? F_HCF : F_VCF, fi);
+    const double bf = fval<UNI>(a, hz ? F_HBF : F_VBF, fi);
     const double two_h = a.two_h, om = a.omega, ct = a.ct, stt = a.st;
     const double gun = sgn * (two_h * du);
     const double gvn = sgn * (two_h * dv);
@@ -472,96 +431,334 @@ __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i
     return 0;
 }
 
+// Slots (A, D, F) of the element's sides on the physical boundary
+// (src/rhs.jl:738-765) from its own traces, into the boundary slot buffer, and
+// the smallest failing _boundary_key of the element (src/rhs.jl:732).
+template <class D, bool UNI, int s>
+__device__ __forceinline__ int boundary_side(int e, int E) {
+    constexpr int NQ = D::NQ;
+    CKArgs& B = *kargs();
+    const int N = B.N, j = e % N, k = e / N;
+    const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
+    int bkey = 0;
+    if (bnd) {
+        const int m = s < 2 ? k : j;
+        const int fidx = s == 0 ? (N + 1) * k : s == 1 ? N + (N + 1) * k : s == 2 ? j : j + N * N;
+        const double* tr = B.tr;
+        double* bs = B.bsl + (long)s * NQ * 3 * N + m;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const double du = tr[(long)((0 + s) * NQ + i) * E + e];
+            const double vt = tr[(long)((4 + s) * NQ + i) * E + e];
+            const double dv = tr[(long)((8 + s) * NQ + i) * E + e];
+            const long fi = (long)i * B.fstride + fidx;
+            double Av, Dd, Ff;
+            const int k2 = boundary_point<UNI>(B, s, fi, i, m, NQ, du, vt, dv, Av, Dd, Ff);
+            if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
+            bs[(long)(i * 3 + 0) * N] = Av;
+            bs[(long)(i * 3 + 1) * N] = Dd;
+            bs[(long)(i * 3 + 2) * N] = Ff;
+        }
+    }
+    return bkey;
+}
+
+template <class D, bool UNI>
+__device__ __forceinline__ void boundary_slots(int e, int E) {
+    int bkey = 0;
+    auto key = [&](int k2) { if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2); };
+    key(boundary_side<D, UNI, 0>(e, E));
+    key(boundary_side<D, UNI, 1>(e, E));
+    key(boundary_side<D, UNI, 2>(e, E));
+    key(boundary_side<D, UNI, 3>(e, E));
+    if (bkey) {
+        atomicMin(&kargs()->err->bkey, bkey);
+        kargs()->err->failed = kargs()->serial;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Traces of every element, and with the global policy the largest Chebyshev
+// step count (src/rhs.jl:889-918) reduced over the grid.
+template <int NU, int NV, int MODE>
+__global__ __launch_bounds__(kWave) void k_traces(const KArgs args) {
+    (void)args;
+    using D = Dims<NU, NV>;
+    constexpr int NH = D::NH;
+    constexpr bool UNI = MODE == MODE_UNIFORM;
+    if (skip_stage(*kargs())) return;   // a failure of an earlier stage ends the run
+    const int e = block_first_element() + threadIdx.x;
+    const int E = kargs()->E;
+    const bool global = kargs()->cheb_global != 0;
+    int n_e = 1;
+    if (e < E) {
+        {
+            const double* yu = kargs()->yu;
+            double U[NU * NU];
+#pragma unroll
+            for (int m = 0; m < NU * NU; ++m) U[m] = yu[(long)m * E + e];
+            trace4<D, NU, true>(ctab(kargs()->tab), D::oE1r, U, kargs()->tr, 0, e, E);
+        }
+        const double* yv = kargs()->yv;
+        double V[NV * NV];
+#pragma unroll
+        for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
+        trace4<D, NV, false>(ctab(kargs()->tab), D::oEr, V, kargs()->tr, 4, e, E);
+        trace4<D, NV, true>(ctab(kargs()->tab), D::oE1r, V, kargs()->tr, 8, e, E);
+        boundary_slots<D, UNI>(e, E);
+        if (global) {
+            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
+            double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
+            bool nan = false;
+            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
+            cdbl* bu = ctab(kargs()->beta);
+#pragma unroll
+            for (int p = 0; p < (NH + 1) / 2; ++p) {
+                const bool mid = (NH & 1) && p == NH / 2;
+                double yp[NH], yq[NH];
+                vq_rows<D>(Ph, p, mid, V, yp, yq);
+#pragma unroll
+                for (int q = 0; q < NH; ++q) {
+                    const double bp = UNI ? bu[p + NH * q] : betav[p + NH * q];
+                    const double g = yp[q] * (2.0 * bp);
+                    gmin = fmin(gmin, g);
+                    gmax = fmax(gmax, g);
+                    nan = nan || (g != g);
+                    if (!mid) {
+                        const double bq = UNI ? bu[NH - 1 - p + NH * q] : betav[NH - 1 - p + NH * q];
+                        const double g2 = yq[q] * (2.0 * bq);
+                        gmin = fmin(gmin, g2);
+                        gmax = fmax(gmax, g2);
+                        nan = nan || (g2 != g2);
+                    }
+                }
+            }
+            double th, de, s1;
+            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
+            if (n_e == 0) {
+                atomicMin(&kargs()->err->mfail, e + 1);
+                n_e = 1;
+            }
+        }
+    }
+    if (global) {
+        for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(&kargs()->cnt[kargs()->slot], n_e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+
 // ---------------------------------------------------------------------------
 // Volume terms (src/rhs.jl:317-333):
 //   out (NO x NO) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S
-// for the modal X (n x n) and point weights W(p, q). The point values of the
-// gradient are formed one mirrored pair of point rows at a time and contracted
-// back at once, so only the modal arrays stay live.
-template <class D, int n, int NO, class WF>
-__device__ __forceinline__ void vol_row(cdbl* P, cdbl* S, const double (&t1)[n], const double (&t2)[n], WF W,
-                                        int r, double (&z1)[NO], double (&z2)[NO]) {
+// for the modal X (n x n) and point weights W (NQ x NQ, p + NQ q): the class
+// record, one for all elements (UNI, read as scalars) or the lane's own. The
+// point values of the gradient are formed one mirrored pair of point rows at a
+// time and contracted back at once, so only the modal arrays stay live.
+// One term of a mirrored pair of point rows (p, q' = NQ-1-p):
+//   x-term (DX): out += S^T (W o (S X P^T)) P over the rows p, q'
+//   y-term:      out += P^T (W o (P X S^T)) S
+// The x-term reads all of P and row p of S, the y-term all of S and row p of
+// P, so each holds half of the 1D tables in SGPRs.
+template <class D, int n, int NO, bool DX>
+__device__ __forceinline__ void vol_term(const double* tab, int p, const double (&X)[n * n], const double (&wp)[D::NQ],
+                                         const double (&wq)[D::NQ], double (&out)[NO * NO]) {
     constexpr int NQ = D::NQ;
-    double g1[NQ], g2[NQ];
-    rows_sym<NQ, n, false>(P, [&](int b) { return t1[b]; }, g1);   // (S X P^T)[r, q]
-    rows_sym<NQ, n, true>(S, [&](int b) { return t2[b]; }, g2);    // (P X S^T)[r, q]
+    cdbl* T = ctab(tab);
+    cdbl* A = T + (DX ? D::oS : D::oP);   // contracted with X's first index, row p
+    cdbl* B = T + (DX ? D::oP : D::oS);   // contracted with X's second index, all rows
+    double tp[n], tq[n];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const double w = W(r, q);
-        g1[q] = w * g1[q];
-        g2[q] = w * g2[q];
+    for (int b = 0; b < n; ++b) row_pair<NQ, n, DX>(A, p, [&](int aa) { return X[aa + n * b]; }, tp[b], tq[b]);
+    double zp[NO], zq[NO];
+    {
+        double g[NQ];
+        rows_sym<NQ, n, !DX>(B, [&](int b) { return tp[b]; }, g);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) g[q] = wp[q] * g[q];
+        cols_sym<NQ, NO, !DX>(B, g, zp);
     }
-    cols_sym<NQ, NO, false>(P, g1, z1);
-    cols_sym<NQ, NO, true>(S, g2, z2);
+    {
+        double g[NQ];
+        rows_sym<NQ, n, !DX>(B, [&](int b) { return tq[b]; }, g);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) g[q] = wq[q] * g[q];
+        cols_sym<NQ, NO, !DX>(B, g, zq);
+    }
+    // A[q', a] = (-1)^a A[p, a] (values), -(-1)^a A[p, a] (derivatives)
+#pragma unroll
+    for (int b = 0; b < NO; ++b) {
+        const double zs = zp[b] + zq[b], zd = zp[b] - zq[b];
+#pragma unroll
+        for (int aa = 0; aa < NO; ++aa) {
+            const bool same = ((aa & 1) == 0) != DX;
+            out[aa + NO * b] = fma(A[p + NQ * aa], same ? zs : zd, out[aa + NO * b]);
+        }
+    }
 }
 
-template <class D, int n, int NO, class WF>
-__device__ __forceinline__ void vol_pass(const double* tab, const double (&X)[n * n], WF W, double (&out)[NO * NO]) {
+// the middle point row (NQ odd)
+template <class D, int n, int NO, bool DX>
+__device__ __forceinline__ void vol_term_mid(const double* tab, const double (&X)[n * n], const double (&wm)[D::NQ],
+                                             double (&out)[NO * NO]) {
+    constexpr int NQ = D::NQ, m = NQ / 2;
+    cdbl* T = ctab(tab);
+    cdbl* A = T + (DX ? D::oS : D::oP);
+    cdbl* B = T + (DX ? D::oP : D::oS);
+    double t[n], z[NO], g[NQ];
+#pragma unroll
+    for (int b = 0; b < n; ++b) t[b] = row_mid<NQ, n, DX>(A, [&](int aa) { return X[aa + n * b]; });
+    rows_sym<NQ, n, !DX>(B, [&](int b) { return t[b]; }, g);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) g[q] = wm[q] * g[q];
+    cols_sym<NQ, NO, !DX>(B, g, z);
+#pragma unroll
+    for (int b = 0; b < NO; ++b)
+#pragma unroll
+        for (int aa = (DX ? 1 : 0); aa < NO; aa += 2) out[aa + NO * b] = fma(A[m + NQ * aa], z[b], out[aa + NO * b]);
+}
+
+// Volume terms (src/rhs.jl:317-333):
+//   out (NO x NO) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S
+// for the modal X (n x n) and the point weights W (NQ x NQ, p + NQ q) of the
+// element's class (Wsrc, read as vector loads). The point values of the
+// gradient are formed one mirrored pair of point rows at a time and contracted
+// back at once, so only the modal arrays stay live.
+template <class D, int n, int NO>
+__device__ __forceinline__ void vol_pass(const double* tab, const double (&X)[n * n], const double* Wsrc,
+                                         double (&out)[NO * NO]) {
     constexpr int NQ = D::NQ;
 #pragma unroll 1
     for (int p = 0; p < NQ / 2; ++p) {
-        cdbl* T = ctab(tab);
-        cdbl* P = T + D::oP;
-        cdbl* S = T + D::oS;
-        double t1p[n], t1q[n], t2p[n], t2q[n];
+        double wp[NQ], wq[NQ];
 #pragma unroll
-        for (int b = 0; b < n; ++b) {
-            row_pair<NQ, n, true>(S, p, [&](int aa) { return X[aa + n * b]; }, t1p[b], t1q[b]);
-            row_pair<NQ, n, false>(P, p, [&](int aa) { return X[aa + n * b]; }, t2p[b], t2q[b]);
-        }
-        double z1p[NO], z2p[NO], z1q[NO], z2q[NO];
-        vol_row<D, n, NO>(P, S, t1p, t2p, W, p, z1p, z2p);
-        vol_row<D, n, NO>(P, S, t1q, t2q, W, NQ - 1 - p, z1q, z2q);
-        // S[q', a] = -(-1)^a S[p, a], P[q', a] = (-1)^a P[p, a]
-#pragma unroll
-        for (int b = 0; b < NO; ++b) {
-            const double s1e = z1p[b] - z1q[b], s1o = z1p[b] + z1q[b];
-            const double s2e = z2p[b] + z2q[b], s2o = z2p[b] - z2q[b];
-#pragma unroll
-            for (int aa = 0; aa < NO; ++aa)
-                out[aa + NO * b] = fma(S[p + NQ * aa], (aa & 1) ? s1o : s1e,
-                                       fma(P[p + NQ * aa], (aa & 1) ? s2o : s2e, out[aa + NO * b]));
-        }
+        for (int q = 0; q < NQ; ++q) { wp[q] = Wsrc[p + NQ * q]; wq[q] = Wsrc[NQ - 1 - p + NQ * q]; }
+        vol_term<D, n, NO, true>(tab, p, X, wp, wq, out);
+        __builtin_amdgcn_sched_barrier(0);
+        vol_term<D, n, NO, false>(tab, p, X, wp, wq, out);
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (NQ & 1) {
-        constexpr int m = NQ / 2;
-        cdbl* T = ctab(tab);
-        cdbl* P = T + D::oP;
-        cdbl* S = T + D::oS;
-        double t1[n], t2[n];
+        double wm[NQ];
 #pragma unroll
-        for (int b = 0; b < n; ++b) {
-            t1[b] = row_mid<NQ, n, true>(S, [&](int aa) { return X[aa + n * b]; });
-            t2[b] = row_mid<NQ, n, false>(P, [&](int aa) { return X[aa + n * b]; });
-        }
-        double z1[NO], z2[NO];
-        vol_row<D, n, NO>(P, S, t1, t2, W, m, z1, z2);
-#pragma unroll
-        for (int b = 0; b < NO; ++b)
-#pragma unroll
-            for (int aa = 0; aa < NO; ++aa)
-                out[aa + NO * b] = (aa & 1) ? fma(S[m + NQ * aa], z1[b], out[aa + NO * b])
-                                            : fma(P[m + NQ * aa], z2[b], out[aa + NO * b]);
+        for (int q = 0; q < NQ; ++q) wm[q] = Wsrc[NQ / 2 + NQ * q];
+        vol_term_mid<D, n, NO, true>(tab, X, wm, out);
+        vol_term_mid<D, n, NO, false>(tab, X, wm, out);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Geometry of side s of element e = j + N k.
-struct SideGeom {
+// Face slots (A, D, F) of side s of element e = j + N k (B, T, L, R) from its
+// own traces and its neighbour's facing ones (interior fluxes
+// src/rhs.jl:620-700; on a physical boundary the slots k_traces formed),
+// reduced to the side's lift vectors Y_A, Y_D, Y_F (src/rhs.jl:434-441).
+template <class D, bool UNI, int s>
+__device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, double* Ys, int lane) {
+    constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU;
+    constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
+    constexpr int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
+    CKArgs& A = *kargs();
     int fidx, nb, m;
-    bool bnd, hz, plus;
-};
+    bool bnd;
+    if (s == 0)      { fidx = j + (N + 1) * k;     nb = e - 1; bnd = j == 0;     m = k; }
+    else if (s == 1) { fidx = j + 1 + (N + 1) * k; nb = e + 1; bnd = j == N - 1; m = k; }
+    else if (s == 2) { fidx = j + N * k;           nb = e - N; bnd = k == 0;     m = j; }
+    else             { fidx = j + N * (k + 1);     nb = e + N; bnd = k == N - 1; m = j; }
+    if (bnd) nb = e;    // not read on the boundary
+    const double* tr = A.tr;
+    double odu[NQ], ov[NQ], odv[NQ], ndu[NQ], nvv[NQ], ndv[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        odu[i] = tr[(long)((0 + s) * NQ + i) * E + e];
+        ov[i] = tr[(long)((4 + s) * NQ + i) * E + e];
+        odv[i] = tr[(long)((8 + s) * NQ + i) * E + e];
+        ndu[i] = tr[(long)((0 + ns) * NQ + i) * E + nb];
+        nvv[i] = tr[(long)((4 + ns) * NQ + i) * E + nb];
+        ndv[i] = tr[(long)((8 + ns) * NQ + i) * E + nb];
+    }
+    double wA[NQ], wD[NQ], wF[NQ];
+    if (bnd) {   // the boundary slots, formed by k_traces
+        CKArgs& B = *kargs();
+        cdbl* Tw = ctab(B.tab) + D::oW;
+        const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            wA[i] = Tw[i] * bs[(long)(i * 3 + 0) * N];
+            wD[i] = Tw[i] * bs[(long)(i * 3 + 1) * N];
+            wF[i] = Tw[i] * bs[(long)(i * 3 + 2) * N];
+        }
+    } else {
+        CKArgs& I = *kargs();
+        cdbl* Tw = ctab(I.tab) + D::oW;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const long fi = (long)i * I.fstride + fidx;
+            double Av, Dd, Ff;
+            if (plus) interior_point<UNI>(I, true, hz, fi, nvv[i], ov[i], ndu[i], odu[i], ndv[i], odv[i], Av, Dd, Ff);
+            else interior_point<UNI>(I, false, hz, fi, ov[i], nvv[i], odu[i], ndu[i], odv[i], ndv[i], Av, Dd, Ff);
+            wA[i] = Tw[i] * Av;
+            wD[i] = Tw[i] * Dd;
+            wF[i] = Tw[i] * Ff;
+        }
+    }
+    cdbl* T = ctab(A.tab);
+    // Y_Z[b] = sum_i P[i, b] w[i] Z[i] (src/rhs.jl:434-441), kept in the lane's
+    // LDS slots [s][A, D, F][b] until the lifts are added
+    double YA[NV], YD[NV], YF[NU];
+    cols_sym<NQ, NV, false>(T + D::oP, wA, YA);
+    cols_sym<NQ, NV, false>(T + D::oP, wD, YD);
+    cols_sym<NQ, NU, false>(T + D::oP, wF, YF);
+    constexpr int y0 = s * (2 * NV + NU);
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        Ys[(y0 + b) * kWave + lane] = YA[b];
+        Ys[(y0 + NV + b) * kWave + lane] = YD[b];
+    }
+#pragma unroll
+    for (int b = 0; b < NU; ++b) Ys[(y0 + 2 * NV + b) * kWave + lane] = YF[b];
+}
 
-__device__ __forceinline__ SideGeom side_geom(int s, int e, int j, int k, int N) {
-    SideGeom g;
-    g.hz = s < 2;
-    g.plus = (s == 0 || s == 2);
-    if (s == 0)      { g.fidx = j + (N + 1) * k;     g.nb = e - 1; g.bnd = j == 0;     g.m = k; }
-    else if (s == 1) { g.fidx = j + 1 + (N + 1) * k; g.nb = e + 1; g.bnd = j == N - 1; g.m = k; }
-    else if (s == 2) { g.fidx = j + N * k;           g.nb = e - N; g.bnd = k == 0;     g.m = j; }
-    else             { g.fidx = j + N * (k + 1);     g.nb = e + N; g.bnd = k == N - 1; g.m = j; }
-    if (g.bnd) g.nb = e;    // not read on the boundary
-    return g;
+// The lifts of side s (src/rhs.jl:449-473): Rv += (2/h) E (x) Y_A + (4/h^2) E1 (x) Y_D
+// and b -= E1 (x) Y_F on a horizontal side, with the factors transposed on a
+// vertical side; Y from the lane's LDS slots.
+template <class D, int s>
+__device__ __forceinline__ void apply_lifts(const double* Ys, int lane, double (&Rv)[D::NV * D::NV],
+                                            double (&bv)[D::NU * D::NU]) {
+    constexpr int NV = D::NV, NU = D::NU;
+    constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
+    constexpr int y0 = s * (2 * NV + NU);
+    double YA[NV], YD[NV], YF[NU];
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        YA[b] = Ys[(y0 + b) * kWave + lane];
+        YD[b] = Ys[(y0 + NV + b) * kWave + lane];
+    }
+#pragma unroll
+    for (int b = 0; b < NU; ++b) YF[b] = Ys[(y0 + 2 * NV + b) * kWave + lane];
+    cdbl* T = ctab(kargs()->tab);
+    constexpr int oE = plus ? D::oTEl : D::oTEr;      // (2/h) E
+    constexpr int oE1 = plus ? D::oFE1l : D::oFE1r;   // (4/h^2) E1
+    constexpr int oF1 = plus ? D::oE1l : D::oE1r;     // E1
+#pragma unroll
+    for (int b = 0; b < NV; ++b)
+#pragma unroll
+        for (int aa = 0; aa < NV; ++aa) {
+            double r = Rv[aa + NV * b];
+            if (hz) {
+                r = fma(T[oE + aa], YA[b], r);
+                r = fma(T[oE1 + aa], YD[b], r);
+            } else {
+                r = fma(YA[aa], T[oE + b], r);
+                r = fma(YD[aa], T[oE1 + b], r);
+            }
+            Rv[aa + NV * b] = r;
+        }
+#pragma unroll
+    for (int b = 0; b < NU; ++b)
+#pragma unroll
+        for (int aa = 0; aa < NU; ++aa)
+            bv[aa + NU * b] = hz ? fma(-T[oF1 + aa], YF[b], bv[aa + NU * b]) : fma(-YF[aa], T[oF1 + b], bv[aa + NU * b]);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,53 +820,46 @@ __device__ __forceinline__ void kd_mid(cdbl* Ph, const double (&d)[D::NV * D::NV
 // RK4 stage update of one field (src/solver.jl:300-321) from the stage's
 // derivative k of the element: stage 0 writes k itself (a plain right-hand
 // side), stage 1 starts the accumulator, stages 2-3 add 2 k, stage 4 forms the
-// new y. Loads are issued in chunks before the arithmetic that needs them.
+// new y. The inputs y and acc are loaded ahead of time (rk_load), so their
+// transfer overlaps the computation of k.
 template <int n>
-__device__ __forceinline__ bool rk_update(int stage, double c, const double (&kv)[n], const double* y0, double* acc,
-                                          double* out, int E, int e) {
+__device__ __forceinline__ void rk_load(int stage, const double* y0, const double* acc, int E, int e,
+                                        double (&y)[n], double (&ac)[n]) {
+    if (stage >= 1) {
+#pragma unroll
+        for (int m = 0; m < n; ++m) y[m] = y0[(long)m * E + e];
+    }
+    if (stage >= 2) {
+#pragma unroll
+        for (int m = 0; m < n; ++m) ac[m] = acc[(long)m * E + e];
+    }
+}
+
+template <int n>
+__device__ __forceinline__ bool rk_store(int stage, double c, const double (&kv)[n], const double (&y)[n],
+                                         const double (&ac)[n], double* acc, double* out, int E, int e) {
     bool fin = true;
     if (stage == 0) {
 #pragma unroll
         for (int m = 0; m < n; ++m) out[(long)m * E + e] = kv[m];
     } else if (stage == 1) {
-        constexpr int C = 12;
 #pragma unroll
-        for (int m0 = 0; m0 < n; m0 += C) {
-            double y[C];
+        for (int m = 0; m < n; ++m) {
+            out[(long)m * E + e] = y[m] + c * kv[m];
+            acc[(long)m * E + e] = kv[m];
+        }
+    } else if (stage < 4) {
 #pragma unroll
-            for (int m = 0; m < C; ++m)
-                if (m0 + m < n) y[m] = y0[(long)(m0 + m) * E + e];
-#pragma unroll
-            for (int m = 0; m < C; ++m)
-                if (m0 + m < n) {
-                    out[(long)(m0 + m) * E + e] = y[m] + c * kv[m0 + m];
-                    acc[(long)(m0 + m) * E + e] = kv[m0 + m];
-                }
+        for (int m = 0; m < n; ++m) {
+            out[(long)m * E + e] = y[m] + c * kv[m];
+            acc[(long)m * E + e] = ac[m] + 2.0 * kv[m];
         }
     } else {
-        constexpr int C = 9;
 #pragma unroll
-        for (int m0 = 0; m0 < n; m0 += C) {
-            double y[C], ac[C];
-#pragma unroll
-            for (int m = 0; m < C; ++m)
-                if (m0 + m < n) {
-                    y[m] = y0[(long)(m0 + m) * E + e];
-                    ac[m] = acc[(long)(m0 + m) * E + e];
-                }
-#pragma unroll
-            for (int m = 0; m < C; ++m)
-                if (m0 + m < n) {
-                    const double k = kv[m0 + m];
-                    if (stage < 4) {
-                        out[(long)(m0 + m) * E + e] = y[m] + c * k;
-                        acc[(long)(m0 + m) * E + e] = ac[m] + 2.0 * k;
-                    } else {
-                        const double yn = y[m] + c * (ac[m] + k);
-                        out[(long)(m0 + m) * E + e] = yn;
-                        fin = fin && isfinite(yn);
-                    }
-                }
+        for (int m = 0; m < n; ++m) {
+            const double yn = y[m] + c * (ac[m] + kv[m]);
+            out[(long)m * E + e] = yn;
+            fin = fin && isfinite(yn);
         }
     }
     return fin;
@@ -687,7 +877,7 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
     constexpr bool UNI = MODE == MODE_UNIFORM;
     constexpr bool FD = MODE != MODE_GENERAL;
     constexpr int nU = NU * NU, nV = NV * NV;
-    __shared__ double Gs[(PL > 0 ? 2 * PL * NH : 1) * kWave];
+    __shared__ double Gs[D::NLDS * kWave];
     if (skip_stage(*kargs())) return;
     const int E = kargs()->E, N = kargs()->N;
     if (blockIdx.x == 0 && threadIdx.x == 0) kargs()->cnt[kargs()->slot ^ 1] = 1;   // the next stage's count
@@ -698,12 +888,19 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
     if (e < E) {
         const int j = e % N, k = e / N;
         const long cc = UNI ? 0 : (long)kargs()->cls_ca[e];
-        auto Wc = [&](int p, int q) -> double {
-            return UNI ? ctab(kargs()->Cw)[p + NQ * q] : kargs()->Cw[cc * NQ * NQ + p + NQ * q];
-        };
-        auto Wa = [&](int p, int q) -> double {
-            return UNI ? ctab(kargs()->Aw)[p + NQ * q] : kargs()->Aw[cc * NQ * NQ + p + NQ * q];
-        };
+        const double* Wc = kargs()->Cw + cc * NQ * NQ;
+        const double* Wa = kargs()->Aw + cc * NQ * NQ;
+
+        // ---- face slots of the four sides and their lift vectors (src/rhs.jl:421-473,
+        // 589-776), into the lane's LDS slots
+        face_side<D, UNI, 0>(e, j, k, N, E, Gs, lane);
+        phase_barrier();
+        face_side<D, UNI, 1>(e, j, k, N, E, Gs, lane);
+        phase_barrier();
+        face_side<D, UNI, 2>(e, j, k, N, E, Gs, lane);
+        phase_barrier();
+        face_side<D, UNI, 3>(e, j, k, N, E, Gs, lane);
+        phase_barrier();
 
         // ---- volume terms: Rv (v-test) and VB (u-test), src/rhs.jl:317-333
         double Rv[nV], bv[nU];
@@ -721,6 +918,7 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
             vol_pass<D, NV, NU>(kargs()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
             vol_pass<D, NV, NV>(kargs()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
         }
+        phase_barrier();
         {
             const double* yu = kargs()->yu;
             double U[nU];
@@ -728,6 +926,7 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
             for (int m = 0; m < nU; ++m) U[m] = yu[(long)m * E + e];
             vol_pass<D, NU, NV>(kargs()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
         }
+        phase_barrier();
         // scale and source, in the order of the reference's sum (src/rhs.jl:340-350)
         {
             const double s = -kargs()->four_hh;
@@ -745,77 +944,12 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
             }
         }
 
-        // ---- face slots of the four sides, lifted into Rv (A, D) and b (F)
-        // (src/rhs.jl:421-473, 589-776)
-        int bkey = 0;
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s) {
-            CKArgs& A = *kargs();
-            const SideGeom g = side_geom(s, e, j, k, N);
-            const int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
-            const double* tr = A.tr;
-            double odu[NQ], ov[NQ], odv[NQ], ndu[NQ], nvv[NQ], ndv[NQ];
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                odu[i] = tr[(long)((0 + s) * NQ + i) * E + e];
-                ov[i] = tr[(long)((4 + s) * NQ + i) * E + e];
-                odv[i] = tr[(long)((8 + s) * NQ + i) * E + e];
-                ndu[i] = tr[(long)((0 + ns) * NQ + i) * E + g.nb];
-                nvv[i] = tr[(long)((4 + ns) * NQ + i) * E + g.nb];
-                ndv[i] = tr[(long)((8 + ns) * NQ + i) * E + g.nb];
-            }
-            double wA[NQ], wD[NQ], wF[NQ];
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                const long fi = (long)i * A.fstride + g.fidx;
-                double Av, Dd, Ff;
-                if (g.bnd) {
-                    const int k2 = boundary_point<UNI>(A, s, fi, i, g.m, NQ, odu[i], ov[i], odv[i], Av, Dd, Ff);
-                    if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
-                } else if (g.plus) {
-                    interior_point<UNI>(A, true, g.hz, fi, nvv[i], ov[i], ndu[i], odu[i], ndv[i], odv[i], Av, Dd, Ff);
-                } else {
-                    interior_point<UNI>(A, false, g.hz, fi, ov[i], nvv[i], odu[i], ndu[i], odv[i], ndv[i], Av, Dd, Ff);
-                }
-                const double wi = ctab(A.tab)[D::oW + i];
-                wA[i] = wi * Av;
-                wD[i] = wi * Dd;
-                wF[i] = wi * Ff;
-            }
-            // Y_Z[b] = sum_i P[i, b] w[i] Z[i] (src/rhs.jl:434-441)
-            cdbl* T = ctab(A.tab);
-            double YA[NV], YD[NV], YF[NU];
-            cols_sym<NQ, NV, false>(T + D::oP, wA, YA);
-            cols_sym<NQ, NV, false>(T + D::oP, wD, YD);
-            cols_sym<NQ, NU, false>(T + D::oP, wF, YF);
-            const int oE = g.plus ? D::oEl : D::oEr;
-            const int oE1 = g.plus ? D::oE1l : D::oE1r;
-            const double two_h = A.two_h, four_hh = A.four_hh;
-#pragma unroll
-            for (int b = 0; b < NV; ++b)
-#pragma unroll
-                for (int aa = 0; aa < NV; ++aa) {
-                    double r = Rv[aa + NV * b];
-                    if (g.hz) {
-                        r = r + (two_h * T[oE + aa]) * YA[b];
-                        r = r + (four_hh * T[oE1 + aa]) * YD[b];
-                    } else {
-                        r = r + (two_h * YA[aa]) * T[oE + b];
-                        r = r + (four_hh * YD[aa]) * T[oE1 + b];
-                    }
-                    Rv[aa + NV * b] = r;
-                }
-            // b = VB - Fu, Fu = sum over sides of E1 (x) Y_F (src/rhs.jl:464-473)
-#pragma unroll
-            for (int b = 0; b < NU; ++b)
-#pragma unroll
-                for (int aa = 0; aa < NU; ++aa)
-                    bv[aa + NU * b] -= g.hz ? T[oE1 + aa] * YF[b] : YF[aa] * T[oE1 + b];
-        }
-        if (bkey) {
-            atomicMin(&kargs()->err->bkey, bkey);
-            kargs()->err->failed = kargs()->serial;
-        }
+        // the lifts, side by side B, T, L, R
+        apply_lifts<D, 0>(Gs, lane, Rv, bv);
+        apply_lifts<D, 1>(Gs, lane, Rv, bv);
+        apply_lifts<D, 2>(Gs, lane, Rv, bv);
+        apply_lifts<D, 3>(Gs, lane, Rv, bv);
+        phase_barrier();
         if (mfail_prev != 0x7fffffff) kargs()->err->failed = kargs()->serial;
         bv[0] += v0;   // (src/rhs.jl:366-370)
 
@@ -839,14 +973,16 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
             const double* Dv = kargs()->Ainv + cc * nU;
             cdbl* Du = ctab(kargs()->Ainv);
 #pragma unroll
-            for (int i = 0; i < NU; ++i)
+            for (int i = 0; i < NU; ++i) {
+                const double* Dr = UNI ? Dv : vfresh(Dv);
 #pragma unroll
                 for (int jj = 0; jj < NU; ++jj) {
                     double s = 0.0;
 #pragma unroll
                     for (int b = (jj & 1); b < NU; b += 2) s = fma(Y1[i + NU * b], Q[b + NU * jj], s);
-                    bv[i + NU * jj] = s * (UNI ? Du[i + NU * jj] : Dv[i + NU * jj]);
+                    bv[i + NU * jj] = s * (UNI ? Du[i + NU * jj] : Dr[i + NU * jj]);
                 }
+            }
 #pragma unroll
             for (int jj = 0; jj < NU; ++jj)
 #pragma unroll
@@ -866,23 +1002,30 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
                     ku[aa + NU * b] = s;
                 }
         } else {
-            // the stored inverse of the class, symmetric: row mm is the contiguous column mm
+            // the stored inverse of the class, symmetric: row mm is the contiguous
+            // column mm; the results pass through the lane's LDS slots (the weights
+            // of the mass solve are written there only afterwards)
             const double* Ai = kargs()->Ainv + cc * (nU * nU);
-#pragma unroll
+#pragma unroll 1
             for (int mm = 0; mm < nU; ++mm) {
                 const double* Ar = Ai + (long)nU * mm;
                 double kv = 0.0;
 #pragma unroll
                 for (int jj = 0; jj < nU; ++jj) kv = fma(Ar[jj], bv[jj], kv);
-                ku[mm] = kv;
+                Gs[(mm % D::NLDS) * kWave + lane] = kv;
             }
+#pragma unroll
+            for (int mm = 0; mm < nU; ++mm) ku[mm] = Gs[(mm % D::NLDS) * kWave + lane];
         }
         {
             CKArgs& A = *kargs();
-            if (!rk_update<nU>(A.stage, A.cstage, ku, A.y0u, A.accu, A.outu, E, e))
+            double yu0[nU], acu[nU];
+            rk_load<nU>(A.stage, A.y0u, A.accu, E, e, yu0, acu);
+            if (!rk_store<nU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, e))
                 atomicMin(&kargs()->err->nonfinite, kargs()->step);
         }
 
+        phase_barrier();
         // ---- the mass system M(v_h) X = Rv by Chebyshev iteration
         // (src/rhs.jl:886-939)
         double Gt[PR > 0 ? 2 * PR * NH : 1];
@@ -932,6 +1075,7 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
             }
             n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
         }
+        phase_barrier();
         double X[nV];
 #pragma unroll
         for (int m = 0; m < nV; ++m) X[m] = 0.0;
@@ -980,7 +1124,9 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         }
         {
             CKArgs& A = *kargs();
-            if (!rk_update<nV>(A.stage, A.cstage, X, A.y0v, A.accv, A.outv, E, e))
+            double yv0[nV], acv[nV];
+            rk_load<nV>(A.stage, A.y0v, A.accv, E, e, yv0, acv);
+            if (!rk_store<nV>(A.stage, A.cstage, X, yv0, acv, A.accv, A.outv, E, e))
                 atomicMin(&kargs()->err->nonfinite, kargs()->step);
         }
     }
