@@ -216,7 +216,9 @@ struct wedg_workspace {
     DevErr* err = nullptr;
     DevErr* herr = nullptr;  // pinned host copy of the failure record
     double *hu = nullptr, *hv = nullptr;   // pinned staging of the state (host <-> device)
-    int* cnt = nullptr;
+    int* cnt = nullptr;                 // per-wave Chebyshev counts [2][waves]
+    unsigned* wstat = nullptr;          // per-wave count statistics [sum | max][waves]
+    unsigned* hstat = nullptr;          // pinned host copy of wstat
     int serial = 0;
     long step_base = 0;
     long stages = 0;                 // stages run since the last reset
@@ -229,6 +231,7 @@ struct wedg_workspace {
     double t_ms[4] = {0, 0, 0, 0};
     long t_n[4] = {0, 0, 0, 0};
     size_t nU() const { return (size_t)p->E * p->nu * p->nu; }
+    size_t nwaves() const { return ((size_t)p->E + kWave - 1) / kWave; }
     size_t nV() const { return (size_t)p->E * p->nv * p->nv; }
 };
 
@@ -465,10 +468,10 @@ static wedg_status reset_err(wedg_workspace* ws) {
     // the pinned record may still be the source of a copy in flight
     HIPOK(hipStreamSynchronize(ws->stream));
     DevErr& h = *ws->herr;
-    h.bkey = INT_MAX; h.mfail = INT_MAX; h.failed = 0; h.nonfinite = INT_MAX; h.maxcheb = 0;
-    h.cheb_wave_sum = 0;
+    h.bkey = INT_MAX; h.mfail = INT_MAX; h.failed = 0; h.nonfinite = INT_MAX;
     ws->stages = 0;
     HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
+    HIPOK(hipMemsetAsync(ws->wstat, 0, 2 * ws->nwaves() * sizeof(unsigned), ws->stream));
     return WEDG_OK;
 }
 
@@ -515,15 +518,15 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     if (he == hipSuccess) he = hipHostMalloc((void**)&ws->hv, nv * sizeof(double), hipHostMallocDefault);
     if (he == hipSuccess) {
         void* c = nullptr;
-        he = hipMalloc(&c, 4 * sizeof(int));
+        const size_t nw = ws->nwaves();
+        he = hipMalloc(&c, 2 * nw * sizeof(int) + 2 * nw * sizeof(unsigned));
         if (he == hipSuccess) {
             ws->allocs.push_back(c);
             ws->cnt = (int*)c;
-            // the counts start at 1 (src/rhs.jl:895); written by a fill, no host copy
-            he = hipMemset(c, 0, 4 * sizeof(int));
-            if (he == hipSuccess) he = hipMemset(c, 1, 1);
-            if (he == hipSuccess) he = hipMemset((char*)c + 4, 1, 1);
+            ws->wstat = (unsigned*)((int*)c + 2 * nw);
+            he = hipMemset(c, 0, 2 * nw * sizeof(int) + 2 * nw * sizeof(unsigned));
         }
+        if (he == hipSuccess) he = hipHostMalloc((void**)&ws->hstat, 2 * nw * sizeof(unsigned), hipHostMallocDefault);
     }
     // the fills above run on the null stream; finish them before the workspace stream is used
     if (he == hipSuccess) he = hipDeviceSynchronize();
@@ -545,6 +548,7 @@ extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     if (ws->herr) (void)hipHostFree(ws->herr);
     if (ws->hu) (void)hipHostFree(ws->hu);
     if (ws->hv) (void)hipHostFree(ws->hv);
+    if (ws->hstat) (void)hipHostFree(ws->hstat);
     if (ws->stream) (void)hipStreamDestroy(ws->stream);
     delete ws;
     return WEDG_OK;
@@ -571,7 +575,7 @@ static KArgs base_args(wedg_workspace* ws) {
     for (int i = 0; i < F_COUNT; ++i) { a.f[i] = p->f[i]; a.fu[i] = p->fu[i]; }
     a.fstride = p->N * (p->N + 1);
     for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
-    a.tr = ws->tr; a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt;
+    a.tr = ws->tr; a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt; a.wstat = ws->wstat;
     return a;
 }
 
@@ -675,8 +679,19 @@ static wedg_status collect(wedg_workspace* ws, int nonfinite_is_error) {
     const DevErr h = *ws->herr;
     wedg_error_info& L = ws->last;
     std::memset(&L, 0, sizeof(L));
-    L.cheb_steps = h.maxcheb;
-    ws->cheb_wave_sum = h.cheb_wave_sum;
+    {
+        const size_t nw = ws->nwaves();
+        HIPOK(hipMemcpyAsync(ws->hstat, ws->wstat, 2 * nw * sizeof(unsigned), hipMemcpyDeviceToHost, ws->stream));
+        HIPOK(hipStreamSynchronize(ws->stream));
+        unsigned long long sum = 0;
+        unsigned mx = 0;
+        for (size_t w = 0; w < nw; ++w) {
+            sum += ws->hstat[w];
+            mx = std::max(mx, ws->hstat[nw + w]);
+        }
+        L.cheb_steps = (int)mx;
+        ws->cheb_wave_sum = sum;
+    }
     const int nq = ws->p->nu, N = ws->p->N;
     if (h.bkey != INT_MAX) {
         L.status = (h.bkey % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
@@ -807,7 +822,7 @@ extern "C" wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* 
 
 extern "C" wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_steps, long* stages) {
     if (!ws) return WEDG_ERR_ARGUMENT;
-    const long waves = (ws->p->E + 63) / 64;
+    const long waves = (long)ws->nwaves();
     if (stages) *stages = ws->stages;
     if (mean_steps)
         *mean_steps = ws->stages > 0 ? (double)ws->cheb_wave_sum / ((double)waves * ws->stages) : 0.0;

@@ -89,9 +89,7 @@ struct DevErr {
     int mfail;         // smallest failing element, 1-based (INT_MAX = none)
     int failed;        // serial of the stage that failed (0 = none)
     int nonfinite;     // first RK4 step with a non-finite entry (INT_MAX = none)
-    int maxcheb;       // largest Chebyshev count taken
-    int pad[3];
-    unsigned long long cheb_wave_sum;   // sum over waves and stages of the count each wave ran
+    int pad[4];
 };
 
 enum { F_HC2 = 0, F_HAF, F_HCF, F_HBF, F_VC2, F_VAF, F_VCF, F_VBF,
@@ -127,7 +125,8 @@ struct KArgs {
     double* tr;                             // traces [arr][i][E]
     double* bsl;                            // boundary slots A, D, F [side][i][3][N]
     DevErr* err;
-    int* cnt;                               // Chebyshev counts, two slots
+    int* cnt;                               // Chebyshev count of every wave [2][waves], by stage parity
+    unsigned* wstat;                        // per wave: sum [waves] and max [waves] of the counts it ran
 };
 
 // ---------------------------------------------------------------------------
@@ -543,7 +542,10 @@ __global__ __launch_bounds__(kWave) void k_traces(const KArgs args) {
     }
     if (global) {
         for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(&kargs()->cnt[kargs()->slot], n_e);
+        if ((threadIdx.x & 63) == 0) {
+            const int nw = (E + kWave - 1) / kWave;
+            kargs()->cnt[kargs()->slot * nw + (block_first_element() / kWave)] = n_e;
+        }
     }
 }
 
@@ -880,10 +882,17 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
     __shared__ double Gs[D::NLDS * kWave];
     if (skip_stage(*kargs())) return;
     const int E = kargs()->E, N = kargs()->N;
-    if (blockIdx.x == 0 && threadIdx.x == 0) kargs()->cnt[kargs()->slot ^ 1] = 1;   // the next stage's count
     const int mfail_prev = kargs()->cheb_global ? kargs()->err->mfail : 0x7fffffff;
     const int lane = threadIdx.x;
     const int e = block_first_element() + lane;
+    const int nw = (E + kWave - 1) / kWave;
+    // the global count: the maximum over the waves' counts of this stage
+    int nglob = 1;
+    if (kargs()->cheb_global) {
+        const int* wc = kargs()->cnt + kargs()->slot * nw;
+        for (int i = lane; i < nw; i += kWave) nglob = max(nglob, wc[i]);
+        for (int off = 32; off > 0; off >>= 1) nglob = max(nglob, __shfl_xor(nglob, off));
+    }
     int ncheb = 0;
     if (e < E) {
         const int j = e % N, k = e / N;
@@ -1085,7 +1094,7 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
                 kargs()->err->failed = kargs()->serial;
             }
         } else {
-            const int n = kargs()->cheb_global ? kargs()->cnt[kargs()->slot] : n_e;
+            const int n = kargs()->cheb_global ? nglob : n_e;
             ncheb = n;
             double r[nV], d[nV];
 #pragma unroll
@@ -1131,9 +1140,11 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         }
     }
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
-    if (lane == 0 && ncheb > 0) {
-        atomicMax(&kargs()->err->maxcheb, ncheb);
-        atomicAdd(&kargs()->err->cheb_wave_sum, (unsigned long long)ncheb);
+    if (lane == 0 && ncheb > 0) {   // this wave's slots only: no atomics
+        unsigned* ws = kargs()->wstat;
+        const int w = block_first_element() / kWave;
+        ws[w] += (unsigned)ncheb;
+        ws[nw + w] = max(ws[nw + w], (unsigned)ncheb);
     }
 }
 
