@@ -5,8 +5,14 @@ elements, 36 modes per field per element, 4,718,592 DOF), mode :initial_value
 with the Gaussian u0 = 0.05 exp(-|x - (1/2,1/2)|^2 / (2 0.05^2)), constant
 coefficients c = 1, alpha = 1e-4, beta = 2, upwind flux, absorbing_2beta on all
 sides, Chebyshev mass solve to 1e-14, FP64 RK4 with dt = h / (10 q_u^2)
-(32,000 steps to T = 0.5). A step is one RK4 step (four right-hand sides) of
-the whole mesh; the state stays in HBM for the timed region.
+(32,000 steps to T = 0.5). The scheme cannot reach T = 0.5 on this instance:
+v_h grows until 1 - 2 beta v_h <= 0 at the centre element near t = 0.131
+(between steps 8,400 and 8,800; the CPU oracle stops the same way on coarser
+meshes), where the mass solve of the reference raises its positivity error.
+The timed run is therefore the part of the run the scheme completes: 8,000 RK4
+steps after a 20-step warmup (t from 3.1e-4 to 0.1253), over which the global
+Chebyshev count rises from 8 to 18. A step is one RK4 step (four right-hand
+sides) of the whole mesh; the state stays in HBM for the timed region.
 
 Prints one JSON line: DOF-stages/s, the roofline of the dominant kernel from
 HIP events on the workspace stream, and the CPU oracle (C/OpenMP restatement
@@ -70,11 +76,12 @@ def flops_per_element_stage(nu, nv, nh, nc):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000, help="timed RK4 steps")
+    ap.add_argument("--steps", type=int, default=8000,
+                    help="timed RK4 steps (default: the 8,000 steps the run completes before the positivity "
+                         "limit near t = 0.131)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--full", action="store_true", help="time all 32,000 steps to T = 0.5")
     ap.add_argument("--cheb", choices=("global", "element"), default="global")
-    ap.add_argument("--cpu-steps", type=int, default=16, help="RK4 steps of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=64, help="RK4 steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
@@ -105,7 +112,7 @@ def main():
     pre, geo = build_problem(q, N)
     t_pre = time.perf_counter() - t_pre
     dt, nsteps_full = time_step(q, N, T)
-    steps = nsteps_full if args.full else args.steps
+    steps = args.steps
     E = N * N
     nu, nv, nh = pre.qu1, pre.qv1, pre.q_high
     dof = E * (nu * nu + nv * nv)
@@ -120,7 +127,6 @@ def main():
     print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
     ws.dev.check(L.wedg_advance(h, 0.0, dt, args.warmup, 0, C.byref(tend)))
     t0 = tend.value
-    # the CPU sample starts from this state, so both time the same window
     uw = np.zeros_like(u)
     vw = np.zeros_like(v)
     ws.dev.check(L.wedg_get_state(h, uw.ctypes.data_as(C.POINTER(C.c_double)), vw.ctypes.data_as(C.POINTER(C.c_double))))
@@ -139,10 +145,19 @@ def main():
     info = ws.dev.last_error()
 
     # per-kernel durations: HIP events around every launch on the workspace
-    # stream, over the same steps from the same state as the timed region
+    # stream, over the same steps from the same state as the timed region (in
+    # two halves; the CPU sample starts from the state at the middle)
     ws.dev.check(L.wedg_set_state(h, uw.ctypes.data_as(C.POINTER(C.c_double)), vw.ctypes.data_as(C.POINTER(C.c_double))))
     L.wedg_timing_enable(h, 1)
-    ws.dev.check(L.wedg_advance(h, t0, dt, steps, 0, C.byref(tend)))
+    h1 = steps // 2
+    ws.dev.check(L.wedg_advance(h, t0, dt, h1, 0, C.byref(tend)))
+    L.wedg_cheb_stats(h, C.byref(mean_nc), C.byref(nst))
+    nc1, ns1 = mean_nc.value, nst.value
+    t_mid = tend.value
+    um = np.zeros_like(u)
+    vm = np.zeros_like(v)
+    ws.dev.check(L.wedg_get_state(h, um.ctypes.data_as(C.POINTER(C.c_double)), vm.ctypes.data_as(C.POINTER(C.c_double))))
+    ws.dev.check(L.wedg_advance(h, t_mid, dt, steps - h1, 0, C.byref(tend)))
     fam = {}
     for i, name in enumerate(FAMILIES):
         ms = C.c_double(0.0)
@@ -151,7 +166,7 @@ def main():
         fam[name] = dict(ms_total=ms.value, launches=n.value, us_avg=1e3 * ms.value / max(1, n.value))
     L.wedg_timing_enable(h, 0)
     L.wedg_cheb_stats(h, C.byref(mean_nc), C.byref(nst))
-    nc_timing = mean_nc.value
+    nc_timing = (nc1 * ns1 + mean_nc.value * nst.value) / max(1, ns1 + nst.value)
 
     # final state is finite (the run is meaningful)
     ws.dev.check(L.wedg_get_state(h, u.ctypes.data_as(C.POINTER(C.c_double)), v.ctypes.data_as(C.POINTER(C.c_double))))
@@ -198,12 +213,12 @@ def main():
         op = ob.OraclePre(pre, N)
         print(f"# cpu oracle {args.cpu_steps} steps on {threads} threads", file=sys.stderr, flush=True)
         c0 = time.perf_counter()
-        ob.rk4_steps(op, uw, vw, t0, dt, args.cpu_steps, cheb_mode=0 if args.cheb == "global" else 1,
+        ob.rk4_steps(op, um, vm, t_mid, dt, args.cpu_steps, cheb_mode=0 if args.cheb == "global" else 1,
                      nthreads=threads)
         cs = time.perf_counter() - c0
         cpu = dict(value=dof * 4 * args.cpu_steps / cs, unit="DOF·stages/s", cores=threads, kind="port",
-                   sample=f"{args.cpu_steps} RK4 steps of the same instance from the state after the warmup "
-                          f"({cs:.1f} s, C/OpenMP oracle)")
+                   sample=f"{args.cpu_steps} RK4 steps of the same instance from the state at the middle of the "
+                          f"timed window (t = {t_mid:.4f}; {cs:.1f} s, C/OpenMP oracle)")
 
     out = dict(metric=METRIC, value=value, unit="DOF·stages/s", n_gpus=1, steps=steps, warmup=args.warmup,
                ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
@@ -211,6 +226,9 @@ def main():
                config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, c=1 alpha=1e-4 beta=2, "
                                     f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
                            elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, full_run_steps=nsteps_full,
+                           timed_window=[t0, t0 + steps * dt],
+                           note="the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "
+                                "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_traces=roof_traces, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
                final_state_finite=finite)

@@ -34,3 +34,63 @@ def rand_state(rng, N, pre, su=1.0, sv=0.05):
     u = np.asfortranarray(su * rng.standard_normal((N, N, pre.qu1, pre.qu1)))
     v = np.asfortranarray(sv * rng.standard_normal((N, N, pre.qv1, pre.qv1)))
     return u, v
+
+
+# ---- the reference's result tables, copied by tests/golden/make_fixtures.py
+import csv as _csv
+import os as _os
+
+_GOLDEN = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "golden")
+
+
+def convergence_row(q, pairing, flux, N):
+    """erru, errv, errux (= errgradu), errE of data/convergence/results.csv."""
+    with open(_os.path.join(_GOLDEN, "convergence_results.csv")) as f:
+        for r in _csv.DictReader(f):
+            if (int(r["q"]), r["pairing"], r["flux"], int(r["N"])) == (q, pairing, flux, N):
+                return dict(erru=float(r["erru"]), errv=float(r["errv"]), errgradu=float(r["errux"]),
+                            errE=float(r["errE"]))
+    raise KeyError((q, pairing, flux, N))
+
+
+def alpha_limit_rows():
+    """(alpha, erru, errv, errE) of data/alpha_limit/order.csv."""
+    with open(_os.path.join(_GOLDEN, "alpha_limit_order.csv")) as f:
+        return [(float(r["alpha"]), dict(erru=float(r["erru"]), errv=float(r["errv"]), errE=float(r["errE"])))
+                for r in _csv.DictReader(f)]
+
+
+# The bump of the alpha -> 0 study (experiments/alpha_limit/run.jl:17-33):
+# u0 = g(s), s = |x - x0|^2 / R^2, g(s) = exp(1 - 1/(1 - s)) for s < 1, else 0,
+# x0 = (1/2, 1/2), R = 1/4, with its first and second derivatives.
+BUMP_X0, BUMP_R = (0.5, 0.5), 0.25
+
+
+def _bump_s(x, y):
+    return ((x - BUMP_X0[0]) ** 2 + (y - BUMP_X0[1]) ** 2) / BUMP_R ** 2
+
+
+def _bump_g(s, order):
+    s = np.asarray(s, dtype=float)
+    inside = s < 1
+    om = np.where(inside, 1 - s, 1.0)
+    g = np.where(inside, np.exp(1 - 1 / om), 0.0)
+    if order == 0:
+        return g
+    if order == 1:
+        return np.where(inside, -g / om ** 2, 0.0)
+    return np.where(inside, g * (1 / om ** 4 - 2 / om ** 3), 0.0)
+
+
+def _bump_d(x, y, ax):
+    d = (x - BUMP_X0[0]) if ax == 0 else (y - BUMP_X0[1])
+    return 2 * d / BUMP_R ** 2
+
+
+BUMP = dict(
+    u0=lambda x, y: _bump_g(_bump_s(x, y), 0),
+    u0_x=lambda x, y: _bump_g(_bump_s(x, y), 1) * _bump_d(x, y, 0),
+    u0_y=lambda x, y: _bump_g(_bump_s(x, y), 1) * _bump_d(x, y, 1),
+    u0_xx=lambda x, y: _bump_g(_bump_s(x, y), 2) * _bump_d(x, y, 0) ** 2 + _bump_g(_bump_s(x, y), 1) * 2 / BUMP_R ** 2,
+    u0_yy=lambda x, y: _bump_g(_bump_s(x, y), 2) * _bump_d(x, y, 1) ** 2 + _bump_g(_bump_s(x, y), 1) * 2 / BUMP_R ** 2,
+)

@@ -13,7 +13,8 @@ import pytest
 import edg_westervelt_amd as w
 from edg_westervelt_amd import precompute
 from oracle import batched as ob
-from tests.cases import MIXED, MIXED2, quadrant_coefficients, rand_state
+from tests.cases import (BUMP, MIXED, MIXED2, alpha_limit_rows, convergence_row, quadrant_coefficients,
+                         rand_state)
 from tests.conftest import rel
 
 pytestmark = pytest.mark.gpu
@@ -212,3 +213,67 @@ def test_dimension_mismatch():
                    0.0, 8, 1 / 8, pre)
     with pytest.raises(w.DimensionMismatch):
         w.edg_rhs(np.zeros((8, 8, 3, 3), order="F"), np.zeros((8, 8, 2, 2), order="F"), 0.0, 8, 1 / 8, pre9)
+
+
+TABLE = [(q, pairing, flux) for q in range(1, 7) for pairing in ("mixed", "matched")
+         for flux in ("upwind", "central", "alternating")]
+
+
+@pytest.mark.parametrize("q,pairing,flux", TABLE)
+def test_convergence_table_device(q, pairing, flux):
+    """solve(q, 8, 0.1) on the device against data/convergence/results.csv (made
+    with CG; the device solves the mass systems by Chebyshev to 1e-14): rtol
+    1e-9 plus 2e-13 absolute, as the oracle's table test."""
+    say(f"case table q={q} {pairing} {flux} N=8")
+    N = 8
+    gold = convergence_row(q, pairing, flux, N)
+    e = w.solve(q, N, 0.1, flux=flux, mixed=pairing == "mixed")
+    for k, g in gold.items():
+        assert abs(getattr(e, k) - g) <= 1e-9 * g + 2e-13, (k, getattr(e, k), g)
+
+
+def test_alpha_limit_device():
+    """:initial_value mode, the bump of experiments/alpha_limit/run.jl, q_u = 6,
+    q_v = 5, N = 20, T = 0.5: difference norms of every alpha of
+    data/alpha_limit/order.csv against alpha = 0. The norms measure differences
+    of two solutions only 1e-5..1e-9 apart, so rounding weighs more as alpha
+    shrinks: rtol max(1e-9, 2e-17 / alpha) (the CPU oracle agrees to 1.3e-9 at
+    alpha = 1e-9 and to 6e-12 at alpha = 1e-5)."""
+    say("case alpha limit")
+    runs = {}
+    for alpha in [0.0] + [a for a, _ in alpha_limit_rows()]:
+        runs[alpha] = w.solve(6, 20, 0.5, flux="upwind", mixed=True, mode="initial_value", initial_profile=BUMP,
+                              coefficients=w.replace_coefficients(w.paper_coefficients(), alpha=alpha),
+                              return_state=True)
+    ref = runs[0.0]
+    for alpha, gold in alpha_limit_rows():
+        d = w.difference_norms(runs[alpha], ref)
+        tol = max(1e-9, 2e-17 / alpha)
+        for k, g in gold.items():
+            assert abs(getattr(d, k) / g - 1) < tol, (alpha, k, getattr(d, k), g)
+
+
+def test_full_size_rhs_and_step():
+    """The 256 x 256, q = 5 instance of the benchmark at full size: from the
+    state after 20 device RK4 steps, one right-hand side and one RK4 step
+    against the oracle (1e-12 relative; the step's increment 1e-11)."""
+    say("case full size q=5 N=256")
+    import bench
+    from edg_westervelt_amd.solver import time_step
+    q, N = 5, 256
+    pre, geo = bench.build_problem(q, N)
+    dt, _ = time_step(q, N, 0.5)
+    ws = w.Workspace(pre, N)
+    u = np.zeros((N, N, 6, 6), order="F")
+    v = np.zeros((N, N, 6, 6), order="F")
+    for _ in range(20):
+        w.rk4_step_(u, v, 0.0, dt, N, 1.0 / N, pre, ws)
+    assert np.abs(v).max() > 1e-3
+    a = ob.rhs(ob.OraclePre(pre, N), u, v, 0.01, nthreads=16)
+    b = w.edg_rhs_(ws, u, v, 0.01, N, 1.0 / N, pre)
+    assert rel(a[0], b[0]) < TOL and rel(a[1], b[1]) < TOL
+    uo, vo, _, _ = ob.rk4_steps(ob.OraclePre(pre, N), u, v, 0.01, dt, 1, nthreads=16)
+    ug = np.array(u, order="F", copy=True)
+    vg = np.array(v, order="F", copy=True)
+    w.rk4_step_(ug, vg, 0.01, dt, N, 1.0 / N, pre, ws)
+    assert rel(uo - u, ug - u) < 1e-11 and rel(vo - v, vg - v) < 1e-11

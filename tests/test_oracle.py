@@ -4,8 +4,9 @@ Each check follows a test of test/runtests.jl or a fixture of data/, restated;
 together they pin every function of oracle/wedg_oracle.c:
 * the element-by-element dense restatement (oracle/dense.py) at <= 1e-12 over
   the reference's matrix (test/runtests.jl:189-293, 716-732);
-* the golden errors of data/convergence/results.csv (:360, :364) and of
-  test/runtests.jl:207-217;
+* the golden errors of data/convergence/results.csv (:360, :364, and every
+  q / pairing / flux of the table at N = 5) and of test/runtests.jl:207-217;
+* the :initial_value mode against data/alpha_limit/order.csv;
 * mass solvers against dense solves (test/runtests.jl:418-444);
 * boundary flux closed forms (test/runtests.jl:295-317);
 * locality and superposition of the boundary sides (test/runtests.jl:319-338);
@@ -24,7 +25,7 @@ from edg_westervelt_amd.coefficients import Coefficients
 from edg_westervelt_amd.quadrature import GaussQCofs, GradJacobiP, JacobiP
 from oracle import batched as ob
 from oracle import dense, norms
-from tests.cases import MIXED, quadrant_coefficients, rand_state
+from tests.cases import BUMP, MIXED, alpha_limit_rows, convergence_row, quadrant_coefficients, rand_state
 from tests.conftest import rel
 
 
@@ -389,3 +390,40 @@ def test_errors_first_in_serial_order():
         with pytest.raises(RuntimeError) as ei:
             ob.rhs(op, u, wv, 0.0)
         assert str(ei.value) == mass_msg
+
+
+TABLE = [(q, pairing, flux) for q in range(1, 7) for pairing in ("mixed", "matched")
+         for flux in ("upwind", "central", "alternating")]
+
+
+@pytest.mark.parametrize("q,pairing,flux", TABLE)
+def test_convergence_table(q, pairing, flux):
+    """Every (q, pairing, flux) of data/convergence/results.csv at N = 5 (CG,
+    T = 0.1): rtol 1e-9 plus 2e-13 absolute. The absolute part is the rounding
+    accumulated by the O(1) solution over the run (the q = 6 errors, 1e-9..1e-6,
+    differ from the table by up to 7e-14 for any summation order); most rows
+    agree to 1e-11."""
+    N = 5
+    gold = convergence_row(q, pairing, flux, N)
+    pre, geo = precompute(q, N, flux=flux, mixed=pairing == "mixed", mass_solver="cg")
+    u, v, t, dt, ns = ob.solve_state(pre, N, 0.1, q)
+    e = norms.mms_errors(u, v, 0.1, N, 1.0 / N, pre, geo)
+    for k, g in gold.items():
+        assert abs(e[k] - g) <= 1e-9 * g + 2e-13, (k, e[k], g)
+
+
+def test_alpha_limit_golden():
+    """:initial_value mode with the bump of experiments/alpha_limit/run.jl: the
+    difference norms of alpha = 1e-5 against alpha = 0 (q_u = 6, q_v = 5, N = 20,
+    T = 0.5) against data/alpha_limit/order.csv, rtol 1e-9."""
+    N, q, T = 20, 6, 0.5
+    res = {}
+    for alpha in (0.0, 1e-5):
+        pre, geo = precompute(q, N, flux="upwind", mixed=True, mode="initial_value", initial_profile=BUMP,
+                              coefficients=replace_coefficients(paper_coefficients(), alpha=alpha))
+        u, v, t, dt, ns = ob.solve_state(pre, N, T, q, nthreads=8)
+        res[alpha] = (u, v)
+    d = norms.difference_norms(*res[1e-5], *res[0.0], N, 1.0 / N, pre, geo)
+    gold = dict(alpha_limit_rows())[1e-5]
+    for k, g in gold.items():
+        assert abs(d[k] / g - 1) < 1e-9, (k, d[k], g)
