@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6        # MI355X FP64 (vector = matrix), TFLOP/s
 METRIC = "FP64 DOF·stages/s for edg_rhs!+RK4"
-FAMILIES = ("traces", "element")   # k_traces; k_element (volume, faces, u-solve, mass, RK4 update)
+FAMILIES = ("prep", "stage")   # k_prep (first stage of a call); k_stage (one RK stage)
 
 
 def gaussian_profile(s=0.05, amp=0.05):
@@ -176,26 +176,24 @@ def main():
     value = dof * stages / elapsed
     ms_per_step = 1e3 * elapsed / steps
 
-    # rooflines: k_element is FP64-bound (its algorithmic flops per element and
-    # stage over its average duration), k_traces HBM-bound (72 + 72 words per
-    # element and stage at q = 5: the state read, the traces written)
+    # roofline of k_stage (one launch per RK stage, FP64-bound): the
+    # algorithmic flops per element and stage over its average duration
     f_total, f_traces = flops_per_element_stage(nu, nv, nh, nc_timing)
-    el_s = fam["element"]["us_avg"] * 1e-6
-    achieved = E * (f_total - f_traces) / el_s / 1e12
+    st_s = fam["stage"]["us_avg"] * 1e-6
+    achieved = E * f_total / st_s / 1e12
     roof = dict(bound="mfma", achieved=achieved, peak=FP64_PEAK_TFS, unit="TFLOP/s",
-                frac=achieved / FP64_PEAK_TFS, traffic=None, kernel="k_element",
-                flops_per_launch=E * (f_total - f_traces), us_avg=fam["element"]["us_avg"])
-    tr_bytes = 8 * E * (nu * nu + nv * nv + 12 * nu)
-    tr_s = fam["traces"]["us_avg"] * 1e-6
-    roof_traces = dict(bound="hbm", achieved=tr_bytes / tr_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       frac=tr_bytes / tr_s / 1e9 / HBM_PEAK_GBS, kernel="k_traces",
-                       bytes_per_launch=tr_bytes, us_avg=fam["traces"]["us_avg"])
+                frac=achieved / FP64_PEAK_TFS, traffic=None, kernel="k_stage",
+                flops_per_launch=E * f_total, us_avg=fam["stage"]["us_avg"])
+    bytes_stage = 8 * 17 * dof / 4
+    roof_hbm = dict(bound="hbm", achieved=bytes_stage / st_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=bytes_stage / st_s / 1e9 / HBM_PEAK_GBS, kernel="k_stage",
+                    bytes_per_launch=bytes_stage, note="compulsory bytes of the three-register RK4, 34 B per DOF")
     traffic_file = args.traffic or os.path.join(ROOT, "profiles", f"traffic_q{q}_n{N}.json")
     if os.path.exists(traffic_file):
         tj = json.load(open(traffic_file))
-        roof["traffic"] = tj.get("k_element")
-        roof_traces["traffic"] = tj.get("k_traces")
-    stage_s = sum(f["us_avg"] for f in fam.values()) * 1e-6
+        roof["traffic"] = tj.get("k_stage")
+        roof_hbm["traffic"] = tj.get("k_stage")
+    stage_s = elapsed / (4 * steps)
     bytes_step = 8 * 17 * dof
     whole = dict(flops_per_stage=E * f_total, tflops=E * f_total / stage_s / 1e12,
                  fp64_frac=E * f_total / stage_s / 1e12 / FP64_PEAK_TFS,
@@ -230,7 +228,7 @@ def main():
                            note="the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "
                                 "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
-               roofline=roof, roofline_traces=roof_traces, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
+               roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
                final_state_finite=finite)
     print(json.dumps(out))
 

@@ -211,7 +211,6 @@ struct wedg_workspace {
     std::vector<void*> allocs;
     double *yu = nullptr, *yv = nullptr, *su = nullptr, *sv = nullptr;
     double *au = nullptr, *av = nullptr, *ou = nullptr, *ov = nullptr;
-    double* tr = nullptr;
     double* bsl = nullptr;   // boundary slots [side][i][A, D, F][N]
     DevErr* err = nullptr;
     DevErr* herr = nullptr;  // pinned host copy of the failure record
@@ -468,7 +467,9 @@ static wedg_status reset_err(wedg_workspace* ws) {
     // the pinned record may still be the source of a copy in flight
     HIPOK(hipStreamSynchronize(ws->stream));
     DevErr& h = *ws->herr;
-    h.bkey = INT_MAX; h.mfail = INT_MAX; h.failed = 0; h.nonfinite = INT_MAX;
+    std::memset(&h, 0, sizeof(h));
+    h.fkey = ~0ull;
+    h.nonfinite = INT_MAX;
     ws->stages = 0;
     HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
     HIPOK(hipMemsetAsync(ws->wstat, 0, 2 * ws->nwaves() * sizeof(unsigned), ws->stream));
@@ -506,7 +507,6 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     const size_t nu = ws->nU(), nv = ws->nV();
     ws->yu = A(nu); ws->yv = A(nv); ws->su = A(nu); ws->sv = A(nv);
     ws->au = A(nu); ws->av = A(nv); ws->ou = A(nu); ws->ov = A(nv);
-    ws->tr = A((size_t)12 * p->nu * p->E);
     ws->bsl = A((size_t)4 * p->nu * 3 * p->N);
     if (he == hipSuccess) {
         void* e = nullptr;
@@ -575,7 +575,7 @@ static KArgs base_args(wedg_workspace* ws) {
     for (int i = 0; i < F_COUNT; ++i) { a.f[i] = p->f[i]; a.fu[i] = p->fu[i]; }
     a.fstride = p->N * (p->N + 1);
     for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
-    a.tr = ws->tr; a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt; a.wstat = ws->wstat;
+    a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt; a.wstat = ws->wstat;
     return a;
 }
 
@@ -627,10 +627,13 @@ static wedg_status run_kernel(wedg_workspace* ws, const KArgs& a, int which) {
     return WEDG_OK;
 }
 
-// One stage: k_traces on Y_s, then k_element writing the stage output.
+// One stage: k_stage on Y_s (after k_prep when it is the first stage of a
+// call), writing the stage output and, when has_next, preparing the stage that
+// follows at time t_next. The output must not alias Y_s: a stage reads its
+// neighbours' input state.
 static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, const double* yv,
                              const double* y0u, const double* y0v, double* outu, double* outv,
-                             double t, double c, long step) {
+                             double t, double c, long step, bool first, bool has_next, double t_next) {
     KArgs a = base_args(ws);
     a.stage = stage;
     a.serial = ++ws->serial;
@@ -641,21 +644,31 @@ static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, co
     stage_time(ws->p, t, a);
     a.yu = yu; a.yv = yv; a.y0u = y0u; a.y0v = y0v;
     a.accu = ws->au; a.accv = ws->av; a.outu = outu; a.outv = outv;
-    ++ws->stages;
-    for (int which = 0; which < 2; ++which) {
-        wedg_status st = run_kernel(ws, a, which);
-        if (st != WEDG_OK) return st;
+    a.has_next = has_next ? 1 : 0;
+    if (has_next) {
+        KArgs nx;
+        std::memset(&nx, 0, sizeof(nx));
+        stage_time(ws->p, t_next, nx);
+        a.nct = nx.ct;
+        a.nst = nx.st;
     }
-    return WEDG_OK;
+    ++ws->stages;
+    wedg_status st;
+    if (first && (st = run_kernel(ws, a, 0)) != WEDG_OK) return st;
+    return run_kernel(ws, a, 1);
 }
 
-static wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step) {
+// One RK4 step (src/solver.jl:300-321) on the device state: the stage values
+// alternate between the S (su, sv) and O (ou, ov) buffers, so no stage writes
+// the state it reads; the last stage updates y in place (it reads only its own
+// element of y). first: the first step of a call; last: the last one.
+static wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step, bool first, bool last) {
     wedg_status s;
-    const double hd = 0.5 * dt;
-    if ((s = run_stage(ws, 1, ws->yu, ws->yv, ws->yu, ws->yv, ws->su, ws->sv, t, hd, step))) return s;
-    if ((s = run_stage(ws, 2, ws->su, ws->sv, ws->yu, ws->yv, ws->su, ws->sv, t + 0.5 * dt, hd, step))) return s;
-    if ((s = run_stage(ws, 3, ws->su, ws->sv, ws->yu, ws->yv, ws->su, ws->sv, t + 0.5 * dt, dt, step))) return s;
-    return run_stage(ws, 4, ws->su, ws->sv, ws->yu, ws->yv, ws->yu, ws->yv, t + dt, dt / 6.0, step);
+    const double hd = 0.5 * dt, th = t + 0.5 * dt, tf = t + dt;
+    if ((s = run_stage(ws, 1, ws->yu, ws->yv, ws->yu, ws->yv, ws->su, ws->sv, t, hd, step, first, true, th))) return s;
+    if ((s = run_stage(ws, 2, ws->su, ws->sv, ws->yu, ws->yv, ws->ou, ws->ov, th, hd, step, false, true, th))) return s;
+    if ((s = run_stage(ws, 3, ws->ou, ws->ov, ws->yu, ws->yv, ws->su, ws->sv, th, dt, step, false, true, tf))) return s;
+    return run_stage(ws, 4, ws->su, ws->sv, ws->yu, ws->yv, ws->yu, ws->yv, tf, dt / 6.0, step, false, !last, tf);
 }
 
 // Reads the failure record and maps it to a status (boundary before mass,
@@ -693,15 +706,17 @@ static wedg_status collect(wedg_workspace* ws, int nonfinite_is_error) {
         ws->cheb_wave_sum = sum;
     }
     const int nq = ws->p->nu, N = ws->p->N;
-    if (h.bkey != INT_MAX) {
-        L.status = (h.bkey % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
-        L.key = h.bkey;
-        L.side = (h.bkey - 1) / (2 * nq * N);
-        return (wedg_status)L.status;
-    }
-    if (h.mfail != INT_MAX) {
-        L.status = WEDG_ERR_MASS;
-        L.element = h.mfail;
+    if (h.fkey != ~0ull) {
+        const int kind = (int)((h.fkey >> 32) & 1u);
+        const long key = (long)(h.fkey & 0xffffffffull);
+        if (kind == 0) {   // a boundary weight (src/rhs.jl:112-116)
+            L.status = (key % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
+            L.key = key;
+            L.side = (int)((key - 1) / (2 * nq * N));
+            return (wedg_status)L.status;
+        }
+        L.status = WEDG_ERR_MASS;   // src/rhs.jl:916
+        L.element = key;
         return WEDG_ERR_MASS;
     }
     if (h.nonfinite != INT_MAX) {
@@ -719,7 +734,7 @@ extern "C" wedg_status wedg_rhs_device(wedg_workspace* ws, const double* u, cons
     if (!ws || !u || !v || !ut || !vt) return WEDG_ERR_ARGUMENT;
     wedg_status s;
     if ((s = reset_err(ws))) return s;
-    return run_stage(ws, 0, u, v, u, v, ut, vt, t, 0.0, 0);
+    return run_stage(ws, 0, u, v, u, v, ut, vt, t, 0.0, 0, true, false, t);
 }
 
 extern "C" wedg_status wedg_sync(wedg_workspace* ws) {
@@ -735,7 +750,7 @@ extern "C" wedg_status wedg_rhs(wedg_workspace* ws, const double* u, const doubl
     if ((s = stage_in(ws, ws->su, ws->hu, u, ws->nU()))) return s;
     if ((s = stage_in(ws, ws->sv, ws->hv, v, ws->nV()))) return s;
     if ((s = reset_err(ws))) return s;
-    if ((s = run_stage(ws, 0, ws->su, ws->sv, ws->su, ws->sv, ws->ou, ws->ov, t, 0.0, 0))) return s;
+    if ((s = run_stage(ws, 0, ws->su, ws->sv, ws->su, ws->sv, ws->ou, ws->ov, t, 0.0, 0, true, false, t))) return s;
     if ((s = collect(ws, 0))) return s;
     if ((s = stage_out(ws, ut, ws->hu, ws->ou, ws->nU()))) return s;
     return stage_out(ws, vt, ws->hv, ws->ov, ws->nV());
@@ -772,7 +787,7 @@ extern "C" wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double 
     if ((s = reset_err(ws))) return s;
     double t = t0;
     for (long it = 0; it < nsteps; ++it) {
-        if ((s = rk4_device(ws, t, dt, it + 1))) return s;
+        if ((s = rk4_device(ws, t, dt, it + 1, it == 0, it == nsteps - 1))) return s;
         t += dt;
     }
     if (t_end) *t_end = t;
@@ -793,7 +808,7 @@ extern "C" wedg_status wedg_rk4_step(wedg_workspace* ws, double* u, double* v, d
     wedg_status s;
     if ((s = wedg_set_state(ws, u, v))) return s;
     if ((s = reset_err(ws))) return s;
-    if ((s = rk4_device(ws, t, dt, 1))) return s;
+    if ((s = rk4_device(ws, t, dt, 1, true, true))) return s;
     s = collect(ws, 0);
     if (finite) *finite = ws->last.step == 0;
     if (s) return s;

@@ -11,12 +11,12 @@ template <int NU, int NV>
 static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) {
     const int grid = (a.E + kWave - 1) / kWave;
     if (which == 0) {
-        if (mode == MODE_UNIFORM) k_traces<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
-        else k_traces<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
+        if (mode == MODE_UNIFORM) k_prep<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
+        else k_prep<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
     } else if (which == 1) {
-        if (mode == MODE_UNIFORM) k_element<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
-        else if (mode == MODE_SEPARABLE) k_element<NU, NV, MODE_SEPARABLE><<<grid, kWave, 0, s>>>(a);
-        else k_element<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
+        if (mode == MODE_UNIFORM) k_stage<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
+        else if (mode == MODE_SEPARABLE) k_stage<NU, NV, MODE_SEPARABLE><<<grid, kWave, 0, s>>>(a);
+        else k_stage<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }

@@ -14,18 +14,18 @@
 // into its even and odd modes (rows_sym / cols_sym), which halves its FMAs; the
 // results differ from the plain sums by rounding only.
 //
-// Two kernels per right-hand side:
-//   k_traces   the traces of every element on its four sides (reference
-//              _traces!/_trace4!, src/rhs.jl:384-410, 573-580) into a trace
-//              buffer, and with the global step-count policy the Chebyshev
-//              intervals (src/rhs.jl:889-918) reduced to one count;
-//   k_element  the volume terms (src/rhs.jl:317-333) and source
-//              (src/rhs.jl:340-350), each element's own face slots from its
-//              traces and its neighbours' (interior fluxes src/rhs.jl:620-700,
-//              boundary fluxes src/rhs.jl:65-128, 738-765), the lifts
-//              (src/rhs.jl:421-473), the u-equation (src/rhs.jl:361-382), the
-//              Chebyshev mass solve (src/rhs.jl:889-939) and the RK4 stage
-//              update (src/solver.jl:300-321).
+// One kernel per RK stage, k_stage: each element forms its own traces and its
+// neighbours' facing ones from the state (reference _traces!/_trace4!,
+// src/rhs.jl:384-410, 573-580), its face slots (interior fluxes
+// src/rhs.jl:620-700), the volume terms (src/rhs.jl:317-333) and source
+// (src/rhs.jl:340-350), the lifts (src/rhs.jl:421-473), the u-equation
+// (src/rhs.jl:361-382), the Chebyshev mass solve (src/rhs.jl:889-939) and the
+// RK4 stage update (src/solver.jl:300-321), and then, from the state it has
+// just written, what the next stage needs before it can start: the a-priori
+// Chebyshev count of every element reduced per wave (the global policy takes
+// the maximum, src/rhs.jl:889-918) and the slots of the physical-boundary sides
+// with their weight checks (src/rhs.jl:65-128, 702-776). k_prep does the same
+// for the first stage of a call.
 // Every face slot is computed by the element that owns it, so no slot has two
 // writers and no floating-point atomics exist; results do not depend on the
 // launch shape.
@@ -33,6 +33,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace wedg {
 
@@ -85,11 +87,13 @@ struct FaceArr {
 
 // Failure record of the device (reset by the host before each public call).
 struct DevErr {
-    int bkey;          // smallest _boundary_key (INT_MAX = none)
-    int mfail;         // smallest failing element, 1-based (INT_MAX = none)
-    int failed;        // serial of the stage that failed (0 = none)
+    // The first failure in serial order (all ones = none): the stage's serial,
+    // then the kind (0: a boundary weight, before 1: the mass matrix), then the
+    // key (the _boundary_key, or the element + 1), packed so that atomicMin
+    // keeps the failure the reference reports.
+    unsigned long long fkey;
     int nonfinite;     // first RK4 step with a non-finite entry (INT_MAX = none)
-    int pad[4];
+    int pad[5];
 };
 
 enum { F_HC2 = 0, F_HAF, F_HCF, F_HBF, F_VC2, F_VAF, F_VCF, F_VBF,
@@ -102,6 +106,8 @@ struct KArgs {
     int nproj, pad0;
     double two_h, four_hh, theta, tau, lu, lv, omega, cheb_L;
     double ct, st, srct, cstage;
+    double nct, nst;                // time factors of the next stage (its boundary data)
+    int has_next, pad2;             // 1: prepare the next stage from this stage's output
     double pc[5];                   // coefficients of the projections
     double gam[4], eta[4], sig[4], kb[4], kap[4];
     const double* tab;
@@ -122,7 +128,6 @@ struct KArgs {
     const double* y0u; const double* y0v;   // base state y of the step
     double* accu; double* accv;             // RK4 accumulator
     double* outu; double* outv;             // Y_{s+1}, y (last stage) or (u_t, v_t)
-    double* tr;                             // traces [arr][i][E]
     double* bsl;                            // boundary slots A, D, F [side][i][3][N]
     DevErr* err;
     int* cnt;                               // Chebyshev count of every wave [2][waves], by stage parity
@@ -193,10 +198,10 @@ __device__ __forceinline__ void phase_barrier() {
 #endif
 }
 
+// a failure of an earlier stage ends the run
 template <class KA>
 __device__ __forceinline__ bool skip_stage(const KA& a) {
-    const int fl = a.err->failed;
-    return fl != 0 && fl < a.serial;
+    return (a.err->fkey >> 33) < (unsigned long long)(unsigned)a.serial;
 }
 
 // First element of the block. Blocks are dealt to the 8 XCDs in turn; this
@@ -261,47 +266,6 @@ __device__ __forceinline__ void cols_sym(cdbl* T, const double (&y)[NR], double 
         if ((NR & 1) && same) acc = fma(T[H + NR * a], y[H], acc);
         z[a] = acc;
     }
-}
-
-// ---------------------------------------------------------------------------
-// traces of one field X (n x n) on the four sides (src/rhs.jl:388-410): for
-// side B  o[i] = sum_b P[i, b] sum_a El[a] X[a, b], T with Er, L/R the same on
-// the transpose. El = (-1)^a Er for values, -(-1)^a Er for derivatives (DER).
-template <class D, int n, bool DER>
-__device__ __forceinline__ void trace4(cdbl* T, int oEr, const double (&X)[n * n], double* __restrict__ tr,
-                                       int arr0, int e, int E) {
-    constexpr int NQ = D::NQ;
-    double sB[n], sT[n], sL[n], sR[n];
-#pragma unroll
-    for (int b = 0; b < n; ++b) {
-        double ev = 0.0, od = 0.0, evt = 0.0, odt = 0.0;
-#pragma unroll
-        for (int a = 0; a < n; ++a) {
-            const double er = T[oEr + a];
-            if (a & 1) {
-                od = fma(er, X[a + n * b], od);
-                odt = fma(er, X[b + n * a], odt);
-            } else {
-                ev = fma(er, X[a + n * b], ev);
-                evt = fma(er, X[b + n * a], evt);
-            }
-        }
-        sT[b] = ev + od;
-        sB[b] = DER ? od - ev : ev - od;
-        sR[b] = evt + odt;
-        sL[b] = DER ? odt - evt : evt - odt;
-    }
-    cdbl* P = T + D::oP;
-    auto emit = [&](const double (&sv)[n], int sd) {
-        double o[NQ];
-        rows_sym<NQ, n, false>(P, [&](int b) { return sv[b]; }, o);
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) tr[(long)((arr0 + sd) * NQ + i) * E + e] = o[i];
-    };
-    emit(sB, 0);
-    emit(sT, 1);
-    emit(sL, 2);
-    emit(sR, 3);
 }
 
 // Interval data of the Chebyshev iteration from the extremes of g
@@ -389,14 +353,15 @@ __device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, 
 // returns the point's failing _boundary_key (src/rhs.jl:732) or 0.
 template <bool UNI, class KA>
 __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i, int m, int nq, double du,
-                                              double vt, double dv, double& A, double& Dd, double& Ff) {
+                                              double vt, double dv, double ct, double stt, double& A, double& Dd,
+                                              double& Ff) {
     const bool hz = s < 2;
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
     const double c2f = fval<UNI>(a, hz ? F_HC2 : F_VC2, fi);
     const double af = fval<UNI>(a, hz ? F_HAF : F_VAF, fi);
     const double cf = fval<UNI>(a, hz ? F_HCF : F_VCF, fi);
     const double bf = fval<UNI>(a, hz ? F_HBF : F_VBF, fi);
-    const double two_h = a.two_h, om = a.omega, ct = a.ct, stt = a.st;
+    const double two_h = a.two_h, om = a.omega;
     const double gun = sgn * (two_h * du);
     const double gvn = sgn * (two_h * dv);
     const double spv = a.bcd[2 * s][i + nq * m], spnv = a.bcd[2 * s + 1][i + nq * m];
@@ -430,127 +395,151 @@ __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i
     return 0;
 }
 
-// Slots (A, D, F) of the element's sides on the physical boundary
-// (src/rhs.jl:738-765) from its own traces, into the boundary slot buffer, and
-// the smallest failing _boundary_key of the element (src/rhs.jl:732).
-template <class D, bool UNI, int s>
-__device__ __forceinline__ int boundary_side(int e, int E) {
+// trace of the field X (n x n) on side sd of its element (0 B, 1 T, 2 L, 3 R;
+// src/rhs.jl:388-410): o[i] = sum_b P[i, b] sum_a E[a] X[a, b] with E the row
+// of the side's end (B, T contract the first index, L, R the second). Erow is
+// the + end's row (Ebr or E1br); the - end's is (-1)^a times it for values,
+// -(-1)^a for derivatives (DER).
+template <class D, int n, bool DER, int sd>
+__device__ __forceinline__ void side_trace(cdbl* T, int oEr, const double (&X)[n * n], double (&o)[D::NQ]) {
     constexpr int NQ = D::NQ;
-    CKArgs& B = *kargs();
-    const int N = B.N, j = e % N, k = e / N;
-    const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
-    int bkey = 0;
-    if (bnd) {
-        const int m = s < 2 ? k : j;
-        const int fidx = s == 0 ? (N + 1) * k : s == 1 ? N + (N + 1) * k : s == 2 ? j : j + N * N;
-        const double* tr = B.tr;
-        double* bs = B.bsl + (long)s * NQ * 3 * N + m;
+    constexpr bool second = sd >= 2, plus_end = (sd & 1) != 0;
+    double sv[n];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const double du = tr[(long)((0 + s) * NQ + i) * E + e];
-            const double vt = tr[(long)((4 + s) * NQ + i) * E + e];
-            const double dv = tr[(long)((8 + s) * NQ + i) * E + e];
-            const long fi = (long)i * B.fstride + fidx;
-            double Av, Dd, Ff;
-            const int k2 = boundary_point<UNI>(B, s, fi, i, m, NQ, du, vt, dv, Av, Dd, Ff);
-            if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
-            bs[(long)(i * 3 + 0) * N] = Av;
-            bs[(long)(i * 3 + 1) * N] = Dd;
-            bs[(long)(i * 3 + 2) * N] = Ff;
+    for (int b = 0; b < n; ++b) {
+        double ev = 0.0, od = 0.0;
+#pragma unroll
+        for (int a = 0; a < n; ++a) {
+            const double x = second ? X[b + n * a] : X[a + n * b];
+            if (a & 1) od = fma(T[oEr + a], x, od);
+            else ev = fma(T[oEr + a], x, ev);
+        }
+        sv[b] = plus_end ? ev + od : (DER ? od - ev : ev - od);
+    }
+    rows_sym<NQ, n, false>(T + D::oP, [&](int b) { return sv[b]; }, o);
+}
+
+__device__ __forceinline__ unsigned long long fail_key(int serial, int kind, int key) {
+    return ((unsigned long long)(unsigned)serial << 33) | ((unsigned long long)kind << 32) | (unsigned)key;
+}
+
+// What the stage of serial `serial` needs of its input state (u at Up in
+// memory, v in V) before it runs: its a-priori Chebyshev count (with the global
+// policy; src/rhs.jl:889-918) and, on the physical boundary, the slots A, D, F
+// of the element's boundary sides at the stage's time factors (ct, st)
+// (src/rhs.jl:738-765) into the boundary slot buffer. Failures are recorded
+// with the stage's serial: the weight checks of the boundary (kind 0, the
+// _boundary_key, src/rhs.jl:732) before the mass matrix (kind 1, the element).
+// Returns the element's count (1 when not counted or failing).
+template <class D, bool UNI>
+__device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D::NV * D::NV], int e, int E, int serial,
+                                          double ct, double st, bool count) {
+    constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV, NH = D::NH;
+    int n_e = 1;
+    if (count) {
+        CKArgs& A = *kargs();
+        const double* betav = A.beta + (UNI ? 0 : (long)A.cls_b[e] * NH * NH);
+        cdbl* Ph = ctab(A.tab) + D::oPh;
+        cdbl* bu = ctab(A.beta);
+        double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
+        bool nan = false;
+#pragma unroll
+        for (int p = 0; p < (NH + 1) / 2; ++p) {
+            const bool mid = (NH & 1) && p == NH / 2;
+            double yp[NH], yq[NH];
+            vq_rows<D>(Ph, p, mid, V, yp, yq);
+#pragma unroll
+            for (int q = 0; q < NH; ++q) {
+                const double g = yp[q] * (2.0 * (UNI ? bu[p + NH * q] : betav[p + NH * q]));
+                gmin = fmin(gmin, g);
+                gmax = fmax(gmax, g);
+                nan = nan || (g != g);
+                if (!mid) {
+                    const int pq = NH - 1 - p;
+                    const double g2 = yq[q] * (2.0 * (UNI ? bu[pq + NH * q] : betav[pq + NH * q]));
+                    gmin = fmin(gmin, g2);
+                    gmax = fmax(gmax, g2);
+                    nan = nan || (g2 != g2);
+                }
+            }
+        }
+        double th, de, s1;
+        n_e = cheb_interval(gmin, gmax, nan, A.cheb_L, th, de, s1);
+        if (n_e == 0) {
+            atomicMin(&A.err->fkey, fail_key(serial, 1, e + 1));
+            n_e = 1;
         }
     }
-    return bkey;
+    CKArgs& B = *kargs();
+    const int N = B.N, j = e % N, k = e / N;
+    if (j == 0 || j == N - 1 || k == 0 || k == N - 1) {
+        double U[NU * NU];
+#pragma unroll
+        for (int m = 0; m < NU * NU; ++m) U[m] = Up[(long)m * E + e];
+        int bkey = 0;
+        auto side = [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
+            if (!bnd) return;
+            CKArgs& A = *kargs();
+            cdbl* T = ctab(A.tab);
+            double du[NQ], vt[NQ], dv[NQ];
+            side_trace<D, NU, true, s>(T, D::oE1r, U, du);
+            side_trace<D, NV, false, s>(T, D::oEr, V, vt);
+            side_trace<D, NV, true, s>(T, D::oE1r, V, dv);
+            const int m = s < 2 ? k : j;
+            const int fidx = s == 0 ? (N + 1) * k : s == 1 ? N + (N + 1) * k : s == 2 ? j : j + N * N;
+            double* bs = A.bsl + (long)s * NQ * 3 * N + m;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const long fi = (long)i * A.fstride + fidx;
+                double Av, Dd, Ff;
+                const int k2 = boundary_point<UNI>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
+                if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
+                bs[(long)(i * 3 + 0) * N] = Av;
+                bs[(long)(i * 3 + 1) * N] = Dd;
+                bs[(long)(i * 3 + 2) * N] = Ff;
+            }
+        };
+        side(std::integral_constant<int, 0>());
+        side(std::integral_constant<int, 1>());
+        side(std::integral_constant<int, 2>());
+        side(std::integral_constant<int, 3>());
+        if (bkey) atomicMin(&B.err->fkey, fail_key(serial, 0, bkey));
+    }
+    return n_e;
 }
 
-template <class D, bool UNI>
-__device__ __forceinline__ void boundary_slots(int e, int E) {
-    int bkey = 0;
-    auto key = [&](int k2) { if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2); };
-    key(boundary_side<D, UNI, 0>(e, E));
-    key(boundary_side<D, UNI, 1>(e, E));
-    key(boundary_side<D, UNI, 2>(e, E));
-    key(boundary_side<D, UNI, 3>(e, E));
-    if (bkey) {
-        atomicMin(&kargs()->err->bkey, bkey);
-        kargs()->err->failed = kargs()->serial;
+// Writes the wave's largest count into the per-wave slot of the stage parity.
+__device__ __forceinline__ void wave_count(int n_e, int slot, int E) {
+    for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
+    if ((threadIdx.x & 63) == 0) {
+        const int nw = (E + kWave - 1) / kWave;
+        kargs()->cnt[slot * nw + (block_first_element() / kWave)] = n_e;
     }
 }
 
-// ---------------------------------------------------------------------------
-// Traces of every element, and with the global policy the largest Chebyshev
-// step count (src/rhs.jl:889-918) reduced over the grid.
+// The first stage of a call: the count and boundary slots of its input state.
 template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_traces(const KArgs args) {
+__global__ __launch_bounds__(kWave) void k_prep(const KArgs args) {
     (void)args;
     using D = Dims<NU, NV>;
-    constexpr int NH = D::NH;
     constexpr bool UNI = MODE == MODE_UNIFORM;
-    if (skip_stage(*kargs())) return;   // a failure of an earlier stage ends the run
-    const int e = block_first_element() + threadIdx.x;
     const int E = kargs()->E;
+    const int e = block_first_element() + threadIdx.x;
     const bool global = kargs()->cheb_global != 0;
     int n_e = 1;
     if (e < E) {
-        {
-            const double* yu = kargs()->yu;
-            double U[NU * NU];
-#pragma unroll
-            for (int m = 0; m < NU * NU; ++m) U[m] = yu[(long)m * E + e];
-            trace4<D, NU, true>(ctab(kargs()->tab), D::oE1r, U, kargs()->tr, 0, e, E);
-        }
         const double* yv = kargs()->yv;
         double V[NV * NV];
 #pragma unroll
         for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
-        trace4<D, NV, false>(ctab(kargs()->tab), D::oEr, V, kargs()->tr, 4, e, E);
-        trace4<D, NV, true>(ctab(kargs()->tab), D::oE1r, V, kargs()->tr, 8, e, E);
-        boundary_slots<D, UNI>(e, E);
-        if (global) {
-            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
-            double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
-            bool nan = false;
-            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
-            cdbl* bu = ctab(kargs()->beta);
-#pragma unroll
-            for (int p = 0; p < (NH + 1) / 2; ++p) {
-                const bool mid = (NH & 1) && p == NH / 2;
-                double yp[NH], yq[NH];
-                vq_rows<D>(Ph, p, mid, V, yp, yq);
-#pragma unroll
-                for (int q = 0; q < NH; ++q) {
-                    const double bp = UNI ? bu[p + NH * q] : betav[p + NH * q];
-                    const double g = yp[q] * (2.0 * bp);
-                    gmin = fmin(gmin, g);
-                    gmax = fmax(gmax, g);
-                    nan = nan || (g != g);
-                    if (!mid) {
-                        const double bq = UNI ? bu[NH - 1 - p + NH * q] : betav[NH - 1 - p + NH * q];
-                        const double g2 = yq[q] * (2.0 * bq);
-                        gmin = fmin(gmin, g2);
-                        gmax = fmax(gmax, g2);
-                        nan = nan || (g2 != g2);
-                    }
-                }
-            }
-            double th, de, s1;
-            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
-            if (n_e == 0) {
-                atomicMin(&kargs()->err->mfail, e + 1);
-                n_e = 1;
-            }
-        }
+        n_e = stage_prep<D, UNI>(kargs()->yu, V, e, E, kargs()->serial, kargs()->ct, kargs()->st, global);
     }
-    if (global) {
-        for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
-        if ((threadIdx.x & 63) == 0) {
-            const int nw = (E + kWave - 1) / kWave;
-            kargs()->cnt[kargs()->slot * nw + (block_first_element() / kWave)] = n_e;
-        }
-    }
+    if (global) wave_count(n_e, kargs()->slot, E);
 }
 
 // ---------------------------------------------------------------------------
-
 // ---------------------------------------------------------------------------
 // Volume terms (src/rhs.jl:317-333):
 //   out (NO x NO) += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S
@@ -652,11 +641,13 @@ __device__ __forceinline__ void vol_pass(const double* tab, const double (&X)[n 
 
 // ---------------------------------------------------------------------------
 // Face slots (A, D, F) of side s of element e = j + N k (B, T, L, R) from its
-// own traces and its neighbour's facing ones (interior fluxes
-// src/rhs.jl:620-700; on a physical boundary the slots k_traces formed),
-// reduced to the side's lift vectors Y_A, Y_D, Y_F (src/rhs.jl:434-441).
+// own traces (from U, V) and its neighbour's facing ones (from the neighbour's
+// state, read here), the interior fluxes of src/rhs.jl:620-700; on a physical
+// boundary the slots stage_prep formed. Reduced to the side's lift vectors
+// Y_A, Y_D, Y_F (src/rhs.jl:434-441), kept in the lane's LDS slots.
 template <class D, bool UNI, int s>
-__device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, double* Ys, int lane) {
+__device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, const double (&U)[D::NU * D::NU],
+                                          const double (&V)[D::NV * D::NV], double* Ys, int lane) {
     constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
@@ -667,30 +658,32 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, dou
     else if (s == 1) { fidx = j + 1 + (N + 1) * k; nb = e + 1; bnd = j == N - 1; m = k; }
     else if (s == 2) { fidx = j + N * k;           nb = e - N; bnd = k == 0;     m = j; }
     else             { fidx = j + N * (k + 1);     nb = e + N; bnd = k == N - 1; m = j; }
-    if (bnd) nb = e;    // not read on the boundary
-    const double* tr = A.tr;
+    if (bnd) nb = e;    // a boundary lane reads its own state here; its slots come from stage_prep
     double odu[NQ], ov[NQ], odv[NQ], ndu[NQ], nvv[NQ], ndv[NQ];
+    {
+        cdbl* T = ctab(A.tab);
+        side_trace<D, NU, true, s>(T, D::oE1r, U, odu);
+        side_trace<D, NV, false, s>(T, D::oEr, V, ov);
+        side_trace<D, NV, true, s>(T, D::oE1r, V, odv);
+    }
+    {
+        const double* yu = A.yu;
+        double X[NU * NU];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        odu[i] = tr[(long)((0 + s) * NQ + i) * E + e];
-        ov[i] = tr[(long)((4 + s) * NQ + i) * E + e];
-        odv[i] = tr[(long)((8 + s) * NQ + i) * E + e];
-        ndu[i] = tr[(long)((0 + ns) * NQ + i) * E + nb];
-        nvv[i] = tr[(long)((4 + ns) * NQ + i) * E + nb];
-        ndv[i] = tr[(long)((8 + ns) * NQ + i) * E + nb];
+        for (int mm = 0; mm < NU * NU; ++mm) X[mm] = yu[(long)mm * E + nb];
+        side_trace<D, NU, true, ns>(ctab(A.tab), D::oE1r, X, ndu);
+    }
+    {
+        const double* yv = A.yv;
+        double X[NV * NV];
+#pragma unroll
+        for (int mm = 0; mm < NV * NV; ++mm) X[mm] = yv[(long)mm * E + nb];
+        cdbl* T = ctab(A.tab);
+        side_trace<D, NV, false, ns>(T, D::oEr, X, nvv);
+        side_trace<D, NV, true, ns>(T, D::oE1r, X, ndv);
     }
     double wA[NQ], wD[NQ], wF[NQ];
-    if (bnd) {   // the boundary slots, formed by k_traces
-        CKArgs& B = *kargs();
-        cdbl* Tw = ctab(B.tab) + D::oW;
-        const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            wA[i] = Tw[i] * bs[(long)(i * 3 + 0) * N];
-            wD[i] = Tw[i] * bs[(long)(i * 3 + 1) * N];
-            wF[i] = Tw[i] * bs[(long)(i * 3 + 2) * N];
-        }
-    } else {
+    {
         CKArgs& I = *kargs();
         cdbl* Tw = ctab(I.tab) + D::oW;
 #pragma unroll
@@ -704,9 +697,18 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, dou
             wF[i] = Tw[i] * Ff;
         }
     }
+    if (bnd) {
+        CKArgs& B = *kargs();
+        cdbl* Tw = ctab(B.tab) + D::oW;
+        const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            wA[i] = Tw[i] * bs[(long)(i * 3 + 0) * N];
+            wD[i] = Tw[i] * bs[(long)(i * 3 + 1) * N];
+            wF[i] = Tw[i] * bs[(long)(i * 3 + 2) * N];
+        }
+    }
     cdbl* T = ctab(A.tab);
-    // Y_Z[b] = sum_i P[i, b] w[i] Z[i] (src/rhs.jl:434-441), kept in the lane's
-    // LDS slots [s][A, D, F][b] until the lifts are added
     double YA[NV], YD[NV], YF[NU];
     cols_sym<NQ, NV, false>(T + D::oP, wA, YA);
     cols_sym<NQ, NV, false>(T + D::oP, wD, YD);
@@ -867,12 +869,14 @@ __device__ __forceinline__ bool rk_store(int stage, double c, const double (&kv)
     return fin;
 }
 
-// The element kernel: volume terms, source, face slots and lifts, u-equation,
-// mass solve and the RK4 stage update of one element per lane. One wave per
-// block: the Chebyshev weights G = 2 beta v_q W of the first mirrored row
-// pairs live in LDS ([entry][lane], conflict-free), the rest in registers.
+// One RK stage of one element per lane (one launch per stage): face slots and
+// lifts, volume terms, source, u-equation, mass solve, the RK4 stage update,
+// and what the next stage needs of the new state (stage_prep). One wave per
+// block: the lane's LDS slots hold first the lift vectors of its four sides,
+// then the Chebyshev weights G = 2 beta v_q W of the first mirrored row pairs
+// ([entry][lane], conflict-free); the rest of G stays in registers.
 template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
+__global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
     (void)args;
     using D = Dims<NU, NV>;
     constexpr int NQ = D::NQ, NH = D::NH, NP = D::NP, PL = D::PL, PR = D::PR;
@@ -882,33 +886,42 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
     __shared__ double Gs[D::NLDS * kWave];
     if (skip_stage(*kargs())) return;
     const int E = kargs()->E, N = kargs()->N;
-    const int mfail_prev = kargs()->cheb_global ? kargs()->err->mfail : 0x7fffffff;
+    const bool global = kargs()->cheb_global != 0;
     const int lane = threadIdx.x;
     const int e = block_first_element() + lane;
     const int nw = (E + kWave - 1) / kWave;
     // the global count: the maximum over the waves' counts of this stage
     int nglob = 1;
-    if (kargs()->cheb_global) {
+    if (global) {
         const int* wc = kargs()->cnt + kargs()->slot * nw;
         for (int i = lane; i < nw; i += kWave) nglob = max(nglob, wc[i]);
         for (int off = 32; off > 0; off >>= 1) nglob = max(nglob, __shfl_xor(nglob, off));
     }
-    int ncheb = 0;
+    int ncheb = 0, n_next = 1;
     if (e < E) {
         const int j = e % N, k = e / N;
         const long cc = UNI ? 0 : (long)kargs()->cls_ca[e];
         const double* Wc = kargs()->Cw + cc * NQ * NQ;
         const double* Wa = kargs()->Aw + cc * NQ * NQ;
 
+        double U[nU], V[nV];
+        {
+            const double* yu = kargs()->yu;
+            const double* yv = kargs()->yv;
+#pragma unroll
+            for (int m = 0; m < nU; ++m) U[m] = yu[(long)m * E + e];
+#pragma unroll
+            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
+        }
         // ---- face slots of the four sides and their lift vectors (src/rhs.jl:421-473,
         // 589-776), into the lane's LDS slots
-        face_side<D, UNI, 0>(e, j, k, N, E, Gs, lane);
+        face_side<D, UNI, 0>(e, j, k, N, E, U, V, Gs, lane);
         phase_barrier();
-        face_side<D, UNI, 1>(e, j, k, N, E, Gs, lane);
+        face_side<D, UNI, 1>(e, j, k, N, E, U, V, Gs, lane);
         phase_barrier();
-        face_side<D, UNI, 2>(e, j, k, N, E, Gs, lane);
+        face_side<D, UNI, 2>(e, j, k, N, E, U, V, Gs, lane);
         phase_barrier();
-        face_side<D, UNI, 3>(e, j, k, N, E, Gs, lane);
+        face_side<D, UNI, 3>(e, j, k, N, E, U, V, Gs, lane);
         phase_barrier();
 
         // ---- volume terms: Rv (v-test) and VB (u-test), src/rhs.jl:317-333
@@ -917,24 +930,11 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         for (int m = 0; m < nV; ++m) Rv[m] = 0.0;
 #pragma unroll
         for (int m = 0; m < nU; ++m) bv[m] = 0.0;
-        double v0;
-        {
-            const double* yv = kargs()->yv;
-            double V[nV];
-#pragma unroll
-            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
-            v0 = V[0];
-            vol_pass<D, NV, NU>(kargs()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
-            vol_pass<D, NV, NV>(kargs()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
-        }
+        const double v0 = V[0];
+        vol_pass<D, NV, NU>(kargs()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
+        vol_pass<D, NV, NV>(kargs()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
         phase_barrier();
-        {
-            const double* yu = kargs()->yu;
-            double U[nU];
-#pragma unroll
-            for (int m = 0; m < nU; ++m) U[m] = yu[(long)m * E + e];
-            vol_pass<D, NU, NV>(kargs()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
-        }
+        vol_pass<D, NU, NV>(kargs()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
         phase_barrier();
         // scale and source, in the order of the reference's sum (src/rhs.jl:340-350)
         {
@@ -959,7 +959,6 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         apply_lifts<D, 2>(Gs, lane, Rv, bv);
         apply_lifts<D, 3>(Gs, lane, Rv, bv);
         phase_barrier();
-        if (mfail_prev != 0x7fffffff) kargs()->err->failed = kargs()->serial;
         bv[0] += v0;   // (src/rhs.jl:366-370)
 
         // ---- u-equation u_t = A_e^-1 b (src/rhs.jl:361-382) and the u part of
@@ -1088,18 +1087,18 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         double X[nV];
 #pragma unroll
         for (int m = 0; m < nV; ++m) X[m] = 0.0;
-        if (n_e == 0) {
-            if (!kargs()->cheb_global) {
-                atomicMin(&kargs()->err->mfail, e + 1);
-                kargs()->err->failed = kargs()->serial;
-            }
+        if (n_e == 0) {   // with the global policy stage_prep has recorded it already
+            if (!global) atomicMin(&kargs()->err->fkey, fail_key(kargs()->serial, 1, e + 1));
         } else {
-            const int n = kargs()->cheb_global ? nglob : n_e;
+            const int n = global ? nglob : n_e;
             ncheb = n;
             double r[nV], d[nV];
 #pragma unroll
             for (int m = 0; m < nV; ++m) { r[m] = Rv[m]; d[m] = r[m] / th; }
             double rho = 1.0 / s1;
+            // the table, one scalar load for the whole iteration (the mirrored
+            // half of P_h, 27 values at q = 5, stays in SGPRs)
+            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
             for (int step = 1; step <= n; ++step) {
 #pragma unroll
                 for (int m = 0; m < nV; ++m) X[m] += d[m];
@@ -1107,7 +1106,6 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
                 // r <- r - (d - K d)
 #pragma unroll
                 for (int m = 0; m < nV; ++m) r[m] = r[m] - d[m];
-                cdbl* Ph = ctab(kargs()->tab) + D::oPh;
 #pragma unroll
                 for (int p = 0; p < NP; ++p) {
                     double Gp[NH], Gq[NH];
@@ -1134,11 +1132,22 @@ __global__ __launch_bounds__(kWave) void k_element(const KArgs args) {
         {
             CKArgs& A = *kargs();
             double yv0[nV], acv[nV];
-            rk_load<nV>(A.stage, A.y0v, A.accv, E, e, yv0, acv);
-            if (!rk_store<nV>(A.stage, A.cstage, X, yv0, acv, A.accv, A.outv, E, e))
+            const int stage = A.stage;
+            const double c = A.cstage;
+            rk_load<nV>(stage, A.y0v, A.accv, E, e, yv0, acv);
+            if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e))
                 atomicMin(&kargs()->err->nonfinite, kargs()->step);
+            if (A.has_next) {
+                // the new state v, as rk_store wrote it; u is read back from outu
+                double Vn[nV];
+#pragma unroll
+                for (int m = 0; m < nV; ++m) Vn[m] = stage < 4 ? yv0[m] + c * X[m] : yv0[m] + c * (acv[m] + X[m]);
+                CKArgs& B = *kargs();
+                n_next = stage_prep<D, UNI>(B.outu, Vn, e, E, B.serial + 1, B.nct, B.nst, global);
+            }
         }
     }
+    if (global && kargs()->has_next) wave_count(n_next, kargs()->slot ^ 1, E);
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
     if (lane == 0 && ncheb > 0) {   // this wave's slots only: no atomics
         unsigned* ws = kargs()->wstat;

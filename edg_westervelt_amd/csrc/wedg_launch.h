@@ -5,7 +5,7 @@
 
 namespace wedg {
 
-// which: 0 = k_traces, 1 = k_element; mode: MODE_GENERAL, MODE_SEPARABLE or MODE_UNIFORM
+// which: 0 = k_prep, 1 = k_stage; mode: MODE_GENERAL, MODE_SEPARABLE or MODE_UNIFORM
 hipError_t launch_q1(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 hipError_t launch_q2(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 hipError_t launch_q3(const KArgs& a, int nv, int which, hipStream_t s, int mode);
