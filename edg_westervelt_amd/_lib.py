@@ -21,6 +21,7 @@ OK, ERR_BOUNDARY_TRACE, ERR_BOUNDARY_DATA, ERR_MASS, ERR_DIMENSION, ERR_NONFINIT
 CHEB_GLOBAL, CHEB_ELEMENT = 0, 1
 
 _dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
 
 
 class WedgSide(C.Structure):
@@ -77,6 +78,11 @@ def _load():
         "wedg_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)],
         "wedg_cheb_stats": [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "wedg_probe_copy": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
+        "wedg_problem_set_diag": [vp, _dp, _dp, _dp],
+        "wedg_advance_receivers": [vp, C.c_double, C.c_double, C.c_long, C.c_int, _ip, _ip, _dp, C.c_int, _dp,
+                                   C.POINTER(C.c_double)],
+        "wedg_diagnostics": [vp, _dp, _dp, _dp, C.c_int, _dp, _dp],
+        "wedg_snapshot": [vp, _dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -180,6 +180,9 @@ struct wedg_problem {
     double* proj[5] = {};   // PiSP, PiBSP2, PiCdiv, PiDdiv, src
     int has_proj[5] = {};
     size_t device_bytes = 0;
+    // energy diagnostics (wedg_problem_set_diag): classes of (c^2, beta) at the standard points
+    int* dcls = nullptr;
+    double *dc2 = nullptr, *dbq = nullptr, *dW2d = nullptr;
 
     // Host data reaches the device only through this pinned buffer: copies
     // straight from pageable memory would make the runtime register (pin) the
@@ -218,6 +221,11 @@ struct wedg_workspace {
     int* cnt = nullptr;                 // per-wave Chebyshev counts [2][waves]
     unsigned* wstat = nullptr;          // per-wave count statistics [sum | max][waves]
     unsigned* hstat = nullptr;          // pinned host copy of wstat
+    // source-mode diagnostics: device outputs and their pinned host copies
+    double* drecv = nullptr; double* hrecv = nullptr; size_t nrecv_cap = 0;
+    double* ddiag = nullptr; double* hdiag = nullptr;   // E, vmax, ndmin [3][E], slice [N nq], eta0 [nv]
+    int* drecv_elem = nullptr; double* drecv_rows = nullptr; int nrecv_dev = 0;
+    void* hrows = nullptr;   // pinned staging of the receiver rows and elements
     int serial = 0;
     long step_base = 0;
     long stages = 0;                 // stages run since the last reset
@@ -549,6 +557,10 @@ extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     if (ws->hu) (void)hipHostFree(ws->hu);
     if (ws->hv) (void)hipHostFree(ws->hv);
     if (ws->hstat) (void)hipHostFree(ws->hstat);
+    if (ws->hrecv) (void)hipHostFree(ws->hrecv);
+    if (ws->hdiag) (void)hipHostFree(ws->hdiag);
+    if (ws->hrows) (void)hipHostFree(ws->hrows);
+    if (ws->drecv) (void)hipFree(ws->drecv);
     if (ws->stream) (void)hipStreamDestroy(ws->stream);
     delete ws;
     return WEDG_OK;
@@ -889,5 +901,177 @@ extern "C" wedg_status wedg_device_info(char* name, size_t len, int* cu_count, d
     }
     if (cu_count) *cu_count = prop.multiProcessorCount;
     if (hbm_gbs) *hbm_gbs = 2.0 * prop.memoryClockRate * 1e3 * (prop.memoryBusWidth / 8) / 1e9;
+    return WEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Source-mode diagnostics (src/solver.jl:856-958).
+
+static hipError_t launch_diag(wedg_problem* p, const DiagArgs* d, const RecvArgs* r, hipStream_t s) {
+    switch (p->qu) {
+        case 1: return launch_diag_q1(d, r, p->nv, s);
+        case 2: return launch_diag_q2(d, r, p->nv, s);
+        case 3: return launch_diag_q3(d, r, p->nv, s);
+        case 4: return launch_diag_q4(d, r, p->nv, s);
+        case 5: return launch_diag_q5(d, r, p->nv, s);
+        case 6: return launch_diag_q6(d, r, p->nv, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" wedg_status wedg_problem_set_diag(wedg_problem* p, const double* err_c2, const double* err_beta,
+                                             const double* W2d) {
+    if (!p || !err_c2 || !err_beta || !W2d) return WEDG_ERR_ARGUMENT;
+    const int N = p->N, nq = p->nu;
+    const long E = p->E;
+    const size_t len = 2 * (size_t)nq * nq;
+    std::vector<int> cls;
+    std::vector<double> uniq;
+    // geo.err_c2, geo.err_beta are (N, N, nq, nq): [j, k, p, q] with j fastest
+    dedup(E, len, [&](long e, double* b) {
+        const long j = e % N, k = e / N;
+        for (int q = 0; q < nq; ++q)
+            for (int pp = 0; pp < nq; ++pp) {
+                const long idx = j + (long)N * (k + (long)N * (pp + (long)nq * q));
+                b[pp + nq * q] = err_c2[idx];
+                b[nq * nq + pp + nq * q] = err_beta[idx];
+            }
+    }, cls, uniq);
+    const int nc = (int)(uniq.size() / len);
+    std::vector<double> c2((size_t)nc * nq * nq), bq((size_t)nc * nq * nq);
+    for (int c = 0; c < nc; ++c) {
+        std::memcpy(&c2[(size_t)c * nq * nq], &uniq[(size_t)c * len], nq * nq * sizeof(double));
+        std::memcpy(&bq[(size_t)c * nq * nq], &uniq[(size_t)c * len + nq * nq], nq * nq * sizeof(double));
+    }
+    p->stage_bytes = (size_t)1 << 20;
+    if (hipHostMalloc(&p->stage, p->stage_bytes, hipHostMallocDefault) != hipSuccess) return WEDG_ERR_HIP;
+    hipError_t he = hipSuccess;
+    p->dcls = p->upload(cls.data(), cls.size(), &he);
+    if (he == hipSuccess) p->dc2 = p->upload(c2.data(), c2.size(), &he);
+    if (he == hipSuccess) p->dbq = p->upload(bq.data(), bq.size(), &he);
+    if (he == hipSuccess) p->dW2d = p->upload(W2d, (size_t)nq * nq, &he);
+    (void)hipHostFree(p->stage);
+    p->stage = nullptr;
+    return he == hipSuccess ? WEDG_OK : WEDG_ERR_HIP;
+}
+
+extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, double dt, long nsteps, int nrecv,
+                                              const int* recv_j, const int* recv_k, const double* rows,
+                                              int stop_on_nonfinite, double* recv, double* t_end) {
+    if (!ws || nsteps < 0 || nrecv < 0 || (nrecv > 0 && (!recv_j || !recv_k || !rows || !recv))) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    const int N = p->N, nv = p->nv;
+    for (int r = 0; r < nrecv; ++r)
+        if (recv_j[r] < 0 || recv_j[r] >= N || recv_k[r] < 0 || recv_k[r] >= N) return WEDG_ERR_ARGUMENT;
+    const size_t need = (size_t)nsteps * nrecv;
+    HIPOK(hipStreamSynchronize(ws->stream));
+    if (need > ws->nrecv_cap) {
+        if (ws->drecv) (void)hipFree(ws->drecv);
+        if (ws->hrecv) (void)hipHostFree(ws->hrecv);
+        ws->drecv = nullptr;
+        ws->hrecv = nullptr;
+        ws->nrecv_cap = 0;
+        HIPOK(hipMalloc((void**)&ws->drecv, need * sizeof(double)));
+        HIPOK(hipHostMalloc((void**)&ws->hrecv, need * sizeof(double), hipHostMallocDefault));
+        ws->nrecv_cap = need;
+    }
+    const size_t rbytes = (size_t)nrecv * (sizeof(int) + 2 * nv * sizeof(double));
+    if (nrecv > ws->nrecv_dev) {   // receiver basis rows, then elements, on the device
+        if (ws->hrows) (void)hipHostFree(ws->hrows);
+        ws->hrows = nullptr;
+        ws->nrecv_dev = 0;
+        void* d = nullptr;
+        HIPOK(hipMalloc(&d, rbytes));
+        ws->allocs.push_back(d);
+        HIPOK(hipHostMalloc(&ws->hrows, rbytes, hipHostMallocDefault));
+        ws->drecv_rows = (double*)d;
+        ws->nrecv_dev = nrecv;
+    }
+    if (nrecv > 0) {
+        ws->drecv_elem = (int*)(ws->drecv_rows + (size_t)2 * nv * nrecv);
+        double* hr = (double*)ws->hrows;
+        int* he_ = (int*)(hr + (size_t)2 * nv * nrecv);
+        std::memcpy(hr, rows, (size_t)2 * nv * nrecv * sizeof(double));
+        for (int r = 0; r < nrecv; ++r) he_[r] = recv_j[r] + N * recv_k[r];
+        HIPOK(hipMemcpyAsync(ws->drecv_rows, ws->hrows, rbytes, hipMemcpyHostToDevice, ws->stream));
+        HIPOK(hipStreamSynchronize(ws->stream));
+    }
+    wedg_status s;
+    if ((s = reset_err(ws))) return s;
+    double t = t0;
+    RecvArgs ra;
+    ra.v = ws->yv; ra.E = p->E; ra.nrecv = nrecv; ra.elem = ws->drecv_elem; ra.rows = ws->drecv_rows;
+    ra.out = ws->drecv;
+    for (long it = 0; it < nsteps; ++it) {
+        if ((s = rk4_device(ws, t, dt, it + 1, it == 0, it == nsteps - 1))) return s;
+        t += dt;
+        if (nrecv > 0) {
+            ra.step = it;
+            HIPOK(launch_diag(p, nullptr, &ra, ws->stream));
+        }
+    }
+    if (t_end) *t_end = t;
+    if ((s = collect(ws, stop_on_nonfinite))) return s;
+    if (need > 0) {
+        HIPOK(hipMemcpyAsync(ws->hrecv, ws->drecv, need * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+        HIPOK(hipStreamSynchronize(ws->stream));
+        std::memcpy(recv, ws->hrecv, need * sizeof(double));
+    }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* vmax, double* ndmin, int j0,
+                                        const double* eta0, double* slice) {
+    if (!ws || !E || !vmax || !ndmin || !slice || !eta0) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    if (!p->dcls) return WEDG_ERR_ARGUMENT;   // wedg_problem_set_diag first
+    const int N = p->N, nq = p->nu, nv = p->nv;
+    if (j0 < 0 || j0 >= N) return WEDG_ERR_ARGUMENT;
+    const size_t n = 3 * (size_t)p->E + (size_t)N * nq + nv;
+    HIPOK(hipStreamSynchronize(ws->stream));
+    if (!ws->ddiag) {
+        void* d = nullptr;
+        HIPOK(hipMalloc(&d, n * sizeof(double)));
+        ws->allocs.push_back(d);
+        ws->ddiag = (double*)d;
+        HIPOK(hipHostMalloc((void**)&ws->hdiag, n * sizeof(double), hipHostMallocDefault));
+    }
+    double* dslice = ws->ddiag + 3 * (size_t)p->E;
+    double* deta = dslice + (size_t)N * nq;
+    std::memcpy(ws->hdiag, eta0, nv * sizeof(double));
+    HIPOK(hipMemcpyAsync(deta, ws->hdiag, nv * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    DiagArgs d;
+    d.u = ws->yu; d.v = ws->yv; d.N = N; d.E = p->E; d.j0 = j0; d.pad = 0; d.h = 1.0 / N;
+    d.tab = p->tab; d.W2d = p->dW2d; d.cls = p->dcls; d.c2 = p->dc2; d.bq = p->dbq; d.eta0 = deta;
+    d.E_out = ws->ddiag; d.vmax_out = ws->ddiag + p->E; d.nd_out = ws->ddiag + 2 * (size_t)p->E; d.slice = dslice;
+    d.grid = nullptr;
+    HIPOK(launch_diag(p, &d, nullptr, ws->stream));
+    HIPOK(hipMemcpyAsync(ws->hdiag, ws->ddiag, (3 * (size_t)p->E + (size_t)N * nq) * sizeof(double),
+                         hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(E, ws->hdiag, p->E * sizeof(double));
+    std::memcpy(vmax, ws->hdiag + p->E, p->E * sizeof(double));
+    std::memcpy(ndmin, ws->hdiag + 2 * (size_t)p->E, p->E * sizeof(double));
+    std::memcpy(slice, ws->hdiag + 3 * (size_t)p->E, (size_t)N * nq * sizeof(double));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_snapshot(wedg_workspace* ws, double* grid) {
+    if (!ws || !grid) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    if (!p->dcls) return WEDG_ERR_ARGUMENT;   // wedg_problem_set_diag first
+    const size_t n = (size_t)p->E * p->nu * p->nu;
+    HIPOK(hipStreamSynchronize(ws->stream));
+    // the grid (E nu^2 values) is formed in the RK4 accumulator, which holds
+    // nothing between calls, and read back through the pinned state staging
+    double* dg = ws->au;
+    DiagArgs d{};
+    d.u = ws->yu; d.v = ws->yv; d.N = p->N; d.E = p->E; d.j0 = 0; d.h = 1.0 / p->N;
+    d.tab = p->tab; d.W2d = p->dW2d; d.cls = p->dcls; d.c2 = p->dc2; d.bq = p->dbq; d.eta0 = nullptr;
+    d.grid = dg;
+    HIPOK(launch_diag(p, &d, nullptr, ws->stream));
+    HIPOK(hipMemcpyAsync(ws->hu, dg, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(grid, ws->hu, n * sizeof(double));
     return WEDG_OK;
 }

@@ -23,4 +23,11 @@ static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) 
     return hipGetLastError();
 }
 
+template <int NU, int NV>
+static hipError_t launch_diag_nv(const DiagArgs* d, const RecvArgs* r, hipStream_t s) {
+    if (d) k_diag<NU, NV><<<(d->E + kWave - 1) / kWave, kWave, 0, s>>>(*d);
+    if (r) k_recv<NV><<<(r->nrecv + kWave - 1) / kWave, kWave, 0, s>>>(*r);
+    return hipGetLastError();
+}
+
 }  // namespace wedg

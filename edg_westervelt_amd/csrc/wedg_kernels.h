@@ -1158,3 +1158,146 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
 }
 
 }  // namespace wedg
+
+namespace wedg {
+
+// ---------------------------------------------------------------------------
+// Source-mode diagnostics (src/solver.jl:856-958).
+
+struct RecvArgs {
+    const double* v;        // the state v (N, N, n_v, n_v)
+    int E, nrecv;
+    const int* elem;        // element of each receiver
+    const double* rows;     // per receiver: P_v row at eta, then at xi ((q_v+1) each)
+    double* out;            // [step][receiver]
+    long step;
+};
+
+// v_h at the receivers: eta^T V xi with the basis rows at the receiver's local
+// coordinates (src/solver.jl:899-905). One lane per receiver.
+template <int NV>
+__global__ __launch_bounds__(kWave) void k_recv(const RecvArgs a) {
+    const int r = blockIdx.x * kWave + threadIdx.x;
+    if (r >= a.nrecv) return;
+    const int e = a.elem[r];
+    const double* eta = a.rows + (long)r * 2 * NV;
+    const double* xi = eta + NV;
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+        double t = 0.0;
+#pragma unroll
+        for (int aa = 0; aa < NV; ++aa) t += eta[aa] * a.v[(long)(aa + NV * b) * a.E + e];
+        s += t * xi[b];
+    }
+    a.out[a.step * a.nrecv + r] = s;
+}
+
+struct DiagArgs {
+    const double* u; const double* v;
+    int N, E, j0, pad;
+    double h;
+    const double* tab;      // the problem's table buffer
+    const double* W2d;      // (nq, nq)
+    const int* cls;         // class of (c^2, beta) at the standard points per element
+    const double* c2;       // per class (nq, nq)
+    const double* bq;
+    const double* eta0;     // P_v row at the slice's local coordinate
+    double* E_out; double* vmax_out; double* nd_out;   // per element, or all null
+    double* slice;          // [k][i] along element row j0, or null
+    double* grid;           // v_h at every point, (N nq) x (N nq) column-major with
+                            // row j nq + p (y) and column k nq + q (x), or null
+};
+
+// The element values of the energy diagnostics (src/solver.jl:870-889): the
+// Westervelt energy (h^2/4) sum W (1/2 c^2 |grad u_h|^2 + (1/2 - 2/3 beta v_h) v_h^2),
+// max |v_h| and min (1 - 2 beta v_h) at the standard points, and on element row
+// j0 the slice P_v (eta0^T V)^T (src/solver.jl:893-896), and the snapshot grid
+// P_v V P_v^T of every element (src/solver.jl:919-923).
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_diag(const DiagArgs a) {
+    using D = Dims<NU, NV>;
+    constexpr int NQ = D::NQ;
+    const int e = blockIdx.x * kWave + threadIdx.x;
+    if (e >= a.E) return;
+    const double* P = a.tab + D::oP;   // P_u (NQ x NU); P_v = its first NV columns
+    const double* S = a.tab + D::oS;
+    double U[NU * NU], V[NV * NV];
+#pragma unroll
+    for (int m = 0; m < NU * NU; ++m) U[m] = a.u[(long)m * a.E + e];
+#pragma unroll
+    for (int m = 0; m < NV * NV; ++m) V[m] = a.v[(long)m * a.E + e];
+    const long cc = a.cls[e];
+    const double* c2 = a.c2 + cc * NQ * NQ;
+    const double* bq = a.bq + cc * NQ * NQ;
+    const double two_h = 2.0 / a.h;
+    const int N = a.N, j = e % N, k = e / N;
+    const long ld = (long)N * NQ;
+    double Esum = 0.0, vmax = 0.0, nd = 1.0 / 0.0;
+#pragma unroll 1
+    for (int p = 0; p < NQ; ++p) {
+        // row p of P U S^T, S U P^T (the gradient) and P V P^T
+        double tpu[NU], tsu[NU], tpv[NV];
+#pragma unroll
+        for (int b = 0; b < NU; ++b) {
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < NU; ++aa) {
+                s1 += P[p + NQ * aa] * U[aa + NU * b];
+                s2 += S[p + NQ * aa] * U[aa + NU * b];
+            }
+            tpu[b] = s1;
+            tsu[b] = s2;
+        }
+#pragma unroll
+        for (int b = 0; b < NV; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < NV; ++aa) s += P[p + NQ * aa] * V[aa + NV * b];
+            tpv[b] = s;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double ux = 0.0, uy = 0.0, vh = 0.0;
+#pragma unroll
+            for (int b = 0; b < NU; ++b) {
+                ux += tpu[b] * S[q + NQ * b];
+                uy += tsu[b] * P[q + NQ * b];
+            }
+#pragma unroll
+            for (int b = 0; b < NV; ++b) vh += tpv[b] * P[q + NQ * b];
+            ux = two_h * ux;
+            uy = two_h * uy;
+            const double cq = c2[p + NQ * q], bb = bq[p + NQ * q];
+            const double z = a.W2d[p + NQ * q] * (0.5 * cq * (ux * ux + uy * uy) + (0.5 - (2.0 / 3.0) * bb * vh) * (vh * vh));
+            Esum += z;
+            vmax = fmax(vmax, fabs(vh));
+            nd = fmin(nd, 1.0 - 2.0 * bb * vh);
+            if (a.grid) a.grid[(long)(j * NQ + p) + ld * (k * NQ + q)] = vh;
+        }
+    }
+    if (a.E_out) {
+        a.E_out[e] = (a.h * a.h / 4.0) * Esum;
+        a.vmax_out[e] = vmax;
+        a.nd_out[e] = nd;
+    }
+    if (a.slice && j == a.j0) {
+        double t[NV];
+#pragma unroll
+        for (int b = 0; b < NV; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < NV; ++aa) s += a.eta0[aa] * V[aa + NV * b];
+            t[b] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NV; ++b) s += P[i + NQ * b] * t[b];
+            a.slice[(long)k * NQ + i] = s;
+        }
+    }
+}
+
+}  // namespace wedg

@@ -1,8 +1,9 @@
 """The reference's hot-path API on the device: ``Workspace``, ``edg_rhs_``
 (``edg_rhs!``), ``edg_rhs``, ``rk4_step_`` (``rk4_step!``), ``rk4_step``,
-``solve`` and ``difference_norms``.
+``solve`` (with the source-mode time series of ``_run_source``) and
+``difference_norms``.
 
-Mirrors src/rhs.jl:166-220, 475-571 and src/solver.jl:88-339, 772-836 of the
+Mirrors src/rhs.jl:166-220, 475-571 and src/solver.jl:88-339, 772-958 of the
 reference (names, argument meaning and error behaviour). Julia's mutating
 ``f!`` is spelled ``f_`` here. Every right-hand side and RK4 step runs in the
 HIP kernels of libwedg.so; the state and work arrays stay on the device during
@@ -19,11 +20,13 @@ import numpy as np
 from . import _lib
 from ._lib import DimensionMismatch, WedgError, lib
 from .precompute import GaussianSource, precompute
+from .quadrature import JacobiP
 
 __all__ = ["Workspace", "edg_rhs_", "edg_rhs", "rk4_step_", "rk4_step", "solve", "difference_norms",
            "DimensionMismatch", "WedgError"]
 
 _dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
 
 
 def _ptr(a):
@@ -134,20 +137,21 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
           sigma: float = 0.0, boundary="absorbing_2beta", sigma_dirichlet=None, kappa: float = 0.0,
           phase=(0.0, 0.0), coefficients=None, mode: str = "mms", kx: int = 1, ky: int = 1,
           dt_fac=10, mass_solver: str = "chebyshev", nblocks=None, initial_profile=None,
-          return_state: bool = False, source: GaussianSource = None, cheb: str = "global"):
+          return_state: bool = False, source: GaussianSource = None, cheb: str = "global",
+          record_every: int = 20, slice_y: float = 0.5,
+          receivers=((0.45, 0.5), (0.7, 0.5), (0.85, 0.5)), snap_times=()):
     """Integrate the semi-discrete scheme to time T on the device (src/solver.jl:167-228).
 
     Returns the errors (erru, errv, errgradu, errE) at T in :mms mode, or the
     final state with the quadrature operators (``return_state=True``). The
     time loop runs ``nsteps`` RK4 steps on the device-resident state with
-    t += dt after every step, as the reference's loop.
+    t += dt after every step, as the reference's loop. :source mode returns
+    the time series of ``_run_source`` (src/solver.jl:872-958) whatever
+    ``return_state`` says, as the reference does.
     """
     mode = str(mode).lstrip(":")
     if mode not in ("mms", "initial_value", "source"):
         raise ValueError(f"unknown mode {mode} (expected :mms, :initial_value or :source)")
-    if mode == "source" and not return_state:
-        raise NotImplementedError("source-mode time series are not produced by this build; "
-                                  "use return_state=True for the final state")
     pre, geo = precompute(q, N, mixed=mixed, flux=flux, flux_params=flux_params, sigma=sigma,
                           boundary=boundary, sigma_dirichlet=sigma_dirichlet, kappa=kappa, phase=phase,
                           coefficients=coefficients, mode=mode, kx=kx, ky=ky, source=source,
@@ -159,8 +163,11 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
     v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
     st = lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v))
     ws.dev.check(st)
+    if mode == "source":
+        return _run_source(ws, dt, nsteps, N, h, pre, geo, record_every=record_every, slice_y=slice_y,
+                           receivers=receivers, snap_times=snap_times)
     tend = C.c_double(0.0)
-    st = lib.wedg_advance(ws.dev.h, 0.0, float(dt), int(nsteps), int(mode == "source"), C.byref(tend))
+    st = lib.wedg_advance(ws.dev.h, 0.0, float(dt), int(nsteps), 0, C.byref(tend))
     ws.dev.check(st)
     st = lib.wedg_get_state(ws.dev.h, _ptr(u), _ptr(v))
     ws.dev.check(st)
@@ -170,6 +177,123 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
     if mode != "mms":
         raise ValueError("errors are only available in :mms mode; use return_state = true")
     return mms_errors(u, v, T, N, h, pre, geo, kx, ky, phase)
+
+
+def locate(s: float, N: int):
+    """Element index (1-based) and local coordinate in [-1, 1] of the physical
+    coordinate s on an N-element line (src/solver.jl:880)."""
+    i = min(max(int(math.ceil(s * N)), 1), N)
+    return i, min(max(2 * (s * N - (i - 1)) - 1, -1.0), 1.0)
+
+
+def vrow(r: float, qv1: int):
+    """The modal basis of v at the local coordinate r, P_{d-1}(r) for d = 1..q_v+1
+    (src/solver.jl:881)."""
+    return np.array([JacobiP(float(r), 0.0, 0.0, d) for d in range(qv1)])
+
+
+def _events(dt: float, nsteps: int, record_every: int, snap_times):
+    """The steps (1-based) after which _run_source records diagnostics or takes a
+    snapshot, with t after each step accumulated as t += dt (src/solver.jl:936-951):
+    at most one snapshot per step, the earliest open snapshot time at or before t."""
+    left = sorted(float(x) for x in snap_times)
+    rec, snap, t = [], [], 0.0
+    for it in range(1, nsteps + 1):
+        t += dt
+        if it % record_every == 0 or it == nsteps:
+            rec.append(it)
+        if left and t >= left[0]:
+            snap.append(it)
+            left.pop(0)
+    return rec, snap
+
+
+def _run_source(ws: Workspace, dt: float, nsteps: int, N: int, h: float, pre, geo, *, record_every: int,
+                slice_y: float, receivers, snap_times):
+    """Source mode (src/solver.jl:872-958): RK4 on the device with v_h at the
+    receivers after every step, the energy diagnostics and the slice along
+    y = slice_y every ``record_every`` steps and after the last, and snapshots of
+    v_h on the point grid at the first step with t >= each snapshot time.
+
+    The receivers, the element values of the diagnostics, the slice and the
+    snapshot grids are formed on the device (``wedg_advance_receivers``,
+    ``wedg_diagnostics``, ``wedg_snapshot``); the host accumulates the element
+    values serially in the order row j, column k, as the reference does."""
+    if record_every < 1:
+        raise ValueError(f"record_every must be ≥ 1 (got {record_every})")
+    qu1, qv1 = pre.qu1, pre.qv1
+    dh = ws.dev.h
+    c2 = np.asfortranarray(geo.err_c2, dtype=np.float64)
+    bq = np.asfortranarray(geo.err_beta, dtype=np.float64)
+    W2d = np.asfortranarray(geo.W2d, dtype=np.float64)
+    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _ptr(c2), _ptr(bq), _ptr(W2d)))
+
+    j0, eta0 = locate(slice_y, N)
+    Pveta0 = np.ascontiguousarray(vrow(eta0, qv1))
+    slice_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
+    receivers = [tuple(map(float, r)) for r in receivers]
+    nr = len(receivers)
+    rj = np.zeros(max(nr, 1), dtype=np.int32)
+    rk = np.zeros(max(nr, 1), dtype=np.int32)
+    rows = np.zeros((max(nr, 1), 2 * qv1))
+    for i, (xr, yr) in enumerate(receivers):
+        kr, xir = locate(xr, N)
+        jr, etar = locate(yr, N)
+        rj[i], rk[i] = jr - 1, kr - 1
+        rows[i, :qv1] = vrow(etar, qv1)
+        rows[i, qv1:] = vrow(xir, qv1)
+    rows = np.ascontiguousarray(rows)
+
+    rec_steps, snap_steps = _events(dt, nsteps, record_every, snap_times)
+    stops = sorted(set(rec_steps) | set(snap_steps))
+    rec_set, snap_set = set(rec_steps), set(snap_steps)
+    E_el = np.zeros(N * N); vmax_el = np.zeros(N * N); nd_el = np.zeros(N * N)
+    sl = np.zeros(N * qu1)
+    order = (np.arange(N)[:, None] + N * np.arange(N)[None, :]).ravel()   # e for j outer, k inner
+    rec_t, rec_slice, rec_E, rec_vmax, rec_nd = [], [], [], [], []
+    recv_all = np.zeros((nsteps, nr))
+    snaps = []
+    t = 0.0
+    done = 0
+    tend = C.c_double(0.0)
+    for stop in stops:
+        n = stop - done
+        buf = np.zeros(max(n * nr, 1))
+        st = lib.wedg_advance_receivers(dh, t, float(dt), int(n), nr, rj.ctypes.data_as(_ip),
+                                        rk.ctypes.data_as(_ip), _ptr(rows), 1, _ptr(buf), C.byref(tend))
+        if st == _lib.ERR_NONFINITE:
+            tb = t
+            for _ in range(ws.dev.last_error().step):
+                tb += dt
+            raise WedgError(f"blow-up at t={tb!r}")
+        ws.dev.check(st)
+        t = tend.value
+        recv_all[done:stop] = buf[:n * nr].reshape(n, nr)
+        done = stop
+        if stop in rec_set:
+            ws.dev.check(lib.wedg_diagnostics(dh, _ptr(E_el), _ptr(vmax_el), _ptr(nd_el), j0 - 1,
+                                              _ptr(Pveta0), _ptr(sl)))
+            rec_t.append(t)
+            rec_slice.append(sl.copy())
+            rec_E.append(float(np.add.accumulate(np.concatenate(([0.0], E_el[order])))[-1]))
+            rec_vmax.append(float(max(0.0, vmax_el.max())))
+            rec_nd.append(float(min(np.inf, nd_el.min())))
+        if stop in snap_set:
+            G = np.zeros((N * qu1, N * qu1), order="F")
+            ws.dev.check(lib.wedg_snapshot(dh, _ptr(G)))
+            snaps.append((t, G))
+    recv_t = np.zeros(nsteps)
+    tt = 0.0
+    for i in range(nsteps):   # t after each step, accumulated as the time loop does
+        tt += dt
+        recv_t[i] = tt
+    grid_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
+    grid_y = np.concatenate([geo.yloc[j, :] for j in range(N)])
+    return SimpleNamespace(t=np.array(rec_t), slice_x=slice_x,
+                           slices=np.array(rec_slice).T if rec_slice else np.zeros((N * qu1, 0)),
+                           energy=np.array(rec_E), vmax=np.array(rec_vmax), ndmin=np.array(rec_nd),
+                           recv_t=recv_t, recv=recv_all, receivers=receivers, snaps=snaps,
+                           grid_x=grid_x, grid_y=grid_y, dt=dt)
 
 
 def mms_errors(u, v, T, N, h, pre, geo, kx=1, ky=1, phase=(0.0, 0.0)):

@@ -138,10 +138,44 @@ wedg_status wedg_state_device(wedg_workspace* ws, double** u, double** v);
 
 wedg_status wedg_last_error(const wedg_workspace* ws, wedg_error_info* info);
 
+/* Source mode (src/solver.jl:872-958). */
+
+/* The coefficients of the energy diagnostics: geo.err_c2 and geo.err_beta,
+ * (N, N, q_u+1, q_u+1) as built by the reference's geometry (src/solver.jl:224),
+ * and the 2D weights geo.W2d ((q_u+1)^2). Called once per problem before
+ * wedg_diagnostics. */
+wedg_status wedg_problem_set_diag(wedg_problem* p, const double* err_c2, const double* err_beta,
+                                  const double* W2d);
+
+/* wedg_advance with the receiver time series of _run_source (src/solver.jl:899-905,
+ * 938-941): after every step, v_h at each receiver, eta_r^T V[j_r, k_r] xi_r, into
+ * recv[step * nrecv + r] (nsteps x nrecv). recv_j/recv_k are the 0-based element
+ * row/column of locate(y_r)/locate(x_r) and rows holds per receiver the modal rows
+ * vrow(eta_r) then vrow(xi_r), 2 (q_v+1) values. */
+wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, double dt, long nsteps, int nrecv,
+                                   const int* recv_j, const int* recv_k, const double* rows,
+                                   int stop_on_nonfinite, double* recv, double* t_end);
+
+/* The record-step diagnostics of the current state (src/solver.jl:856-889, 893-896,
+ * 907-918): per element (e = j + N k) the energy (h^2/4) sum W2d (1/2 c^2 |grad u_h|^2
+ * + (1/2 - 2/3 beta v_h) v_h^2), max |v_h| and min (1 - 2 beta v_h) at the points,
+ * into E, vmax, ndmin (N^2 each, for the caller's serial accumulation in row-j,
+ * column-k order), and the slice P_v (eta0^T V[j0, k])^T for k = 0..N-1 into slice
+ * (N (q_u+1) values). eta0 is vrow(eta) of locate(slice_y), q_v+1 values. */
+wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* vmax, double* ndmin, int j0,
+                             const double* eta0, double* slice);
+
+/* The snapshot vgrid of _run_source (src/solver.jl:919-923): P_v V[j, k] P_v^T of
+ * every element into grid, (N (q_u+1)) x (N (q_u+1)) column-major, row j (q_u+1) + p
+ * (the y points), column k (q_u+1) + q (the x points). */
+wedg_status wedg_snapshot(wedg_workspace* ws, double* grid);
+
 /* Kernel timing with HIP events on the workspace stream over the next run(s):
  * enable, then read the accumulated per-kernel-family milliseconds and counts.
- * family: 0 = the traces kernel, 1 = the element kernel (volume terms, source, faces,
- * lifts, u-solve, mass solve and the RK4 update); 2 and 3 are not used. */
+ * family: 0 = k_prep (Chebyshev count and boundary slots of the first stage of a call),
+ * 1 = k_stage (one RK4 stage: traces, faces, volume terms, source, lifts, u-solve,
+ * mass solve, the RK4 update and the next stage's count and boundary slots);
+ * 2 and 3 are not used. */
 wedg_status wedg_timing_enable(wedg_workspace* ws, int on);
 wedg_status wedg_timing_read(wedg_workspace* ws, int family, double* ms, long* launches);
 

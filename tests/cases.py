@@ -94,3 +94,42 @@ BUMP = dict(
     u0_xx=lambda x, y: _bump_g(_bump_s(x, y), 2) * _bump_d(x, y, 0) ** 2 + _bump_g(_bump_s(x, y), 1) * 2 / BUMP_R ** 2,
     u0_yy=lambda x, y: _bump_g(_bump_s(x, y), 2) * _bump_d(x, y, 1) ** 2 + _bump_g(_bump_s(x, y), 1) * 2 / BUMP_R ** 2,
 )
+
+
+# The wave-steepening study of experiments/steepening/run.jl:20-34: q_u = 4,
+# q_v = 3, N = 64, T = 1.8, upwind, c = 1, alpha = 1e-6, zero initial data, the
+# Gaussian source at (0.15, 0.5) with r0 = 0.05, omega = 16 pi, T_s = 1,
+# record_every = 32, receivers (0.45, 0.5), (0.7, 0.5), (0.85, 0.5), snapshots
+# at t = 0.5, 0.8, 1.1; case -> (beta, boundary).
+STEEPENING = {"nl": (8.0, "absorbing_2beta"), "lin": (0.0, "absorbing_2beta"), "nl_dbc": (8.0, "dirichlet")}
+STEEPENING_RUN = dict(q=4, N=64, T=1.8, record_every=32, receivers=((0.45, 0.5), (0.7, 0.5), (0.85, 0.5)),
+                      snap_times=(0.5, 0.8, 1.1))
+
+
+def _rows(name):
+    with open(_os.path.join(_GOLDEN, name)) as f:
+        return [[float(x) for x in line.split(",")] for line in f.read().splitlines() if line]
+
+
+def steepening_series(case):
+    """(t, energy, max|v_h|, min(1 - 2 beta v_h)) rows of data/steepening/<case>/series.csv."""
+    return _rows(f"steepening_{case}_series.csv")
+
+
+def steepening_recv(case):
+    """(step, t, v_1, v_2, v_3) rows: every 64th step of data/steepening/<case>/recv.csv."""
+    return _rows(f"steepening_{case}_recv.csv")
+
+
+def steepening_grid():
+    """(grid_x, grid_y) rows of data/steepening/nl/grid.csv."""
+    return _rows("steepening_nl_grid.csv")
+
+
+def steepening_snapshot():
+    """(rows, cols, values) of the every-8th subsample of data/steepening/nl/snap_0.80.csv."""
+    with open(_os.path.join(_GOLDEN, "steepening_nl_snap_0.80.csv")) as f:
+        lines = f.read().splitlines()
+    cols = [int(c) for c in lines[0].split(",")[1:]]
+    r = [[float(x) for x in line.split(",")] for line in lines[1:] if line]
+    return [int(x[0]) for x in r], cols, [x[1:] for x in r]

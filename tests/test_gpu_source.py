@@ -1,0 +1,151 @@
+"""Source mode on the device (src/solver.jl:856-958): the diagnostics kernels
+against the oracle on the same state, a short source-mode run against the
+oracle's loop, and the three wave-steepening runs of
+experiments/steepening/run.jl at full length against the reference's series,
+receiver and snapshot data (data/steepening/*).
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import edg_westervelt_amd as w
+from edg_westervelt_amd import _lib, precompute
+from edg_westervelt_amd._lib import lib
+from oracle import batched as ob
+from oracle import diagnostics as od
+from tests.cases import (MIXED, STEEPENING, STEEPENING_RUN, quadrant_coefficients, rand_state,
+                         steepening_grid, steepening_recv, steepening_series, steepening_snapshot)
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+@pytest.mark.parametrize("q,N,mixed", [(3, 6, True), (2, 5, False), (5, 4, True)])
+def test_diagnostics_device(q, N, mixed):
+    """k_diag and k_recv on a random state with piecewise c, beta: the element
+    energies, max |v_h|, min (1 - 2 beta v_h), the slice and the snapshot grid
+    against the oracle's restatement (1e-13 relative), and the receivers over
+    three device RK4 steps against the oracle's steps (1e-12)."""
+    say(f"case diagnostics q={q} N={N} mixed={mixed}")
+    rng = np.random.default_rng(100 + q)
+    co = quadrant_coefficients(0.5, 0.5, c=(1.0, 0.7, 0.8, 0.6), alpha=(1e-3, 1e-3, 2e-3, 2e-3),
+                               beta=(0.9, 0.5, 0.3, 0.7), smooth=True)
+    pre, geo = precompute(q, N, mixed=mixed, mode="source", boundary=MIXED, coefficients=co)
+    u, v = rand_state(rng, N, pre, su=1e-3, sv=5e-3)
+    ws = w.Workspace(pre, N)
+    dh = ws.dev.h
+    ws.dev.check(lib.wedg_set_state(dh, _p(u), _p(v)))
+    c2 = np.asfortranarray(geo.err_c2); bq = np.asfortranarray(geo.err_beta); W2d = np.asfortranarray(geo.W2d)
+    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _p(c2), _p(bq), _p(W2d)))
+    E = np.zeros(N * N); vm = np.zeros(N * N); nd = np.zeros(N * N); sl = np.zeros(N * pre.qu1)
+    for slice_y in (0.55, 0.0, 1.0):
+        j0, eta0 = od.locate(slice_y, N)
+        r0 = od.vrow(eta0, pre.qv1)
+        ws.dev.check(lib.wedg_diagnostics(dh, _p(E), _p(vm), _p(nd), j0 - 1, _p(r0), _p(sl)))
+        Eo, vmo, ndo = od.energy_elements(u, v, N, 1.0 / N, pre, geo)
+        assert rel(Eo, E) < 1e-13 and rel(vmo, vm) < 1e-13 and rel(ndo, nd) < 1e-13
+        assert rel(od.slice_values(v, N, pre, slice_y), sl) < 1e-13
+    G = np.zeros((N * pre.qu1, N * pre.qu1), order="F")
+    ws.dev.check(lib.wedg_snapshot(dh, _p(G)))
+    assert rel(od.grid(v, N, pre), G) < 1e-13
+    rc = [(0.45, 0.5), (0.05, 0.93), (1.0, 0.0), (0.71, 0.28)]
+    rj = np.zeros(len(rc), dtype=np.int32); rk = np.zeros(len(rc), dtype=np.int32)
+    rows = np.zeros((len(rc), 2 * pre.qv1))
+    for i, (x, y) in enumerate(rc):
+        kr, xi = od.locate(x, N); jr, eta = od.locate(y, N)
+        rj[i], rk[i] = jr - 1, kr - 1
+        rows[i] = np.concatenate([od.vrow(eta, pre.qv1), od.vrow(xi, pre.qv1)])
+    dt = 1e-4
+    rec = np.zeros(3 * len(rc))
+    tend = C.c_double(0.0)
+    ws.dev.check(lib.wedg_advance_receivers(dh, 0.2, dt, 3, len(rc), rj.ctypes.data_as(_ip), rk.ctypes.data_as(_ip),
+                                            _p(rows), 1, _p(rec), C.byref(tend)))
+    op = ob.OraclePre(pre, N)
+    uo, vo, t = u, v, 0.2
+    for s in range(3):
+        uo, vo, _, _ = ob.rk4_steps(op, uo, vo, t, dt, 1)
+        t += dt
+        assert rel(od.receiver_values(vo, N, pre, rc), rec[s * len(rc):(s + 1) * len(rc)]) < 1e-12, s
+    assert tend.value == t
+    ud = np.zeros_like(u); vd = np.zeros_like(v)
+    ws.dev.check(lib.wedg_get_state(dh, _p(ud), _p(vd)))
+    assert rel(uo, ud) < 1e-12 and rel(vo, vd) < 1e-12
+
+
+def test_source_mode_short_run():
+    """solve(mode = :source) against the oracle's restatement of the loop
+    (q = 2, N = 8, T = 0.05: 16 steps, record_every = 5, two snapshot times
+    inside one step): every series 1e-12 relative."""
+    say("case short source run")
+    q, N, T = 2, 8, 0.05
+    kw = dict(mixed=True, mode="source", boundary="absorbing_2beta",
+              coefficients=w.constant_coefficients(c=1.0, alpha=1e-3, beta=4.0),
+              source=w.GaussianSource(x0=(0.3, 0.55), r0=0.1, amplitude=2.0))
+    run = dict(record_every=5, slice_y=0.3, receivers=[(0.45, 0.5), (0.3, 0.55), (0.99, 0.01)],
+               snap_times=(0.011, 0.01, 0.03))
+    H = w.solve(q, N, T, **kw, **run)
+    pre, geo = precompute(q, N, **kw)
+    dt, nsteps = w.time_step(q, N, T)
+    O = od.run_source(pre, geo, N, dt, nsteps, **run)
+    assert list(H.t) == O["t"] and list(H.recv_t) == O["recv_t"]
+    assert rel(np.array(O["recv"]), H.recv) < 1e-12
+    assert rel(np.array(O["slices"]).T, H.slices) < 1e-12
+    for k in ("energy", "vmax", "ndmin"):
+        assert rel(np.array(O[k]), getattr(H, k)) < 1e-12, k
+    assert [s[0] for s in H.snaps] == [s[0] for s in O["snaps"]] and len(H.snaps) == 3
+    for (_, a), (_, b) in zip(O["snaps"], H.snaps):
+        assert rel(a, b) < 1e-12
+    assert H.dt == dt and H.receivers == [tuple(r) for r in run["receivers"]]
+
+
+@pytest.mark.parametrize("case", ["nl", "lin", "nl_dbc"])
+def test_steepening(case):
+    """experiments/steepening/run.jl on the device, all 18432 steps, against
+    data/steepening/<case>: the time axis exactly; the energy, max |v_h| and
+    min (1 - 2 beta v_h) of all 576 records, 1e-9 relative; v_h at the three
+    receivers every 64th step, 1e-8 of the series' largest |v_h|; for nl the
+    snapshot at t = 0.80 (every 8th point), 1e-8 of its largest |v_h|, and the
+    grid coordinates."""
+    beta, bnd = STEEPENING[case]
+    R = STEEPENING_RUN
+    say(f"case steepening {case}")
+    H = w.solve(R["q"], R["N"], R["T"], flux="upwind", mixed=True, mode="source", boundary=bnd,
+                coefficients=w.constant_coefficients(c=1.0, alpha=1e-6, beta=beta),
+                source=w.GaussianSource(x0=(0.15, 0.5), r0=0.05, omega=16 * math.pi, Ts=1.0),
+                record_every=R["record_every"], receivers=list(R["receivers"]), snap_times=list(R["snap_times"]))
+    ser = np.array(steepening_series(case))
+    assert np.array_equal(H.t, ser[:, 0])
+    dev = {k: float(np.max(np.abs(getattr(H, k) / ser[:, i] - 1))) for i, k in ((1, "energy"), (2, "vmax"), (3, "ndmin"))}
+    rv = np.array(steepening_recv(case))
+    steps = rv[:, 0].astype(int) - 1
+    assert np.array_equal(H.recv_t[steps], rv[:, 1])
+    dev["recv"] = float(np.abs(H.recv[steps] - rv[:, 2:]).max() / np.abs(rv[:, 2:]).max())
+    if case == "nl":
+        rows, cols, vals = steepening_snapshot()
+        snap = [G for (t, G) in H.snaps if f"{t:.2f}" == "0.80"]
+        assert len(snap) == 1 and len(H.snaps) == 3
+        G = snap[0][np.ix_(rows, cols)]
+        vals = np.array(vals)
+        dev["snap"] = float(np.abs(G - vals).max() / np.abs(vals).max())
+        g = np.array(steepening_grid())
+        assert np.abs(H.grid_x - g[:, 0]).max() < 1e-16 and np.abs(H.grid_y - g[:, 1]).max() < 1e-16
+    say(f"steepening {case}: " + " ".join(f"{k} {x:.2e}" for k, x in dev.items()))
+    for k in ("energy", "vmax", "ndmin"):
+        assert dev[k] < 1e-9, (k, dev[k])
+    assert dev["recv"] < 1e-8
+    if case == "nl":
+        assert dev["snap"] < 1e-8
