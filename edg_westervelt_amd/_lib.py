@@ -83,6 +83,8 @@ def _load():
                                    C.POINTER(C.c_double)],
         "wedg_diagnostics": [vp, _dp, _dp, _dp, C.c_int, _dp, _dp],
         "wedg_snapshot": [vp, _dp],
+        "wedg_mms_errors": [vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp],
+        "wedg_difference_norms": [vp, _dp, _dp, _dp, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -226,6 +226,7 @@ struct wedg_workspace {
     double* ddiag = nullptr; double* hdiag = nullptr;   // E, vmax, ndmin [3][E], slice [N nq], eta0 [nv]
     int* drecv_elem = nullptr; double* drecv_rows = nullptr; int nrecv_dev = 0;
     void* hrows = nullptr;   // pinned staging of the receiver rows and elements
+    double* dnorm = nullptr; double* hnorm = nullptr;   // error norms: [4][E] sums, then the 1D tables
     int serial = 0;
     long step_base = 0;
     long stages = 0;                 // stages run since the last reset
@@ -560,6 +561,7 @@ extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     if (ws->hrecv) (void)hipHostFree(ws->hrecv);
     if (ws->hdiag) (void)hipHostFree(ws->hdiag);
     if (ws->hrows) (void)hipHostFree(ws->hrows);
+    if (ws->hnorm) (void)hipHostFree(ws->hnorm);
     if (ws->drecv) (void)hipFree(ws->drecv);
     if (ws->stream) (void)hipStreamDestroy(ws->stream);
     delete ws;
@@ -907,14 +909,15 @@ extern "C" wedg_status wedg_device_info(char* name, size_t len, int* cu_count, d
 // ---------------------------------------------------------------------------
 // Source-mode diagnostics (src/solver.jl:856-958).
 
-static hipError_t launch_diag(wedg_problem* p, const DiagArgs* d, const RecvArgs* r, hipStream_t s) {
+static hipError_t launch_diag(wedg_problem* p, const DiagArgs* d, const RecvArgs* r, const NormArgs* n,
+                              hipStream_t s) {
     switch (p->qu) {
-        case 1: return launch_diag_q1(d, r, p->nv, s);
-        case 2: return launch_diag_q2(d, r, p->nv, s);
-        case 3: return launch_diag_q3(d, r, p->nv, s);
-        case 4: return launch_diag_q4(d, r, p->nv, s);
-        case 5: return launch_diag_q5(d, r, p->nv, s);
-        case 6: return launch_diag_q6(d, r, p->nv, s);
+        case 1: return launch_diag_q1(d, r, n, p->nv, s);
+        case 2: return launch_diag_q2(d, r, n, p->nv, s);
+        case 3: return launch_diag_q3(d, r, n, p->nv, s);
+        case 4: return launch_diag_q4(d, r, n, p->nv, s);
+        case 5: return launch_diag_q5(d, r, n, p->nv, s);
+        case 6: return launch_diag_q6(d, r, n, p->nv, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -978,6 +981,7 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
     const size_t rbytes = (size_t)nrecv * (sizeof(int) + 2 * nv * sizeof(double));
     if (nrecv > ws->nrecv_dev) {   // receiver basis rows, then elements, on the device
         if (ws->hrows) (void)hipHostFree(ws->hrows);
+    if (ws->hnorm) (void)hipHostFree(ws->hnorm);
         ws->hrows = nullptr;
         ws->nrecv_dev = 0;
         void* d = nullptr;
@@ -1007,7 +1011,7 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
         t += dt;
         if (nrecv > 0) {
             ra.step = it;
-            HIPOK(launch_diag(p, nullptr, &ra, ws->stream));
+            HIPOK(launch_diag(p, nullptr, &ra, nullptr, ws->stream));
         }
     }
     if (t_end) *t_end = t;
@@ -1045,7 +1049,7 @@ extern "C" wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* v
     d.tab = p->tab; d.W2d = p->dW2d; d.cls = p->dcls; d.c2 = p->dc2; d.bq = p->dbq; d.eta0 = deta;
     d.E_out = ws->ddiag; d.vmax_out = ws->ddiag + p->E; d.nd_out = ws->ddiag + 2 * (size_t)p->E; d.slice = dslice;
     d.grid = nullptr;
-    HIPOK(launch_diag(p, &d, nullptr, ws->stream));
+    HIPOK(launch_diag(p, &d, nullptr, nullptr, ws->stream));
     HIPOK(hipMemcpyAsync(ws->hdiag, ws->ddiag, (3 * (size_t)p->E + (size_t)N * nq) * sizeof(double),
                          hipMemcpyDeviceToHost, ws->stream));
     HIPOK(hipStreamSynchronize(ws->stream));
@@ -1069,9 +1073,83 @@ extern "C" wedg_status wedg_snapshot(wedg_workspace* ws, double* grid) {
     d.u = ws->yu; d.v = ws->yv; d.N = p->N; d.E = p->E; d.j0 = 0; d.h = 1.0 / p->N;
     d.tab = p->tab; d.W2d = p->dW2d; d.cls = p->dcls; d.c2 = p->dc2; d.bq = p->dbq; d.eta0 = nullptr;
     d.grid = dg;
-    HIPOK(launch_diag(p, &d, nullptr, ws->stream));
+    HIPOK(launch_diag(p, &d, nullptr, nullptr, ws->stream));
     HIPOK(hipMemcpyAsync(ws->hu, dg, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
     HIPOK(hipStreamSynchronize(ws->stream));
     std::memcpy(grid, ws->hu, n * sizeof(double));
     return WEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Error norms (src/solver.jl:774-836): per-element quadrature sums on the device.
+
+static wedg_status norms_buffers(wedg_workspace* ws) {
+    wedg_problem* p = ws->p;
+    if (ws->dnorm) return WEDG_OK;
+    const size_t n = 4 * (size_t)p->E + 4 * (size_t)p->N * p->nu;
+    void* d = nullptr;
+    HIPOK(hipMalloc(&d, n * sizeof(double)));
+    ws->allocs.push_back(d);
+    ws->dnorm = (double*)d;
+    HIPOK(hipHostMalloc((void**)&ws->hnorm, n * sizeof(double), hipHostMallocDefault));
+    return WEDG_OK;
+}
+
+static NormArgs norm_args(wedg_workspace* ws) {
+    wedg_problem* p = ws->p;
+    NormArgs a{};
+    a.N = p->N; a.E = p->E; a.h = 1.0 / p->N;
+    a.tab = p->tab; a.W2d = p->dW2d; a.cls = p->dcls; a.c2 = p->dc2;
+    a.out = ws->dnorm;
+    return a;
+}
+
+static wedg_status norms_out(wedg_workspace* ws, double* elem) {
+    const size_t n = 4 * (size_t)ws->p->E;
+    HIPOK(hipMemcpyAsync(ws->hnorm, ws->dnorm, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(elem, ws->hnorm, n * sizeof(double));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_mms_errors(wedg_workspace* ws, double cu, double cv, double tpx, double tpy,
+                                       const double* cx, const double* sx, const double* cy, const double* sy,
+                                       double* elem) {
+    if (!ws || !cx || !sx || !cy || !sy || !elem) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    if (!p->dcls) return WEDG_ERR_ARGUMENT;   // wedg_problem_set_diag first
+    wedg_status s;
+    HIPOK(hipStreamSynchronize(ws->stream));
+    if ((s = norms_buffers(ws))) return s;
+    const size_t m = (size_t)p->N * p->nu;
+    double* dt = ws->dnorm + 4 * (size_t)p->E;
+    std::memcpy(ws->hnorm, cx, m * sizeof(double));
+    std::memcpy(ws->hnorm + m, sx, m * sizeof(double));
+    std::memcpy(ws->hnorm + 2 * m, cy, m * sizeof(double));
+    std::memcpy(ws->hnorm + 3 * m, sy, m * sizeof(double));
+    HIPOK(hipMemcpyAsync(dt, ws->hnorm, 4 * m * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+    NormArgs a = norm_args(ws);
+    a.u = ws->yu; a.v = ws->yv; a.ub = nullptr; a.vb = nullptr;
+    a.cu = cu; a.cv = cv; a.tpx = tpx; a.tpy = tpy;
+    a.cx = dt; a.sx = dt + m; a.cy = dt + 2 * m; a.sy = dt + 3 * m;
+    HIPOK(launch_diag(p, nullptr, nullptr, &a, ws->stream));
+    return norms_out(ws, elem);
+}
+
+extern "C" wedg_status wedg_difference_norms(wedg_workspace* ws, const double* uA, const double* vA,
+                                             const double* uB, const double* vB, double* elem) {
+    if (!ws || !uA || !vA || !uB || !vB || !elem) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    if (!p->dcls) return WEDG_ERR_ARGUMENT;
+    wedg_status s;
+    if ((s = norms_buffers(ws))) return s;
+    // the two states go to the stage buffers, which hold nothing between calls
+    if ((s = stage_in(ws, ws->su, ws->hu, uA, ws->nU()))) return s;
+    if ((s = stage_in(ws, ws->sv, ws->hv, vA, ws->nV()))) return s;
+    if ((s = stage_in(ws, ws->ou, ws->hu, uB, ws->nU()))) return s;
+    if ((s = stage_in(ws, ws->ov, ws->hv, vB, ws->nV()))) return s;
+    NormArgs a = norm_args(ws);
+    a.u = ws->su; a.v = ws->sv; a.ub = ws->ou; a.vb = ws->ov;
+    HIPOK(launch_diag(p, nullptr, nullptr, &a, ws->stream));
+    return norms_out(ws, elem);
 }

@@ -24,9 +24,10 @@ static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) 
 }
 
 template <int NU, int NV>
-static hipError_t launch_diag_nv(const DiagArgs* d, const RecvArgs* r, hipStream_t s) {
+static hipError_t launch_diag_nv(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, hipStream_t s) {
     if (d) k_diag<NU, NV><<<(d->E + kWave - 1) / kWave, kWave, 0, s>>>(*d);
     if (r) k_recv<NV><<<(r->nrecv + kWave - 1) / kWave, kWave, 0, s>>>(*r);
+    if (n) k_norms<NU, NV><<<(n->E + kWave - 1) / kWave, kWave, 0, s>>>(*n);
     return hipGetLastError();
 }
 

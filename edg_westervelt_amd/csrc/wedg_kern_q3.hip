@@ -11,9 +11,9 @@ hipError_t launch_q3(const KArgs& a, int nv, int which, hipStream_t s, int mode)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_diag_q3(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s) {
-    if (nv == 4) return launch_diag_nv<4, 4>(d, r, s);
-    if (nv == 3) return launch_diag_nv<4, 3>(d, r, s);
+hipError_t launch_diag_q3(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s) {
+    if (nv == 4) return launch_diag_nv<4, 4>(d, r, n, s);
+    if (nv == 3) return launch_diag_nv<4, 3>(d, r, n, s);
     return hipErrorInvalidValue;
 }
 

@@ -11,9 +11,9 @@ hipError_t launch_q4(const KArgs& a, int nv, int which, hipStream_t s, int mode)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_diag_q4(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s) {
-    if (nv == 5) return launch_diag_nv<5, 5>(d, r, s);
-    if (nv == 4) return launch_diag_nv<5, 4>(d, r, s);
+hipError_t launch_diag_q4(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s) {
+    if (nv == 5) return launch_diag_nv<5, 5>(d, r, n, s);
+    if (nv == 4) return launch_diag_nv<5, 4>(d, r, n, s);
     return hipErrorInvalidValue;
 }
 

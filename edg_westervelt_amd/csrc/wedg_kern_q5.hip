@@ -11,9 +11,9 @@ hipError_t launch_q5(const KArgs& a, int nv, int which, hipStream_t s, int mode)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_diag_q5(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s) {
-    if (nv == 6) return launch_diag_nv<6, 6>(d, r, s);
-    if (nv == 5) return launch_diag_nv<6, 5>(d, r, s);
+hipError_t launch_diag_q5(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s) {
+    if (nv == 6) return launch_diag_nv<6, 6>(d, r, n, s);
+    if (nv == 5) return launch_diag_nv<6, 5>(d, r, n, s);
     return hipErrorInvalidValue;
 }
 

@@ -11,9 +11,9 @@ hipError_t launch_q6(const KArgs& a, int nv, int which, hipStream_t s, int mode)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_diag_q6(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s) {
-    if (nv == 7) return launch_diag_nv<7, 7>(d, r, s);
-    if (nv == 6) return launch_diag_nv<7, 6>(d, r, s);
+hipError_t launch_diag_q6(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s) {
+    if (nv == 7) return launch_diag_nv<7, 7>(d, r, n, s);
+    if (nv == 6) return launch_diag_nv<7, 6>(d, r, n, s);
     return hipErrorInvalidValue;
 }
 

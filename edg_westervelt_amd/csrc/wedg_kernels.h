@@ -1301,3 +1301,108 @@ __global__ __launch_bounds__(kWave) void k_diag(const DiagArgs a) {
 }
 
 }  // namespace wedg
+
+namespace wedg {
+
+// ---------------------------------------------------------------------------
+// Error norms (src/solver.jl:774-836): per element the quadrature sums of
+//   MMS:  W (u* - u_h)^2, W (v* - v_h)^2, W |grad u* - grad_h u_h|^2,
+//         W (1/2 c^2 |grad u* - grad_h u_h|^2 + 1/2 (v* - v_h)^2)
+//         against u* = (cos(wT) - 1) Phi, v* = -w sin(wT) Phi with
+//         Phi = cos(2 pi kx x + px) sin(2 pi ky y + py), its x and y derivatives;
+//   DIFF: the same sums of the difference of two states (e_u, e_v) with u* = v* = 0
+//         (erru, errv, errE; the gradient sum is not used).
+struct NormArgs {
+    const double* u; const double* v;      // the state (MMS) or state A (DIFF)
+    const double* ub; const double* vb;    // state B (DIFF), else null
+    int N, E;
+    double h, cu, cv, tpx, tpy;            // cos(wT) - 1, -w sin(wT), 2 pi kx, 2 pi ky
+    const double* tab;
+    const double* W2d;
+    const int* cls; const double* c2;      // classes of c^2 at the points (wedg_problem_set_diag)
+    const double* cx; const double* sx;    // cos, sin(2 pi kx x + px) at the x points [k][q]
+    const double* cy; const double* sy;    // cos, sin(2 pi ky y + py) at the y points [j][p]
+    double* out;                            // [4][E]
+};
+
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_norms(const NormArgs a) {
+    using D = Dims<NU, NV>;
+    constexpr int NQ = D::NQ;
+    const int e = blockIdx.x * kWave + threadIdx.x;
+    if (e >= a.E) return;
+    const bool diff = a.ub != nullptr;
+    const double* P = a.tab + D::oP;
+    const double* S = a.tab + D::oS;
+    double U[NU * NU], V[NV * NV];
+#pragma unroll
+    for (int m = 0; m < NU * NU; ++m) U[m] = a.u[(long)m * a.E + e] - (diff ? a.ub[(long)m * a.E + e] : 0.0);
+#pragma unroll
+    for (int m = 0; m < NV * NV; ++m) V[m] = a.v[(long)m * a.E + e] - (diff ? a.vb[(long)m * a.E + e] : 0.0);
+    const int N = a.N, j = e % N, k = e / N;
+    const double* c2 = a.c2 + (long)a.cls[e] * NQ * NQ;
+    const double two_h = 2.0 / a.h;
+    double su = 0.0, sv = 0.0, sg = 0.0, sE = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < NQ; ++p) {
+        double tpu[NU], tsu[NU], tpv[NV];
+#pragma unroll
+        for (int b = 0; b < NU; ++b) {
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < NU; ++aa) {
+                s1 += P[p + NQ * aa] * U[aa + NU * b];
+                s2 += S[p + NQ * aa] * U[aa + NU * b];
+            }
+            tpu[b] = s1;
+            tsu[b] = s2;
+        }
+#pragma unroll
+        for (int b = 0; b < NV; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < NV; ++aa) s += P[p + NQ * aa] * V[aa + NV * b];
+            tpv[b] = s;
+        }
+        const double syv = diff ? 0.0 : a.sy[(long)j * NQ + p], cyv = diff ? 0.0 : a.cy[(long)j * NQ + p];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double uh = 0.0, ux = 0.0, uy = 0.0, vh = 0.0;
+#pragma unroll
+            for (int b = 0; b < NU; ++b) {
+                uh += tpu[b] * P[q + NQ * b];
+                ux += tpu[b] * S[q + NQ * b];
+                uy += tsu[b] * P[q + NQ * b];
+            }
+#pragma unroll
+            for (int b = 0; b < NV; ++b) vh += tpv[b] * P[q + NQ * b];
+            ux = two_h * ux;
+            uy = two_h * uy;
+            double eu, ev, ex, ey;
+            if (diff) {
+                eu = uh; ev = vh; ex = ux; ey = uy;
+            } else {
+                const double cxv = a.cx[(long)k * NQ + q], sxv = a.sx[(long)k * NQ + q];
+                const double sp = cxv * syv;
+                const double spx = -a.tpx * sxv * syv;
+                const double spy = a.tpy * cxv * cyv;
+                eu = a.cu * sp - uh;
+                ev = a.cv * sp - vh;
+                ex = a.cu * spx - ux;
+                ey = a.cu * spy - uy;
+            }
+            const double w = a.W2d[p + NQ * q];
+            const double g2 = ex * ex + ey * ey;
+            su += w * (eu * eu);
+            sv += w * (ev * ev);
+            sg += w * g2;
+            sE += w * (0.5 * c2[p + NQ * q] * g2 + 0.5 * (ev * ev));
+        }
+    }
+    a.out[e] = su;
+    a.out[(long)a.E + e] = sv;
+    a.out[2L * a.E + e] = sg;
+    a.out[3L * a.E + e] = sE;
+}
+
+}  // namespace wedg

@@ -13,12 +13,12 @@ hipError_t launch_q4(const KArgs& a, int nv, int which, hipStream_t s, int mode)
 hipError_t launch_q5(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 hipError_t launch_q6(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 
-// the diagnostics kernels (either argument may be null)
-hipError_t launch_diag_q1(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
-hipError_t launch_diag_q2(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
-hipError_t launch_diag_q3(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
-hipError_t launch_diag_q4(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
-hipError_t launch_diag_q5(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
-hipError_t launch_diag_q6(const DiagArgs* d, const RecvArgs* r, int nv, hipStream_t s);
+// the diagnostics and norm kernels (any argument may be null)
+hipError_t launch_diag_q1(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
+hipError_t launch_diag_q2(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
+hipError_t launch_diag_q3(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
+hipError_t launch_diag_q4(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
+hipError_t launch_diag_q5(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
+hipError_t launch_diag_q6(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);
 
 }  // namespace wedg

@@ -169,14 +169,52 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
     tend = C.c_double(0.0)
     st = lib.wedg_advance(ws.dev.h, 0.0, float(dt), int(nsteps), 0, C.byref(tend))
     ws.dev.check(st)
-    st = lib.wedg_get_state(ws.dev.h, _ptr(u), _ptr(v))
-    ws.dev.check(st)
     if return_state:
+        st = lib.wedg_get_state(ws.dev.h, _ptr(u), _ptr(v))
+        ws.dev.check(st)
+        _set_diag(ws, geo)
         return SimpleNamespace(u=u, v=v, N=N, h=h, P_u=pre.P_u, P_v=pre.P_v, S_u=geo.S_u, W2d=geo.W2d,
-                               err_c2=geo.err_c2, t=tend.value, dt=dt, nsteps=nsteps)
+                               err_c2=geo.err_c2, t=tend.value, dt=dt, nsteps=nsteps, workspace=ws)
     if mode != "mms":
         raise ValueError("errors are only available in :mms mode; use return_state = true")
-    return mms_errors(u, v, T, N, h, pre, geo, kx, ky, phase)
+    return _device_mms_errors(ws, T, N, h, pre, geo, kx, ky, phase)
+
+
+def _set_diag(ws: Workspace, geo):
+    """The point coefficients of the diagnostics and norms on the device, once per workspace."""
+    if getattr(ws, "_diag_set", False):
+        return
+    c2 = np.asfortranarray(geo.err_c2, dtype=np.float64)
+    bq = np.asfortranarray(geo.err_beta, dtype=np.float64)
+    W2d = np.asfortranarray(geo.W2d, dtype=np.float64)
+    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _ptr(c2), _ptr(bq), _ptr(W2d)))
+    ws._diag_set = True
+
+
+def _serial_sums(elem, N):
+    """Sums of the element values elem[c, e] (e = j + N k) over j outer, k inner,
+    one after the other, as the reference's loops add them (src/solver.jl:781, 825)."""
+    order = (np.arange(N)[:, None] + N * np.arange(N)[None, :]).ravel()
+    return [float(np.add.accumulate(np.concatenate(([0.0], elem[c, order])))[-1]) for c in range(elem.shape[0])]
+
+
+def _device_mms_errors(ws: Workspace, T, N, h, pre, geo, kx=1, ky=1, phase=(0.0, 0.0)):
+    """_mms_errors (src/solver.jl:774-810) of the workspace state on the device."""
+    _set_diag(ws, geo)
+    px, py = phase
+    tpx, tpy = 2 * math.pi * kx, 2 * math.pi * ky
+    cx = np.ascontiguousarray(np.cos(tpx * geo.xloc + px))
+    sx = np.ascontiguousarray(np.sin(tpx * geo.xloc + px))
+    cy = np.ascontiguousarray(np.cos(tpy * geo.yloc + py))
+    sy = np.ascontiguousarray(np.sin(tpy * geo.yloc + py))
+    ct, st = math.cos(pre.omega * T), math.sin(pre.omega * T)
+    elem = np.zeros((4, N * N))
+    ws.dev.check(lib.wedg_mms_errors(ws.dev.h, ct - 1.0, -pre.omega * st, tpx, tpy, _ptr(cx), _ptr(sx), _ptr(cy),
+                                     _ptr(sy), _ptr(elem)))
+    eu, ev, eg, eE = _serial_sums(elem, N)
+    s = h / 2
+    return SimpleNamespace(erru=s * math.sqrt(eu), errv=s * math.sqrt(ev), errgradu=s * math.sqrt(eg),
+                           errE=s * math.sqrt(eE))
 
 
 def locate(s: float, N: int):
@@ -298,48 +336,27 @@ def _run_source(ws: Workspace, dt: float, nsteps: int, N: int, h: float, pre, ge
 
 def mms_errors(u, v, T, N, h, pre, geo, kx=1, ky=1, phase=(0.0, 0.0)):
     """||e_u||, ||e_v||, ||grad_h e_u||, ||e||_en at T against u = cos(wT) Phi
-    (src/solver.jl:774-810), vectorized over the elements."""
-    px, py = phase
-    W = geo.W2d
-    ct, st = math.cos(pre.omega * T), math.sin(pre.omega * T)
-    tpx, tpy = 2 * math.pi * kx, 2 * math.pi * ky
-    X = geo.xloc[None, :, None, :]          # [j, k, iy, ix]
-    Y = geo.yloc[:, None, :, None]
-    sp = np.cos(tpx * X + px) * np.sin(tpy * Y + py)
-    spx = -tpx * np.sin(tpx * X + px) * np.sin(tpy * Y + py)
-    spy = tpy * np.cos(tpx * X + px) * np.cos(tpy * Y + py)
-    P_u, P_v, S_u = pre.P_u, pre.P_v, geo.S_u
-    uh = np.einsum("pa,jkab,qb->jkpq", P_u, u, P_u, optimize=True)
-    vh = np.einsum("pa,jkab,qb->jkpq", P_v, v, P_v, optimize=True)
-    uxh = (2 / h) * np.einsum("pa,jkab,qb->jkpq", P_u, u, S_u, optimize=True)
-    uyh = (2 / h) * np.einsum("pa,jkab,qb->jkpq", S_u, u, P_u, optimize=True)
-    eu = (ct - 1.0) * sp - uh
-    ev = -pre.omega * st * sp - vh
-    ex = (ct - 1.0) * spx - uxh
-    ey = (ct - 1.0) * spy - uyh
-    s = h / 2
-    return SimpleNamespace(
-        erru=s * math.sqrt(np.sum(W * eu ** 2)),
-        errv=s * math.sqrt(np.sum(W * ev ** 2)),
-        errgradu=s * math.sqrt(np.sum(W * (ex ** 2 + ey ** 2))),
-        errE=s * math.sqrt(np.sum(W * (0.5 * geo.err_c2 * (ex ** 2 + ey ** 2) + 0.5 * ev ** 2))),
-    )
+    (src/solver.jl:774-810) of the host state (u, v), on the device."""
+    u = _state(u); v = _state(v)
+    ws = Workspace(pre, N)
+    ws._check_sizes(u, v, N, pre)
+    ws.dev.check(lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v)))
+    return _device_mms_errors(ws, T, N, h, pre, geo, kx, ky, phase)
 
 
 def difference_norms(A, B):
     """L2 norms of u_A - u_B, v_A - v_B and the energy norm of the difference of
-    two final states of ``solve(...; return_state=True)`` (src/solver.jl:819-836)."""
-    h = B.h
-    W = B.W2d
-    eu = A.u - B.u
-    ev = A.v - B.v
-    du = np.einsum("pa,jkab,qb->jkpq", B.P_u, eu, B.P_u, optimize=True)
-    dux = (2 / h) * np.einsum("pa,jkab,qb->jkpq", B.P_u, eu, B.S_u, optimize=True)
-    duy = (2 / h) * np.einsum("pa,jkab,qb->jkpq", B.S_u, eu, B.P_u, optimize=True)
-    dv = np.einsum("pa,jkab,qb->jkpq", B.P_v, ev, B.P_v, optimize=True)
-    s = h / 2
-    return SimpleNamespace(
-        erru=s * math.sqrt(np.sum(W * du ** 2)),
-        errv=s * math.sqrt(np.sum(W * dv ** 2)),
-        errE=s * math.sqrt(np.sum(W * (0.5 * B.err_c2 * (dux ** 2 + duy ** 2) + 0.5 * dv ** 2))),
-    )
+    two final states of ``solve(...; return_state=True)`` (src/solver.jl:819-836),
+    on the device of B's workspace (B's tables and c^2, as the reference)."""
+    ws = getattr(B, "workspace", None)
+    if ws is None:
+        raise ValueError("difference_norms needs B from solve(...; return_state = true) (its device workspace)")
+    N = B.N
+    uA = _state(A.u); vA = _state(A.v); uB = _state(B.u); vB = _state(B.v)
+    if uA.shape != uB.shape or vA.shape != vB.shape:
+        raise DimensionMismatch("difference_norms: the two states do not have the same shape")
+    elem = np.zeros((4, N * N))
+    ws.dev.check(lib.wedg_difference_norms(ws.dev.h, _ptr(uA), _ptr(vA), _ptr(uB), _ptr(vB), _ptr(elem)))
+    eu, ev, _, eE = _serial_sums(elem, N)
+    s = B.h / 2
+    return SimpleNamespace(erru=s * math.sqrt(eu), errv=s * math.sqrt(ev), errE=s * math.sqrt(eE))

@@ -170,6 +170,27 @@ wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* vmax, double
  * (the y points), column k (q_u+1) + q (the x points). */
 wedg_status wedg_snapshot(wedg_workspace* ws, double* grid);
 
+/* Error norms (src/solver.jl:772-836), after wedg_problem_set_diag. Both give per
+ * element e = j + N k the quadrature sums elem[c * N^2 + e] (c = 0..3) that the
+ * reference accumulates over j outer, k inner before h/2 sqrt(.):
+ *
+ * _mms_errors (src/solver.jl:774-810) of the workspace state: W (u* - u_h)^2,
+ * W (v* - v_h)^2, W |grad u* - grad_h u_h|^2 and W (1/2 c^2 |grad u* - grad_h u_h|^2
+ * + 1/2 (v* - v_h)^2) with u* = cu Phi, v* = cv Phi, cu = cos(wT) - 1,
+ * cv = -w sin(wT), Phi = cos(tpx x + px) sin(tpy y + py), tpx = 2 pi kx,
+ * tpy = 2 pi ky. cx, sx hold cos and sin of tpx x + px at the x points of element
+ * column k ([k][q], N (q_u+1) values), cy, sy those of tpy y + py at the y points
+ * of element row j ([j][p]).
+ *
+ * difference_norms (src/solver.jl:819-836) of two host states A, B (the
+ * workspace's layout): the sums of e = A - B (the third is the gradient sum,
+ * which the reference does not report). */
+wedg_status wedg_mms_errors(wedg_workspace* ws, double cu, double cv, double tpx, double tpy,
+                            const double* cx, const double* sx, const double* cy, const double* sy,
+                            double* elem);
+wedg_status wedg_difference_norms(wedg_workspace* ws, const double* uA, const double* vA,
+                                  const double* uB, const double* vB, double* elem);
+
 /* Kernel timing with HIP events on the workspace stream over the next run(s):
  * enable, then read the accumulated per-kernel-family milliseconds and counts.
  * family: 0 = k_prep (Chebyshev count and boundary slots of the first stage of a call),

@@ -7,6 +7,7 @@ receiver and snapshot data (data/steepening/*).
 import ctypes as C
 import math
 import sys
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -149,3 +150,30 @@ def test_steepening(case):
     assert dev["recv"] < 1e-12
     if case == "nl":
         assert dev["snap"] < 1e-12
+
+
+@pytest.mark.parametrize("q,N,mixed", [(3, 6, True), (5, 4, False), (1, 7, True)])
+def test_norms_device(q, N, mixed):
+    """k_norms: _mms_errors and difference_norms of random states with varying
+    c (the paper's coefficients) against the oracle's loop restatements
+    (oracle/norms.py), 1e-13 relative."""
+    from oracle import norms
+    say(f"case norms q={q} N={N} mixed={mixed}")
+    rng = np.random.default_rng(200 + q)
+    co = w.paper_coefficients()
+    pre, geo = precompute(q, N, mixed=mixed, mode="mms", kx=2, ky=1, phase=(0.3, -0.2), coefficients=co)
+    u, v = rand_state(rng, N, pre, su=0.1, sv=0.05)
+    h = 1.0 / N
+    for T in (0.0, 0.37):
+        d = w.mms_errors(u, v, T, N, h, pre, geo, kx=2, ky=1, phase=(0.3, -0.2))
+        o = norms.mms_errors(u, v, T, N, h, pre, geo, kx=2, ky=1, phase=(0.3, -0.2))
+        for k2 in ("erru", "errv", "errgradu", "errE"):
+            assert abs(getattr(d, k2) / o[k2] - 1) < 1e-13, (T, k2, getattr(d, k2), o[k2])
+    # difference norms: B from a device solve, A a perturbed copy of it
+    B = w.solve(q, N, 0.05, mixed=mixed, mode="mms", kx=2, ky=1, phase=(0.3, -0.2), coefficients=co,
+                return_state=True)
+    A = SimpleNamespace(u=B.u + 1e-3 * rng.standard_normal(B.u.shape), v=B.v + 1e-3 * rng.standard_normal(B.v.shape))
+    d = w.difference_norms(A, B)
+    o = norms.difference_norms(A.u, A.v, B.u, B.v, N, h, pre, geo)
+    for k2 in ("erru", "errv", "errE"):
+        assert abs(getattr(d, k2) / o[k2] - 1) < 1e-13, (k2, getattr(d, k2), o[k2])
