@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
@@ -49,7 +50,7 @@ class WedgPre(C.Structure):
 
 class WedgErrorInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("side", C.c_int), ("key", C.c_long), ("element", C.c_long),
-                ("step", C.c_long), ("cheb_steps", C.c_int)]
+                ("step", C.c_long), ("cheb_steps", C.c_int), ("cg_unconverged", C.c_int)]
 
 
 def _load():
@@ -120,6 +121,10 @@ def error_message(pre, info: WedgErrorInfo) -> str:
             return f"boundary weight 1 - {k}βv_h ≤ 0 at a face point ({bs['kind']} side)"
         return f"boundary weight 1 - {k}βg ≤ 0 for the boundary data g ({bs['kind']} side)"
     if st == ERR_MASS:
+        if getattr(pre, "mass_solver", "chebyshev") == "cg":   # (j, k), src/rhs.jl:845
+            N = pre.N
+            j, k = (info.element - 1) % N + 1, (info.element - 1) // N + 1
+            return f"mass matrix not positive definite on element ({j}, {k}): 1 - 2βv_h ≤ 0"
         return f"mass matrix not positive definite on element {info.element}: 1 - 2βv_h ≤ 0"
     if st == ERR_NONFINITE:
         return f"non-finite state after step {info.step}"
@@ -134,14 +139,14 @@ class Problem:
     """Device copy of one problem's tables (``wedg_problem_create``)."""
 
     def __init__(self, pre, N: int):
-        if pre.mass_solver != "chebyshev":
-            raise NotImplementedError("the device mass solver is Chebyshev iteration (mass_solver = :chebyshev)")
+        if pre.mass_solver not in ("chebyshev", "cg"):
+            raise ValueError(f"unknown mass_solver {pre.mass_solver} (expected :cg or :chebyshev)")
         s = WedgPre()
         s.N, s.qu1, s.qv1, s.q_high = N, pre.qu1, pre.qv1, pre.q_high
         s.omega, s.theta, s.tau = pre.omega, pre.theta, pre.tau
         s.lambda_u, s.lambda_v = pre.lambda_u, pre.lambda_v
         s.mms_on, s.src_on, s.one_sided = int(pre.mms_on), int(pre.src_on), int(pre.one_sided)
-        s.mass_solver = 0
+        s.mass_solver = 1 if pre.mass_solver == "cg" else 0
         s.mass_tol = pre.mass_tol
         s.src_omega, s.src_Ts = pre.source.omega, pre.source.Ts
         for i, side in enumerate(SIDES):
@@ -194,6 +199,10 @@ class DeviceWorkspace:
 
     def check(self, st: int):
         if st == OK:
+            info = self.last_error()
+            if info.cg_unconverged > 0:   # src/rhs.jl:855-858
+                warnings.warn(f"mass solve: {info.cg_unconverged} element systems did not reach the tolerance "
+                              f"{self.problem.pre.mass_tol}")
             return
         info = self.last_error()
         if info.status == OK:

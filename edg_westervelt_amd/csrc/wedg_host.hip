@@ -162,6 +162,7 @@ bool all_equal(const double* p, size_t n) {
 struct wedg_problem {
     int N = 0, E = 0, nu = 0, nv = 0, nh = 0, qu = 0;
     double omega = 0, theta = 0, tau = 0, lu = 0, lv = 0, mass_tol = 1e-14;
+    int mass_cg = 0;   // conjugate gradients (mass_solver = :cg) instead of Chebyshev iteration
     int mms_on = 0, src_on = 0, one_sided = 0;
     double src_omega = 0, src_Ts = 0;
     wedg_side bc[4];
@@ -260,7 +261,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
     *out = nullptr;
     const int N = pre->N, nu = pre->qu1, nv = pre->qv1, nh = pre->q_high;
     if (N < 1 || nu < 2 || nu > 7 || (nv != nu && nv != nu - 1)) return WEDG_ERR_ARGUMENT;
-    if (pre->mass_solver != 0) return WEDG_ERR_ARGUMENT;   // Chebyshev only on the device
+    if (pre->mass_solver != 0 && pre->mass_solver != 1) return WEDG_ERR_ARGUMENT;
     const int qv = nv - 1;
     int nh_expect = (3 * qv + 1) / 2 + 1;
     if (nh_expect < nu) nh_expect = nu;
@@ -274,6 +275,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
     p->N = N; p->E = N * N; p->nu = nu; p->nv = nv; p->nh = nh; p->qu = nu - 1;
     p->omega = pre->omega; p->theta = pre->theta; p->tau = pre->tau;
     p->lu = pre->lambda_u; p->lv = pre->lambda_v; p->mass_tol = pre->mass_tol;
+    p->mass_cg = pre->mass_solver == 1;
     p->mms_on = pre->mms_on; p->src_on = pre->src_on; p->one_sided = pre->one_sided;
     p->src_omega = pre->src_omega; p->src_Ts = pre->src_Ts;
     for (int s = 0; s < 4; ++s) p->bc[s] = pre->bc[s];
@@ -330,7 +332,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         const int n = nu * nu;
         std::vector<double> cw((size_t)nc * nq * nq), aw((size_t)nc * nq * nq), ai((size_t)nc * n * n);
         std::vector<double> dfac((size_t)nc * n);
-        bool separable = true;
+        bool separable = !p->mass_cg;   // conjugate gradients run with the general kernel only
         for (int c = 0; c < nc; ++c) {
             std::memcpy(&cw[(size_t)c * nq * nq], &uniq[(size_t)c * len], nq * nq * sizeof(double));
             std::memcpy(&aw[(size_t)c * nq * nq], &uniq[(size_t)c * len + nq * nq], nq * nq * sizeof(double));
@@ -387,7 +389,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         if (fd) UP(p->Ainv, dfac.data(), dfac.size());
         else UP(p->Ainv, ai.data(), ai.size());
         UP(p->cls_ca, cls.data(), cls.size());
-        p->mode = fd ? MODE_SEPARABLE : MODE_GENERAL;
+        p->mode = fd ? MODE_SEPARABLE : (p->mass_cg ? MODE_GENERAL_CG : MODE_GENERAL);
     }
     // beta at the high-order points per element -> classes
     {
@@ -573,7 +575,9 @@ static KArgs base_args(wedg_workspace* ws) {
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.N = p->N; a.E = p->E;
-    a.cheb_global = ws->cheb_global;
+    a.cheb_global = p->mass_cg ? 0 : ws->cheb_global;   // no counts with conjugate gradients
+    a.mass_cg = p->mass_cg;
+    a.mass_tol = p->mass_tol;
     a.one_sided = p->one_sided;
     const double h = 1.0 / p->N;
     a.two_h = 2.0 / h; a.four_hh = 4.0 / (h * h);
@@ -717,6 +721,7 @@ static wedg_status collect(wedg_workspace* ws, int nonfinite_is_error) {
             mx = std::max(mx, ws->hstat[nw + w]);
         }
         L.cheb_steps = (int)mx;
+        L.cg_unconverged = h.cg_unconverged;
         ws->cheb_wave_sum = sum;
     }
     const int nq = ws->p->nu, N = ws->p->N;

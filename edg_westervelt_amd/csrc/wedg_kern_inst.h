@@ -16,6 +16,7 @@ static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) 
     } else if (which == 1) {
         if (mode == MODE_UNIFORM) k_stage<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
         else if (mode == MODE_SEPARABLE) k_stage<NU, NV, MODE_SEPARABLE><<<grid, kWave, 0, s>>>(a);
+        else if (mode == MODE_GENERAL_CG) k_stage<NU, NV, MODE_GENERAL_CG><<<grid, kWave, 0, s>>>(a);
         else k_stage<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;

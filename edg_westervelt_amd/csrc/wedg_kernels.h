@@ -47,6 +47,7 @@ enum {
     MODE_GENERAL = 0,     // per-element classes, stored inverse of the u-stiffness
     MODE_SEPARABLE = 1,   // per-element classes, c^2 constant on each element: spectral u-solve
     MODE_UNIFORM = 2,     // one class, one beta record, every face array one value; spectral u-solve
+    MODE_GENERAL_CG = 3,  // as MODE_GENERAL, mass systems by conjugate gradients (mass_solver = :cg)
 };
 
 template <int NU_, int NV_>
@@ -94,7 +95,8 @@ struct DevErr {
     // keeps the failure the reference reports.
     unsigned long long fkey;
     int nonfinite;     // first RK4 step with a non-finite entry (INT_MAX = none)
-    int pad[5];
+    int cg_unconverged;   // element systems conjugate gradients left above the tolerance
+    int pad[4];
 };
 
 enum { F_HC2 = 0, F_HAF, F_HCF, F_HBF, F_VC2, F_VAF, F_VCF, F_VBF,
@@ -104,8 +106,9 @@ struct KArgs {
     int N, E, stage, cheb_global;
     int serial, slot, one_sided, step;
     int kind[4];
-    int nproj, pad0;
+    int nproj, mass_cg;             // mass_cg: conjugate gradients instead of Chebyshev iteration
     double two_h, four_hh, theta, tau, lu, lv, omega, cheb_L;
+    double mass_tol;
     double ct, st, srct, cstage;
     double nct, nst;                // time factors of the next stage (its boundary data)
     int has_next, pad2;             // 1: prepare the next stage from this stage's output
@@ -882,7 +885,8 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
     using D = Dims<NU, NV>;
     constexpr int NQ = D::NQ, NH = D::NH, NP = D::NP, PL = D::PL, PR = D::PR;
     constexpr bool UNI = MODE == MODE_UNIFORM;
-    constexpr bool FD = MODE != MODE_GENERAL;
+    constexpr bool FD = MODE == MODE_UNIFORM || MODE == MODE_SEPARABLE;
+    constexpr bool CG = MODE == MODE_GENERAL_CG;
     constexpr int nU = NU * NU, nV = NV * NV;
     __shared__ double Gs[D::NLDS * kWave];
     if (skip_stage(*kargs())) return;
@@ -1097,7 +1101,82 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
         double X[nV];
 #pragma unroll
         for (int m = 0; m < nV; ++m) X[m] = 0.0;
-        if (n_e == 0) {   // with the global policy stage_prep has recorded it already
+        if constexpr (CG) {
+            // conjugate gradients on M X = Rv, M = I - K, element by element
+            // (src/rhs.jl:824-853): X0 = Rv, r0 = K Rv, until |r| <= tol |Rv| or
+            // n_v^2 + 10 steps; a lane that has converged keeps its values.
+            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
+            auto apply_K = [&](const double (&d)[nV], double (&out)[nV]) {
+#pragma unroll
+                for (int m = 0; m < nV; ++m) out[m] = 0.0;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    double Gp[NH], Gq[NH];
+#pragma unroll
+                    for (int q = 0; q < NH; ++q) {
+                        if (p < PL) {
+                            Gp[q] = Gs[((2 * p) * NH + q) * kWave + lane];
+                            Gq[q] = Gs[((2 * p + 1) * NH + q) * kWave + lane];
+                        } else {
+                            Gp[q] = Gt[(2 * (p - PL)) * NH + q];
+                            Gq[q] = Gt[(2 * (p - PL) + 1) * NH + q];
+                        }
+                    }
+                    kd_pair<D>(Ph, p, d, Gp, Gq, out);
+                }
+                if (NH & 1) kd_mid<D>(Ph, d, Gm, out);
+            };
+            const double tol = kargs()->mass_tol;
+            double r[nV], pv[nV], Ap[nV];
+#pragma unroll
+            for (int m = 0; m < nV; ++m) X[m] = Rv[m];
+            apply_K(X, r);
+            double rr = 0.0, bb = 0.0;
+#pragma unroll
+            for (int m = 0; m < nV; ++m) {
+                pv[m] = r[m];
+                rr = fma(r[m], r[m], rr);
+                bb = fma(Rv[m], Rv[m], bb);
+            }
+            const double r0 = sqrt(bb);
+            bool act = sqrt(rr) > tol * r0, bad = false;
+            const int maxit = nV + 10;   // n_v^2 + 10 (src/rhs.jl:826)
+            int it = 0;
+            for (; it < maxit; ++it) {
+                int any = act ? 1 : 0;
+                for (int off = 32; off > 0; off >>= 1) any = max(any, __shfl_xor(any, off));
+                if (!any) break;
+                apply_K(pv, Ap);
+                double pAp = 0.0;
+#pragma unroll
+                for (int m = 0; m < nV; ++m) {
+                    Ap[m] = pv[m] - Ap[m];
+                    pAp = fma(pv[m], Ap[m], pAp);
+                }
+                if (act && !(pAp > 0.0)) {
+                    atomicMin(&kargs()->err->fkey, fail_key(kargs()->serial, 1, e + 1));
+                    act = false;
+                    bad = true;
+                }
+                if (act) {
+                    const double a = rr / pAp;
+                    double rrn = 0.0;
+#pragma unroll
+                    for (int m = 0; m < nV; ++m) {
+                        X[m] = X[m] + a * pv[m];
+                        r[m] = r[m] - a * Ap[m];
+                        rrn = fma(r[m], r[m], rrn);
+                    }
+                    const double bta = rrn / rr;
+#pragma unroll
+                    for (int m = 0; m < nV; ++m) pv[m] = r[m] + bta * pv[m];
+                    rr = rrn;
+                    act = sqrt(rr) > tol * r0;
+                }
+            }
+            if (!bad && sqrt(rr) > tol * r0) atomicAdd(&kargs()->err->cg_unconverged, 1);
+            ncheb = it;
+        } else if (n_e == 0) {   // with the global policy stage_prep has recorded it already
             if (!global) atomicMin(&kargs()->err->fkey, fail_key(kargs()->serial, 1, e + 1));
         } else {
             const int n = global ? nglob : n_e;

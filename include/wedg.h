@@ -58,7 +58,7 @@ typedef struct wedg_pre {
     double omega;                              /* 2 pi sqrt(kx^2 + ky^2) */
     double theta, tau, lambda_u, lambda_v;     /* flux parameters (src/solver.jl:16-21) */
     int mms_on, src_on, one_sided;
-    int mass_solver;                           /* 0 = Chebyshev (the only device solver) */
+    int mass_solver;                           /* 0 = Chebyshev, 1 = conjugate gradients (src/rhs.jl:824-853) */
     double mass_tol;                           /* 1e-14 */
     double src_omega, src_Ts;                  /* GaussianSource omega, Ts (src/solver.jl:80-86) */
     wedg_side bc[4];                           /* bot, top, lft, rgt */
@@ -87,7 +87,9 @@ typedef struct wedg_error_info {
     long key;           /* _boundary_key (src/rhs.jl:732) */
     long element;       /* first failing element, linear and 1-based (src/rhs.jl:916) */
     long step;          /* step (1-based, within the last advance) of a failure */
-    int cheb_steps;     /* largest Chebyshev step count of the last evaluation */
+    int cheb_steps;     /* largest Chebyshev step count (conjugate gradients: iterations) of the last call */
+    int cg_unconverged; /* conjugate gradients: element systems of the last call left above the
+                         * tolerance (the reference warns, src/rhs.jl:855-858) */
 } wedg_error_info;
 
 /* Chebyshev step-count policies. */

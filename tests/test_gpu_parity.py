@@ -220,16 +220,53 @@ TABLE = [(q, pairing, flux) for q in range(1, 7) for pairing in ("mixed", "match
 
 
 @pytest.mark.parametrize("q,pairing,flux", TABLE)
-def test_convergence_table_device(q, pairing, flux):
+@pytest.mark.parametrize("mass_solver", ["chebyshev", "cg"])
+def test_convergence_table_device(q, pairing, flux, mass_solver):
     """solve(q, 8, 0.1) on the device against data/convergence/results.csv (made
-    with CG; the device solves the mass systems by Chebyshev to 1e-14): rtol
-    1e-9 plus 2e-13 absolute, as the oracle's table test."""
-    say(f"case table q={q} {pairing} {flux} N=8")
+    with CG), with both mass solvers (Chebyshev and CG to 1e-14): rtol 1e-9 plus
+    2e-13 absolute, as the oracle's table test."""
+    say(f"case table q={q} {pairing} {flux} N=8 {mass_solver}")
     N = 8
     gold = convergence_row(q, pairing, flux, N)
-    e = w.solve(q, N, 0.1, flux=flux, mixed=pairing == "mixed")
+    e = w.solve(q, N, 0.1, flux=flux, mixed=pairing == "mixed", mass_solver=mass_solver)
     for k, g in gold.items():
         assert abs(getattr(e, k) - g) <= 1e-9 * g + 2e-13, (k, getattr(e, k), g)
+
+
+@pytest.mark.parametrize("q,N,mixed,mode", [(3, 7, True, "mms"), (5, 6, False, "initial_value"),
+                                            (2, 9, True, "source"), (6, 4, True, "mms")])
+def test_cg_rhs_device(q, N, mixed, mode):
+    """edg_rhs with mass_solver = :cg (src/rhs.jl:824-853) against the oracle's CG,
+    1e-12 relative, and one RK4 step."""
+    say(f"case cg q={q} N={N} mixed={mixed} {mode}")
+    rng = np.random.default_rng(300 + q)
+    pre, _ = precompute(q, N, mixed=mixed, mode=mode, mass_solver="cg", boundary=MIXED)
+    u, v = rand_state(rng, N, pre, su=1e-2, sv=5e-3)
+    a = ob.rhs(ob.OraclePre(pre, N), u, v, 0.3)
+    b = w.edg_rhs(u, v, 0.3, N, 1.0 / N, pre)
+    assert rel(a[0], b[0]) < TOL and rel(a[1], b[1]) < TOL
+    dt = 1e-4
+    uo, vo, _, _ = ob.rk4_steps(ob.OraclePre(pre, N), u, v, 0.3, dt, 1)
+    ud, vd = w.rk4_step(u, v, 0.3, dt, N, 1.0 / N, pre)
+    assert rel(uo - u, ud - u) < 1e-11 and rel(vo - v, vd - v) < 1e-11
+
+
+def test_cg_not_positive_definite_device():
+    """A state with 1 - 2 beta v_h < 0 inside one element: CG's p.Mp <= 0 gives the
+    reference's message with the element as (j, k) (src/rhs.jl:845), as the
+    oracle."""
+    say("case cg indefinite")
+    q, N = 3, 5
+    pre, _ = precompute(q, N, mixed=True, mode="initial_value", mass_solver="cg",
+                        coefficients=w.constant_coefficients(c=1.0, alpha=1e-3, beta=2.0))
+    u, v = rand_state(np.random.default_rng(7), N, pre, su=1e-2, sv=1e-4)
+    v[2, 3, :, :] = 0.0
+    v[2, 3, 0, 0] = 2.0   # v_h = 1 on element (3, 4): 1 - 2 beta v_h = -3
+    with pytest.raises(RuntimeError) as ro:
+        ob.rhs(ob.OraclePre(pre, N), u, v, 0.0)
+    with pytest.raises(w.WedgError) as rd:
+        w.edg_rhs(u, v, 0.0, N, 1.0 / N, pre)
+    assert str(rd.value) == str(ro.value), (str(rd.value), str(ro.value))
 
 
 def test_alpha_limit_device():
