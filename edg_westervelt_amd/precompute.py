@@ -11,6 +11,7 @@ runs inside a time step.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass
 from types import SimpleNamespace
 from typing import Callable, Optional
@@ -278,14 +279,25 @@ def precompute(q: int, N: int, *, mixed: bool = True, flux: str = "upwind", flux
     order = np.lexsort((iu, ju))
     iu, ju = iu[order], ju[order]
     packs = np.empty((uniq.shape[0], npk))
-    for c_i in range(uniq.shape[0]):
-        cwv_c = uniq[c_i]
-        Mj = Phi_xi_u.T @ (cwv_c[:, None] * Phi_xi_u) + Phi_eta_u.T @ (cwv_c[:, None] * Phi_eta_u)
-        Mj[0, 0] += 1.0
-        L = np.linalg.cholesky(Mj)
-        Linv = np.linalg.inv(L)
-        Aj = Linv.T @ Linv
-        packs[c_i] = Aj[iu, ju]
+
+    def classes(c0, c1):
+        # the classes' matrices, factors and inverses as stacks (LAPACK per
+        # matrix, as the loop over the elements calls it)
+        cw_c = uniq[c0:c1]
+        Mj = (np.matmul(Phi_xi_u.T[None, :, :], cw_c[:, :, None] * Phi_xi_u[None, :, :])
+              + np.matmul(Phi_eta_u.T[None, :, :], cw_c[:, :, None] * Phi_eta_u[None, :, :]))
+        Mj[:, 0, 0] += 1.0
+        Linv = np.linalg.inv(np.linalg.cholesky(Mj))
+        packs[c0:c1] = np.matmul(np.swapaxes(Linv, 1, 2), Linv)[:, iu, ju]
+
+    nc = uniq.shape[0]
+    chunk = 1024
+    if nc <= chunk:
+        classes(0, nc)
+    else:   # numpy's linalg releases the GIL: chunks on the host cores
+        import concurrent.futures as cf
+        with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            list(ex.map(lambda c0: classes(c0, min(c0 + chunk, nc)), range(0, nc, chunk)))
     Ainv_u[:, :] = packs[inv_idx].T
 
     # ---- face coefficients seen from inside every element (src/solver.jl:538-552) ----
