@@ -897,10 +897,11 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
     const int nw = (E + kWave - 1) / kWave;
     // Staggered start on grids that fill the chip: with one wave per SIMD all
     // waves would load (and store) at the same moments. The waves start in four
-    // groups ~3.5 us apart (s_sleep 127 = 8128 clocks), which spreads the memory
-    // phases of the whole stage over time; waves with physical-boundary elements
-    // (whose epilogue forms the boundary slots) take the two early groups.
-    // Measured on the 256 x 256, q = 5 instance: 5% less time per stage.
+    // groups, each one s_sleep 127 (64 x 127 clocks) later than the one before,
+    // which spreads the memory phases of the whole stage over time; waves with
+    // physical-boundary elements (whose epilogue forms the boundary slots) take
+    // the two early groups. Measured on the 256 x 256, q = 5 instance: 5% less
+    // time per stage.
     if (nw >= kStaggerWaves) {
         int b = 0;
         if (e < E) {
@@ -912,12 +913,34 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
         const int g = b ? (w & 1) : 2 + (w & 1);
         for (int i = 0; i < g; ++i) __builtin_amdgcn_s_sleep(127);
     }
-    // the global count: the maximum over the waves' counts of this stage
+    // the global count, the maximum over the waves' counts of this stage: the
+    // slots are read (eight at a time) before the state loads are issued and
+    // reduced after, so the two latencies overlap; lanes past E load element E - 1
     int nglob = 1;
-    if (global) {
-        const int* wc = kargs()->cnt + kargs()->slot * nw;
-        for (int i = lane; i < nw; i += kWave) nglob = max(nglob, wc[i]);
-        for (int off = 32; off > 0; off >>= 1) nglob = max(nglob, __shfl_xor(nglob, off));
+    double U[nU], V[nV];
+    {
+        int m = 1;
+        if (global) {
+            const int* wc = kargs()->cnt + kargs()->slot * nw;
+            for (int i0 = lane; i0 < nw; i0 += 8 * kWave) {
+                int t[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t[u] = i0 + u * kWave < nw ? wc[i0 + u * kWave] : 1;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m = max(m, t[u]);
+            }
+        }
+        const int ec = e < E ? e : E - 1;
+        const double* yu = kargs()->yu;
+        const double* yv = kargs()->yv;
+#pragma unroll
+        for (int mm = 0; mm < nU; ++mm) U[mm] = yu[(long)mm * E + ec];
+#pragma unroll
+        for (int mm = 0; mm < nV; ++mm) V[mm] = yv[(long)mm * E + ec];
+        if (global) {
+            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+            nglob = m;
+        }
     }
     int ncheb = 0, n_next = 1;
     if (e < E) {
@@ -926,15 +949,6 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
         const double* Wc = kargs()->Cw + cc * NQ * NQ;
         const double* Wa = kargs()->Aw + cc * NQ * NQ;
 
-        double U[nU], V[nV];
-        {
-            const double* yu = kargs()->yu;
-            const double* yv = kargs()->yv;
-#pragma unroll
-            for (int m = 0; m < nU; ++m) U[m] = yu[(long)m * E + e];
-#pragma unroll
-            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
-        }
         // ---- face slots of the four sides and their lift vectors (src/rhs.jl:421-473,
         // 589-776), into the lane's LDS slots
         face_side<D, UNI, 0>(e, j, k, N, E, U, V, Gs, lane);
