@@ -314,3 +314,17 @@ def test_full_size_rhs_and_step():
     vg = np.array(v, order="F", copy=True)
     w.rk4_step_(ug, vg, 0.01, dt, N, 1.0 / N, pre, ws)
     assert rel(uo - u, ug - u) < 1e-11 and rel(vo - v, vg - v) < 1e-11
+
+
+@pytest.mark.parametrize("q,N,mixed,mode,solver", [(3, 1, True, "mms", "chebyshev"), (2, 1, False, "source", "cg"),
+                                                   (4, 2, True, "initial_value", "chebyshev"),
+                                                   (5, 3, False, "mms", "cg")])
+def test_smallest_meshes_device(q, N, mixed, mode, solver):
+    """N = 1 (one element with four boundary sides), 2 and 3, both mass solvers,
+    the reference's mixed boundary kinds: RHS vs the oracle, 1e-12."""
+    say(f"case small mesh q={q} N={N} {mode} {solver}")
+    rng = np.random.default_rng(400 + N)
+    pre, _ = precompute(q, N, mixed=mixed, mode=mode, boundary=MIXED, mass_solver=solver)
+    u, v = rand_state(rng, N, pre, su=1e-2, sv=5e-3)
+    a, b = both(pre, N, u, v, 0.2)
+    assert rel(a[0], b[0]) < TOL and rel(a[1], b[1]) < TOL
