@@ -8,6 +8,10 @@
  *   edg_rhs!(ws,u,v,t,…) -> wedg_rhs / wedg_rhs_device (src/rhs.jl:490-494)
  *   rk4_step!(u,v,t,dt,…)-> wedg_rk4_step           (src/solver.jl:289-297)
  *   the loop of solve    -> wedg_advance            (src/solver.jl:213-216, 937-940)
+ *   _run_source          -> wedg_advance_receivers, wedg_diagnostics, wedg_snapshot
+ *                           (src/solver.jl:856-958)
+ *   _mms_errors, difference_norms -> wedg_mms_errors, wedg_difference_norms
+ *                           (src/solver.jl:772-836)
  * Arrays are the reference's column-major Julia arrays, byte for byte: a Julia
  * caller passes pointer(pre.Cw), pointer(u), … unchanged. The state u is
  * (N, N, q_u+1, q_u+1) and v is (N, N, q_v+1, q_v+1), element e = j + N k
