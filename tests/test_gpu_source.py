@@ -177,3 +177,42 @@ def test_norms_device(q, N, mixed):
     o = norms.difference_norms(A.u, A.v, B.u, B.v, N, h, pre, geo)
     for k2 in ("erru", "errv", "errE"):
         assert abs(getattr(d, k2) / o[k2] - 1) < 1e-13, (k2, getattr(d, k2), o[k2])
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("mixed", [True, False])
+def test_energy_diagnostics_all_degrees(q, mixed):
+    """The element energies, max |v_h| and min (1 - 2 beta v_h) of k_diag for every
+    q and pairing with beta = 6 (test/runtests.jl:603-625's cases) against the
+    oracle's loop, 1e-13 relative."""
+    say(f"case energy diag q={q} mixed={mixed}")
+    N = 5
+    pre, geo = precompute(q, N, mixed=mixed, mode="source",
+                          coefficients=w.replace_coefficients(w.paper_coefficients(), beta=6.0))
+    rng = np.random.default_rng(q)
+    u, v = rand_state(rng, N, pre, su=1e-2, sv=1e-2)
+    ws = w.Workspace(pre, N)
+    dh = ws.dev.h
+    ws.dev.check(lib.wedg_set_state(dh, _p(u), _p(v)))
+    c2 = np.asfortranarray(geo.err_c2); bq = np.asfortranarray(geo.err_beta); W2d = np.asfortranarray(geo.W2d)
+    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _p(c2), _p(bq), _p(W2d)))
+    E = np.zeros(N * N); vm = np.zeros(N * N); nd = np.zeros(N * N); sl = np.zeros(N * pre.qu1)
+    r0 = od.vrow(0.0, pre.qv1)
+    ws.dev.check(lib.wedg_diagnostics(dh, _p(E), _p(vm), _p(nd), 2, _p(r0), _p(sl)))
+    Eo, vmo, ndo = od.energy_elements(u, v, N, 1.0 / N, pre, geo)
+    assert rel(Eo, E) < 1e-13 and rel(vmo, vm) < 1e-13 and rel(ndo, nd) < 1e-13
+
+
+@pytest.mark.parametrize("flux", ["upwind", "central"])
+def test_source_energy_decay(flux):
+    """test/runtests.jl:839-850 on the device: quadrant coefficients, a source that
+    ends at t = 0.25, solve(3, 8, 0.75; mode = :source, record_every = 5): after
+    the source the recorded energy never rises by more than 1e-10 E[1] and ends
+    below 0.95 of its first value."""
+    say(f"case energy decay {flux}")
+    co = quadrant_coefficients(0.5, 0.5, alpha=(1e-5, 4e-6, 2e-5, 1e-6), beta=(1e-4, 3e-4, 0.0, 2e-4))
+    src = w.GaussianSource(x0=(0.45, 0.55), r0=0.08, amplitude=50.0, omega=8 * math.pi, Ts=0.25)
+    H = w.solve(3, 8, 0.75, mode="source", coefficients=co, flux=flux, source=src, record_every=5)
+    E = H.energy[H.t > 0.25]
+    assert np.max(np.diff(E)) <= 1e-10 * E[0]
+    assert E[-1] / E[0] < 0.95
