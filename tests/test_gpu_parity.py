@@ -5,6 +5,7 @@ oracle on the same seeded inputs, over the reference's own test matrix
 Tolerance: 1e-12 relative per right-hand side (max |a - b| / max |a|, the
 reference's rhs_rel), as the reference's own batched-vs-reference tests.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -328,3 +329,34 @@ def test_smallest_meshes_device(q, N, mixed, mode, solver):
     u, v = rand_state(rng, N, pre, su=1e-2, sv=5e-3)
     a, b = both(pre, N, u, v, 0.2)
     assert rel(a[0], b[0]) < TOL and rel(a[1], b[1]) < TOL
+
+
+def test_chebyshev_counts_device():
+    """test/runtests.jl:506-518's state (q = 4, N = 9, beta = 8, v raised on two
+    element columns): the global a-priori Chebyshev count of the device equals
+    the oracle's and exceeds 20 (the reference's test asserts a block maximum
+    above 20); with per-element counts the wave without raised elements runs
+    fewer steps, so the mean over the waves is below the global count; the RHS
+    agrees with the oracle (1e-12) under both policies."""
+    say("case chebyshev counts")
+    N = 9
+    co = w.replace_coefficients(w.paper_coefficients(), beta=8.0)
+    pre, _ = precompute(4, N, coefficients=co, mode="source")
+    rng = np.random.default_rng(5)
+    u = np.asfortranarray(1e-3 * rng.standard_normal((N, N, pre.qu1, pre.qu1)))
+    v = np.asfortranarray(1e-4 * rng.standard_normal((N, N, pre.qv1, pre.qv1)))
+    v[:, 0:2, 1, 0] = 0.04
+    a = ob.rhs(ob.OraclePre(pre, N), u, v, 0.1)
+    ws = w.Workspace(pre, N)
+    ut, vt = w.edg_rhs_(ws, u, v, 0.1, N, 1.0 / N, pre)
+    assert rel(a[0], ut) < TOL and rel(a[1], vt) < TOL
+    assert ws.cheb_steps == a[2].cheb_steps and ws.cheb_steps > 20, (ws.cheb_steps, a[2].cheb_steps)
+    ae = ob.rhs(ob.OraclePre(pre, N), u, v, 0.1, cheb_mode=1)
+    be = w.edg_rhs(u, v, 0.1, N, 1.0 / N, pre, cheb="element")
+    assert rel(ae[0], be[0]) < TOL and rel(ae[1], be[1]) < TOL
+    from edg_westervelt_amd import _lib as L
+    mean = C.c_double(0.0); nst = C.c_long(0)
+    wse = w.Workspace(pre, N, cheb="element")
+    w.edg_rhs_(wse, u, v, 0.1, N, 1.0 / N, pre)
+    L.lib.wedg_cheb_stats(wse.dev.h, C.byref(mean), C.byref(nst))
+    assert mean.value < ws.cheb_steps
