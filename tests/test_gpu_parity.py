@@ -360,3 +360,28 @@ def test_chebyshev_counts_device():
     w.edg_rhs_(wse, u, v, 0.1, N, 1.0 / N, pre)
     L.lib.wedg_cheb_stats(wse.dev.h, C.byref(mean), C.byref(nst))
     assert mean.value < ws.cheb_steps
+
+
+def test_solve_deterministic_and_concurrent():
+    """Results do not depend on the launch or on other work (test/runtests.jl:521-546):
+    two solves of the same problem are bitwise equal, also when they run at the
+    same time from two host threads on their own workspaces and streams."""
+    import threading
+    say("case determinism")
+    kw = dict(mixed=True, mode="initial_value", boundary=MIXED, sigma=0.3, sigma_dirichlet=0.6, kappa=0.7,
+              return_state=True)
+    r1 = w.solve(3, 12, 0.02, **kw)
+    r2 = w.solve(3, 12, 0.02, **kw)
+    assert np.array_equal(r1.u, r2.u) and np.array_equal(r1.v, r2.v)
+    out = [None, None]
+
+    def run(i):
+        out[i] = w.solve(3, 12, 0.02, **kw)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for r in out:
+        assert np.array_equal(r.u, r1.u) and np.array_equal(r.v, r1.v)
