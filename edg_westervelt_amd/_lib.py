@@ -86,6 +86,10 @@ def _load():
         "wedg_snapshot": [vp, _dp],
         "wedg_mms_errors": [vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp],
         "wedg_difference_norms": [vp, _dp, _dp, _dp, _dp, _dp],
+        "wedg_batch_create": [C.POINTER(vp), C.c_int, C.POINTER(vp)],
+        "wedg_batch_destroy": [vp],
+        "wedg_batch_advance": [vp, _dp, _dp, C.POINTER(C.c_long), C.c_int, _dp, _ip],
+        "wedg_batch_time": [vp, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,193 @@
+"""Batched instances on the device: many independent problems (uneven N, q_u,
+q_v, coefficients, boundary kinds, time steps and step counts) advanced
+together, one kernel launch per RK stage for each group of instances with the
+same (q_u, q_v) and coefficient layout (``wedg_batch_*`` of include/wedg.h).
+
+The reference advances one instance per ``solve`` (src/solver.jl:167-228);
+every instance here gets the stage arguments its own ``wedg_advance`` would
+give it, so a batched run returns bitwise the states of separate runs.
+
+``uneven_instances`` draws the 64-instance workload of uneven problems
+(initial-value mode, piecewise-constant coefficients on a 4 x 4 grid of
+mesh-aligned blocks); ``solve_batch`` builds and runs such a set.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+from ._lib import WedgError, lib
+from .coefficients import Coefficients, _zero
+from .precompute import precompute
+from .solver import Workspace, _ptr, time_step
+
+__all__ = ["Batch", "piecewise_constant", "block_coefficients", "gaussian_profile", "uneven_instances",
+           "build_instance", "solve_batch"]
+
+
+class Batch:
+    """Workspaces advanced together (``wedg_batch_create``); each keeps its own
+    state, which ``wedg_get_state`` / ``Workspace`` calls read as usual."""
+
+    def __init__(self, workspaces):
+        self.workspaces = list(workspaces)
+        n = len(self.workspaces)
+        arr = (C.c_void_p * n)(*[w.dev.h.value for w in self.workspaces])
+        h = C.c_void_p()
+        st = lib.wedg_batch_create(arr, n, C.byref(h))
+        if st != _lib.OK:
+            raise WedgError(f"wedg_batch_create failed with status {st}")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib.wedg_batch_destroy(h)
+            self.h = None
+
+    def advance(self, t0, dt, nsteps, stop_on_nonfinite: bool = False, raise_errors: bool = True):
+        """nsteps[i] RK4 steps of dt[i] from t0[i] for instance i. Returns the
+        final times and the per-instance statuses; with ``raise_errors`` the
+        first failing instance raises the reference's message."""
+        n = len(self.workspaces)
+        dt = np.ascontiguousarray(np.asarray(dt, dtype=np.float64))
+        if dt.shape != (n,) or len(nsteps) != n or np.ndim(t0) not in (0, 1) or np.size(t0) not in (1, n):
+            raise ValueError("t0, dt and nsteps need one entry per instance")
+        t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)))
+        ns = (C.c_long * n)(*[int(x) for x in nsteps])
+        tend = np.zeros(n)
+        status = np.zeros(n, dtype=np.int32)
+        st = lib.wedg_batch_advance(self.h, _ptr(t0), _ptr(dt), ns, int(stop_on_nonfinite), _ptr(tend),
+                                    status.ctypes.data_as(C.POINTER(C.c_int)))
+        if raise_errors:
+            for i, w in enumerate(self.workspaces):
+                if status[i] != _lib.OK:
+                    w.dev.check(int(status[i]))
+            if st != _lib.OK:
+                raise WedgError(f"wedg_batch_advance failed with status {st}")
+        return tend, status
+
+    def device_ms(self) -> float:
+        """Device time of the last advance (HIP events on the batch stream), ms."""
+        ms = C.c_double(0.0)
+        lib.wedg_batch_time(self.h, C.byref(ms))
+        return ms.value
+
+
+def piecewise_constant(values) -> callable:
+    """f(x, y) = values[bx, by] on a B x B grid of blocks of the unit square,
+    bx = floor(B x) (the last block closed on the right), for numpy arrays or
+    scalars."""
+    vals = np.asarray(values, dtype=np.float64)
+    B = vals.shape[0]
+
+    def f(x, y):
+        bx = np.clip(np.floor(np.asarray(x) * B).astype(int), 0, B - 1)
+        by = np.clip(np.floor(np.asarray(y) * B).astype(int), 0, B - 1)
+        return vals[bx, by]
+
+    return f
+
+
+def block_coefficients(c, alpha, beta) -> Coefficients:
+    """Piecewise-constant c, alpha, beta (B x B block tables) with zero
+    derivatives inside the blocks (the jumps are taken by ``precompute``'s
+    one-sided face values)."""
+    return Coefficients(piecewise_constant(c), _zero, _zero, piecewise_constant(alpha), _zero, _zero,
+                        piecewise_constant(beta))
+
+
+def gaussian_profile(x0=0.5, y0=0.5, s=0.05, amp=0.05) -> dict:
+    """u0 = amp exp(-((x-x0)^2 + (y-y0)^2) / (2 s^2)) with its analytic first
+    and second derivatives (the ``initial_profile`` of :initial_value mode)."""
+    def u0(x, y):
+        return amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * s * s))
+
+    return dict(u0=u0,
+                u0_x=lambda x, y: -(x - x0) / s ** 2 * u0(x, y),
+                u0_y=lambda x, y: -(y - y0) / s ** 2 * u0(x, y),
+                u0_xx=lambda x, y: ((x - x0) ** 2 / s ** 4 - 1 / s ** 2) * u0(x, y),
+                u0_yy=lambda x, y: ((y - y0) ** 2 / s ** 4 - 1 / s ** 2) * u0(x, y))
+
+
+KINDS = ("absorbing_2beta", "absorbing", "dirichlet", "wall")
+FLUXES = ("upwind", "central", "alternating")
+
+
+def uneven_instances(seed: int = 0, count: int = 64, T: float = 0.25):
+    """The uneven batched workload, drawn with numpy's default_rng(seed), per
+    instance in this order: N in {16, 24, 32, 48, 64}; q_u in {2..6}; q_v =
+    q_u - 1 or q_u; the flux (upwind, central, alternating); per 4 x 4 block c
+    in U[0.8, 1.2], alpha log-uniform in [1e-6, 1e-3], beta in U[0, 8]; the
+    Gaussian's centre in [0.3, 0.7]^2 and width in U[0.03, 0.08]; each side's
+    kind from (absorbing_2beta, absorbing, dirichlet, wall); sigma in U[-1, 1]
+    (Dirichlet sides take |sigma|, which must be >= 0) and kappa in U[0, 1].
+    The amplitude keeps 1 - 2 beta |v| >= 0.5 with a margin of 4 on the
+    free-space bound |v| <= c_max |grad u0|_max = c_max amp / (s sqrt(e))
+    (reflections from walls, Dirichlet sides and coefficient jumps raise |v|
+    above it): amp = min(0.05, s sqrt(e) / (16 beta_max c_max)). Every run
+    goes to T."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        N = int(rng.choice([16, 24, 32, 48, 64]))
+        q = int(rng.integers(2, 7))
+        mixed = bool(rng.integers(0, 2))
+        flux = FLUXES[int(rng.integers(0, 3))]
+        c = rng.uniform(0.8, 1.2, (4, 4))
+        alpha = 10.0 ** rng.uniform(-6.0, -3.0, (4, 4))
+        beta = rng.uniform(0.0, 8.0, (4, 4))
+        x0, y0 = rng.uniform(0.3, 0.7, 2)
+        s = float(rng.uniform(0.03, 0.08))
+        kinds = [KINDS[int(k)] for k in rng.integers(0, 4, 4)]
+        sigma = float(rng.uniform(-1.0, 1.0))
+        kappa = float(rng.uniform(0.0, 1.0))
+        amp = min(0.05, s * math.sqrt(math.e) / (16.0 * float(beta.max()) * float(c.max())))
+        out.append(SimpleNamespace(index=i, N=N, q=q, mixed=mixed, flux=flux, c=c, alpha=alpha, beta=beta,
+                                   x0=float(x0), y0=float(y0), s=s, amp=amp,
+                                   boundary=dict(zip(("bot", "top", "lft", "rgt"), kinds)),
+                                   sigma=sigma, sigma_dirichlet=abs(sigma), kappa=kappa, T=T))
+    return out
+
+
+def build_instance(inst, dt_fac=10, cheb: str = "global"):
+    """precompute, a device workspace with the zero initial state (u - u0 = 0,
+    v = 0) and the time step of one instance (src/solver.jl:167-213)."""
+    pre, geo = precompute(inst.q, inst.N, mixed=inst.mixed, flux=inst.flux, sigma=inst.sigma,
+                          boundary=inst.boundary, sigma_dirichlet=inst.sigma_dirichlet, kappa=inst.kappa,
+                          coefficients=block_coefficients(inst.c, inst.alpha, inst.beta), mode="initial_value",
+                          initial_profile=gaussian_profile(inst.x0, inst.y0, inst.s, inst.amp))
+    dt, nsteps = time_step(inst.q, inst.N, inst.T, dt_fac)
+    ws = Workspace(pre, inst.N, cheb=cheb)
+    set_zero_state(ws)
+    return SimpleNamespace(inst=inst, pre=pre, geo=geo, ws=ws, dt=dt, nsteps=nsteps,
+                           dof=inst.N * inst.N * (pre.qu1 ** 2 + pre.qv1 ** 2))
+
+
+def set_zero_state(ws: Workspace):
+    u = np.zeros((ws.N, ws.N, ws.qu1, ws.qu1), order="F")
+    v = np.zeros((ws.N, ws.N, ws.qv1, ws.qv1), order="F")
+    ws.dev.check(lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v)))
+
+
+def get_state(ws: Workspace):
+    u = np.zeros((ws.N, ws.N, ws.qu1, ws.qu1), order="F")
+    v = np.zeros((ws.N, ws.N, ws.qv1, ws.qv1), order="F")
+    ws.dev.check(lib.wedg_get_state(ws.dev.h, _ptr(u), _ptr(v)))
+    return u, v
+
+
+def solve_batch(instances, dt_fac=10, cheb: str = "global"):
+    """Builds the instances and advances them together to their T. Returns the
+    built instances (with ``u``, ``v``, ``t`` set) and the Batch."""
+    built = [build_instance(x, dt_fac=dt_fac, cheb=cheb) for x in instances]
+    b = Batch([x.ws for x in built])
+    tend, _ = b.advance(0.0, [x.dt for x in built], [x.nsteps for x in built])
+    for x, t in zip(built, tend):
+        x.u, x.v = get_state(x.ws)
+        x.t = float(t)
+    return built, b

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -645,13 +646,12 @@ static wedg_status run_kernel(wedg_workspace* ws, const KArgs& a, int which) {
     return WEDG_OK;
 }
 
-// One stage: k_stage on Y_s (after k_prep when it is the first stage of a
-// call), writing the stage output and, when has_next, preparing the stage that
-// follows at time t_next. The output must not alias Y_s: a stage reads its
-// neighbours' input state.
-static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, const double* yv,
-                             const double* y0u, const double* y0v, double* outu, double* outv,
-                             double t, double c, long step, bool first, bool has_next, double t_next) {
+// The arguments of one stage: k_stage on Y_s, writing the stage output and,
+// when has_next, preparing the stage that follows at time t_next. The output
+// must not alias Y_s: a stage reads its neighbours' input state.
+static KArgs stage_args(wedg_workspace* ws, int stage, const double* yu, const double* yv,
+                        const double* y0u, const double* y0v, double* outu, double* outv,
+                        double t, double c, long step, bool has_next, double t_next) {
     KArgs a = base_args(ws);
     a.stage = stage;
     a.serial = ++ws->serial;
@@ -671,22 +671,42 @@ static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, co
         a.nst = nx.st;
     }
     ++ws->stages;
+    return a;
+}
+
+// One stage (after k_prep when it is the first stage of a call).
+static wedg_status run_stage(wedg_workspace* ws, int stage, const double* yu, const double* yv,
+                             const double* y0u, const double* y0v, double* outu, double* outv,
+                             double t, double c, long step, bool first, bool has_next, double t_next) {
+    const KArgs a = stage_args(ws, stage, yu, yv, y0u, y0v, outu, outv, t, c, step, has_next, t_next);
     wedg_status st;
     if (first && (st = run_kernel(ws, a, 0)) != WEDG_OK) return st;
     return run_kernel(ws, a, 1);
 }
 
-// One RK4 step (src/solver.jl:300-321) on the device state: the stage values
-// alternate between the S (su, sv) and O (ou, ov) buffers, so no stage writes
-// the state it reads; the last stage updates y in place (it reads only its own
-// element of y). first: the first step of a call; last: the last one.
-static wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step, bool first, bool last) {
-    wedg_status s;
+// Stage s (1..4) of one RK4 step (src/solver.jl:300-321) on the device state:
+// the stage values alternate between the S (su, sv) and O (ou, ov) buffers, so
+// no stage writes the state it reads; the last stage updates y in place (it
+// reads only its own element of y). last: the last step of a call.
+static KArgs rk4_args(wedg_workspace* ws, int s, double t, double dt, long step, bool last) {
     const double hd = 0.5 * dt, th = t + 0.5 * dt, tf = t + dt;
-    if ((s = run_stage(ws, 1, ws->yu, ws->yv, ws->yu, ws->yv, ws->su, ws->sv, t, hd, step, first, true, th))) return s;
-    if ((s = run_stage(ws, 2, ws->su, ws->sv, ws->yu, ws->yv, ws->ou, ws->ov, th, hd, step, false, true, th))) return s;
-    if ((s = run_stage(ws, 3, ws->ou, ws->ov, ws->yu, ws->yv, ws->su, ws->sv, th, dt, step, false, true, tf))) return s;
-    return run_stage(ws, 4, ws->su, ws->sv, ws->yu, ws->yv, ws->yu, ws->yv, tf, dt / 6.0, step, false, !last, tf);
+    switch (s) {
+        case 1: return stage_args(ws, 1, ws->yu, ws->yv, ws->yu, ws->yv, ws->su, ws->sv, t, hd, step, true, th);
+        case 2: return stage_args(ws, 2, ws->su, ws->sv, ws->yu, ws->yv, ws->ou, ws->ov, th, hd, step, true, th);
+        case 3: return stage_args(ws, 3, ws->ou, ws->ov, ws->yu, ws->yv, ws->su, ws->sv, th, dt, step, true, tf);
+        default: return stage_args(ws, 4, ws->su, ws->sv, ws->yu, ws->yv, ws->yu, ws->yv, tf, dt / 6.0, step, !last, tf);
+    }
+}
+
+// One RK4 step; first: the first step of a call.
+static wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step, bool first, bool last) {
+    wedg_status st;
+    for (int s = 1; s <= 4; ++s) {
+        const KArgs a = rk4_args(ws, s, t, dt, step, last);
+        if (s == 1 && first && (st = run_kernel(ws, a, 0)) != WEDG_OK) return st;
+        if ((st = run_kernel(ws, a, 1)) != WEDG_OK) return st;
+    }
+    return WEDG_OK;
 }
 
 // Reads the failure record and maps it to a status (boundary before mass,
@@ -1157,4 +1177,277 @@ extern "C" wedg_status wedg_difference_norms(wedg_workspace* ws, const double* u
     a.u = ws->su; a.v = ws->sv; a.ub = ws->ou; a.vb = ws->ov;
     HIPOK(launch_diag(p, nullptr, nullptr, &a, ws->stream));
     return norms_out(ws, elem);
+}
+
+// ---------------------------------------------------------------------------
+// Batched instances: the workspaces of one (n_u, n_v, mode) group advance in
+// one launch per RK stage; every instance keeps its own tables, state, time,
+// step count and failure record, and gets exactly the stage arguments (and so
+// the results) its own wedg_advance would give it.
+
+static hipError_t launch_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode) {
+    switch (qu) {
+        case 1: return launch_batch_q1(b, grid, nv, which, s, mode);
+        case 2: return launch_batch_q2(b, grid, nv, which, s, mode);
+        case 3: return launch_batch_q3(b, grid, nv, which, s, mode);
+        case 4: return launch_batch_q4(b, grid, nv, which, s, mode);
+        case 5: return launch_batch_q5(b, grid, nv, which, s, mode);
+        case 6: return launch_batch_q6(b, grid, nv, which, s, mode);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+struct BatchGroup {
+    int qu = 0, nv = 0, mode = 0;
+    int lane = 0;                 // the stream that runs the group
+    std::vector<int> inst;        // the group's workspaces (indexes into wedg_batch::ws)
+    std::vector<int> order;       // inst by decreasing step count: the active ones are a prefix
+    std::vector<int> bfirst;      // first block of order[i]; bfirst[n] = all blocks
+    int* dmap = nullptr;          // device: binst [blocks], then bfirst [instances + 1]
+};
+
+// One stream of a batch and the stage arguments of its groups for a chunk of
+// steps, staged through two pinned buffers.
+struct BatchLane {
+    hipStream_t stream = nullptr;
+    size_t ninst = 0;             // instances of the lane's groups
+    size_t cap = 0;               // KArgs per buffer
+    KArgs* hargs[2] = {nullptr, nullptr};
+    KArgs* dargs[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    hipEvent_t done = nullptr;
+};
+
+struct wedg_batch {
+    std::vector<wedg_workspace*> ws;
+    std::vector<BatchGroup> groups;
+    std::vector<BatchLane> lanes;
+    hipStream_t stream = nullptr; // block maps, timing events, the join of the lanes
+    std::vector<void*> allocs;
+    long chunk = 1;               // steps per chunk of stage arguments
+    int* hmap = nullptr;          // pinned staging of the block maps
+    size_t map_ints = 0;
+    double last_ms = 0.0;         // device time of the last advance (stream events)
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+// Streams of a batch: the hardware queues of a process (GPU_MAX_HW_QUEUES, 4
+// by default), so that the lanes run concurrently.
+static int batch_lanes(int ngroups) {
+    int q = 4;
+    if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::max(1, std::atoi(e));
+    return std::max(1, std::min(ngroups, q));
+}
+
+// The FP64 operations of one element and stage without Chebyshev steps (traces,
+// volume terms, u-solve, lifts, Chebyshev set-up): the weight of a group when
+// the groups are dealt to the lanes.
+static double stage_cost(int nu, int nv, int nh) {
+    const double q = nu, v = nv, h = nh;
+    return 2 * (4 * (q * q + q * q) + 8 * (v * v + q * v)) +
+           2 * (2 * (q * q * q + q * q * q) + 2 * (q * v * v + q * q * v) + 2 * (v * q * q + v * v * q) +
+                2 * (q * q * q + q * q * q)) +
+           2 * q * q * q * q + 4 * (3 * q * (2 * v + q) + 3 * (2 * v * v + q * q)) + 2 * (h * v * v + h * h * v);
+}
+
+extern "C" wedg_status wedg_batch_destroy(wedg_batch* b) {
+    if (!b) return WEDG_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& l : b->lanes) {
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
+        for (int i = 0; i < 2; ++i) {
+            if (l.hargs[i]) (void)hipHostFree(l.hargs[i]);
+            if (l.copied[i]) (void)hipEventDestroy(l.copied[i]);
+        }
+        if (l.done) (void)hipEventDestroy(l.done);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+    }
+    if (b->hmap) (void)hipHostFree(b->hmap);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    for (void* a : b->allocs) (void)hipFree(a);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_batch** out) {
+    if (!ws || !out || n <= 0) return WEDG_ERR_ARGUMENT;
+    for (int i = 0; i < n; ++i) {
+        if (!ws[i]) return WEDG_ERR_ARGUMENT;
+        for (int j = 0; j < i; ++j)
+            if (ws[j] == ws[i]) return WEDG_ERR_ARGUMENT;   // one state per instance
+    }
+    auto* b = new wedg_batch();
+    b->ws.assign(ws, ws + n);
+    for (int i = 0; i < n; ++i) {
+        const wedg_problem* p = ws[i]->p;
+        auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BatchGroup& g) {
+            return g.qu == p->qu && g.nv == p->nv && g.mode == p->mode;
+        });
+        if (it == b->groups.end()) {
+            b->groups.emplace_back();
+            it = b->groups.end() - 1;
+            it->qu = p->qu; it->nv = p->nv; it->mode = p->mode;
+        }
+        it->inst.push_back(i);
+    }
+    // groups to lanes, largest first, each to the least loaded lane; the weight
+    // of a group is its cost per element and stage times its largest mesh
+    b->lanes.resize(batch_lanes((int)b->groups.size()));
+    {
+        std::vector<std::pair<double, int>> w;
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            const wedg_problem* p0 = ws[b->groups[g].inst[0]]->p;
+            double m = 0;
+            for (int i : b->groups[g].inst) m = std::max(m, (double)ws[i]->p->N);
+            w.push_back({stage_cost(p0->nu, p0->nv, p0->nh) * m, (int)g});
+        }
+        std::stable_sort(w.begin(), w.end(), [](auto& x, auto& y) { return x.first > y.first; });
+        std::vector<double> load(b->lanes.size(), 0.0);
+        for (auto& x : w) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[l] += x.first;
+            b->groups[x.second].lane = l;
+            b->lanes[l].ninst += b->groups[x.second].inst.size();
+        }
+    }
+    hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    // chunks of steps whose arguments fill about 4 MB over all lanes
+    b->chunk = (long)std::max<size_t>(1, ((size_t)4 << 20) / (4 * (size_t)n * sizeof(KArgs)));
+    for (auto& l : b->lanes) {
+        l.cap = (size_t)b->chunk * 4 * std::max<size_t>(1, l.ninst);
+        if (he == hipSuccess) he = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && he == hipSuccess; ++i) {
+            he = hipHostMalloc((void**)&l.hargs[i], l.cap * sizeof(KArgs), hipHostMallocDefault);
+            void* d = nullptr;
+            if (he == hipSuccess) he = hipMalloc(&d, l.cap * sizeof(KArgs));
+            if (he == hipSuccess) { b->allocs.push_back(d); l.dargs[i] = (KArgs*)d; }
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&l.copied[i], hipEventDisableTiming);
+        }
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
+    }
+    for (auto& g : b->groups) {
+        size_t blocks = 0;
+        for (int i : g.inst) blocks += ws[i]->nwaves();
+        const size_t ints = blocks + g.inst.size() + 1;
+        b->map_ints += ints;
+        void* d = nullptr;
+        if (he == hipSuccess) he = hipMalloc(&d, ints * sizeof(int));
+        if (he == hipSuccess) { b->allocs.push_back(d); g.dmap = (int*)d; }
+    }
+    if (he == hipSuccess) he = hipHostMalloc((void**)&b->hmap, b->map_ints * sizeof(int), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipEventCreate(&b->t0);
+    if (he == hipSuccess) he = hipEventCreate(&b->t1);
+    if (he != hipSuccess) { wedg_batch_destroy(b); return WEDG_ERR_HIP; }
+    *out = b;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
+                                          int stop_on_nonfinite, double* t_end, int* status) {
+    if (!b || !t0 || !dt || !nsteps) return WEDG_ERR_ARGUMENT;
+    const int n = (int)b->ws.size();
+    for (int i = 0; i < n; ++i)
+        if (nsteps[i] < 0) return WEDG_ERR_ARGUMENT;
+    wedg_status s;
+    for (int i = 0; i < n; ++i) {
+        if ((s = reset_err(b->ws[i]))) return s;
+        HIPOK(hipStreamSynchronize(b->ws[i]->stream));
+    }
+    // block maps: the instances of a group by decreasing step count
+    HIPOK(hipStreamSynchronize(b->stream));   // the pinned maps may be the source of a copy in flight
+    size_t mo = 0;
+    long maxsteps = 0;
+    for (auto& g : b->groups) {
+        g.order = g.inst;
+        std::stable_sort(g.order.begin(), g.order.end(), [&](int x, int y) { return nsteps[x] > nsteps[y]; });
+        g.bfirst.assign(1, 0);
+        int* m = b->hmap + mo;
+        int blk = 0;
+        for (size_t r = 0; r < g.order.size(); ++r) {
+            const int nw = (int)b->ws[g.order[r]]->nwaves();
+            for (int w = 0; w < nw; ++w) m[blk + w] = (int)r;
+            blk += nw;
+            g.bfirst.push_back(blk);
+        }
+        std::copy(g.bfirst.begin(), g.bfirst.end(), m + blk);
+        const size_t ints = blk + g.bfirst.size();
+        HIPOK(hipMemcpyAsync(g.dmap, m, ints * sizeof(int), hipMemcpyHostToDevice, b->stream));
+        mo += ints;
+        if (!g.order.empty()) maxsteps = std::max(maxsteps, nsteps[g.order[0]]);
+    }
+    HIPOK(hipEventRecord(b->t0, b->stream));
+    for (auto& l : b->lanes) HIPOK(hipStreamWaitEvent(l.stream, b->t0, 0));
+    std::vector<double> t(t0, t0 + n);   // every instance's time: t += dt after each of its steps
+    std::vector<double> tc;              // [instance][step of the chunk]: the time at the step's start
+    struct Launch { int g, which, grid; size_t off; };
+    std::vector<Launch> L;
+    int buf = 0;
+    for (long c0 = 0; c0 < maxsteps; c0 += b->chunk, buf ^= 1) {
+        const long c1 = std::min(maxsteps, c0 + b->chunk), len = c1 - c0;
+        tc.resize((size_t)n * len);
+        for (int i = 0; i < n; ++i)
+            for (long j = 0; j < len; ++j) {
+                tc[(size_t)i * len + j] = t[i];
+                if (c0 + j < nsteps[i]) t[i] += dt[i];
+            }
+        for (size_t li = 0; li < b->lanes.size(); ++li) {
+            BatchLane& l = b->lanes[li];
+            HIPOK(hipEventSynchronize(l.copied[buf]));   // the pinned buffer's last copy has run
+            KArgs* H = l.hargs[buf];
+            size_t k = 0;
+            L.clear();
+            for (long it = c0; it < c1; ++it)
+                for (int st = 1; st <= 4; ++st)
+                    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+                        const BatchGroup& g = b->groups[gi];
+                        if (g.lane != (int)li) continue;
+                        size_t na = 0;
+                        while (na < g.order.size() && nsteps[g.order[na]] > it) ++na;
+                        if (na == 0) continue;
+                        const size_t off = k;
+                        for (size_t r = 0; r < na; ++r) {
+                            const int i = g.order[r];
+                            H[k++] = rk4_args(b->ws[i], st, tc[(size_t)i * len + (it - c0)], dt[i], it + 1,
+                                              it == nsteps[i] - 1);
+                        }
+                        if (it == 0 && st == 1) L.push_back({(int)gi, 0, g.bfirst[na], off});
+                        L.push_back({(int)gi, 1, g.bfirst[na], off});
+                    }
+            if (k == 0) continue;
+            HIPOK(hipMemcpyAsync(l.dargs[buf], H, k * sizeof(KArgs), hipMemcpyHostToDevice, l.stream));
+            HIPOK(hipEventRecord(l.copied[buf], l.stream));
+            for (const Launch& x : L) {
+                const BatchGroup& g = b->groups[x.g];
+                BatchArgs ba;
+                ba.args = l.dargs[buf] + x.off;
+                ba.binst = g.dmap;
+                ba.bfirst = g.dmap + g.bfirst.back();
+                HIPOK(launch_batch(ba, x.grid, g.qu, g.nv, x.which, l.stream, g.mode));
+            }
+        }
+    }
+    for (auto& l : b->lanes) {
+        HIPOK(hipEventRecord(l.done, l.stream));
+        HIPOK(hipStreamWaitEvent(b->stream, l.done, 0));
+    }
+    HIPOK(hipEventRecord(b->t1, b->stream));
+    HIPOK(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    b->last_ms = hipEventElapsedTime(&ms, b->t0, b->t1) == hipSuccess ? ms : 0.0;
+    wedg_status first = WEDG_OK;
+    for (int i = 0; i < n; ++i) {
+        const wedg_status si = collect(b->ws[i], stop_on_nonfinite);
+        if (status) status[i] = si;
+        if (t_end) t_end[i] = t[i];
+        if (si != WEDG_OK && first == WEDG_OK) first = si;
+    }
+    return first;
+}
+
+extern "C" wedg_status wedg_batch_time(const wedg_batch* b, double* ms) {
+    if (!b || !ms) return WEDG_ERR_ARGUMENT;
+    *ms = b->last_ms;
+    return WEDG_OK;
 }

@@ -1,0 +1,14 @@
+// Batched kernel instantiations for q_u = 4 (n_u = 5), both pairings q_v = q_u - 1 and q_v = q_u.
+#include "wedg_kernels.h"
+#include "wedg_launch.h"
+#include "wedg_kern_inst.h"
+
+namespace wedg {
+
+hipError_t launch_batch_q4(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode) {
+    if (nv == 5) return launch_batch_nv<5, 5>(b, grid, which, s, mode);
+    if (nv == 4) return launch_batch_nv<5, 4>(b, grid, which, s, mode);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace wedg

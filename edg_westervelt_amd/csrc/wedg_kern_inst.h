@@ -24,6 +24,24 @@ static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) 
     return hipGetLastError();
 }
 
+// The batched launches of one (n_u, n_v) pair: grid blocks over the instances
+// of b; used by the batched instantiation files only.
+template <int NU, int NV>
+static hipError_t launch_batch_nv(const BatchArgs& b, int grid, int which, hipStream_t s, int mode) {
+    if (which == 0) {
+        if (mode == MODE_UNIFORM) k_prep<NU, NV, MODE_UNIFORM, true><<<grid, kWave, 0, s>>>(b);
+        else k_prep<NU, NV, MODE_GENERAL, true><<<grid, kWave, 0, s>>>(b);
+    } else if (which == 1) {
+        if (mode == MODE_UNIFORM) k_stage<NU, NV, MODE_UNIFORM, true><<<grid, kWave, 0, s>>>(b);
+        else if (mode == MODE_SEPARABLE) k_stage<NU, NV, MODE_SEPARABLE, true><<<grid, kWave, 0, s>>>(b);
+        else if (mode == MODE_GENERAL_CG) k_stage<NU, NV, MODE_GENERAL_CG, true><<<grid, kWave, 0, s>>>(b);
+        else k_stage<NU, NV, MODE_GENERAL, true><<<grid, kWave, 0, s>>>(b);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int NU, int NV>
 static hipError_t launch_diag_nv(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, hipStream_t s) {
     if (d) k_diag<NU, NV><<<(d->E + kWave - 1) / kWave, kWave, 0, s>>>(*d);

@@ -50,8 +50,9 @@ enum {
     MODE_GENERAL_CG = 3,  // as MODE_GENERAL, mass systems by conjugate gradients (mass_solver = :cg)
 };
 
-template <int NU_, int NV_>
+template <int NU_, int NV_, bool BT_ = false>
 struct Dims {
+    static constexpr bool BT = BT_;   // a batched launch (BatchArgs): the block's instance selects its KArgs
     static constexpr int NU = NU_, NV = NV_, NQ = NU_;
     static constexpr int QU = NU_ - 1, QV = NV_ - 1;
     static constexpr int NH0 = (3 * QV + 1) / 2 + 1;                 // ceil(3 q_v / 2) + 1
@@ -159,21 +160,47 @@ __device__ __forceinline__ cdbl* ctab(const double* p) {
 }
 
 
+// A batched launch advances the instances of one (n_u, n_v, mode) group at
+// once: each instance owns a contiguous range of blocks and its own KArgs.
+struct BatchArgs {
+    const KArgs* args;    // [instances] this stage's arguments of every instance
+    const int* binst;     // [blocks] the instance of each block
+    const int* bfirst;    // [instances] the first block of each instance
+};
+
 // The kernel's own arguments read through the kernarg segment pointer in the
 // constant address space, opaque to the optimizer like ctab(): fields are
 // loaded (scalar) where they are used instead of being kept in SGPRs across
-// the whole kernel. Every kernel takes one KArgs by value, at offset 0.
+// the whole kernel. A plain kernel takes one KArgs by value, at offset 0; a
+// batched one (BT) a BatchArgs, whose block-to-instance lookup is a pair of
+// scalar loads from read-only memory that the compiler forms once.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const WEDG_CONST_AS KArgs CKArgs;
+typedef const WEDG_CONST_AS BatchArgs CBatchArgs;
+typedef const WEDG_CONST_AS int cint;
+__device__ __forceinline__ CBatchArgs* bargs() { return (CBatchArgs*)__builtin_amdgcn_kernarg_segment_ptr(); }
+__device__ __forceinline__ int block_instance() { return ((cint*)bargs()->binst)[blockIdx.x]; }
+template <bool BT = false>
 __device__ __forceinline__ CKArgs* kargs() {
-    CKArgs* p = (CKArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    CKArgs* p;
+    if constexpr (BT) p = (CKArgs*)bargs()->args + block_instance();
+    else p = (CKArgs*)__builtin_amdgcn_kernarg_segment_ptr();
     asm volatile("" : "+s"(p));
     return p;
 }
 #else
 typedef const KArgs CKArgs;
-extern const KArgs* wedg_emu_kargs;   // set by a host build that runs the kernels on the CPU
-__device__ __forceinline__ CKArgs* kargs() { return wedg_emu_kargs; }
+typedef const BatchArgs CBatchArgs;
+typedef const int cint;
+extern const KArgs* wedg_emu_kargs;        // set by a host build that runs the kernels on the CPU
+extern const BatchArgs* wedg_emu_bargs;
+__device__ __forceinline__ CBatchArgs* bargs() { return wedg_emu_bargs; }
+__device__ __forceinline__ int block_instance() { return bargs()->binst[blockIdx.x]; }
+template <bool BT = false>
+__device__ __forceinline__ CKArgs* kargs() {
+    if constexpr (BT) return bargs()->args + block_instance();
+    else return wedg_emu_kargs;
+}
 #endif
 
 // A per-lane pointer, opaque to the optimizer: loads through it are not
@@ -210,11 +237,17 @@ __device__ __forceinline__ bool skip_stage(const KA& a) {
 
 // First element of the block. Blocks are dealt to the 8 XCDs in turn; this
 // gives each XCD a contiguous band of elements, so the neighbours across k
-// live in the same L2 except at the band edges.
+// live in the same L2 except at the band edges. In a batched launch, the
+// block's place in its instance's range.
+template <bool BT = false>
 __device__ __forceinline__ int block_first_element() {
-    const int b = blockIdx.x, nb = gridDim.x;
-    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-    return lb * kWave;
+    if constexpr (BT) {
+        return ((int)blockIdx.x - ((cint*)bargs()->bfirst)[block_instance()]) * kWave;
+    } else {
+        const int b = blockIdx.x, nb = gridDim.x;
+        const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+        return lb * kWave;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -441,7 +474,7 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
     constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV, NH = D::NH;
     int n_e = 1;
     if (count) {
-        CKArgs& A = *kargs();
+        CKArgs& A = *kargs<D::BT>();
         const double* betav = A.beta + (UNI ? 0 : (long)A.cls_b[e] * NH * NH);
         cdbl* Ph = ctab(A.tab) + D::oPh;
         cdbl* bu = ctab(A.beta);
@@ -474,7 +507,7 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
             n_e = 1;
         }
     }
-    CKArgs& B = *kargs();
+    CKArgs& B = *kargs<D::BT>();
     const int N = B.N, j = e % N, k = e / N;
     if (j == 0 || j == N - 1 || k == 0 || k == N - 1) {
         double U[NU * NU];
@@ -485,7 +518,7 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
             constexpr int s = decltype(sc)::value;
             const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
             if (!bnd) return;
-            CKArgs& A = *kargs();
+            CKArgs& A = *kargs<D::BT>();
             cdbl* T = ctab(A.tab);
             double du[NQ], vt[NQ], dv[NQ];
             side_trace<D, NU, true, s>(T, D::oE1r, U, du);
@@ -515,32 +548,33 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
 }
 
 // Writes the wave's largest count into the per-wave slot of the stage parity.
+template <bool BT>
 __device__ __forceinline__ void wave_count(int n_e, int slot, int E) {
     for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
     if ((threadIdx.x & 63) == 0) {
         const int nw = (E + kWave - 1) / kWave;
-        kargs()->cnt[slot * nw + (block_first_element() / kWave)] = n_e;
+        kargs<BT>()->cnt[slot * nw + (block_first_element<BT>() / kWave)] = n_e;
     }
 }
 
 // The first stage of a call: the count and boundary slots of its input state.
-template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_prep(const KArgs args) {
+template <int NU, int NV, int MODE, bool BT = false>
+__global__ __launch_bounds__(kWave) void k_prep(const std::conditional_t<BT, BatchArgs, KArgs> args) {
     (void)args;
-    using D = Dims<NU, NV>;
+    using D = Dims<NU, NV, BT>;
     constexpr bool UNI = MODE == MODE_UNIFORM;
-    const int E = kargs()->E;
-    const int e = block_first_element() + threadIdx.x;
-    const bool global = kargs()->cheb_global != 0;
+    const int E = kargs<D::BT>()->E;
+    const int e = block_first_element<D::BT>() + threadIdx.x;
+    const bool global = kargs<D::BT>()->cheb_global != 0;
     int n_e = 1;
     if (e < E) {
-        const double* yv = kargs()->yv;
+        const double* yv = kargs<D::BT>()->yv;
         double V[NV * NV];
 #pragma unroll
         for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
-        n_e = stage_prep<D, UNI>(kargs()->yu, V, e, E, kargs()->serial, kargs()->ct, kargs()->st, global);
+        n_e = stage_prep<D, UNI>(kargs<D::BT>()->yu, V, e, E, kargs<D::BT>()->serial, kargs<D::BT>()->ct, kargs<D::BT>()->st, global);
     }
-    if (global) wave_count(n_e, kargs()->slot, E);
+    if (global) wave_count<D::BT>(n_e, kargs<D::BT>()->slot, E);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,7 +689,7 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, con
     constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int ns = s ^ 1;   // the neighbour's facing side: B <-> T, L <-> R
-    CKArgs& A = *kargs();
+    CKArgs& A = *kargs<D::BT>();
     int fidx, nb, m;
     bool bnd;
     if (s == 0)      { fidx = j + (N + 1) * k;     nb = e - 1; bnd = j == 0;     m = k; }
@@ -688,7 +722,7 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, con
     }
     double wA[NQ], wD[NQ], wF[NQ];
     {
-        CKArgs& I = *kargs();
+        CKArgs& I = *kargs<D::BT>();
         cdbl* Tw = ctab(I.tab) + D::oW;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
@@ -702,7 +736,7 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, con
         }
     }
     if (bnd) {
-        CKArgs& B = *kargs();
+        CKArgs& B = *kargs<D::BT>();
         cdbl* Tw = ctab(B.tab) + D::oW;
         const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
 #pragma unroll
@@ -744,7 +778,7 @@ __device__ __forceinline__ void apply_lifts(const double* Ys, int lane, double (
     }
 #pragma unroll
     for (int b = 0; b < NU; ++b) YF[b] = Ys[(y0 + 2 * NV + b) * kWave + lane];
-    cdbl* T = ctab(kargs()->tab);
+    cdbl* T = ctab(kargs<D::BT>()->tab);
     constexpr int oE = plus ? D::oTEl : D::oTEr;      // (2/h) E
     constexpr int oE1 = plus ? D::oFE1l : D::oFE1r;   // (4/h^2) E1
     constexpr int oF1 = plus ? D::oE1l : D::oE1r;     // E1
@@ -879,21 +913,21 @@ __device__ __forceinline__ bool rk_store(int stage, double c, const double (&kv)
 // block: the lane's LDS slots hold first the lift vectors of its four sides,
 // then the Chebyshev weights G = 2 beta v_q W of the first mirrored row pairs
 // ([entry][lane], conflict-free); the rest of G stays in registers.
-template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
+template <int NU, int NV, int MODE, bool BT = false>
+__global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, BatchArgs, KArgs> args) {
     (void)args;
-    using D = Dims<NU, NV>;
+    using D = Dims<NU, NV, BT>;
     constexpr int NQ = D::NQ, NH = D::NH, NP = D::NP, PL = D::PL, PR = D::PR;
     constexpr bool UNI = MODE == MODE_UNIFORM;
     constexpr bool FD = MODE == MODE_UNIFORM || MODE == MODE_SEPARABLE;
     constexpr bool CG = MODE == MODE_GENERAL_CG;
     constexpr int nU = NU * NU, nV = NV * NV;
     __shared__ double Gs[D::NLDS * kWave];
-    if (skip_stage(*kargs())) return;
-    const int E = kargs()->E, N = kargs()->N;
-    const bool global = kargs()->cheb_global != 0;
+    if (skip_stage(*kargs<D::BT>())) return;
+    const int E = kargs<D::BT>()->E, N = kargs<D::BT>()->N;
+    const bool global = kargs<D::BT>()->cheb_global != 0;
     const int lane = threadIdx.x;
-    const int e = block_first_element() + lane;
+    const int e = block_first_element<D::BT>() + lane;
     const int nw = (E + kWave - 1) / kWave;
     // Staggered start on grids that fill the chip: with one wave per SIMD all
     // waves would load (and store) at the same moments. The waves start in four
@@ -909,7 +943,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             b = (jj == 0 || jj == N - 1 || kk == 0 || kk == N - 1) ? 1 : 0;
         }
         for (int off = 32; off > 0; off >>= 1) b = max(b, __shfl_xor(b, off));
-        const int w = block_first_element() / kWave;
+        const int w = block_first_element<D::BT>() / kWave;
         const int g = b ? (w & 1) : 2 + (w & 1);
         for (int i = 0; i < g; ++i) __builtin_amdgcn_s_sleep(127);
     }
@@ -921,7 +955,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
     {
         int m = 1;
         if (global) {
-            const int* wc = kargs()->cnt + kargs()->slot * nw;
+            const int* wc = kargs<D::BT>()->cnt + kargs<D::BT>()->slot * nw;
             for (int i0 = lane; i0 < nw; i0 += 8 * kWave) {
                 int t[8];
 #pragma unroll
@@ -931,8 +965,8 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             }
         }
         const int ec = e < E ? e : E - 1;
-        const double* yu = kargs()->yu;
-        const double* yv = kargs()->yv;
+        const double* yu = kargs<D::BT>()->yu;
+        const double* yv = kargs<D::BT>()->yv;
 #pragma unroll
         for (int mm = 0; mm < nU; ++mm) U[mm] = yu[(long)mm * E + ec];
 #pragma unroll
@@ -945,9 +979,9 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
     int ncheb = 0, n_next = 1;
     if (e < E) {
         const int j = e % N, k = e / N;
-        const long cc = UNI ? 0 : (long)kargs()->cls_ca[e];
-        const double* Wc = kargs()->Cw + cc * NQ * NQ;
-        const double* Wa = kargs()->Aw + cc * NQ * NQ;
+        const long cc = UNI ? 0 : (long)kargs<D::BT>()->cls_ca[e];
+        const double* Wc = kargs<D::BT>()->Cw + cc * NQ * NQ;
+        const double* Wa = kargs<D::BT>()->Aw + cc * NQ * NQ;
 
         // ---- face slots of the four sides and their lift vectors (src/rhs.jl:421-473,
         // 589-776), into the lane's LDS slots
@@ -967,20 +1001,20 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
 #pragma unroll
         for (int m = 0; m < nU; ++m) bv[m] = 0.0;
         const double v0 = V[0];
-        vol_pass<D, NV, NU>(kargs()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
-        vol_pass<D, NV, NV>(kargs()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
+        vol_pass<D, NV, NU>(kargs<D::BT>()->tab, V, Wc, bv);    // (c^2 grad phi_u, grad v_h)
+        vol_pass<D, NV, NV>(kargs<D::BT>()->tab, V, Wa, Rv);    // (alpha grad phi_v, grad v_h)
         phase_barrier();
-        vol_pass<D, NU, NV>(kargs()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
+        vol_pass<D, NU, NV>(kargs<D::BT>()->tab, U, Wc, Rv);    // (c^2 grad phi_v, grad u_h)
         phase_barrier();
         // scale and source, in the order of the reference's sum (src/rhs.jl:340-350)
         {
-            const double s = -kargs()->four_hh;
+            const double s = -kargs<D::BT>()->four_hh;
 #pragma unroll
             for (int m = 0; m < nV; ++m) Rv[m] = s * Rv[m];
-            const int np = kargs()->nproj;
+            const int np = kargs<D::BT>()->nproj;
             for (int t = 0; t < np; ++t) {
-                const double pc = kargs()->pc[t];
-                const double* pr = kargs()->proj[t];
+                const double pc = kargs<D::BT>()->pc[t];
+                const double* pr = kargs<D::BT>()->proj[t];
                 double pv[nV];
 #pragma unroll
                 for (int m = 0; m < nV; ++m) pv[m] = pr[(long)m * E + e];
@@ -1003,7 +1037,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
         if (FD) {
             // A_e = c^2 (I (x) K + K (x) I) + e0 e0^T with K = Q diag(lambda) Q^T:
             // A_e^-1 b = (Q (x) Q) (Dc o ((Q (x) Q)^T b)); Q pairs modes of one parity.
-            cdbl* Q = ctab(kargs()->tab) + D::oQ;
+            cdbl* Q = ctab(kargs<D::BT>()->tab) + D::oQ;
             double Y1[nU];
 #pragma unroll
             for (int b = 0; b < NU; ++b)
@@ -1014,8 +1048,8 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
                     for (int aa = (i & 1); aa < NU; aa += 2) s = fma(Q[aa + NU * i], bv[aa + NU * b], s);
                     Y1[i + NU * b] = s;
                 }
-            const double* Dv = kargs()->Ainv + cc * nU;
-            cdbl* Du = ctab(kargs()->Ainv);
+            const double* Dv = kargs<D::BT>()->Ainv + cc * nU;
+            cdbl* Du = ctab(kargs<D::BT>()->Ainv);
 #pragma unroll
             for (int i = 0; i < NU; ++i) {
                 const double* Dr = UNI ? Dv : vfresh(Dv);
@@ -1049,7 +1083,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             // the stored inverse of the class, symmetric: row mm is the contiguous
             // column mm; the results pass through the lane's LDS slots (the weights
             // of the mass solve are written there only afterwards)
-            const double* Ai = kargs()->Ainv + cc * (nU * nU);
+            const double* Ai = kargs<D::BT>()->Ainv + cc * (nU * nU);
 #pragma unroll 1
             for (int mm = 0; mm < nU; ++mm) {
                 const double* Ar = Ai + (long)nU * mm;
@@ -1062,11 +1096,11 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             for (int mm = 0; mm < nU; ++mm) ku[mm] = Gs[(mm % D::NLDS) * kWave + lane];
         }
         {
-            CKArgs& A = *kargs();
+            CKArgs& A = *kargs<D::BT>();
             double yu0[nU], acu[nU];
             rk_load<nU>(A.stage, A.y0u, A.accu, E, e, yu0, acu);
             if (!rk_store<nU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, e))
-                atomicMin(&kargs()->err->nonfinite, kargs()->step);
+                atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
         }
 
         phase_barrier();
@@ -1077,17 +1111,17 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
         double th = 1.0, de = 1.0, s1 = 1.0;
         int n_e;
         {
-            const double* yv = kargs()->yv;
+            const double* yv = kargs<D::BT>()->yv;
             double V[nV];
 #pragma unroll
             for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
-            const double* betav = kargs()->beta + (UNI ? 0 : (long)kargs()->cls_b[e] * NH * NH);
+            const double* betav = kargs<D::BT>()->beta + (UNI ? 0 : (long)kargs<D::BT>()->cls_b[e] * NH * NH);
             double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
             bool nan = false;
 #pragma unroll
             for (int p = 0; p < (NH + 1) / 2; ++p) {
-                cdbl* T = ctab(kargs()->tab);
-                cdbl* bu = ctab(kargs()->beta);
+                cdbl* T = ctab(kargs<D::BT>()->tab);
+                cdbl* bu = ctab(kargs<D::BT>()->beta);
                 const bool mid = (NH & 1) && p == NH / 2;
                 double yp[NH], yq[NH];
                 vq_rows<D>(T + D::oPh, p, mid, V, yp, yq);
@@ -1117,7 +1151,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
                     }
                 }
             }
-            n_e = cheb_interval(gmin, gmax, nan, kargs()->cheb_L, th, de, s1);
+            n_e = cheb_interval(gmin, gmax, nan, kargs<D::BT>()->cheb_L, th, de, s1);
         }
         phase_barrier();
         double X[nV];
@@ -1127,7 +1161,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             // conjugate gradients on M X = Rv, M = I - K, element by element
             // (src/rhs.jl:824-853): X0 = Rv, r0 = K Rv, until |r| <= tol |Rv| or
             // n_v^2 + 10 steps; a lane that has converged keeps its values.
-            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
+            cdbl* Ph = ctab(kargs<D::BT>()->tab) + D::oPh;
             auto apply_K = [&](const double (&d)[nV], double (&out)[nV]) {
 #pragma unroll
                 for (int m = 0; m < nV; ++m) out[m] = 0.0;
@@ -1148,7 +1182,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
                 }
                 if (NH & 1) kd_mid<D>(Ph, d, Gm, out);
             };
-            const double tol = kargs()->mass_tol;
+            const double tol = kargs<D::BT>()->mass_tol;
             double r[nV], pv[nV], Ap[nV];
 #pragma unroll
             for (int m = 0; m < nV; ++m) X[m] = Rv[m];
@@ -1176,7 +1210,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
                     pAp = fma(pv[m], Ap[m], pAp);
                 }
                 if (act && !(pAp > 0.0)) {
-                    atomicMin(&kargs()->err->fkey, fail_key(kargs()->serial, 1, e + 1));
+                    atomicMin(&kargs<D::BT>()->err->fkey, fail_key(kargs<D::BT>()->serial, 1, e + 1));
                     act = false;
                     bad = true;
                 }
@@ -1196,10 +1230,10 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
                     act = sqrt(rr) > tol * r0;
                 }
             }
-            if (!bad && sqrt(rr) > tol * r0) atomicAdd(&kargs()->err->cg_unconverged, 1);
+            if (!bad && sqrt(rr) > tol * r0) atomicAdd(&kargs<D::BT>()->err->cg_unconverged, 1);
             ncheb = it;
         } else if (n_e == 0) {   // with the global policy stage_prep has recorded it already
-            if (!global) atomicMin(&kargs()->err->fkey, fail_key(kargs()->serial, 1, e + 1));
+            if (!global) atomicMin(&kargs<D::BT>()->err->fkey, fail_key(kargs<D::BT>()->serial, 1, e + 1));
         } else {
             const int n = global ? nglob : n_e;
             ncheb = n;
@@ -1209,7 +1243,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             double rho = 1.0 / s1;
             // the table, one scalar load for the whole iteration (the mirrored
             // half of P_h, 27 values at q = 5, stays in SGPRs)
-            cdbl* Ph = ctab(kargs()->tab) + D::oPh;
+            cdbl* Ph = ctab(kargs<D::BT>()->tab) + D::oPh;
             for (int step = 1; step <= n; ++step) {
 #pragma unroll
                 for (int m = 0; m < nV; ++m) X[m] += d[m];
@@ -1241,28 +1275,28 @@ __global__ __launch_bounds__(kWave) void k_stage(const KArgs args) {
             }
         }
         {
-            CKArgs& A = *kargs();
+            CKArgs& A = *kargs<D::BT>();
             double yv0[nV], acv[nV];
             const int stage = A.stage;
             const double c = A.cstage;
             rk_load<nV>(stage, A.y0v, A.accv, E, e, yv0, acv);
             if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e))
-                atomicMin(&kargs()->err->nonfinite, kargs()->step);
+                atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
             if (A.has_next) {
                 // the new state v, as rk_store wrote it; u is read back from outu
                 double Vn[nV];
 #pragma unroll
                 for (int m = 0; m < nV; ++m) Vn[m] = stage < 4 ? yv0[m] + c * X[m] : yv0[m] + c * (acv[m] + X[m]);
-                CKArgs& B = *kargs();
+                CKArgs& B = *kargs<D::BT>();
                 n_next = stage_prep<D, UNI>(B.outu, Vn, e, E, B.serial + 1, B.nct, B.nst, global);
             }
         }
     }
-    if (global && kargs()->has_next) wave_count(n_next, kargs()->slot ^ 1, E);
+    if (global && kargs<D::BT>()->has_next) wave_count<D::BT>(n_next, kargs<D::BT>()->slot ^ 1, E);
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
     if (lane == 0 && ncheb > 0) {   // this wave's slots only: no atomics
-        unsigned* ws = kargs()->wstat;
-        const int w = block_first_element() / kWave;
+        unsigned* ws = kargs<D::BT>()->wstat;
+        const int w = block_first_element<D::BT>() / kWave;
         ws[w] += (unsigned)ncheb;
         ws[nw + w] = max(ws[nw + w], (unsigned)ncheb);
     }

@@ -215,6 +215,25 @@ wedg_status wedg_cheb_stats(const wedg_workspace* ws, double* mean_steps, long* 
  * byte counters of a profiler; *ms receives the average time per copy. */
 wedg_status wedg_probe_copy(size_t n, int repeats, double* ms);
 
+/* Batched instances. The reference advances one instance per call (the loop
+ * of solve, src/solver.jl:213-216); a batch advances n independent workspaces
+ * (uneven N, q_u, q_v, coefficients, boundary kinds, time steps and step
+ * counts) together: one kernel launch per RK stage for each group of
+ * instances with the same (q_u, q_v) and coefficient layout, on the batch's
+ * own stream. Instance i runs nsteps[i] steps of dt[i] from t0[i] exactly as
+ * wedg_advance(ws[i], t0[i], dt[i], nsteps[i], ...) would (same stage
+ * arguments, bitwise the same state); status[i] and t_end[i] (either may be
+ * null) receive its status and final time, wedg_last_error(ws[i]) its failure
+ * details; the return value is the first non-zero status. A workspace may
+ * appear in a batch once; it must not be used elsewhere while the batch runs.
+ * wedg_batch_time gives the device time of the last advance in ms. */
+typedef struct wedg_batch wedg_batch;
+wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_batch** out);
+wedg_status wedg_batch_destroy(wedg_batch* b);
+wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
+                               int stop_on_nonfinite, double* t_end, int* status);
+wedg_status wedg_batch_time(const wedg_batch* b, double* ms);
+
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);
 wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);

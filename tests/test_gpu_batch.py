@@ -1,0 +1,148 @@
+"""Batched instances on the device (wedg_batch_*, through the C ABI): every
+instance of a batch ends bitwise where its own wedg_advance takes it (the same
+kernels get the same stage arguments), within the RK4 tolerance of the oracle,
+and a failing instance reports its own error without touching the others.
+
+The instances are the uneven workload of edg_westervelt_amd.batch
+(initial-value mode, piecewise-constant coefficients with jumps on a 4 x 4
+block grid, every flux family and boundary kind, q_u 2..6, both pairings);
+the tests run a few steps of each.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import edg_westervelt_amd.batch as B
+from edg_westervelt_amd import _lib
+from edg_westervelt_amd._lib import lib
+from oracle import batched as ob
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def solo(b, nsteps, v0=None):
+    """The instance alone from the zero state (or v0): state, end time, status, error info."""
+    B.set_zero_state(b.ws)
+    if v0 is not None:
+        u = np.zeros((b.ws.N, b.ws.N, b.ws.qu1, b.ws.qu1), order="F")
+        b.ws.dev.check(lib.wedg_set_state(b.ws.dev.h, u.ctypes.data_as(C.POINTER(C.c_double)),
+                                          v0.ctypes.data_as(C.POINTER(C.c_double))))
+    t = C.c_double(0.0)
+    st = lib.wedg_advance(b.ws.dev.h, 0.0, b.dt, int(nsteps), 0, C.byref(t))
+    info = b.ws.dev.last_error()
+    u, v = B.get_state(b.ws)
+    return u, v, t.value, st, (info.status, info.side, info.key, info.element)
+
+
+def describe(b):
+    x = b.inst
+    return (f"instance {x.index}: q_u={x.q} mixed={x.mixed} N={x.N} {x.flux} {x.boundary} "
+            f"one_sided={b.pre.one_sided}")
+
+
+def test_batch_bitwise_equals_separate_runs():
+    insts = B.uneven_instances(seed=0, count=12)
+    steps = [3, 7, 0, 5, 1, 9, 4, 2, 6, 8, 3, 5]
+    built = [B.build_instance(x) for x in insts]
+    ref = []
+    for b, n in zip(built, steps):
+        say("solo", describe(b), "steps", n)
+        ref.append(solo(b, n))
+    for b in built:
+        B.set_zero_state(b.ws)
+    say("batch of", len(built))
+    bt = B.Batch([b.ws for b in built])
+    tend, status = bt.advance(0.0, [b.dt for b in built], steps)
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        assert status[i] == _lib.OK
+        assert tend[i] == ref[i][2]
+        assert np.array_equal(u, ref[i][0]) and np.array_equal(v, ref[i][1]), describe(b)
+    # a second advance of the same batch continues from the states it left
+    tend2, _ = bt.advance(tend, [b.dt for b in built], [2] * len(built))
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        B.set_zero_state(b.ws)
+        uu, vv, t, _, _ = solo(b, steps[i] + 2)
+        assert np.array_equal(u, uu) and np.array_equal(v, vv), describe(b)
+        assert tend2[i] == t
+
+
+def test_batch_matches_oracle():
+    """Four instances of different groups, four steps each, against the oracle's RK4."""
+    insts = B.uneven_instances(seed=0, count=16)
+    pick = [0, 3, 7, 12]   # q_u 5, 6, 3 and 2
+    built = [B.build_instance(insts[i]) for i in pick]
+    bt = B.Batch([b.ws for b in built])
+    bt.advance(0.0, [b.dt for b in built], [4] * len(built))
+    for b in built:
+        say("oracle", describe(b))
+        u, v = B.get_state(b.ws)
+        z_u = np.zeros_like(u, order="F")
+        z_v = np.zeros_like(v, order="F")
+        uo, vo, _, _ = ob.rk4_steps(ob.OraclePre(b.pre, b.inst.N), z_u, z_v, 0.0, b.dt, 4)
+        assert np.abs(vo).max() > 0
+        assert rel(uo, u) < 1e-11 and rel(vo, v) < 1e-11, describe(b)
+
+
+def test_batch_failure_stays_with_its_instance():
+    """One instance starts with v_h = 1 (1 - 2 beta v_h <= 0 wherever beta >= 1/2):
+    it stops with the error (and message) its own run gives; the others are
+    unaffected."""
+    insts = B.uneven_instances(seed=0, count=6)
+    built = [B.build_instance(x) for x in insts]
+    bad = 2
+    v0 = np.zeros((built[bad].inst.N, built[bad].inst.N, built[bad].ws.qv1, built[bad].ws.qv1), order="F")
+    v0[:, :, 0, 0] = 2.0   # v_h = 2 L_0(y) L_0(x) = 1
+    steps = [3] * len(built)
+    ref = [solo(b, 3, v0 if i == bad else None) for i, b in enumerate(built)]
+    assert ref[bad][3] != _lib.OK
+    say("bad instance status", ref[bad][3], ref[bad][4])
+    for b in built:
+        B.set_zero_state(b.ws)
+    u = np.zeros((built[bad].ws.N, built[bad].ws.N, built[bad].ws.qu1, built[bad].ws.qu1), order="F")
+    built[bad].ws.dev.check(lib.wedg_set_state(built[bad].ws.dev.h, u.ctypes.data_as(C.POINTER(C.c_double)),
+                                               v0.ctypes.data_as(C.POINTER(C.c_double))))
+    bt = B.Batch([b.ws for b in built])
+    _, status = bt.advance(0.0, [b.dt for b in built], steps, raise_errors=False)
+    for i, b in enumerate(built):
+        info = b.ws.dev.last_error()
+        assert status[i] == ref[i][3]
+        assert (info.status, info.side, info.key, info.element) == ref[i][4]
+        if i != bad:
+            uu, vv = B.get_state(b.ws)
+            assert np.array_equal(uu, ref[i][0]) and np.array_equal(vv, ref[i][1]), describe(b)
+    with pytest.raises(_lib.WedgError) as e:
+        bt.advance(0.0, [b.dt for b in built], [1] * len(built))
+    assert str(e.value) == _lib.error_message(built[bad].pre, built[bad].ws.dev.last_error())
+
+
+def test_batch_full_workload_two_steps():
+    """All 64 instances of the uneven workload (every group, N up to 64), two steps."""
+    built = [B.build_instance(x) for x in B.uneven_instances(seed=0, count=64)]
+    ref = [solo(b, 2)[:2] for b in built]
+    for b in built:
+        B.set_zero_state(b.ws)
+    bt = B.Batch([b.ws for b in built])
+    _, status = bt.advance(0.0, [b.dt for b in built], [2] * len(built))
+    assert (status == _lib.OK).all()
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        assert np.array_equal(u, ref[i][0]) and np.array_equal(v, ref[i][1]), describe(b)
+
+
+def test_batch_rejects_a_workspace_twice():
+    b = B.build_instance(B.uneven_instances(seed=0, count=1)[0])
+    arr = (C.c_void_p * 2)(b.ws.dev.h.value, b.ws.dev.h.value)
+    h = C.c_void_p()
+    assert lib.wedg_batch_create(arr, 2, C.byref(h)) == _lib.ERR_ARGUMENT
+    with pytest.raises(ValueError):
+        B.Batch([b.ws]).advance(0.0, [b.dt, b.dt], [1, 1])
