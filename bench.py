@@ -17,6 +17,15 @@ sides) of the whole mesh; the state stays in HBM for the timed region.
 Prints one JSON line: DOF-stages/s, the roofline of the dominant kernel from
 HIP events on the workspace stream, and the CPU oracle (C/OpenMP restatement
 of the reference's algorithm) timed on a bounded sample on this host.
+
+The line also carries the batched speedup ("batched"): the 64 uneven
+instances of edg_westervelt_amd.batch.uneven_instances(seed=0) (N in
+{16..64}, q_u in 2..6, both pairings, every flux and boundary kind,
+piecewise-constant coefficients with jumps, 240 to 5,760 steps to T = 0.25)
+advanced together to T (one launch per RK stage per basis group, the groups on
+concurrent streams) against the same 64 solves one after the other
+(wedg_advance each), in aggregate DOF-stages/s; the batched states are checked
+bitwise against the sequential ones.
 """
 from __future__ import annotations
 
@@ -73,6 +82,51 @@ def flops_per_element_stage(nu, nv, nh, nc):
     return base + (nc - 1) * step, traces
 
 
+def bench_batched(count=64, seed=0, sequential=True):
+    """Aggregate DOF·stages/s of the uneven batch: together vs one after the other."""
+    import edg_westervelt_amd.batch as B
+    from edg_westervelt_amd._lib import lib
+    t = time.perf_counter()
+    built = [B.build_instance(x) for x in B.uneven_instances(seed=seed, count=count)]
+    t_build = time.perf_counter() - t
+    work = sum(4 * b.nsteps * b.dof for b in built)
+    bt = B.Batch([b.ws for b in built])
+    bt.advance(0.0, [b.dt for b in built], [1] * count)          # warm-up step, then back to t = 0
+    for b in built:
+        B.set_zero_state(b.ws)
+    print(f"# batched: {count} instances together", file=sys.stderr, flush=True)
+    c0 = time.perf_counter()
+    bt.advance(0.0, [b.dt for b in built], [b.nsteps for b in built])
+    wb = time.perf_counter() - c0
+    dev_ms = bt.device_ms()
+    states = [B.get_state(b.ws) for b in built]
+    seq = 0.0
+    same = True
+    longest = 0.0
+    if sequential:
+        print(f"# batched: {count} instances one after the other", file=sys.stderr, flush=True)
+    for i, b in enumerate(built if sequential else []):
+        B.set_zero_state(b.ws)
+        tend = C.c_double(0.0)
+        c0 = time.perf_counter()
+        b.ws.dev.check(lib.wedg_advance(b.ws.dev.h, 0.0, b.dt, b.nsteps, 0, C.byref(tend)))
+        d = time.perf_counter() - c0
+        seq += d
+        longest = max(longest, d)
+        u, v = B.get_state(b.ws)
+        same = same and np.array_equal(u, states[i][0]) and np.array_equal(v, states[i][1])
+    groups = len({(b.inst.q, b.inst.mixed, b.ws.problem.pre.one_sided) for b in built})
+    if not sequential:
+        return dict(value=work / wb, unit="DOF·stages/s", wall_s=wb, device_s=dev_ms / 1e3, dof_stages=work,
+                    instances=count)
+    return dict(value=work / wb, unit="DOF·stages/s", sequential_value=work / seq, speedup=seq / wb,
+                wall_s=wb, device_s=dev_ms / 1e3, sequential_s=seq, longest_instance_s=longest,
+                dof_stages=work, instances=count, groups=groups, bitwise_equal_to_sequential=bool(same),
+                workload=f"edg_westervelt_amd.batch.uneven_instances(seed={seed}, count={count}), T = 0.25, "
+                         "initial_value Gaussian, piecewise-constant c, alpha, beta on 4x4 blocks",
+                build_s=round(t_build, 2))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +139,10 @@ def main():
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--batched", choices=("on", "off", "only"), default="on",
+                    help="the uneven-batch speedup: with the headline (on), not at all (off), or alone (only)")
+    ap.add_argument("--batched-sequential", choices=("on", "off"), default="on",
+                    help="off: time only the batched run (for a kernel trace of it)")
     ap.add_argument("--probe-copy", type=float, default=0.0,
                     help="only run a plain copy kernel of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -105,6 +163,11 @@ def main():
             raise SystemExit(f"copy probe failed ({st})")
         print(json.dumps(dict(copy_probe_bytes_each_way=8 * n, launches=11, ms_avg=ms.value,
                               gbs=2 * 8 * n / ms.value / 1e6)))
+        return
+    if args.batched == "only":
+        bb = bench_batched(sequential=args.batched_sequential == "on")
+        print(json.dumps(dict(metric="batched speedup (uneven instances)", value=bb.get("speedup"), unit="x",
+                              n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched=bb)))
         return
 
     q, N, T = args.q, args.N, 0.5
@@ -201,6 +264,8 @@ def main():
                  hbm_frac_model=bytes_step / 4 / stage_s / 1e9 / HBM_PEAK_GBS,
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
+    batched = bench_batched() if args.batched == "on" else None
+
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
     # problem, a bounded sample of RK4 steps from the same initial state
     cpu = None
@@ -229,7 +294,7 @@ def main():
                                 "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
-               final_state_finite=finite)
+               batched=batched, final_state_finite=finite)
     print(json.dumps(out))
 
 

@@ -1210,7 +1210,6 @@ struct BatchGroup {
 // steps, staged through two pinned buffers.
 struct BatchLane {
     hipStream_t stream = nullptr;
-    size_t ninst = 0;             // instances of the lane's groups
     size_t cap = 0;               // KArgs per buffer
     KArgs* hargs[2] = {nullptr, nullptr};
     KArgs* dargs[2] = {nullptr, nullptr};
@@ -1292,31 +1291,12 @@ extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_
         }
         it->inst.push_back(i);
     }
-    // groups to lanes, largest first, each to the least loaded lane; the weight
-    // of a group is its cost per element and stage times its largest mesh
     b->lanes.resize(batch_lanes((int)b->groups.size()));
-    {
-        std::vector<std::pair<double, int>> w;
-        for (size_t g = 0; g < b->groups.size(); ++g) {
-            const wedg_problem* p0 = ws[b->groups[g].inst[0]]->p;
-            double m = 0;
-            for (int i : b->groups[g].inst) m = std::max(m, (double)ws[i]->p->N);
-            w.push_back({stage_cost(p0->nu, p0->nv, p0->nh) * m, (int)g});
-        }
-        std::stable_sort(w.begin(), w.end(), [](auto& x, auto& y) { return x.first > y.first; });
-        std::vector<double> load(b->lanes.size(), 0.0);
-        for (auto& x : w) {
-            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[l] += x.first;
-            b->groups[x.second].lane = l;
-            b->lanes[l].ninst += b->groups[x.second].inst.size();
-        }
-    }
     hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     // chunks of steps whose arguments fill about 4 MB over all lanes
     b->chunk = (long)std::max<size_t>(1, ((size_t)4 << 20) / (4 * (size_t)n * sizeof(KArgs)));
     for (auto& l : b->lanes) {
-        l.cap = (size_t)b->chunk * 4 * std::max<size_t>(1, l.ninst);
+        l.cap = (size_t)b->chunk * 4 * (size_t)n;   // any share of the groups
         if (he == hipSuccess) he = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
         for (int i = 0; i < 2 && he == hipSuccess; ++i) {
             he = hipHostMalloc((void**)&l.hargs[i], l.cap * sizeof(KArgs), hipHostMallocDefault);
@@ -1376,6 +1356,23 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
         HIPOK(hipMemcpyAsync(g.dmap, m, ints * sizeof(int), hipMemcpyHostToDevice, b->stream));
         mo += ints;
         if (!g.order.empty()) maxsteps = std::max(maxsteps, nsteps[g.order[0]]);
+    }
+    // groups to lanes, largest first, each to the least loaded lane; the weight
+    // of a group is its cost per element and stage (the stage time of a wave,
+    // one element per lane) times its largest step count
+    {
+        std::vector<std::pair<double, int>> w;
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            const wedg_problem* p0 = b->ws[b->groups[g].order[0]]->p;
+            w.push_back({stage_cost(p0->nu, p0->nv, p0->nh) * (double)nsteps[b->groups[g].order[0]], (int)g});
+        }
+        std::stable_sort(w.begin(), w.end(), [](auto& x, auto& y) { return x.first > y.first; });
+        std::vector<double> load(b->lanes.size(), 0.0);
+        for (auto& x : w) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[l] += x.first;
+            b->groups[x.second].lane = l;
+        }
     }
     HIPOK(hipEventRecord(b->t0, b->stream));
     for (auto& l : b->lanes) HIPOK(hipStreamWaitEvent(l.stream, b->t0, 0));
