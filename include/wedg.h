@@ -12,6 +12,8 @@
  *                           (src/solver.jl:856-958)
  *   _mms_errors, difference_norms -> wedg_mms_errors, wedg_difference_norms
  *                           (src/solver.jl:772-836)
+ *   many solves at once  -> wedg_batch_create, wedg_batch_advance (no counterpart:
+ *                           independent instances advanced together)
  * Arrays are the reference's column-major Julia arrays, byte for byte: a Julia
  * caller passes pointer(pre.Cw), pointer(u), … unchanged. The state u is
  * (N, N, q_u+1, q_u+1) and v is (N, N, q_v+1, q_v+1), element e = j + N k
