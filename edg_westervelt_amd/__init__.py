@@ -8,15 +8,15 @@ and raises if it is missing.
 """
 from .quadrature import JacobiP, GradJacobiP, GaussQCofs
 from .coefficients import (Coefficients, paper_coefficients, replace_coefficients,
-                           constant_coefficients)
+                           constant_coefficients, element_coefficients)
 from .precompute import flux_parameters, GaussianSource, precompute, BOUNDARY_KINDS
 
 _DEVICE_API = ("Workspace", "edg_rhs_", "edg_rhs", "rk4_step_", "rk4_step", "solve",
                "difference_norms", "DimensionMismatch", "WedgError", "mms_errors", "time_step")
 
 __all__ = ["JacobiP", "GradJacobiP", "GaussQCofs", "Coefficients", "paper_coefficients",
-           "replace_coefficients", "constant_coefficients", "flux_parameters", "GaussianSource",
-           "precompute", "BOUNDARY_KINDS", *_DEVICE_API]
+           "replace_coefficients", "constant_coefficients", "element_coefficients", "flux_parameters",
+           "GaussianSource", "precompute", "BOUNDARY_KINDS", *_DEVICE_API]
 
 
 def __getattr__(name):

@@ -14,7 +14,7 @@ from typing import Callable
 import numpy as np
 
 __all__ = ["Coefficients", "paper_coefficients", "replace_coefficients",
-           "constant_coefficients", "evaluate"]
+           "constant_coefficients", "element_coefficients", "evaluate"]
 
 PI = math.pi
 
@@ -108,3 +108,30 @@ def replace_coefficients(coef: Coefficients, c=None, alpha=None, beta=None) -> C
 def constant_coefficients(*, c, alpha, beta) -> Coefficients:
     """Constant coefficients (src/coefficients.jl:83-84)."""
     return replace_coefficients(paper_coefficients(), c=c, alpha=alpha, beta=beta)
+
+
+def _element_field(A) -> Callable:
+    """f(x, y) = A[j, k] on element (j, k) of the N x N mesh (row j in y, column k
+    in x, the reference's element indexing), for numpy arrays or scalars."""
+    A = np.asarray(A, dtype=np.float64)
+    N = A.shape[0]
+    if A.shape != (N, N):
+        raise ValueError(f"per-element fields are (N, N) arrays (got shape {A.shape})")
+
+    def f(x, y):
+        k = np.clip(np.floor(np.asarray(x) * N).astype(int), 0, N - 1)
+        j = np.clip(np.floor(np.asarray(y) * N).astype(int), 0, N - 1)
+        return A[j, k]
+
+    return f
+
+
+def element_coefficients(*, c, alpha, beta) -> Coefficients:
+    """Coefficients constant on every element, given as (N, N) arrays of
+    per-element values (or numbers, constant everywhere), with zero derivatives
+    inside the elements. They jump across element faces, so ``precompute``
+    takes one-sided face values from each element (src/solver.jl:371-408,
+    742); the point evaluations of the arrays replace closures."""
+    def field(a):
+        return _const(float(a)) if np.ndim(a) == 0 else _element_field(a)
+    return Coefficients(field(c), _zero, _zero, field(alpha), _zero, _zero, field(beta))

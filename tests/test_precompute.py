@@ -184,3 +184,46 @@ def test_initial_profile_equals_default():
     B, _ = precompute(3, 6, mode="initial_value", initial_profile=prof)
     for name in ("PiCdiv", "bcB_sp", "bcB_spy", "bcL_spx", "hJC"):
         assert np.allclose(getattr(A, name), getattr(B, name), rtol=1e-13, atol=1e-12), name
+
+
+def test_element_coefficient_arrays():
+    """Per-element arrays: Cw, Aw and beta_hq carry each element's own value, every
+    interior face takes the value of the element on each side (bot/lft = the
+    element above/right, the M arrays the one below/left) and the jumps make
+    the faces one-sided (src/solver.jl:371-408, 716-770)."""
+    from edg_westervelt_amd import element_coefficients
+    rng = np.random.default_rng(4)
+    N, q = 6, 3
+    c = rng.uniform(0.8, 1.2, (N, N))
+    alpha = 10.0 ** rng.uniform(-6, -3, (N, N))
+    beta = rng.uniform(0, 8, (N, N))
+    pre, _ = precompute(q, N, mixed=True, mode="initial_value", coefficients=element_coefficients(
+        c=c, alpha=alpha, beta=beta))
+    assert pre.one_sided
+    nq = pre.qu1
+    w = pre.w
+    Cw = np.asarray(pre.Cw).reshape((nq, N, N, nq), order="F")
+    Aw = np.asarray(pre.Aw).reshape((nq, N, N, nq), order="F")
+    nh = pre.q_high
+    Bq = np.asarray(pre.beta_hq).reshape((nh, N, N, nh), order="F")
+    for j in range(N):
+        for k in range(N):
+            assert np.allclose(Cw[:, j, k, :], c[j, k] ** 2 * np.outer(w, w), rtol=1e-14, atol=0)
+            assert np.allclose(Aw[:, j, k, :], alpha[j, k] * np.outer(w, w), rtol=1e-14, atol=0)
+            assert (Bq[:, j, k, :] == beta[j, k]).all()
+    hc2, hc2m = np.asarray(pre.hc2), np.asarray(pre.hc2m)
+    vc2, vc2m = np.asarray(pre.vc2), np.asarray(pre.vc2m)
+    for jf in range(1, N):
+        for k in range(N):
+            assert np.allclose(hc2[:, jf, k], c[jf, k] ** 2, rtol=1e-14, atol=0)
+            assert np.allclose(hc2m[:, jf, k], c[jf - 1, k] ** 2, rtol=1e-14, atol=0)
+    for j in range(N):
+        for kf in range(1, N):
+            assert np.allclose(vc2[:, j, kf], c[j, kf] ** 2, rtol=1e-14, atol=0)
+            assert np.allclose(vc2m[:, j, kf], c[j, kf - 1] ** 2, rtol=1e-14, atol=0)
+    # numbers are constants, the same as constant_coefficients
+    A, _ = precompute(q, N, mode="initial_value", coefficients=element_coefficients(c=1.0, alpha=1e-4, beta=2.0))
+    B, _ = precompute(q, N, mode="initial_value", coefficients=constant_coefficients(c=1.0, alpha=1e-4, beta=2.0))
+    assert np.array_equal(A.Cw, B.Cw) and np.array_equal(A.hc2, B.hc2) and A.one_sided == B.one_sided
+    with pytest.raises(ValueError):
+        element_coefficients(c=np.ones((3, 4)), alpha=0.0, beta=0.0)
