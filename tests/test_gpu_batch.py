@@ -146,3 +146,58 @@ def test_batch_rejects_a_workspace_twice():
     assert lib.wedg_batch_create(arr, 2, C.byref(h)) == _lib.ERR_ARGUMENT
     with pytest.raises(ValueError):
         B.Batch([b.ws]).advance(0.0, [b.dt, b.dt], [1, 1])
+
+
+def test_batch_mixes_every_kernel_layout_and_policy():
+    """One batch with every specialisation of the stage kernel — per-element
+    classes with the stored inverse (paper coefficients), with the spectral
+    u-solve (quadrant jumps on element faces), one class (constant
+    coefficients), conjugate gradients — both count policies in one group, a
+    source-mode instance and random states: bitwise the solo runs."""
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import precompute
+    from tests.cases import MIXED, quadrant_coefficients, rand_state
+    rng = np.random.default_rng(21)
+    cc = w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0)
+    cases = [
+        dict(q=3, N=10, kw=dict(mixed=True, mode="mms"), cheb="global"),
+        dict(q=3, N=12, kw=dict(mixed=False, mode="mms"), cheb="element"),
+        dict(q=3, N=7, kw=dict(mixed=False, mode="mms", flux="central"), cheb="global"),
+        dict(q=4, N=16, kw=dict(mode="initial_value", coefficients=cc, boundary=MIXED, sigma_dirichlet=0.3,
+                                initial_profile=B.gaussian_profile(0.5, 0.5, 0.08, 0.01)), cheb="global"),
+        dict(q=4, N=8, kw=dict(mode="source", coefficients=cc, boundary="wall", kappa=0.5,
+                               source=w.GaussianSource(x0=(0.5, 0.7), r0=0.05)), cheb="global"),
+        dict(q=2, N=9, kw=dict(mode="mms", mass_solver="cg"), cheb="global"),
+        dict(q=5, N=20, kw=dict(mode="initial_value", coefficients=quadrant_coefficients(0.5, 0.5),
+                                flux="alternating"), cheb="global"),
+    ]
+    built = []
+    for c in cases:
+        pre, _ = precompute(c["q"], c["N"], **c["kw"])
+        ws = w.Workspace(pre, c["N"], cheb=c["cheb"])
+        u, v = rand_state(rng, c["N"], pre, su=1e-5, sv=1e-3)
+        built.append((c, ws, u, v))
+    steps = [3, 2, 4, 3, 5, 2, 3]
+    dts = [1e-4, 2e-4, 1.5e-4, 1e-4, 2e-4, 1e-4, 5e-5]
+    P = C.POINTER(C.c_double)
+
+    def put(ws, u, v):
+        ws.dev.check(lib.wedg_set_state(ws.dev.h, u.ctypes.data_as(P), v.ctypes.data_as(P)))
+
+    ref = []
+    for (c, ws, u, v), n, dt in zip(built, steps, dts):
+        say("solo", c["q"], c["N"], c["kw"].get("mode"), c["kw"].get("mass_solver", "chebyshev"), c["cheb"])
+        put(ws, u, v)
+        t = C.c_double(0.0)
+        ws.dev.check(lib.wedg_advance(ws.dev.h, 0.05, dt, n, 0, C.byref(t)))
+        ref.append(B.get_state(ws) + (t.value, ws.dev.last_error().cheb_steps))
+    for c, ws, u, v in built:
+        put(ws, u, v)
+    bt = B.Batch([x[1] for x in built])
+    tend, status = bt.advance(0.05, dts, steps)
+    assert (status == _lib.OK).all()
+    for i, (c, ws, u, v) in enumerate(built):
+        uu, vv = B.get_state(ws)
+        assert np.array_equal(uu, ref[i][0]) and np.array_equal(vv, ref[i][1]), c
+        assert tend[i] == ref[i][2] and ws.dev.last_error().cheb_steps == ref[i][3]
+        assert np.abs(vv - v).max() > 0
