@@ -201,3 +201,71 @@ def test_batch_mixes_every_kernel_layout_and_policy():
         assert np.array_equal(uu, ref[i][0]) and np.array_equal(vv, ref[i][1]), c
         assert tend[i] == ref[i][2] and ws.dev.last_error().cheb_steps == ref[i][3]
         assert np.abs(vv - v).max() > 0
+
+
+def _plain(q, N, **kw):
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import precompute
+    pre, _ = precompute(q, N, **kw)
+    return w.Workspace(pre, N)
+
+
+def _run_both(wss, states, dts, steps):
+    """Solo runs, then the batch from the same states; returns both results."""
+    P = C.POINTER(C.c_double)
+
+    def put(ws, u, v):
+        ws.dev.check(lib.wedg_set_state(ws.dev.h, u.ctypes.data_as(P), v.ctypes.data_as(P)))
+
+    ref = []
+    for ws, (u, v), dt, n in zip(wss, states, dts, steps):
+        put(ws, u, v)
+        t = C.c_double(0.0)
+        ws.dev.check(lib.wedg_advance(ws.dev.h, 0.0, dt, n, 0, C.byref(t)))
+        ref.append(B.get_state(ws) + (t.value,))
+    for ws, (u, v) in zip(wss, states):
+        put(ws, u, v)
+    tend, status = B.Batch(wss).advance(0.0, dts, steps)
+    got = [B.get_state(ws) for ws in wss]
+    return ref, got, tend, status
+
+
+def test_batch_edge_cases():
+    """The smallest meshes (N = 1, 2, 3: one partly filled wave), a batch of one,
+    and an advance of zero steps (the states stay as they are)."""
+    from tests.cases import rand_state
+    rng = np.random.default_rng(8)
+    wss = [_plain(1, 1, mode="mms"), _plain(2, 2, mode="mms", mixed=False), _plain(3, 3, mode="mms", flux="central"),
+           _plain(2, 1, mode="mms", mixed=False)]
+    states = [rand_state(rng, ws.N, ws.pre, su=1e-3, sv=1e-3) for ws in wss]
+    ref, got, tend, status = _run_both(wss, states, [1e-3] * 4, [3, 2, 4, 1])
+    assert (status == _lib.OK).all()
+    for r, g in zip(ref, got):
+        assert np.array_equal(r[0], g[0]) and np.array_equal(r[1], g[1])
+    assert list(tend) == [r[2] for r in ref]
+    # one instance
+    ref, got, tend, _ = _run_both(wss[2:3], states[2:3], [2e-3], [3])
+    assert np.array_equal(ref[0][0], got[0][0]) and np.array_equal(ref[0][1], got[0][1])
+    # zero steps: nothing runs, the states and times are unchanged
+    ref, got, tend, status = _run_both(wss, states, [1e-3] * 4, [0, 0, 0, 0])
+    assert (status == _lib.OK).all() and list(tend) == [0.0] * 4
+    for (u, v), g in zip(states, got):
+        assert np.array_equal(u, g[0]) and np.array_equal(v, g[1])
+
+
+def test_batch_with_the_full_size_instance():
+    """The 256 x 256, q = 5 instance (1,024 waves: the staggered start of the
+    stage kernel) in one batch with a small one, two steps: bitwise the solo runs."""
+    import edg_westervelt_amd as w
+    cc = w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0)
+    big = _plain(5, 256, mixed=False, mode="initial_value", coefficients=cc,
+                 initial_profile=B.gaussian_profile(0.5, 0.5, 0.05, 0.05))
+    sb = B.build_instance(B.uneven_instances(seed=0, count=1)[0])
+    small = sb.ws
+    zero = [(np.zeros((ws.N, ws.N, ws.qu1, ws.qu1), order="F"), np.zeros((ws.N, ws.N, ws.qv1, ws.qv1), order="F"))
+            for ws in (big, small)]
+    ref, got, tend, status = _run_both([big, small], zero, [1.5625e-5, sb.dt], [2, 3])
+    assert (status == _lib.OK).all()
+    for r, g in zip(ref, got):
+        assert np.array_equal(r[0], g[0]) and np.array_equal(r[1], g[1])
+        assert np.abs(g[1]).max() > 0
