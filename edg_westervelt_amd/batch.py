@@ -9,7 +9,8 @@ give it, so a batched run returns bitwise the states of separate runs.
 
 ``uneven_instances`` draws the 64-instance workload of uneven problems
 (initial-value mode, piecewise-constant coefficients on a 4 x 4 grid of
-mesh-aligned blocks); ``solve_batch`` builds and runs such a set.
+mesh-aligned blocks); ``solve_batch`` builds and runs such a set;
+``solve_many`` is ``solve`` for a list of runs.
 """
 from __future__ import annotations
 
@@ -26,7 +27,7 @@ from .precompute import precompute
 from .solver import Workspace, _ptr, time_step
 
 __all__ = ["Batch", "piecewise_constant", "block_coefficients", "gaussian_profile", "uneven_instances",
-           "build_instance", "solve_batch"]
+           "build_instance", "solve_batch", "solve_many"]
 
 
 class Batch:
@@ -191,3 +192,54 @@ def solve_batch(instances, dt_fac=10, cheb: str = "global"):
         x.u, x.v = get_state(x.ws)
         x.t = float(t)
     return built, b
+
+
+def solve_many(runs, cheb: str = "global"):
+    """``solve`` for many runs at once: ``runs`` is a list of dicts with the
+    arguments of ``solve`` (``q``, ``N``, ``T`` and its keywords); returns, per
+    run, what ``solve`` returns for it — the errors at T in :mms mode, the
+    final state with ``return_state=True`` — from states advanced together in
+    one batch (bitwise the states of separate ``solve`` calls). :source mode,
+    whose host loop stops at every record step, is not batched here."""
+    from .solver import _device_mms_errors, _set_diag
+    setups = []
+    for r in runs:
+        r = dict(r)
+        q, N, T = r.pop("q"), r.pop("N"), r.pop("T")
+        return_state = r.pop("return_state", False)
+        dt_fac = r.pop("dt_fac", 10)
+        nblocks = r.pop("nblocks", None)
+        run_cheb = r.pop("cheb", cheb)
+        for k in ("record_every", "slice_y", "receivers", "snap_times"):   # :source mode only
+            r.pop(k, None)
+        mode = str(r.get("mode", "mms")).lstrip(":")
+        if mode not in ("mms", "initial_value", "source"):
+            raise ValueError(f"unknown mode {mode} (expected :mms, :initial_value or :source)")
+        if mode == "source":
+            raise ValueError("solve_many does not batch :source mode; call solve for it")
+        if mode != "mms" and not return_state:
+            raise ValueError("errors are only available in :mms mode; use return_state = true")
+        r["mode"] = mode
+        pre, geo = precompute(q, N, **r)
+        dt, nsteps = time_step(q, N, T, dt_fac)
+        ws = Workspace(pre, N, nblocks=nblocks, cheb=run_cheb)
+        set_zero_state(ws)
+        setups.append(SimpleNamespace(q=q, N=N, T=T, pre=pre, geo=geo, ws=ws, dt=dt, nsteps=nsteps,
+                                      return_state=return_state, kx=r.get("kx", 1), ky=r.get("ky", 1),
+                                      phase=r.get("phase", (0.0, 0.0))))
+    if not setups:
+        return []
+    b = Batch([s.ws for s in setups])
+    tend, _ = b.advance(0.0, [s.dt for s in setups], [s.nsteps for s in setups])
+    out = []
+    for s, t in zip(setups, tend):
+        h = 1.0 / s.N
+        if s.return_state:
+            u, v = get_state(s.ws)
+            _set_diag(s.ws, s.geo)
+            out.append(SimpleNamespace(u=u, v=v, N=s.N, h=h, P_u=s.pre.P_u, P_v=s.pre.P_v, S_u=s.geo.S_u,
+                                       W2d=s.geo.W2d, err_c2=s.geo.err_c2, t=float(t), dt=s.dt, nsteps=s.nsteps,
+                                       workspace=s.ws))
+        else:
+            out.append(_device_mms_errors(s.ws, s.T, s.N, h, s.pre, s.geo, s.kx, s.ky, s.phase))
+    return out

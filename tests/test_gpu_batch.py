@@ -269,3 +269,29 @@ def test_batch_with_the_full_size_instance():
     for r, g in zip(ref, got):
         assert np.array_equal(r[0], g[0]) and np.array_equal(r[1], g[1])
         assert np.abs(g[1]).max() > 0
+
+
+def test_solve_many_equals_solve():
+    """solve_many returns what solve returns for each run: the :mms errors
+    (bitwise: the same states and the same norm kernels) and the final states."""
+    import edg_westervelt_amd as w
+    runs = [dict(q=3, N=10, T=0.1, flux="upwind", mixed=True),
+            dict(q=2, N=8, T=0.05, flux="central", mixed=False, boundary="absorbing", sigma=0.3),
+            dict(q=4, N=6, T=0.05, mass_solver="cg"),
+            dict(q=3, N=8, T=0.02, mode="initial_value", return_state=True,
+                 coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0),
+                 initial_profile=B.gaussian_profile(0.5, 0.5, 0.1, 0.01))]
+    got = B.solve_many(runs)
+    for r, g in zip(runs, got):
+        r = dict(r)
+        ref = w.solve(r.pop("q"), r.pop("N"), r.pop("T"), **r)
+        if r.get("return_state"):
+            assert np.array_equal(ref.u, g.u) and np.array_equal(ref.v, g.v) and ref.t == g.t
+        else:
+            assert (ref.erru, ref.errv, ref.errgradu, ref.errE) == (g.erru, g.errv, g.errgradu, g.errE)
+    # the cfg1 run's errors are the reference's (data/convergence/results.csv row 364 was made with CG)
+    assert abs(got[0].erru / 1.0717765588e-04 - 1) < 1e-3
+    with pytest.raises(ValueError, match="source"):
+        B.solve_many([dict(q=2, N=4, T=0.01, mode="source")])
+    with pytest.raises(ValueError, match="return_state"):
+        B.solve_many([dict(q=2, N=4, T=0.01, mode="initial_value")])
