@@ -82,12 +82,12 @@ def flops_per_element_stage(nu, nv, nh, nc):
     return base + (nc - 1) * step, traces
 
 
-def bench_batched(count=64, seed=0, sequential=True):
+def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
     """Aggregate DOF·stages/s of the uneven batch: together vs one after the other."""
     import edg_westervelt_amd.batch as B
     from edg_westervelt_amd._lib import lib
     t = time.perf_counter()
-    built = [B.build_instance(x) for x in B.uneven_instances(seed=seed, count=count)]
+    built = [B.build_instance(x, cheb=cheb) for x in B.uneven_instances(seed=seed, count=count)]
     t_build = time.perf_counter() - t
     work = sum(4 * b.nsteps * b.dof for b in built)
     bt = B.Batch([b.ws for b in built])
@@ -124,7 +124,7 @@ def bench_batched(count=64, seed=0, sequential=True):
                 dof_stages=work, instances=count, groups=groups, bitwise_equal_to_sequential=bool(same),
                 workload=f"edg_westervelt_amd.batch.uneven_instances(seed={seed}, count={count}), T = 0.25, "
                          "initial_value Gaussian, piecewise-constant c, alpha, beta on 4x4 blocks",
-                build_s=round(t_build, 2))
+                cheb_policy=cheb, build_s=round(t_build, 2))
 
 
 def main():
@@ -165,7 +165,7 @@ def main():
                               gbs=2 * 8 * n / ms.value / 1e6)))
         return
     if args.batched == "only":
-        bb = bench_batched(sequential=args.batched_sequential == "on")
+        bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb)
         print(json.dumps(dict(metric="batched speedup (uneven instances)", value=bb.get("speedup"), unit="x",
                               n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched=bb)))
         return
@@ -264,7 +264,7 @@ def main():
                  hbm_frac_model=bytes_step / 4 / stage_s / 1e9 / HBM_PEAK_GBS,
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
-    batched = bench_batched() if args.batched == "on" else None
+    batched = bench_batched(cheb=args.cheb) if args.batched == "on" else None
 
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
     # problem, a bounded sample of RK4 steps from the same initial state
