@@ -91,8 +91,10 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
     t_build = time.perf_counter() - t
     work = sum(4 * b.nsteps * b.dof for b in built)
     bt = B.Batch([b.ws for b in built])
-    bt.advance(0.0, [b.dt for b in built], [1] * count)          # warm-up step, then back to t = 0
-    for b in built:
+    bt.advance(0.0, [b.dt for b in built], [1] * count)          # warm-up step
+    # launch parameters by the autotuner (32-step probes of every stream count and chunk)
+    tune_ms, choice = bt.tune(0.0, [b.dt for b in built], probe_steps=32)
+    for b in built:                                              # back to t = 0
         B.set_zero_state(b.ws)
     print(f"# batched: {count} instances together", file=sys.stderr, flush=True)
     c0 = time.perf_counter()
@@ -118,13 +120,15 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
     groups = len({(b.inst.q, b.inst.mixed, b.ws.problem.pre.one_sided) for b in built})
     if not sequential:
         return dict(value=work / wb, unit="DOF·stages/s", wall_s=wb, device_s=dev_ms / 1e3, dof_stages=work,
-                    instances=count)
+                    instances=count, tuner=dict(lanes=choice[0], chunk_steps=choice[1]))
     return dict(value=work / wb, unit="DOF·stages/s", sequential_value=work / seq, speedup=seq / wb,
                 wall_s=wb, device_s=dev_ms / 1e3, sequential_s=seq, longest_instance_s=longest,
                 dof_stages=work, instances=count, groups=groups, bitwise_equal_to_sequential=bool(same),
                 workload=f"edg_westervelt_amd.batch.uneven_instances(seed={seed}, count={count}), T = 0.25, "
                          "initial_value Gaussian, piecewise-constant c, alpha, beta on 4x4 blocks",
-                cheb_policy=cheb, build_s=round(t_build, 2))
+                cheb_policy=cheb, build_s=round(t_build, 2),
+                tuner=dict(choice=dict(lanes=choice[0], chunk_steps=choice[1]),
+                           probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()}))
 
 
 def main():

@@ -90,6 +90,8 @@ def _load():
         "wedg_batch_destroy": [vp],
         "wedg_batch_advance": [vp, _dp, _dp, C.POINTER(C.c_long), C.c_int, _dp, _ip],
         "wedg_batch_time": [vp, C.POINTER(C.c_double)],
+        "wedg_batch_configure": [vp, C.c_int, C.c_long],
+        "wedg_batch_config": [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -72,6 +72,43 @@ class Batch:
                 raise WedgError(f"wedg_batch_advance failed with status {st}")
         return tend, status
 
+    def configure(self, lanes: int, chunk_steps: int = 0):
+        """Streams and argument chunk of the launches (``wedg_batch_configure``)."""
+        st = lib.wedg_batch_configure(self.h, int(lanes), int(chunk_steps))
+        if st != _lib.OK:
+            raise WedgError(f"wedg_batch_configure failed with status {st}")
+
+    def config(self):
+        lanes, chunk = C.c_int(0), C.c_long(0)
+        lib.wedg_batch_config(self.h, C.byref(lanes), C.byref(chunk))
+        return lanes.value, chunk.value
+
+    def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8), chunks=(0, 4), repeats: int = 2):
+        """Autotuner of the launch parameters: times ``probe_steps`` steps of the
+        batch (device time between the batch's events, best of ``repeats``) for
+        every (lanes, chunk) pair, keeps the fastest and puts every state back
+        as it was. Returns {(lanes, chunk): ms} and the choice."""
+        saved = [get_state(w) for w in self.workspaces]
+        n = len(self.workspaces)
+        times = {}
+        try:
+            for nl in lanes:
+                for ch in chunks:
+                    self.configure(nl, ch)
+                    best = math.inf
+                    for _ in range(repeats):
+                        for w, (u, v) in zip(self.workspaces, saved):
+                            w.dev.check(lib.wedg_set_state(w.dev.h, _ptr(u), _ptr(v)))
+                        self.advance(t0, dt, [probe_steps] * n)
+                        best = min(best, self.device_ms())
+                    times[(nl, ch)] = best
+        finally:
+            for w, (u, v) in zip(self.workspaces, saved):
+                w.dev.check(lib.wedg_set_state(w.dev.h, _ptr(u), _ptr(v)))
+        choice = min(times, key=times.get)
+        self.configure(*choice)
+        return times, choice
+
     def device_ms(self) -> float:
         """Device time of the last advance (HIP events on the batch stream), ms."""
         ms = C.c_double(0.0)

@@ -1230,6 +1230,8 @@ struct wedg_batch {
     hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
+constexpr int kMaxBatchLanes = 16;
+
 // Streams of a batch: the hardware queues of a process (GPU_MAX_HW_QUEUES, 4
 // by default), so that the lanes run concurrently.
 static int batch_lanes(int ngroups) {
@@ -1249,18 +1251,46 @@ static double stage_cost(int nu, int nv, int nh) {
            2 * q * q * q * q + 4 * (3 * q * (2 * v + q) + 3 * (2 * v * v + q * q)) + 2 * (h * v * v + h * h * v);
 }
 
-extern "C" wedg_status wedg_batch_destroy(wedg_batch* b) {
-    if (!b) return WEDG_OK;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+static void lanes_free(wedg_batch* b) {
     for (auto& l : b->lanes) {
         if (l.stream) (void)hipStreamSynchronize(l.stream);
         for (int i = 0; i < 2; ++i) {
             if (l.hargs[i]) (void)hipHostFree(l.hargs[i]);
+            if (l.dargs[i]) (void)hipFree(l.dargs[i]);
             if (l.copied[i]) (void)hipEventDestroy(l.copied[i]);
         }
         if (l.done) (void)hipEventDestroy(l.done);
         if (l.stream) (void)hipStreamDestroy(l.stream);
     }
+    b->lanes.clear();
+}
+
+// nl streams with stage-argument buffers for chunks of `chunk` steps (0: the
+// chunk whose arguments fill about 4 MB over all lanes)
+static hipError_t lanes_alloc(wedg_batch* b, int nl, long chunk) {
+    const size_t n = b->ws.size();
+    b->chunk = chunk > 0 ? chunk : (long)std::max<size_t>(1, ((size_t)4 << 20) / (4 * n * sizeof(KArgs)));
+    b->lanes.resize(nl);
+    hipError_t he = hipSuccess;
+    for (auto& l : b->lanes) {
+        l.cap = (size_t)b->chunk * 4 * n;   // any share of the groups
+        if (he == hipSuccess) he = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && he == hipSuccess; ++i) {
+            he = hipHostMalloc((void**)&l.hargs[i], l.cap * sizeof(KArgs), hipHostMallocDefault);
+            void* d = nullptr;
+            if (he == hipSuccess) he = hipMalloc(&d, l.cap * sizeof(KArgs));
+            if (he == hipSuccess) l.dargs[i] = (KArgs*)d;
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&l.copied[i], hipEventDisableTiming);
+        }
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
+    }
+    return he;
+}
+
+extern "C" wedg_status wedg_batch_destroy(wedg_batch* b) {
+    if (!b) return WEDG_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    lanes_free(b);
     if (b->hmap) (void)hipHostFree(b->hmap);
     if (b->t0) (void)hipEventDestroy(b->t0);
     if (b->t1) (void)hipEventDestroy(b->t1);
@@ -1291,22 +1321,8 @@ extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_
         }
         it->inst.push_back(i);
     }
-    b->lanes.resize(batch_lanes((int)b->groups.size()));
     hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    // chunks of steps whose arguments fill about 4 MB over all lanes
-    b->chunk = (long)std::max<size_t>(1, ((size_t)4 << 20) / (4 * (size_t)n * sizeof(KArgs)));
-    for (auto& l : b->lanes) {
-        l.cap = (size_t)b->chunk * 4 * (size_t)n;   // any share of the groups
-        if (he == hipSuccess) he = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
-        for (int i = 0; i < 2 && he == hipSuccess; ++i) {
-            he = hipHostMalloc((void**)&l.hargs[i], l.cap * sizeof(KArgs), hipHostMallocDefault);
-            void* d = nullptr;
-            if (he == hipSuccess) he = hipMalloc(&d, l.cap * sizeof(KArgs));
-            if (he == hipSuccess) { b->allocs.push_back(d); l.dargs[i] = (KArgs*)d; }
-            if (he == hipSuccess) he = hipEventCreateWithFlags(&l.copied[i], hipEventDisableTiming);
-        }
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
-    }
+    if (he == hipSuccess) he = lanes_alloc(b, batch_lanes((int)b->groups.size()), 0);
     for (auto& g : b->groups) {
         size_t blocks = 0;
         for (int i : g.inst) blocks += ws[i]->nwaves();
@@ -1330,6 +1346,8 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
     const int n = (int)b->ws.size();
     for (int i = 0; i < n; ++i)
         if (nsteps[i] < 0) return WEDG_ERR_ARGUMENT;
+    for (const auto& l : b->lanes)
+        if (!l.stream || !l.dargs[0] || !l.dargs[1] || !l.hargs[0] || !l.hargs[1]) return WEDG_ERR_HIP;
     wedg_status s;
     for (int i = 0; i < n; ++i) {
         if ((s = reset_err(b->ws[i]))) return s;
@@ -1441,6 +1459,25 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
         if (si != WEDG_OK && first == WEDG_OK) first = si;
     }
     return first;
+}
+
+extern "C" wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps) {
+    if (!b || lanes < 1 || lanes > kMaxBatchLanes || chunk_steps < 0) return WEDG_ERR_ARGUMENT;
+    HIPOK(hipStreamSynchronize(b->stream));
+    lanes_free(b);
+    if (lanes_alloc(b, lanes, chunk_steps) != hipSuccess) {
+        lanes_free(b);
+        (void)lanes_alloc(b, 1, 0);   // leave a usable batch if the larger set did not fit
+        return WEDG_ERR_HIP;
+    }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps) {
+    if (!b || !lanes || !chunk_steps) return WEDG_ERR_ARGUMENT;
+    *lanes = (int)b->lanes.size();
+    *chunk_steps = b->chunk;
+    return WEDG_OK;
 }
 
 extern "C" wedg_status wedg_batch_time(const wedg_batch* b, double* ms) {

@@ -235,6 +235,12 @@ wedg_status wedg_batch_destroy(wedg_batch* b);
 wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
                                int stop_on_nonfinite, double* t_end, int* status);
 wedg_status wedg_batch_time(const wedg_batch* b, double* ms);
+/* Launch parameters of a batch (they change no result): the number of streams
+ * the groups are dealt to (1..16; default: the process's hardware queues,
+ * GPU_MAX_HW_QUEUES or 4) and the steps whose stage arguments are staged per
+ * copy (0: about 4 MB of arguments). wedg_batch_config reads them back. */
+wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps);
+wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps);
 
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);

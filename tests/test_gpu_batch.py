@@ -295,3 +295,30 @@ def test_solve_many_equals_solve():
         B.solve_many([dict(q=2, N=4, T=0.01, mode="source")])
     with pytest.raises(ValueError, match="return_state"):
         B.solve_many([dict(q=2, N=4, T=0.01, mode="initial_value")])
+
+
+def test_batch_tuner_changes_no_result():
+    """The autotuner tries stream counts and argument chunks, restores every
+    state, and the tuned batch still gives bitwise the solo runs."""
+    insts = B.uneven_instances(seed=0, count=8)
+    built = [B.build_instance(x) for x in insts]
+    steps = [5, 3, 6, 2, 4, 5, 1, 3]
+    ref = [solo(b, n)[:3] for b, n in zip(built, steps)]
+    for b in built:
+        B.set_zero_state(b.ws)
+    bt = B.Batch([b.ws for b in built])
+    times, choice = bt.tune(0.0, [b.dt for b in built], probe_steps=4, lanes=(1, 2, 4), chunks=(0, 1), repeats=1)
+    say("tuner", times, choice)
+    assert set(times) == {(l, c) for l in (1, 2, 4) for c in (0, 1)} and choice in times
+    assert bt.config()[0] == choice[0]
+    for b in built:
+        u, v = B.get_state(b.ws)
+        assert not u.any() and not v.any()   # the probes' states were put back
+    tend, status = bt.advance(0.0, [b.dt for b in built], steps)
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        assert np.array_equal(u, ref[i][0]) and np.array_equal(v, ref[i][1]) and tend[i] == ref[i][2]
+    with pytest.raises(_lib.WedgError):
+        bt.configure(0)
+    with pytest.raises(_lib.WedgError):
+        bt.configure(17)
