@@ -227,7 +227,8 @@ wedg_status wedg_probe_copy(size_t n, int repeats, double* ms);
  * arguments, bitwise the same state); status[i] and t_end[i] (either may be
  * null) receive its status and final time, wedg_last_error(ws[i]) its failure
  * details; the return value is the first non-zero status. A workspace may
- * appear in a batch once; it must not be used elsewhere while the batch runs.
+ * appear in a batch once; it must not be used elsewhere while the batch runs,
+ * and it must outlive the batch. A batch, like a workspace, is not re-entrant.
  * wedg_batch_time gives the device time of the last advance in ms. */
 typedef struct wedg_batch wedg_batch;
 wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_batch** out);
