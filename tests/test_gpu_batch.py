@@ -322,3 +322,46 @@ def test_batch_tuner_changes_no_result():
         bt.configure(0)
     with pytest.raises(_lib.WedgError):
         bt.configure(17)
+
+
+def test_batch_blow_up_instance():
+    """An instance that blows up (beta = 0, a time step far above the stable
+    one) stops with the error its own run reports — the non-finite state
+    fails the next stage's Chebyshev interval — at the same element and step,
+    with stop_on_nonfinite; the other instance finishes unaffected."""
+    import edg_westervelt_amd as w
+    lin = w.constant_coefficients(c=1.0, alpha=1e-4, beta=0.0)
+    wss = [_plain(2, 4, mode="initial_value", coefficients=lin,
+                  initial_profile=B.gaussian_profile(0.5, 0.5, 0.1, 0.01)),
+           _plain(3, 6, mode="mms")]
+    dts, steps = [0.5, 1e-3], [400, 5]
+    zero = [(np.zeros((ws.N, ws.N, ws.qu1, ws.qu1), order="F"), np.zeros((ws.N, ws.N, ws.qv1, ws.qv1), order="F"))
+            for ws in wss]
+    P = C.POINTER(C.c_double)
+
+    def reset():
+        for ws, (u, v) in zip(wss, zero):
+            ws.dev.check(lib.wedg_set_state(ws.dev.h, u.ctypes.data_as(P), v.ctypes.data_as(P)))
+
+    def info(ws):
+        e = ws.dev.last_error()
+        return (e.status, e.side, e.key, e.element, e.step)
+
+    reset()
+    ref = []
+    for ws, dt, n in zip(wss, dts, steps):
+        t = C.c_double(0.0)
+        st = lib.wedg_advance(ws.dev.h, 0.0, dt, n, 1, C.byref(t))
+        ref.append((st, info(ws), B.get_state(ws)))
+    assert ref[0][0] != _lib.OK and ref[1][0] == _lib.OK
+    say("blow-up instance:", ref[0][1])
+    reset()
+    bt = B.Batch(wss)
+    _, status = bt.advance(0.0, dts, steps, stop_on_nonfinite=True, raise_errors=False)
+    assert list(status) == [ref[0][0], ref[1][0]]
+    assert info(wss[0]) == ref[0][1]
+    u, v = B.get_state(wss[1])
+    assert np.array_equal(u, ref[1][2][0]) and np.array_equal(v, ref[1][2][1])
+    reset()
+    with pytest.raises(_lib.WedgError):
+        bt.advance(0.0, dts, steps, stop_on_nonfinite=True)
