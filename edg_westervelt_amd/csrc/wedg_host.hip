@@ -1006,7 +1006,6 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
     const size_t rbytes = (size_t)nrecv * (sizeof(int) + 2 * nv * sizeof(double));
     if (nrecv > ws->nrecv_dev) {   // receiver basis rows, then elements, on the device
         if (ws->hrows) (void)hipHostFree(ws->hrows);
-    if (ws->hnorm) (void)hipHostFree(ws->hnorm);
         ws->hrows = nullptr;
         ws->nrecv_dev = 0;
         void* d = nullptr;

@@ -216,3 +216,50 @@ def test_source_energy_decay(flux):
     E = H.energy[H.t > 0.25]
     assert np.max(np.diff(E)) <= 1e-10 * E[0]
     assert E[-1] / E[0] < 0.95
+
+
+def test_norms_after_receivers_on_one_workspace():
+    """Error norms, then a receiver run, then the norms again on one workspace:
+    the receiver buffers grow without touching the norms' staging, and the
+    norms agree with a fresh workspace that took the same steps."""
+    import ctypes as C
+    from edg_westervelt_amd import _lib
+    from edg_westervelt_amd.solver import Workspace, _device_mms_errors, _ptr, time_step, vrow
+    lib = _lib.lib
+    N, q = 6, 2
+    pre, geo = precompute(q, N, mode="mms")
+    dt, _ = time_step(q, N, 0.1)
+
+    def fresh():
+        ws = Workspace(pre, N)
+        u = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+        v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+        ws.dev.check(lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v)))
+        return ws
+
+    def errs(ws, T):
+        e = _device_mms_errors(ws, T, N, 1.0 / N, pre, geo)
+        return (e.erru, e.errv, e.errgradu, e.errE)
+
+    ws = fresh()
+    t = C.c_double(0.0)
+    ws.dev.check(lib.wedg_advance(ws.dev.h, 0.0, dt, 3, 0, C.byref(t)))
+    e1 = errs(ws, t.value)
+    rj = np.array([1, 4], dtype=np.int32)
+    rk = np.array([2, 3], dtype=np.int32)
+    rows = np.ascontiguousarray(np.tile(vrow(0.2, pre.qv1), (2, 2)))
+    recv = np.zeros(2 * 2)
+    ip = C.POINTER(C.c_int)
+    t2 = C.c_double(0.0)
+    ws.dev.check(lib.wedg_advance_receivers(ws.dev.h, t.value, dt, 0, 2, rj.ctypes.data_as(ip), rk.ctypes.data_as(ip),
+                                            _ptr(rows), 0, _ptr(recv), C.byref(t2)))
+    assert errs(ws, t.value) == e1
+    ws.dev.check(lib.wedg_advance_receivers(ws.dev.h, t.value, dt, 2, 2, rj.ctypes.data_as(ip), rk.ctypes.data_as(ip),
+                                            _ptr(rows), 0, _ptr(recv), C.byref(t2)))
+    ref = fresh()
+    t3 = C.c_double(0.0)
+    ref.dev.check(lib.wedg_advance(ref.dev.h, 0.0, dt, 3, 0, C.byref(t3)))
+    ref.dev.check(lib.wedg_advance(ref.dev.h, t3.value, dt, 2, 0, C.byref(t3)))
+    assert t3.value == t2.value
+    assert errs(ws, t2.value) == errs(ref, t3.value)
+    del ws
