@@ -1062,8 +1062,8 @@ extern "C" wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* v
         HIPOK(hipMalloc(&d, n * sizeof(double)));
         ws->allocs.push_back(d);
         ws->ddiag = (double*)d;
-        HIPOK(hipHostMalloc((void**)&ws->hdiag, n * sizeof(double), hipHostMallocDefault));
     }
+    if (!ws->hdiag) HIPOK(hipHostMalloc((void**)&ws->hdiag, n * sizeof(double), hipHostMallocDefault));
     double* dslice = ws->ddiag + 3 * (size_t)p->E;
     double* deta = dslice + (size_t)N * nq;
     std::memcpy(ws->hdiag, eta0, nv * sizeof(double));
@@ -1109,13 +1109,14 @@ extern "C" wedg_status wedg_snapshot(wedg_workspace* ws, double* grid) {
 
 static wedg_status norms_buffers(wedg_workspace* ws) {
     wedg_problem* p = ws->p;
-    if (ws->dnorm) return WEDG_OK;
     const size_t n = 4 * (size_t)p->E + 4 * (size_t)p->N * p->nu;
-    void* d = nullptr;
-    HIPOK(hipMalloc(&d, n * sizeof(double)));
-    ws->allocs.push_back(d);
-    ws->dnorm = (double*)d;
-    HIPOK(hipHostMalloc((void**)&ws->hnorm, n * sizeof(double), hipHostMallocDefault));
+    if (!ws->dnorm) {
+        void* d = nullptr;
+        HIPOK(hipMalloc(&d, n * sizeof(double)));
+        ws->allocs.push_back(d);
+        ws->dnorm = (double*)d;
+    }
+    if (!ws->hnorm) HIPOK(hipHostMalloc((void**)&ws->hnorm, n * sizeof(double), hipHostMallocDefault));
     return WEDG_OK;
 }
 
