@@ -10,7 +10,8 @@ give it, so a batched run returns bitwise the states of separate runs.
 ``uneven_instances`` draws the 64-instance workload of uneven problems
 (initial-value mode, piecewise-constant coefficients on a 4 x 4 grid of
 mesh-aligned blocks); ``solve_batch`` builds and runs such a set;
-``solve_many`` is ``solve`` for a list of runs.
+``solve_many`` is ``solve`` for a list of runs; ``run_source_many`` is
+source-mode ``solve`` (time series, receivers, snapshots) for a list of runs.
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ from .precompute import precompute
 from .solver import Workspace, _ptr, time_step
 
 __all__ = ["Batch", "piecewise_constant", "block_coefficients", "gaussian_profile", "uneven_instances",
-           "build_instance", "solve_batch", "solve_many"]
+           "build_instance", "solve_batch", "solve_many", "run_source_many"]
 
 
 class Batch:
@@ -280,3 +281,67 @@ def solve_many(runs, cheb: str = "global"):
         else:
             out.append(_device_mms_errors(s.ws, s.T, s.N, h, s.pre, s.geo, s.kx, s.ky, s.phase))
     return out
+
+
+def run_source_many(runs, cheb: str = "global"):
+    """``solve(..., mode="source")`` for many runs at once: ``runs`` is a list of
+    dicts with the arguments of ``solve`` (``q``, ``N``, ``T``, ``record_every``,
+    ``slice_y``, ``receivers``, ``snap_times`` and the ``precompute`` keywords);
+    returns per run the time series ``solve`` returns (src/solver.jl:872-958).
+    The host loop stops at the union of the runs' record and snapshot steps;
+    between stops the runs advance in one batch with their receivers recorded
+    after every step (``wedg_batch_advance_receivers``). The series are bitwise
+    those of separate runs."""
+    from .solver import _blow_up, _source_record, _source_result, _source_setup
+    setups = []
+    for r in runs:
+        r = dict(r)
+        q, N, T = r.pop("q"), r.pop("N"), r.pop("T")
+        dt_fac = r.pop("dt_fac", 10)
+        nblocks = r.pop("nblocks", None)
+        run_cheb = r.pop("cheb", cheb)
+        r.pop("return_state", None)   # source mode returns the time series either way
+        kw = dict(record_every=r.pop("record_every", 20), slice_y=r.pop("slice_y", 0.5),
+                  receivers=r.pop("receivers", ((0.45, 0.5), (0.7, 0.5), (0.85, 0.5))),
+                  snap_times=r.pop("snap_times", ()))
+        mode = str(r.pop("mode", "source")).lstrip(":")
+        if mode != "source":
+            raise ValueError("run_source_many runs :source mode; use solve_many for the others")
+        pre, geo = precompute(q, N, mode="source", **r)
+        dt, nsteps = time_step(q, N, T, dt_fac)
+        ws = Workspace(pre, N, nblocks=nblocks, cheb=run_cheb)
+        set_zero_state(ws)
+        S = _source_setup(ws, dt, nsteps, N, pre, geo, **kw)
+        ws.dev.check(lib.wedg_set_receivers(ws.dev.h, S.nr, S.rj.ctypes.data_as(C.POINTER(C.c_int)),
+                                            S.rk.ctypes.data_as(C.POINTER(C.c_int)), _ptr(S.rows)))
+        setups.append(S)
+    if not setups:
+        return []
+    n = len(setups)
+    b = Batch([S.ws for S in setups])
+    stops = sorted(set().union(*[S.stops for S in setups]))
+    dts = np.ascontiguousarray([S.dt for S in setups], dtype=np.float64)
+    for stop in stops:
+        steps = [max(0, min(stop, S.nsteps) - S.done) for S in setups]
+        if not any(steps):
+            continue
+        bufs = [np.zeros(max(k * S.nr, 1)) for k, S in zip(steps, setups)]
+        ptrs = (C.POINTER(C.c_double) * n)(*[_ptr(x) for x in bufs])
+        t0 = np.ascontiguousarray([S.t for S in setups], dtype=np.float64)
+        tend = np.zeros(n)
+        status = np.zeros(n, dtype=np.int32)
+        ns = (C.c_long * n)(*steps)
+        lib.wedg_batch_advance_receivers(b.h, _ptr(t0), _ptr(dts), ns, 1, _ptr(tend),
+                                         status.ctypes.data_as(C.POINTER(C.c_int)), ptrs)
+        for i, S in enumerate(setups):
+            if status[i] == _lib.ERR_NONFINITE:
+                raise _blow_up(S)
+            S.ws.dev.check(int(status[i]))
+        for k, S, buf, te in zip(steps, setups, bufs, tend):
+            if k == 0:
+                continue
+            S.t = float(te)
+            S.recv_all[S.done:S.done + k] = buf[:k * S.nr].reshape(k, S.nr)
+            S.done += k
+            _source_record(S, S.done)
+    return [_source_result(S) for S in setups]

@@ -227,6 +227,7 @@ struct wedg_workspace {
     double* drecv = nullptr; double* hrecv = nullptr; size_t nrecv_cap = 0;
     double* ddiag = nullptr; double* hdiag = nullptr;   // E, vmax, ndmin [3][E], slice [N nq], eta0 [nv]
     int* drecv_elem = nullptr; double* drecv_rows = nullptr; int nrecv_dev = 0;
+    int nrecv_set = 0;       // receivers a batched advance records (wedg_set_receivers)
     void* hrows = nullptr;   // pinned staging of the receiver rows and elements
     double* dnorm = nullptr; double* hnorm = nullptr;   // error norms: [4][E] sums, then the 1D tables
     int serial = 0;
@@ -983,15 +984,8 @@ extern "C" wedg_status wedg_problem_set_diag(wedg_problem* p, const double* err_
     return he == hipSuccess ? WEDG_OK : WEDG_ERR_HIP;
 }
 
-extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, double dt, long nsteps, int nrecv,
-                                              const int* recv_j, const int* recv_k, const double* rows,
-                                              int stop_on_nonfinite, double* recv, double* t_end) {
-    if (!ws || nsteps < 0 || nrecv < 0 || (nrecv > 0 && (!recv_j || !recv_k || !rows || !recv))) return WEDG_ERR_ARGUMENT;
-    wedg_problem* p = ws->p;
-    const int N = p->N, nv = p->nv;
-    for (int r = 0; r < nrecv; ++r)
-        if (recv_j[r] < 0 || recv_j[r] >= N || recv_k[r] < 0 || recv_k[r] >= N) return WEDG_ERR_ARGUMENT;
-    const size_t need = (size_t)nsteps * nrecv;
+// room for `need` receiver values on the device and in pinned memory
+static wedg_status recv_capacity(wedg_workspace* ws, size_t need) {
     HIPOK(hipStreamSynchronize(ws->stream));
     if (need > ws->nrecv_cap) {
         if (ws->drecv) (void)hipFree(ws->drecv);
@@ -1003,6 +997,18 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
         HIPOK(hipHostMalloc((void**)&ws->hrecv, need * sizeof(double), hipHostMallocDefault));
         ws->nrecv_cap = need;
     }
+    return WEDG_OK;
+}
+
+// The receivers' elements and basis rows on the device (checked against the mesh).
+static wedg_status recv_rows(wedg_workspace* ws, int nrecv, const int* recv_j, const int* recv_k,
+                             const double* rows) {
+    if (nrecv < 0 || (nrecv > 0 && (!recv_j || !recv_k || !rows))) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    const int N = p->N, nv = p->nv;
+    for (int r = 0; r < nrecv; ++r)
+        if (recv_j[r] < 0 || recv_j[r] >= N || recv_k[r] < 0 || recv_k[r] >= N) return WEDG_ERR_ARGUMENT;
+    HIPOK(hipStreamSynchronize(ws->stream));
     const size_t rbytes = (size_t)nrecv * (sizeof(int) + 2 * nv * sizeof(double));
     if (nrecv > ws->nrecv_dev) {   // receiver basis rows, then elements, on the device
         if (ws->hrows) (void)hipHostFree(ws->hrows);
@@ -1024,12 +1030,36 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
         HIPOK(hipMemcpyAsync(ws->drecv_rows, ws->hrows, rbytes, hipMemcpyHostToDevice, ws->stream));
         HIPOK(hipStreamSynchronize(ws->stream));
     }
+    return WEDG_OK;
+}
+
+static RecvArgs recv_args(wedg_workspace* ws, int nrecv) {
+    RecvArgs ra;
+    ra.v = ws->yv; ra.E = ws->p->E; ra.nrecv = nrecv; ra.elem = ws->drecv_elem; ra.rows = ws->drecv_rows;
+    ra.out = ws->drecv; ra.step = 0;
+    return ra;
+}
+
+static wedg_status recv_out(wedg_workspace* ws, double* recv, size_t need) {
+    if (need == 0) return WEDG_OK;
+    HIPOK(hipMemcpyAsync(ws->hrecv, ws->drecv, need * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    HIPOK(hipStreamSynchronize(ws->stream));
+    std::memcpy(recv, ws->hrecv, need * sizeof(double));
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, double dt, long nsteps, int nrecv,
+                                              const int* recv_j, const int* recv_k, const double* rows,
+                                              int stop_on_nonfinite, double* recv, double* t_end) {
+    if (!ws || nsteps < 0 || nrecv < 0 || (nrecv > 0 && (!recv_j || !recv_k || !rows || !recv))) return WEDG_ERR_ARGUMENT;
+    wedg_problem* p = ws->p;
+    const size_t need = (size_t)nsteps * nrecv;
     wedg_status s;
+    if ((s = recv_capacity(ws, need))) return s;
+    if ((s = recv_rows(ws, nrecv, recv_j, recv_k, rows))) return s;
     if ((s = reset_err(ws))) return s;
     double t = t0;
-    RecvArgs ra;
-    ra.v = ws->yv; ra.E = p->E; ra.nrecv = nrecv; ra.elem = ws->drecv_elem; ra.rows = ws->drecv_rows;
-    ra.out = ws->drecv;
+    RecvArgs ra = recv_args(ws, nrecv);
     for (long it = 0; it < nsteps; ++it) {
         if ((s = rk4_device(ws, t, dt, it + 1, it == 0, it == nsteps - 1))) return s;
         t += dt;
@@ -1040,12 +1070,15 @@ extern "C" wedg_status wedg_advance_receivers(wedg_workspace* ws, double t0, dou
     }
     if (t_end) *t_end = t;
     if ((s = collect(ws, stop_on_nonfinite))) return s;
-    if (need > 0) {
-        HIPOK(hipMemcpyAsync(ws->hrecv, ws->drecv, need * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-        HIPOK(hipStreamSynchronize(ws->stream));
-        std::memcpy(recv, ws->hrecv, need * sizeof(double));
-    }
-    return WEDG_OK;
+    return recv_out(ws, recv, need);
+}
+
+extern "C" wedg_status wedg_set_receivers(wedg_workspace* ws, int nrecv, const int* recv_j, const int* recv_k,
+                                          const double* rows) {
+    if (!ws) return WEDG_ERR_ARGUMENT;
+    wedg_status s = recv_rows(ws, nrecv, recv_j, recv_k, rows);
+    ws->nrecv_set = s == WEDG_OK ? nrecv : 0;
+    return s;
 }
 
 extern "C" wedg_status wedg_diagnostics(wedg_workspace* ws, double* E, double* vmax, double* ndmin, int j0,
@@ -1340,12 +1373,21 @@ extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_
     return WEDG_OK;
 }
 
-extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
-                                          int stop_on_nonfinite, double* t_end, int* status) {
+// The batched run; recv (may be null): per instance, null or room for
+// nsteps[i] x (its wedg_set_receivers count) values of v_h at its receivers
+// after every step, [step][receiver].
+static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
+                             int stop_on_nonfinite, double* t_end, int* status, double* const* recv) {
     if (!b || !t0 || !dt || !nsteps) return WEDG_ERR_ARGUMENT;
     const int n = (int)b->ws.size();
     for (int i = 0; i < n; ++i)
         if (nsteps[i] < 0) return WEDG_ERR_ARGUMENT;
+    auto records = [&](int i) { return recv && recv[i] && b->ws[i]->nrecv_set > 0; };
+    for (int i = 0; i < n; ++i)
+        if (records(i)) {
+            const wedg_status sc = recv_capacity(b->ws[i], (size_t)nsteps[i] * b->ws[i]->nrecv_set);
+            if (sc) return sc;
+        }
     for (const auto& l : b->lanes)
         if (!l.stream || !l.dargs[0] || !l.dargs[1] || !l.hargs[0] || !l.hargs[1]) return WEDG_ERR_HIP;
     wedg_status s;
@@ -1396,7 +1438,7 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
     for (auto& l : b->lanes) HIPOK(hipStreamWaitEvent(l.stream, b->t0, 0));
     std::vector<double> t(t0, t0 + n);   // every instance's time: t += dt after each of its steps
     std::vector<double> tc;              // [instance][step of the chunk]: the time at the step's start
-    struct Launch { int g, which, grid; size_t off; };
+    struct Launch { int g, which, grid; size_t off; int inst; long step; };
     std::vector<Launch> L;
     int buf = 0;
     for (long c0 = 0; c0 < maxsteps; c0 += b->chunk, buf ^= 1) {
@@ -1427,14 +1469,24 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
                             H[k++] = rk4_args(b->ws[i], st, tc[(size_t)i * len + (it - c0)], dt[i], it + 1,
                                               it == nsteps[i] - 1);
                         }
-                        if (it == 0 && st == 1) L.push_back({(int)gi, 0, g.bfirst[na], off});
-                        L.push_back({(int)gi, 1, g.bfirst[na], off});
+                        if (it == 0 && st == 1) L.push_back({(int)gi, 0, g.bfirst[na], off, -1, 0});
+                        L.push_back({(int)gi, 1, g.bfirst[na], off, -1, 0});
+                        if (st == 4)   // v_h at the receivers after the step
+                            for (size_t r = 0; r < na; ++r)
+                                if (records(g.order[r])) L.push_back({(int)gi, 2, 0, 0, g.order[r], it});
                     }
             if (k == 0) continue;
             HIPOK(hipMemcpyAsync(l.dargs[buf], H, k * sizeof(KArgs), hipMemcpyHostToDevice, l.stream));
             HIPOK(hipEventRecord(l.copied[buf], l.stream));
             for (const Launch& x : L) {
                 const BatchGroup& g = b->groups[x.g];
+                if (x.which == 2) {
+                    wedg_workspace* w = b->ws[x.inst];
+                    RecvArgs ra = recv_args(w, w->nrecv_set);
+                    ra.step = x.step;
+                    HIPOK(launch_diag(w->p, nullptr, &ra, nullptr, l.stream));
+                    continue;
+                }
                 BatchArgs ba;
                 ba.args = l.dargs[buf] + x.off;
                 ba.binst = g.dmap;
@@ -1453,12 +1505,25 @@ extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const
     b->last_ms = hipEventElapsedTime(&ms, b->t0, b->t1) == hipSuccess ? ms : 0.0;
     wedg_status first = WEDG_OK;
     for (int i = 0; i < n; ++i) {
-        const wedg_status si = collect(b->ws[i], stop_on_nonfinite);
+        wedg_status si = collect(b->ws[i], stop_on_nonfinite);
+        if (si == WEDG_OK && records(i)) si = recv_out(b->ws[i], recv[i], (size_t)nsteps[i] * b->ws[i]->nrecv_set);
         if (status) status[i] = si;
         if (t_end) t_end[i] = t[i];
         if (si != WEDG_OK && first == WEDG_OK) first = si;
     }
     return first;
+}
+
+extern "C" wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
+                                          int stop_on_nonfinite, double* t_end, int* status) {
+    return batch_run(b, t0, dt, nsteps, stop_on_nonfinite, t_end, status, nullptr);
+}
+
+extern "C" wedg_status wedg_batch_advance_receivers(wedg_batch* b, const double* t0, const double* dt,
+                                                    const long* nsteps, int stop_on_nonfinite, double* t_end,
+                                                    int* status, double* const* recv) {
+    if (!recv) return WEDG_ERR_ARGUMENT;
+    return batch_run(b, t0, dt, nsteps, stop_on_nonfinite, t_end, status, recv);
 }
 
 extern "C" wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps) {

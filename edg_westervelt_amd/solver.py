@@ -246,6 +246,91 @@ def _events(dt: float, nsteps: int, record_every: int, snap_times):
     return rec, snap
 
 
+def _source_setup(ws: Workspace, dt: float, nsteps: int, N: int, pre, geo, *, record_every: int, slice_y: float,
+                  receivers, snap_times):
+    """The state of one source-mode run (src/solver.jl:872-933): the diagnostics
+    coefficients on the device, the slice row, the receivers' elements and basis
+    rows, the record and snapshot steps, and the series collected so far."""
+    if record_every < 1:
+        raise ValueError(f"record_every must be ≥ 1 (got {record_every})")
+    qu1, qv1 = pre.qu1, pre.qv1
+    c2 = np.asfortranarray(geo.err_c2, dtype=np.float64)
+    bq = np.asfortranarray(geo.err_beta, dtype=np.float64)
+    W2d = np.asfortranarray(geo.W2d, dtype=np.float64)
+    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _ptr(c2), _ptr(bq), _ptr(W2d)))
+    S = SimpleNamespace(ws=ws, dt=dt, nsteps=nsteps, N=N, pre=pre, geo=geo)
+    S.j0, eta0 = locate(slice_y, N)
+    S.Pveta0 = np.ascontiguousarray(vrow(eta0, qv1))
+    S.slice_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
+    S.receivers = [tuple(map(float, r)) for r in receivers]
+    nr = S.nr = len(S.receivers)
+    S.rj = np.zeros(max(nr, 1), dtype=np.int32)
+    S.rk = np.zeros(max(nr, 1), dtype=np.int32)
+    rows = np.zeros((max(nr, 1), 2 * qv1))
+    for i, (xr, yr) in enumerate(S.receivers):
+        kr, xir = locate(xr, N)
+        jr, etar = locate(yr, N)
+        S.rj[i], S.rk[i] = jr - 1, kr - 1
+        rows[i, :qv1] = vrow(etar, qv1)
+        rows[i, qv1:] = vrow(xir, qv1)
+    S.rows = np.ascontiguousarray(rows)
+    rec_steps, snap_steps = _events(dt, nsteps, record_every, snap_times)
+    S.stops = sorted(set(rec_steps) | set(snap_steps))
+    S.rec_set, S.snap_set = set(rec_steps), set(snap_steps)
+    S.E_el = np.zeros(N * N); S.vmax_el = np.zeros(N * N); S.nd_el = np.zeros(N * N)
+    S.sl = np.zeros(N * qu1)
+    S.order = (np.arange(N)[:, None] + N * np.arange(N)[None, :]).ravel()   # e for j outer, k inner
+    S.rec_t, S.rec_slice, S.rec_E, S.rec_vmax, S.rec_nd = [], [], [], [], []
+    S.recv_all = np.zeros((nsteps, nr))
+    S.snaps = []
+    S.t = 0.0
+    S.done = 0
+    return S
+
+
+def _blow_up(S):
+    """The reference's message for a non-finite step (src/solver.jl:940): t
+    accumulated over the steps of the failing call."""
+    tb = S.t
+    for _ in range(S.ws.dev.last_error().step):
+        tb += S.dt
+    return WedgError(f"blow-up at t={tb!r}")
+
+
+def _source_record(S, stop: int):
+    """The diagnostics and snapshot of a record/snapshot step (src/solver.jl:907-923)."""
+    dh = S.ws.dev.h
+    N, qu1 = S.N, S.pre.qu1
+    if stop in S.rec_set:
+        S.ws.dev.check(lib.wedg_diagnostics(dh, _ptr(S.E_el), _ptr(S.vmax_el), _ptr(S.nd_el), S.j0 - 1,
+                                            _ptr(S.Pveta0), _ptr(S.sl)))
+        S.rec_t.append(S.t)
+        S.rec_slice.append(S.sl.copy())
+        S.rec_E.append(float(np.add.accumulate(np.concatenate(([0.0], S.E_el[S.order])))[-1]))
+        S.rec_vmax.append(float(max(0.0, S.vmax_el.max())))
+        S.rec_nd.append(float(min(np.inf, S.nd_el.min())))
+    if stop in S.snap_set:
+        G = np.zeros((N * qu1, N * qu1), order="F")
+        S.ws.dev.check(lib.wedg_snapshot(dh, _ptr(G)))
+        S.snaps.append((S.t, G))
+
+
+def _source_result(S):
+    N, qu1, geo = S.N, S.pre.qu1, S.geo
+    recv_t = np.zeros(S.nsteps)
+    tt = 0.0
+    for i in range(S.nsteps):   # t after each step, accumulated as the time loop does
+        tt += S.dt
+        recv_t[i] = tt
+    grid_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
+    grid_y = np.concatenate([geo.yloc[j, :] for j in range(N)])
+    return SimpleNamespace(t=np.array(S.rec_t), slice_x=S.slice_x,
+                           slices=np.array(S.rec_slice).T if S.rec_slice else np.zeros((N * qu1, 0)),
+                           energy=np.array(S.rec_E), vmax=np.array(S.rec_vmax), ndmin=np.array(S.rec_nd),
+                           recv_t=recv_t, recv=S.recv_all, receivers=S.receivers, snaps=S.snaps,
+                           grid_x=grid_x, grid_y=grid_y, dt=S.dt)
+
+
 def _run_source(ws: Workspace, dt: float, nsteps: int, N: int, h: float, pre, geo, *, record_every: int,
                 slice_y: float, receivers, snap_times):
     """Source mode (src/solver.jl:872-958): RK4 on the device with v_h at the
@@ -257,81 +342,23 @@ def _run_source(ws: Workspace, dt: float, nsteps: int, N: int, h: float, pre, ge
     snapshot grids are formed on the device (``wedg_advance_receivers``,
     ``wedg_diagnostics``, ``wedg_snapshot``); the host accumulates the element
     values serially in the order row j, column k, as the reference does."""
-    if record_every < 1:
-        raise ValueError(f"record_every must be ≥ 1 (got {record_every})")
-    qu1, qv1 = pre.qu1, pre.qv1
-    dh = ws.dev.h
-    c2 = np.asfortranarray(geo.err_c2, dtype=np.float64)
-    bq = np.asfortranarray(geo.err_beta, dtype=np.float64)
-    W2d = np.asfortranarray(geo.W2d, dtype=np.float64)
-    ws.dev.check(lib.wedg_problem_set_diag(ws.problem.h, _ptr(c2), _ptr(bq), _ptr(W2d)))
-
-    j0, eta0 = locate(slice_y, N)
-    Pveta0 = np.ascontiguousarray(vrow(eta0, qv1))
-    slice_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
-    receivers = [tuple(map(float, r)) for r in receivers]
-    nr = len(receivers)
-    rj = np.zeros(max(nr, 1), dtype=np.int32)
-    rk = np.zeros(max(nr, 1), dtype=np.int32)
-    rows = np.zeros((max(nr, 1), 2 * qv1))
-    for i, (xr, yr) in enumerate(receivers):
-        kr, xir = locate(xr, N)
-        jr, etar = locate(yr, N)
-        rj[i], rk[i] = jr - 1, kr - 1
-        rows[i, :qv1] = vrow(etar, qv1)
-        rows[i, qv1:] = vrow(xir, qv1)
-    rows = np.ascontiguousarray(rows)
-
-    rec_steps, snap_steps = _events(dt, nsteps, record_every, snap_times)
-    stops = sorted(set(rec_steps) | set(snap_steps))
-    rec_set, snap_set = set(rec_steps), set(snap_steps)
-    E_el = np.zeros(N * N); vmax_el = np.zeros(N * N); nd_el = np.zeros(N * N)
-    sl = np.zeros(N * qu1)
-    order = (np.arange(N)[:, None] + N * np.arange(N)[None, :]).ravel()   # e for j outer, k inner
-    rec_t, rec_slice, rec_E, rec_vmax, rec_nd = [], [], [], [], []
-    recv_all = np.zeros((nsteps, nr))
-    snaps = []
-    t = 0.0
-    done = 0
+    S = _source_setup(ws, dt, nsteps, N, pre, geo, record_every=record_every, slice_y=slice_y,
+                      receivers=receivers, snap_times=snap_times)
+    nr = S.nr
     tend = C.c_double(0.0)
-    for stop in stops:
-        n = stop - done
+    for stop in S.stops:
+        n = stop - S.done
         buf = np.zeros(max(n * nr, 1))
-        st = lib.wedg_advance_receivers(dh, t, float(dt), int(n), nr, rj.ctypes.data_as(_ip),
-                                        rk.ctypes.data_as(_ip), _ptr(rows), 1, _ptr(buf), C.byref(tend))
+        st = lib.wedg_advance_receivers(ws.dev.h, S.t, float(dt), int(n), nr, S.rj.ctypes.data_as(_ip),
+                                        S.rk.ctypes.data_as(_ip), _ptr(S.rows), 1, _ptr(buf), C.byref(tend))
         if st == _lib.ERR_NONFINITE:
-            tb = t
-            for _ in range(ws.dev.last_error().step):
-                tb += dt
-            raise WedgError(f"blow-up at t={tb!r}")
+            raise _blow_up(S)
         ws.dev.check(st)
-        t = tend.value
-        recv_all[done:stop] = buf[:n * nr].reshape(n, nr)
-        done = stop
-        if stop in rec_set:
-            ws.dev.check(lib.wedg_diagnostics(dh, _ptr(E_el), _ptr(vmax_el), _ptr(nd_el), j0 - 1,
-                                              _ptr(Pveta0), _ptr(sl)))
-            rec_t.append(t)
-            rec_slice.append(sl.copy())
-            rec_E.append(float(np.add.accumulate(np.concatenate(([0.0], E_el[order])))[-1]))
-            rec_vmax.append(float(max(0.0, vmax_el.max())))
-            rec_nd.append(float(min(np.inf, nd_el.min())))
-        if stop in snap_set:
-            G = np.zeros((N * qu1, N * qu1), order="F")
-            ws.dev.check(lib.wedg_snapshot(dh, _ptr(G)))
-            snaps.append((t, G))
-    recv_t = np.zeros(nsteps)
-    tt = 0.0
-    for i in range(nsteps):   # t after each step, accumulated as the time loop does
-        tt += dt
-        recv_t[i] = tt
-    grid_x = np.concatenate([geo.xloc[k, :] for k in range(N)])
-    grid_y = np.concatenate([geo.yloc[j, :] for j in range(N)])
-    return SimpleNamespace(t=np.array(rec_t), slice_x=slice_x,
-                           slices=np.array(rec_slice).T if rec_slice else np.zeros((N * qu1, 0)),
-                           energy=np.array(rec_E), vmax=np.array(rec_vmax), ndmin=np.array(rec_nd),
-                           recv_t=recv_t, recv=recv_all, receivers=receivers, snaps=snaps,
-                           grid_x=grid_x, grid_y=grid_y, dt=dt)
+        S.t = tend.value
+        S.recv_all[S.done:stop] = buf[:n * nr].reshape(n, nr)
+        S.done = stop
+        _source_record(S, stop)
+    return _source_result(S)
 
 
 def mms_errors(u, v, T, N, h, pre, geo, kx=1, ky=1, phase=(0.0, 0.0)):

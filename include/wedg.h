@@ -236,6 +236,17 @@ wedg_status wedg_batch_destroy(wedg_batch* b);
 wedg_status wedg_batch_advance(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
                                int stop_on_nonfinite, double* t_end, int* status);
 wedg_status wedg_batch_time(const wedg_batch* b, double* ms);
+/* Source mode in a batch: wedg_set_receivers stores a workspace's receivers
+ * (the arguments of wedg_advance_receivers); wedg_batch_advance_receivers is
+ * wedg_batch_advance that also records, for every instance i with receivers
+ * and recv[i] non-null, v_h at its receivers after each of its nsteps[i]
+ * steps into recv[i] ([step][receiver]), as wedg_advance_receivers does for
+ * one workspace (src/solver.jl:898-905, 938-941). */
+wedg_status wedg_set_receivers(wedg_workspace* ws, int nrecv, const int* recv_j, const int* recv_k,
+                               const double* rows);
+wedg_status wedg_batch_advance_receivers(wedg_batch* b, const double* t0, const double* dt, const long* nsteps,
+                                         int stop_on_nonfinite, double* t_end, int* status, double* const* recv);
+
 /* Launch parameters of a batch (they change no result): the number of streams
  * the groups are dealt to (1..16; default: the process's hardware queues,
  * GPU_MAX_HW_QUEUES or 4) and the steps whose stage arguments are staged per

@@ -365,3 +365,35 @@ def test_batch_blow_up_instance():
     reset()
     with pytest.raises(_lib.WedgError):
         bt.advance(0.0, dts, steps, stop_on_nonfinite=True)
+
+
+def _same_series(a, b):
+    assert np.array_equal(a.t, b.t) and np.array_equal(a.energy, b.energy)
+    assert np.array_equal(a.vmax, b.vmax) and np.array_equal(a.ndmin, b.ndmin)
+    assert np.array_equal(a.slices, b.slices) and np.array_equal(a.recv, b.recv)
+    assert np.array_equal(a.recv_t, b.recv_t) and len(a.snaps) == len(b.snaps)
+    for (ta, Ga), (tb, Gb) in zip(a.snaps, b.snaps):
+        assert ta == tb and np.array_equal(Ga, Gb)
+
+
+def test_run_source_many_uneven_schedules():
+    """Source-mode runs of different meshes, degrees, end times, record
+    intervals and snapshot times in one batch: every series, receiver row and
+    snapshot bitwise those of separate solve(..., mode="source") calls."""
+    import edg_westervelt_amd as w
+    src = w.GaussianSource(x0=(0.4, 0.5), r0=0.1)
+    cc = w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0)
+    runs = [dict(q=2, N=8, T=0.05, record_every=5, snap_times=(0.02,), source=src, coefficients=cc,
+                 receivers=((0.45, 0.5), (0.7, 0.3))),
+            dict(q=3, N=6, T=0.04, record_every=7, source=src, coefficients=cc, boundary="wall", kappa=0.4,
+                 mixed=False),
+            dict(q=2, N=8, T=0.03, record_every=3, snap_times=(0.01, 0.025), source=src, coefficients=cc,
+                 boundary="dirichlet", receivers=((0.2, 0.8),))]
+    got = B.run_source_many(runs)
+    for r, g in zip(runs, got):
+        r = dict(r)
+        ref = w.solve(r.pop("q"), r.pop("N"), r.pop("T"), mode="source", **r)
+        _same_series(ref, g)
+        assert len(g.t) > 1 and np.abs(g.recv).max() > 0
+    with pytest.raises(ValueError):
+        B.run_source_many([dict(q=2, N=4, T=0.01, mode="mms")])

@@ -113,21 +113,22 @@ def test_source_mode_short_run():
     assert H.dt == dt and H.receivers == [tuple(r) for r in run["receivers"]]
 
 
-@pytest.mark.parametrize("case", ["nl", "lin", "nl_dbc"])
-def test_steepening(case):
-    """experiments/steepening/run.jl on the device, all 18432 steps, against
-    data/steepening/<case>: the time axis exactly; the energy, max |v_h| and
-    min (1 - 2 beta v_h) of all 576 records, 1e-12 relative; v_h at the three
-    receivers every 64th step, 1e-12 of the series' largest |v_h|; for nl the
-    snapshot at t = 0.80 (every 8th point), 1e-12 of its largest |v_h|, and the
-    grid coordinates. (Measured on MI355X: 2e-14 or better for every series.)"""
+def steepening_kwargs(case):
+    """The arguments of solve for experiments/steepening/run.jl:21-31, case <case>."""
     beta, bnd = STEEPENING[case]
     R = STEEPENING_RUN
-    say(f"case steepening {case}")
-    H = w.solve(R["q"], R["N"], R["T"], flux="upwind", mixed=True, mode="source", boundary=bnd,
+    return dict(q=R["q"], N=R["N"], T=R["T"], flux="upwind", mixed=True, mode="source", boundary=bnd,
                 coefficients=w.constant_coefficients(c=1.0, alpha=1e-6, beta=beta),
                 source=w.GaussianSource(x0=(0.15, 0.5), r0=0.05, omega=16 * math.pi, Ts=1.0),
                 record_every=R["record_every"], receivers=list(R["receivers"]), snap_times=list(R["snap_times"]))
+
+
+def check_steepening(case, H):
+    """H against data/steepening/<case>: the time axis exactly; the energy, max
+    |v_h| and min (1 - 2 beta v_h) of all 576 records, 1e-12 relative; v_h at
+    the three receivers every 64th step, 1e-12 of the series' largest |v_h|;
+    for nl the snapshot at t = 0.80 (every 8th point), 1e-12 of its largest
+    |v_h|, and the grid coordinates."""
     ser = np.array(steepening_series(case))
     assert np.array_equal(H.t, ser[:, 0])
     dev = {k: float(np.max(np.abs(getattr(H, k) / ser[:, i] - 1))) for i, k in ((1, "energy"), (2, "vmax"), (3, "ndmin"))}
@@ -150,6 +151,30 @@ def test_steepening(case):
     assert dev["recv"] < 1e-12
     if case == "nl":
         assert dev["snap"] < 1e-12
+
+
+@pytest.mark.parametrize("case", ["nl", "lin", "nl_dbc"])
+def test_steepening(case):
+    """experiments/steepening/run.jl on the device, all 18432 steps, against
+    data/steepening/<case> (check_steepening). (Measured on MI355X: 2e-14 or
+    better for every series.)"""
+    say(f"case steepening {case}")
+    kw = steepening_kwargs(case)
+    H = w.solve(kw.pop("q"), kw.pop("N"), kw.pop("T"), **kw)
+    check_steepening(case, H)
+
+
+def test_steepening_batched():
+    """The three steepening runs in one batch (run_source_many), each against
+    the reference's data as the separate runs are."""
+    from edg_westervelt_amd import batch
+    import time
+    cases = ["nl", "lin", "nl_dbc"]
+    t0 = time.perf_counter()
+    Hs = batch.run_source_many([steepening_kwargs(c) for c in cases])
+    say(f"three steepening runs batched: {time.perf_counter() - t0:.2f} s")
+    for c, H in zip(cases, Hs):
+        check_steepening(c, H)
 
 
 @pytest.mark.parametrize("q,N,mixed", [(3, 6, True), (5, 4, False), (1, 7, True)])
