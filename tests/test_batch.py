@@ -61,3 +61,17 @@ def test_instance_tables(i):
         for k in (0, 2 * nb, x.N - 1):
             c2 = x.c[k // nb, j // nb] ** 2
             assert np.allclose(Cw[:, j, k, :], c2 * np.outer(w, w), rtol=1e-14, atol=0)
+
+
+def test_solve_many_argument_checks():
+    """solve_many and run_source_many refuse what they do not batch before any
+    device work (solve's own messages where solve has one)."""
+    with pytest.raises(ValueError, match="source"):
+        B.solve_many([dict(q=2, N=4, T=0.01, mode="source")])
+    with pytest.raises(ValueError, match="return_state"):
+        B.solve_many([dict(q=2, N=4, T=0.01, mode="initial_value")])
+    with pytest.raises(ValueError, match="unknown mode"):
+        B.solve_many([dict(q=2, N=4, T=0.01, mode="adjoint")])
+    with pytest.raises(ValueError, match="source"):
+        B.run_source_many([dict(q=2, N=4, T=0.01, mode="mms")])
+    assert B.solve_many([]) == [] and B.run_source_many([]) == []
