@@ -131,6 +131,79 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
                            probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()}))
 
 
+def reference_runs():
+    """Wall times (precompute, upload, run and results, on this machine) of the
+    runs the reference times itself, as it states them: README.md:244-245,
+    313-315 and data/convergence/results.csv; its figures are on its own
+    machine (Apple M4 Pro for the README), given here as context."""
+    import csv
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import batch as B
+    from tests.cases import BUMP, STEEPENING_RUN
+
+    def timed(f):
+        t = time.perf_counter()
+        f()
+        return time.perf_counter() - t
+
+    out = {}
+    w.solve(2, 4, 0.01)   # loads the library and the device once
+    print("# reference runs: cfg1, solve(4, 32, 0.1), source 64x64", file=sys.stderr, flush=True)
+    out["cfg1 solve(3, 10, 0.1; flux=:upwind, mixed=true)"] = dict(
+        s=timed(lambda: w.solve(3, 10, 0.1, flux="upwind", mixed=True)),
+        reference="0.1 s (data/convergence/results.csv:364, one Julia thread)")
+    out["solve(4, 32, 0.1)"] = dict(s=timed(lambda: w.solve(4, 32, 0.1)),
+                                    reference="1.2 s with 8 threads, 3.4 s serial (README.md:313-314, M4 Pro)")
+    out["solve(4, 64, 0.05; mode=:source)"] = dict(
+        s=timed(lambda: w.solve(4, 64, 0.05, mode="source")),
+        reference="2.5 s with 8 threads, 13 s serial (README.md:314-315, M4 Pro)")
+    R = STEEPENING_RUN
+    steep = dict(flux="upwind", mixed=True, mode="source",
+                 source=w.GaussianSource(x0=(0.15, 0.5), r0=0.05, omega=16 * math.pi, Ts=1.0),
+                 record_every=R["record_every"], receivers=list(R["receivers"]), snap_times=list(R["snap_times"]))
+    cases = {"nl": (8.0, "absorbing_2beta"), "lin": (0.0, "absorbing_2beta"), "nl_dbc": (8.0, "dirichlet")}
+    print("# reference runs: steepening", file=sys.stderr, flush=True)
+    out["steepening nl (experiments/steepening/run.jl)"] = dict(
+        s=timed(lambda: w.solve(R["q"], R["N"], R["T"], boundary="absorbing_2beta",
+                                coefficients=w.constant_coefficients(c=1.0, alpha=1e-6, beta=8.0), **steep)),
+        reference="about 2.5 min with 8 threads, 11 min on one (README.md:245)")
+    runs = [dict(q=R["q"], N=R["N"], T=R["T"], boundary=b,
+                 coefficients=w.constant_coefficients(c=1.0, alpha=1e-6, beta=beta), **steep)
+            for beta, b in cases.values()]
+    out["steepening, all three cases in one batch (run_source_many)"] = dict(
+        s=timed(lambda: B.run_source_many(runs)), reference="about 2.5 min per case with 8 threads")
+
+    def alpha_table(many):
+        alphas = [0.0, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9]
+        kw = [dict(q=6, N=20, T=0.5, flux="upwind", mixed=True, mode="initial_value", initial_profile=BUMP,
+                   coefficients=w.replace_coefficients(w.paper_coefficients(), alpha=a), return_state=True)
+              for a in alphas]
+        if many:
+            res = B.solve_many(kw)
+        else:
+            res = []
+            for k in kw:
+                k = dict(k)
+                res.append(w.solve(k.pop("q"), k.pop("N"), k.pop("T"), **k))
+        return [w.difference_norms(r, res[0]) for r in res[1:]]
+
+    print("# reference runs: alpha table", file=sys.stderr, flush=True)
+    out["alpha -> 0 table (experiments/alpha_limit/run.jl), runs one after the other"] = dict(
+        s=timed(lambda: alpha_table(False)), reference="about 1 min with 8 threads, 2-3 min on one (README.md:244)")
+    out["alpha -> 0 table, runs in one batch (solve_many)"] = dict(s=timed(lambda: alpha_table(True)),
+                                                                  reference="as above")
+    rows = list(csv.DictReader(open(os.path.join(ROOT, "tests", "golden", "convergence_results.csv"))))
+    conv = [dict(q=int(r["q"]), N=int(r["N"]), T=0.1, flux=r["flux"], mixed=r["pairing"] == "mixed") for r in rows]
+    print("# reference runs: convergence study", file=sys.stderr, flush=True)
+    out["convergence study, 756 solves (experiments/convergence/run_all.sh) in one batch (solve_many)"] = dict(
+        s=timed(lambda: B.solve_many(conv)),
+        reference="about 5 min with 8 processes, 36 min on one core (README.md:240); results.csv secs sum "
+                  "to 3,426 s")
+    for v in out.values():
+        v["s"] = round(v["s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +220,8 @@ def main():
                     help="the uneven-batch speedup: with the headline (on), not at all (off), or alone (only)")
     ap.add_argument("--batched-sequential", choices=("on", "off"), default="on",
                     help="off: time only the batched run (for a kernel trace of it)")
+    ap.add_argument("--reference-runs", action="store_true",
+                    help="only time the runs the reference times itself (README, data files) and exit")
     ap.add_argument("--probe-copy", type=float, default=0.0,
                     help="only run a plain copy kernel of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -167,6 +242,10 @@ def main():
             raise SystemExit(f"copy probe failed ({st})")
         print(json.dumps(dict(copy_probe_bytes_each_way=8 * n, launches=11, ms_avg=ms.value,
                               gbs=2 * 8 * n / ms.value / 1e6)))
+        return
+    if args.reference_runs:
+        print(json.dumps(dict(metric="wall time of the reference's own timed runs", unit="s", n_gpus=1,
+                              higher_is_better=False, dtype="f64", data="synthetic", runs=reference_runs())))
         return
     if args.batched == "only":
         bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb)
