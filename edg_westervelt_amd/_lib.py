@@ -90,10 +90,10 @@ def _load():
         "wedg_batch_destroy": [vp],
         "wedg_batch_advance": [vp, _dp, _dp, C.POINTER(C.c_long), C.c_int, _dp, _ip],
         "wedg_batch_time": [vp, C.POINTER(C.c_double)],
-        "wedg_batch_configure": [vp, C.c_int, C.c_long],
+        "wedg_batch_configure": [vp, C.c_int, C.c_long, C.c_int],
         "wedg_set_receivers": [vp, C.c_int, _ip, _ip, _dp],
         "wedg_batch_advance_receivers": [vp, _dp, _dp, C.POINTER(C.c_long), C.c_int, _dp, _ip, C.POINTER(_dp)],
-        "wedg_batch_config": [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)],
+        "wedg_batch_config": [vp, C.POINTER(C.c_int), C.POINTER(C.c_long), C.POINTER(C.c_int)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -73,16 +73,20 @@ class Batch:
                 raise WedgError(f"wedg_batch_advance failed with status {st}")
         return tend, status
 
-    def configure(self, lanes: int, chunk_steps: int = 0):
-        """Streams and argument chunk of the launches (``wedg_batch_configure``)."""
-        st = lib.wedg_batch_configure(self.h, int(lanes), int(chunk_steps))
+    def configure(self, lanes: int, chunk_steps: int = 0, schedule=None):
+        """Streams, argument chunk and schedule (0: every instance starts at the
+        first step, 1: all end at the last; None keeps it) of the launches
+        (``wedg_batch_configure``)."""
+        if schedule is None:
+            schedule = self.config()[2]
+        st = lib.wedg_batch_configure(self.h, int(lanes), int(chunk_steps), int(schedule))
         if st != _lib.OK:
             raise WedgError(f"wedg_batch_configure failed with status {st}")
 
     def config(self):
-        lanes, chunk = C.c_int(0), C.c_long(0)
-        lib.wedg_batch_config(self.h, C.byref(lanes), C.byref(chunk))
-        return lanes.value, chunk.value
+        lanes, chunk, sched = C.c_int(0), C.c_long(0), C.c_int(0)
+        lib.wedg_batch_config(self.h, C.byref(lanes), C.byref(chunk), C.byref(sched))
+        return lanes.value, chunk.value, sched.value
 
     def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8), chunks=(0, 4), repeats: int = 2):
         """Autotuner of the launch parameters: times ``probe_steps`` steps of the

@@ -1257,6 +1257,7 @@ struct wedg_batch {
     hipStream_t stream = nullptr; // block maps, timing events, the join of the lanes
     std::vector<void*> allocs;
     long chunk = 1;               // steps per chunk of stage arguments
+    int schedule = 0;             // 0: every instance starts at the first step; 1: all end at the last
     int* hmap = nullptr;          // pinned staging of the block maps
     size_t map_ints = 0;
     double last_ms = 0.0;         // device time of the last advance (stream events)
@@ -1436,6 +1437,12 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
     }
     HIPOK(hipEventRecord(b->t0, b->stream));
     for (auto& l : b->lanes) HIPOK(hipStreamWaitEvent(l.stream, b->t0, 0));
+    // instance i runs its steps at the batch's steps o[i] .. o[i] + nsteps[i] - 1;
+    // in both schedules the active instances of a group are a prefix of its order
+    std::vector<long> o(n, 0);
+    if (b->schedule == 1)
+        for (int i = 0; i < n; ++i) o[i] = maxsteps - nsteps[i];
+    auto active = [&](int i, long it) { return it >= o[i] && it < o[i] + nsteps[i]; };
     std::vector<double> t(t0, t0 + n);   // every instance's time: t += dt after each of its steps
     std::vector<double> tc;              // [instance][step of the chunk]: the time at the step's start
     struct Launch { int g, which, grid; size_t off; int inst; long step; };
@@ -1447,7 +1454,7 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
         for (int i = 0; i < n; ++i)
             for (long j = 0; j < len; ++j) {
                 tc[(size_t)i * len + j] = t[i];
-                if (c0 + j < nsteps[i]) t[i] += dt[i];
+                if (active(i, c0 + j)) t[i] += dt[i];
             }
         for (size_t li = 0; li < b->lanes.size(); ++li) {
             BatchLane& l = b->lanes[li];
@@ -1461,19 +1468,25 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
                         const BatchGroup& g = b->groups[gi];
                         if (g.lane != (int)li) continue;
                         size_t na = 0;
-                        while (na < g.order.size() && nsteps[g.order[na]] > it) ++na;
+                        while (na < g.order.size() && active(g.order[na], it)) ++na;
                         if (na == 0) continue;
                         const size_t off = k;
+                        bool starts = false;   // an instance takes its first step here
                         for (size_t r = 0; r < na; ++r) {
                             const int i = g.order[r];
-                            H[k++] = rk4_args(b->ws[i], st, tc[(size_t)i * len + (it - c0)], dt[i], it + 1,
-                                              it == nsteps[i] - 1);
+                            const long s = it - o[i];   // the instance's own step
+                            starts = starts || s == 0;
+                            H[k++] = rk4_args(b->ws[i], st, tc[(size_t)i * len + (it - c0)], dt[i], s + 1,
+                                              s == nsteps[i] - 1);
                         }
-                        if (it == 0 && st == 1) L.push_back({(int)gi, 0, g.bfirst[na], off, -1, 0});
+                        // the first stage of an instance's run needs k_prep; for the instances
+                        // already running it rewrites the values their last stage formed
+                        if (st == 1 && starts) L.push_back({(int)gi, 0, g.bfirst[na], off, -1, 0});
                         L.push_back({(int)gi, 1, g.bfirst[na], off, -1, 0});
                         if (st == 4)   // v_h at the receivers after the step
                             for (size_t r = 0; r < na; ++r)
-                                if (records(g.order[r])) L.push_back({(int)gi, 2, 0, 0, g.order[r], it});
+                                if (records(g.order[r]))
+                                    L.push_back({(int)gi, 2, 0, 0, g.order[r], it - o[g.order[r]]});
                     }
             if (k == 0) continue;
             HIPOK(hipMemcpyAsync(l.dargs[buf], H, k * sizeof(KArgs), hipMemcpyHostToDevice, l.stream));
@@ -1526,8 +1539,10 @@ extern "C" wedg_status wedg_batch_advance_receivers(wedg_batch* b, const double*
     return batch_run(b, t0, dt, nsteps, stop_on_nonfinite, t_end, status, recv);
 }
 
-extern "C" wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps) {
-    if (!b || lanes < 1 || lanes > kMaxBatchLanes || chunk_steps < 0) return WEDG_ERR_ARGUMENT;
+extern "C" wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps, int schedule) {
+    if (!b || lanes < 1 || lanes > kMaxBatchLanes || chunk_steps < 0 || schedule < 0 || schedule > 1)
+        return WEDG_ERR_ARGUMENT;
+    b->schedule = schedule;
     HIPOK(hipStreamSynchronize(b->stream));
     lanes_free(b);
     if (lanes_alloc(b, lanes, chunk_steps) != hipSuccess) {
@@ -1538,10 +1553,11 @@ extern "C" wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk
     return WEDG_OK;
 }
 
-extern "C" wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps) {
-    if (!b || !lanes || !chunk_steps) return WEDG_ERR_ARGUMENT;
+extern "C" wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps, int* schedule) {
+    if (!b || !lanes || !chunk_steps || !schedule) return WEDG_ERR_ARGUMENT;
     *lanes = (int)b->lanes.size();
     *chunk_steps = b->chunk;
+    *schedule = b->schedule;
     return WEDG_OK;
 }
 

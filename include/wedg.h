@@ -249,10 +249,13 @@ wedg_status wedg_batch_advance_receivers(wedg_batch* b, const double* t0, const 
 
 /* Launch parameters of a batch (they change no result): the number of streams
  * the groups are dealt to (1..16; default: the process's hardware queues,
- * GPU_MAX_HW_QUEUES or 4) and the steps whose stage arguments are staged per
- * copy (0: about 4 MB of arguments). wedg_batch_config reads them back. */
-wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps);
-wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps);
+ * GPU_MAX_HW_QUEUES or 4), the steps whose stage arguments are staged per
+ * copy (0: about 4 MB of arguments), and the schedule: 0 (default) starts
+ * every instance at the advance's first step, 1 ends them all at its last
+ * (instance i runs its nsteps[i] steps as the last ones of the advance).
+ * wedg_batch_config reads them back. */
+wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps, int schedule);
+wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps, int* schedule);
 
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);

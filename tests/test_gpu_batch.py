@@ -48,7 +48,11 @@ def describe(b):
             f"one_sided={b.pre.one_sided}")
 
 
-def test_batch_bitwise_equals_separate_runs():
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_batch_bitwise_equals_separate_runs(schedule):
+    """Every instance bitwise as its own run, with the instances starting
+    together (schedule 0) or ending together (1), over two streams and chunks
+    of three steps (so chunks and instance starts cut each other)."""
     insts = B.uneven_instances(seed=0, count=12)
     steps = [3, 7, 0, 5, 1, 9, 4, 2, 6, 8, 3, 5]
     built = [B.build_instance(x) for x in insts]
@@ -58,8 +62,9 @@ def test_batch_bitwise_equals_separate_runs():
         ref.append(solo(b, n))
     for b in built:
         B.set_zero_state(b.ws)
-    say("batch of", len(built))
+    say("batch of", len(built), "schedule", schedule)
     bt = B.Batch([b.ws for b in built])
+    bt.configure(2, 3, schedule)
     tend, status = bt.advance(0.0, [b.dt for b in built], steps)
     for i, b in enumerate(built):
         u, v = B.get_state(b.ws)
