@@ -31,3 +31,30 @@ def test_struct_layout_matches_header():
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     ptrs = re.findall(r"\*(\w+)", body)
     assert tuple(ptrs) == _lib._PTRS
+
+
+def test_entry_points_reject_null_handles():
+    """Every entry point that takes a handle checks its arguments before any
+    HIP call: null handles and pointers give WEDG_ERR_ARGUMENT (the destroy
+    functions accept null, as free does). No device work is done."""
+    import ctypes as C
+    from edg_westervelt_amd import _lib
+    skip = {"wedg_probe_copy", "wedg_device_info", "wedg_version"}   # these touch the device
+    checked = 0
+    for name in declared_functions():
+        if name in skip:
+            continue
+        f = getattr(_lib.lib, name)
+        args = []
+        for t in f.argtypes:
+            if t in (C.c_int, C.c_long, C.c_size_t):
+                args.append(0)
+            elif t is C.c_double:
+                args.append(0.0)
+            else:
+                args.append(None)
+        st = f(*args)
+        expect = _lib.OK if name.endswith("_destroy") else _lib.ERR_ARGUMENT
+        assert st == expect, (name, st)
+        checked += 1
+    assert checked >= 30
