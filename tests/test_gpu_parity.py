@@ -385,3 +385,46 @@ def test_solve_deterministic_and_concurrent():
         t.join()
     for r in out:
         assert np.array_equal(r.u, r1.u) and np.array_equal(r.v, r1.v)
+
+
+def _mirror_x(X):
+    """x -> 1 - x: element column k -> N-1-k, x-mode b (the last index) times (-1)^b."""
+    s = (-1.0) ** np.arange(X.shape[3])
+    return np.asfortranarray(X[:, ::-1, :, :] * s[None, None, None, :])
+
+
+def _mirror_y(X):
+    """y -> 1 - y: element row j -> N-1-j, y-mode a (the third index) times (-1)^a."""
+    s = (-1.0) ** np.arange(X.shape[2])
+    return np.asfortranarray(X[::-1, :, :, :] * s[None, None, :, None])
+
+
+@pytest.mark.parametrize("case", ["full_size", "sides"])
+def test_mirror_symmetry_device(case):
+    """A size-independent property at the benchmark's full size: with the
+    central flux (theta = 1/2), the same kind on opposite sides, constant
+    coefficients and a centred profile, the scheme commutes with the
+    reflections x -> 1-x and y -> 1-y (element order reversed, odd Legendre
+    modes negated): RHS(mirror(u, v)) = mirror(RHS(u, v)) to 1e-12. A swapped
+    side, a transposed contraction or a wrong orientation breaks it (the
+    oracle at N = 6 holds it to 2e-15 and fails it at O(1) with the parity on
+    the wrong mode index)."""
+    from edg_westervelt_amd.batch import gaussian_profile
+    cc = w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0)
+    if case == "full_size":
+        q, N = 5, 256
+        pre, _ = precompute(q, N, mixed=False, flux="central", boundary="absorbing_2beta", mode="initial_value",
+                            coefficients=cc, initial_profile=gaussian_profile(0.5, 0.5, 0.05, 0.05))
+    else:
+        q, N = 4, 20
+        pre, _ = precompute(q, N, mixed=True, flux="central", mode="initial_value", coefficients=cc,
+                            boundary=dict(bot="dirichlet", top="dirichlet", lft="wall", rgt="wall"),
+                            sigma_dirichlet=0.3, kappa=0.5, initial_profile=gaussian_profile(0.5, 0.5, 0.1, 0.01))
+    say(f"case mirror {case} q={q} N={N}")
+    rng = np.random.default_rng(17)
+    u, v = rand_state(rng, N, pre, su=1e-4, sv=1e-3)
+    a = w.edg_rhs(u, v, 0.1, N, 1.0 / N, pre)
+    for R in (_mirror_x, _mirror_y):
+        b = w.edg_rhs(R(u), R(v), 0.1, N, 1.0 / N, pre)
+        assert rel(R(a[0]), b[0]) < TOL and rel(R(a[1]), b[1]) < TOL, (R.__name__, rel(R(a[0]), b[0]),
+                                                                      rel(R(a[1]), b[1]))
