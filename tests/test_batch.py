@@ -75,3 +75,14 @@ def test_solve_many_argument_checks():
     with pytest.raises(ValueError, match="source"):
         B.run_source_many([dict(q=2, N=4, T=0.01, mode="mms")])
     assert B.solve_many([]) == [] and B.run_source_many([]) == []
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before any device work."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and not r.stdout
