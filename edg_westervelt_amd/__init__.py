@@ -4,7 +4,9 @@ of the Westervelt equation (the hot path of WesterveltEDG.jl).
 Host-side table production (``precompute``) is plain numpy; the device API
 (``Workspace``, ``edg_rhs_``, ``edg_rhs``, ``rk4_step_``, ``rk4_step``,
 ``solve``, ``difference_norms``) loads the HIP library libwedg.so on first use
-and raises if it is missing.
+and raises if it is missing. ``edg_westervelt_amd.batch`` advances many
+independent instances together (``Batch``, ``solve_many``,
+``run_source_many``, the uneven workload ``uneven_instances``).
 """
 from .quadrature import JacobiP, GradJacobiP, GaussQCofs
 from .coefficients import (Coefficients, paper_coefficients, replace_coefficients,
