@@ -387,12 +387,16 @@ __device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, 
 }
 
 // Slot values of one point i of a physical boundary side s (src/rhs.jl:738-765);
-// returns the point's failing _boundary_key (src/rhs.jl:732) or 0.
-template <bool UNI, class KA>
+// returns the point's failing _boundary_key (src/rhs.jl:732) or 0. HZ: s is a
+// horizontal side (0 or 1, possibly different in different lanes).
+template <bool UNI, bool HZ, class KA>
 __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i, int m, int nq, double du,
                                               double vt, double dv, double ct, double stt, double& A, double& Dd,
                                               double& Ff) {
-    const bool hz = s < 2;
+    constexpr bool hz = HZ;
+    // the side's parameters: with HZ a choice between the two horizontal sides'
+    // (uniform values), so that no lane-indexed load is needed
+    auto sv = [&](const auto& arr) { return HZ ? (s == 0 ? arr[0] : arr[1]) : arr[s]; };
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
     const double c2f = fval<UNI>(a, hz ? F_HC2 : F_VC2, fi);
     const double af = fval<UNI>(a, hz ? F_HAF : F_VAF, fi);
@@ -401,24 +405,26 @@ __device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i
     const double two_h = a.two_h, om = a.omega;
     const double gun = sgn * (two_h * du);
     const double gvn = sgn * (two_h * dv);
-    const double spv = a.bcd[2 * s][i + nq * m], spnv = a.bcd[2 * s + 1][i + nq * m];
+    const double* bsp = HZ ? (s == 0 ? a.bcd[0] : a.bcd[2]) : a.bcd[2 * s];
+    const double* bspn = HZ ? (s == 0 ? a.bcd[1] : a.bcd[3]) : a.bcd[2 * s + 1];
+    const double spv = bsp[i + nq * m], spnv = bspn[i + nq * m];
     const double gu_ex = sgn * (ct * spnv - spnv);
     const double gv_ex = sgn * (-om * stt * spnv);
     const double v_ex = -om * stt * spv;
     double F1, dvv;
-    if (a.kind[s] == 3) {                                  // wall (src/rhs.jl:91-96)
+    if (sv(a.kind) == 3) {                                 // wall (src/rhs.jl:91-96)
         const double g2 = c2f * gu_ex + af * gv_ex;
         const double rho_w = c2f * gun + af * gvn - g2;
         F1 = g2;
-        dvv = -(a.kap[s] / cf) * rho_w;
+        dvv = -(sv(a.kap) / cf) * rho_w;
     } else {
-        const double kb = a.kb[s];
+        const double kb = sv(a.kb);
         const int check = 1.0 - kb * bf * vt <= 0.0 ? 1 : (1.0 - kb * bf * v_ex <= 0.0 ? 2 : 0);
         if (check) {
             A = 0.0; Dd = 0.0; Ff = 0.0;
             return 2 * (s * nq * a.N + i + m * nq) + check;
         }
-        const double g = a.gam[s], et = a.eta[s], sg = a.sig[s];   // src/rhs.jl:74-85
+        const double g = sv(a.gam), et = sv(a.eta), sg = sv(a.sig);   // src/rhs.jl:74-85
         const double rho_num = et * (c2f * gun + af * gvn) + g * cf * sqrt(1.0 - kb * bf * vt) * vt;
         const double rho_ex = et * (c2f * gu_ex + af * gv_ex) + g * cf * sqrt(1.0 - kb * bf * v_ex) * v_ex;
         const double rho_net = rho_num - rho_ex;
@@ -448,6 +454,26 @@ __device__ __forceinline__ void side_trace(cdbl* T, int oEr, const double (&X)[n
 #pragma unroll
         for (int a = 0; a < n; ++a) {
             const double x = second ? X[b + n * a] : X[a + n * b];
+            if (a & 1) od = fma(T[oEr + a], x, od);
+            else ev = fma(T[oEr + a], x, ev);
+        }
+        sv[b] = plus_end ? ev + od : (DER ? od - ev : ev - od);
+    }
+    rows_sym<NQ, n, false>(T + D::oP, [&](int b) { return sv[b]; }, o);
+}
+
+// side_trace of a horizontal side (B or T) whose end is chosen per lane (plus_end: T)
+template <class D, int n, bool DER>
+__device__ __forceinline__ void side_trace_h(cdbl* T, int oEr, const double (&X)[n * n], double (&o)[D::NQ],
+                                             bool plus_end) {
+    constexpr int NQ = D::NQ;
+    double sv[n];
+#pragma unroll
+    for (int b = 0; b < n; ++b) {
+        double ev = 0.0, od = 0.0;
+#pragma unroll
+        for (int a = 0; a < n; ++a) {
+            const double x = X[a + n * b];
             if (a & 1) od = fma(T[oEr + a], x, od);
             else ev = fma(T[oEr + a], x, ev);
         }
@@ -531,15 +557,44 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
             for (int i = 0; i < NQ; ++i) {
                 const long fi = (long)i * A.fstride + fidx;
                 double Av, Dd, Ff;
-                const int k2 = boundary_point<UNI>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
+                const int k2 = boundary_point<UNI, (s < 2)>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
                 if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
                 bs[(long)(i * 3 + 0) * N] = Av;
                 bs[(long)(i * 3 + 1) * N] = Dd;
                 bs[(long)(i * 3 + 2) * N] = Ff;
             }
         };
-        side(std::integral_constant<int, 0>());
-        side(std::integral_constant<int, 1>());
+        // The bottom (j = 0) and top (j = N - 1) sides in one pass, each lane its own:
+        // in a wave of one column they are two lanes, which would otherwise run the
+        // side's code one after the other. (N = 1: one element has both.)
+        auto hside = [&](int s) {
+            CKArgs& A = *kargs<D::BT>();
+            cdbl* T = ctab(A.tab);
+            const bool pe = s == 1;
+            double du[NQ], vt[NQ], dv[NQ];
+            side_trace_h<D, NU, true>(T, D::oE1r, U, du, pe);
+            side_trace_h<D, NV, false>(T, D::oEr, V, vt, pe);
+            side_trace_h<D, NV, true>(T, D::oE1r, V, dv, pe);
+            const int m = k;
+            const int fidx = pe ? N + (N + 1) * k : (N + 1) * k;
+            double* bs = A.bsl + (long)s * NQ * 3 * N + m;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const long fi = (long)i * A.fstride + fidx;
+                double Av, Dd, Ff;
+                const int k2 = boundary_point<UNI, true>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
+                if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
+                bs[(long)(i * 3 + 0) * N] = Av;
+                bs[(long)(i * 3 + 1) * N] = Dd;
+                bs[(long)(i * 3 + 2) * N] = Ff;
+            }
+        };
+        if (N > 1) {
+            if (j == 0 || j == N - 1) hside(j == 0 ? 0 : 1);
+        } else {
+            side(std::integral_constant<int, 0>());
+            side(std::integral_constant<int, 1>());
+        }
         side(std::integral_constant<int, 2>());
         side(std::integral_constant<int, 3>());
         if (bkey) atomicMin(&B.err->fkey, fail_key(serial, 0, bkey));
