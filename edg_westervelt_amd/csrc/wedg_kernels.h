@@ -14,18 +14,19 @@
 // into its even and odd modes (rows_sym / cols_sym), which halves its FMAs; the
 // results differ from the plain sums by rounding only.
 //
-// One kernel per RK stage, k_stage: each element forms its own traces and its
+// One kernel per RK stage, k_stage: each element forms the slots of its
+// physical-boundary sides with their weight checks (src/rhs.jl:65-128,
+// 702-776) from its stage state, its own traces and its
 // neighbours' facing ones from the state (reference _traces!/_trace4!,
 // src/rhs.jl:384-410, 573-580), its face slots (interior fluxes
 // src/rhs.jl:620-700), the volume terms (src/rhs.jl:317-333) and source
 // (src/rhs.jl:340-350), the lifts (src/rhs.jl:421-473), the u-equation
 // (src/rhs.jl:361-382), the Chebyshev mass solve (src/rhs.jl:889-939) and the
 // RK4 stage update (src/solver.jl:300-321), and then, from the state it has
-// just written, what the next stage needs before it can start: the a-priori
-// Chebyshev count of every element reduced per wave (the global policy takes
-// the maximum, src/rhs.jl:889-918) and the slots of the physical-boundary sides
-// with their weight checks (src/rhs.jl:65-128, 702-776). k_prep does the same
-// for the first stage of a call.
+// just written, what the next stage needs of all elements before it can start:
+// the a-priori Chebyshev count of every element reduced per wave (the global
+// policy takes the maximum, src/rhs.jl:889-918). k_prep does the same for the
+// first stage of a call.
 // Every face slot is computed by the element that owns it, so no slot has two
 // writers and no floating-point atomics exist; results do not depend on the
 // launch shape.
@@ -486,18 +487,15 @@ __device__ __forceinline__ unsigned long long fail_key(int serial, int kind, int
     return ((unsigned long long)(unsigned)serial << 33) | ((unsigned long long)kind << 32) | (unsigned)key;
 }
 
-// What the stage of serial `serial` needs of its input state (u at Up in
-// memory, v in V) before it runs: its a-priori Chebyshev count (with the global
-// policy; src/rhs.jl:889-918) and, on the physical boundary, the slots A, D, F
-// of the element's boundary sides at the stage's time factors (ct, st)
-// (src/rhs.jl:738-765) into the boundary slot buffer. Failures are recorded
-// with the stage's serial: the weight checks of the boundary (kind 0, the
-// _boundary_key, src/rhs.jl:732) before the mass matrix (kind 1, the element).
-// Returns the element's count (1 when not counted or failing).
+// What every element of the stage of serial `serial` must know of the others
+// before the stage starts: its a-priori Chebyshev count from its input v (V)
+// (with the global policy the stage takes the maximum over all elements;
+// src/rhs.jl:889-918). A non-positive mass matrix is recorded with the stage's
+// serial (kind 1, the element). Returns the element's count (1 when not
+// counted or failing).
 template <class D, bool UNI>
-__device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D::NV * D::NV], int e, int E, int serial,
-                                          double ct, double st, bool count) {
-    constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV, NH = D::NH;
+__device__ __forceinline__ int stage_prep(const double (&V)[D::NV * D::NV], int e, int serial, bool count) {
+    constexpr int NH = D::NH;
     int n_e = 1;
     if (count) {
         CKArgs& A = *kargs<D::BT>();
@@ -533,12 +531,21 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
             n_e = 1;
         }
     }
+    return n_e;
+}
+
+// The slots A, D, F of the element's physical-boundary sides (src/rhs.jl:738-765)
+// at the stage's time factors (ct, st), from its stage state (U, V), into the
+// boundary slot buffer, with the weight checks of the boundary (failures kind 0,
+// the _boundary_key of src/rhs.jl:732, with the stage's serial; they come
+// before the stage's mass-matrix failure, kind 1).
+template <class D, bool UNI>
+__device__ __forceinline__ void stage_boundary(const double (&U)[D::NU * D::NU], const double (&V)[D::NV * D::NV],
+                                               int e, int serial, double ct, double st) {
+    constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV;
     CKArgs& B = *kargs<D::BT>();
     const int N = B.N, j = e % N, k = e / N;
     if (j == 0 || j == N - 1 || k == 0 || k == N - 1) {
-        double U[NU * NU];
-#pragma unroll
-        for (int m = 0; m < NU * NU; ++m) U[m] = Up[(long)m * E + e];
         int bkey = 0;
         auto side = [&](auto sc) {
             constexpr int s = decltype(sc)::value;
@@ -599,8 +606,8 @@ __device__ __forceinline__ int stage_prep(const double* Up, const double (&V)[D:
         side(std::integral_constant<int, 3>());
         if (bkey) atomicMin(&B.err->fkey, fail_key(serial, 0, bkey));
     }
-    return n_e;
 }
+
 
 // Writes the wave's largest count into the per-wave slot of the stage parity.
 template <bool BT>
@@ -612,7 +619,7 @@ __device__ __forceinline__ void wave_count(int n_e, int slot, int E) {
     }
 }
 
-// The first stage of a call: the count and boundary slots of its input state.
+// The first stage of a call: the Chebyshev counts of its input state.
 template <int NU, int NV, int MODE, bool BT = false>
 __global__ __launch_bounds__(kWave) void k_prep(const std::conditional_t<BT, BatchArgs, KArgs> args) {
     (void)args;
@@ -627,7 +634,7 @@ __global__ __launch_bounds__(kWave) void k_prep(const std::conditional_t<BT, Bat
         double V[NV * NV];
 #pragma unroll
         for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
-        n_e = stage_prep<D, UNI>(kargs<D::BT>()->yu, V, e, E, kargs<D::BT>()->serial, kargs<D::BT>()->ct, kargs<D::BT>()->st, global);
+        n_e = stage_prep<D, UNI>(V, e, kargs<D::BT>()->serial, global);
     }
     if (global) wave_count<D::BT>(n_e, kargs<D::BT>()->slot, E);
 }
@@ -736,7 +743,7 @@ __device__ __forceinline__ void vol_pass(const double* tab, const double (&X)[n 
 // Face slots (A, D, F) of side s of element e = j + N k (B, T, L, R) from its
 // own traces (from U, V) and its neighbour's facing ones (from the neighbour's
 // state, read here), the interior fluxes of src/rhs.jl:620-700; on a physical
-// boundary the slots stage_prep formed. Reduced to the side's lift vectors
+// boundary the slots stage_boundary formed. Reduced to the side's lift vectors
 // Y_A, Y_D, Y_F (src/rhs.jl:434-441), kept in the lane's LDS slots.
 template <class D, bool UNI, int s>
 __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, const double (&U)[D::NU * D::NU],
@@ -751,7 +758,7 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, con
     else if (s == 1) { fidx = j + 1 + (N + 1) * k; nb = e + 1; bnd = j == N - 1; m = k; }
     else if (s == 2) { fidx = j + N * k;           nb = e - N; bnd = k == 0;     m = j; }
     else             { fidx = j + N * (k + 1);     nb = e + N; bnd = k == N - 1; m = j; }
-    if (bnd) nb = e;    // a boundary lane reads its own state here; its slots come from stage_prep
+    if (bnd) nb = e;    // a boundary lane reads its own state here; its slots come from stage_boundary
     double odu[NQ], ov[NQ], odv[NQ], ndu[NQ], nvv[NQ], ndv[NQ];
     {
         cdbl* T = ctab(A.tab);
@@ -964,7 +971,7 @@ __device__ __forceinline__ bool rk_store(int stage, double c, const double (&kv)
 
 // One RK stage of one element per lane (one launch per stage): face slots and
 // lifts, volume terms, source, u-equation, mass solve, the RK4 stage update,
-// and what the next stage needs of the new state (stage_prep). One wave per
+// and the next stage's Chebyshev count of the new state (stage_prep). One wave per
 // block: the lane's LDS slots hold first the lift vectors of its four sides,
 // then the Chebyshev weights G = 2 beta v_q W of the first mirrored row pairs
 // ([entry][lane], conflict-free); the rest of G stays in registers.
@@ -988,7 +995,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
     // waves would load (and store) at the same moments. The waves start in four
     // groups, each one s_sleep 127 (64 x 127 clocks) later than the one before,
     // which spreads the memory phases of the whole stage over time; waves with
-    // physical-boundary elements (whose epilogue forms the boundary slots) take
+    // physical-boundary elements (which form their boundary slots first) take
     // the two early groups. Measured on the 256 x 256, q = 5 instance: 5% less
     // time per stage.
     if (nw >= kStaggerWaves) {
@@ -1034,6 +1041,11 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
     int ncheb = 0, n_next = 1;
     if (e < E) {
         const int j = e % N, k = e / N;
+        // the slots of the element's physical-boundary sides, from the state just loaded
+        if (j == 0 || j == N - 1 || k == 0 || k == N - 1) {
+            CKArgs& A = *kargs<D::BT>();
+            stage_boundary<D, UNI>(U, V, e, A.serial, A.ct, A.st);
+        }
         const long cc = UNI ? 0 : (long)kargs<D::BT>()->cls_ca[e];
         const double* Wc = kargs<D::BT>()->Cw + cc * NQ * NQ;
         const double* Wa = kargs<D::BT>()->Aw + cc * NQ * NQ;
@@ -1343,7 +1355,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
 #pragma unroll
                 for (int m = 0; m < nV; ++m) Vn[m] = stage < 4 ? yv0[m] + c * X[m] : yv0[m] + c * (acv[m] + X[m]);
                 CKArgs& B = *kargs<D::BT>();
-                n_next = stage_prep<D, UNI>(B.outu, Vn, e, E, B.serial + 1, B.nct, B.nst, global);
+                n_next = stage_prep<D, UNI>(Vn, e, B.serial + 1, global);
             }
         }
     }
