@@ -48,6 +48,19 @@ class WedgPre(C.Structure):
                 + [(n, _dp) for n in _PTRS])
 
 
+class WedgAdjDesc(C.Structure):
+    _fields_ = ([("N", C.c_int), ("qu1", C.c_int), ("qv1", C.c_int),
+                 ("theta", C.c_double), ("tau", C.c_double), ("lambda_u", C.c_double), ("lambda_v", C.c_double),
+                 ("bc", WedgSide * 4), ("mass_tol", C.c_double), ("src_omega", C.c_double), ("src_Ts", C.c_double)]
+                + [(n, _dp) for n in ("w", "P_u", "S_u", "Ebl_u", "Ebr_u", "E1bl_u", "E1br_u", "P_h_v", "W2d_h",
+                                      "src_projE", "alpha")])
+
+
+class WedgAdjStats(C.Structure):
+    _fields_ = [("forward_steps", C.c_long), ("taped_steps", C.c_long), ("adjoint_steps", C.c_long),
+                ("snapshots_taken", C.c_long), ("snapshots_used", C.c_int), ("pad", C.c_int)]
+
+
 class WedgErrorInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("side", C.c_int), ("key", C.c_long), ("element", C.c_long),
                 ("step", C.c_long), ("cheb_steps", C.c_int), ("cg_unconverged", C.c_int)]
@@ -94,6 +107,15 @@ def _load():
         "wedg_set_receivers": [vp, C.c_int, _ip, _ip, _dp],
         "wedg_batch_advance_receivers": [vp, _dp, _dp, C.POINTER(C.c_long), C.c_int, _dp, _ip, C.POINTER(_dp)],
         "wedg_batch_config": [vp, C.POINTER(C.c_int), C.POINTER(C.c_long), C.POINTER(C.c_int)],
+        "wedg_adj_create": [C.POINTER(WedgAdjDesc), C.c_int, C.POINTER(vp)],
+        "wedg_adj_destroy": [vp],
+        "wedg_adj_set_params": [vp, _dp, _dp],
+        "wedg_adj_set_target": [vp, C.c_long, _dp],
+        "wedg_adj_forward": [vp, C.c_double, C.c_long, _dp, _dp],
+        "wedg_adj_gradient": [vp, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp],
+        "wedg_adj_last_stats": [vp, C.POINTER(WedgAdjStats)],
+        "wedg_adj_rhs_vjp": [vp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
+        "wedg_adj_rhs_tl": [vp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,194 @@
+"""Gradients of a source-mode boundary misfit with respect to per-element β and
+c², by the discrete adjoint of the device RK4 run (``wedg_adj_*`` of libwedg.so).
+
+The forward map is the reference's ``solve(...; mode = :source)``
+(src/solver.jl:167-228, 892-958) with coefficients constant on every element:
+zero initial state, ``rk4_step!`` (src/solver.jl:289-321) of ``edg_rhs!``
+(src/rhs.jl:490-550) with the Gaussian source (src/solver.jl:80-86, 647-662),
+t += dt after every step. The objective, which the reference does not have, is
+the L² misfit of u_h on the bottom side y = 0 after every step,
+
+    J = ½ Σ_n Σ_k Σ_i (h/2) w_i (u_h(x_ki, 0, t_n) − g_n[k, i])²,
+
+with g the target traces ([n, k, i]); ``SourceAdjoint.traces`` gives the traces
+of a run, so a target is the traces of a run with other coefficients.
+
+``SourceAdjoint.gradient`` returns J, dJ/dc² and dJ/dβ ((N, N) arrays, row j in
+y, column k in x) by the adjoint of every stage and of RK4, with binomial
+checkpointing of the RK4 states; ``misfit_rrule`` packages it as a ChainRules
+``rrule`` does: the value and a pullback. ``rhs_vjp`` / ``rhs_tl`` are the
+vector-Jacobian and Jacobian-vector products of one right-hand side.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+from ._lib import WedgError, lib
+from .coefficients import constant_coefficients
+from .precompute import GaussianSource, precompute
+
+__all__ = ["SourceAdjoint", "misfit_rrule"]
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _field(a, N, name):
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 0:
+        return np.full((N, N), float(a), order="F")
+    if a.shape != (N, N):
+        raise ValueError(f"{name} must be a number or an (N, N) array (got shape {a.shape})")
+    return np.asfortranarray(a)
+
+
+class SourceAdjoint:
+    """A source-mode problem with per-element coefficients on the device, its
+    forward runs, misfit and gradient.
+
+    Arguments as ``solve`` (src/solver.jl:167-188) where they apply; ``alpha``
+    is a number or an (N, N) array and stays fixed, c² and β are set with
+    ``set_params``. ``cheb`` is the Chebyshev step-count policy of the forward
+    steps ("global": the reference's; "element")."""
+
+    def __init__(self, q: int, N: int, *, mixed: bool = False, flux: str = "upwind", flux_params=None,
+                 sigma: float = 0.0, boundary="absorbing_2beta", sigma_dirichlet=None, kappa: float = 0.0,
+                 alpha=1e-5, source: GaussianSource = None, cheb: str = "global"):
+        source = source or GaussianSource()
+        if sigma_dirichlet is None:
+            sigma_dirichlet = sigma
+        pre, _ = precompute(q, N, mixed=mixed, flux=flux, flux_params=flux_params, sigma=sigma,
+                            boundary=boundary, sigma_dirichlet=sigma_dirichlet, kappa=kappa, mode="source",
+                            source=source, coefficients=constant_coefficients(c=1.0, alpha=1e-5, beta=0.0))
+        self.q, self.N, self.h = q, N, 1.0 / N
+        self.qu1, self.qv1 = pre.qu1, pre.qv1
+        self.pre = pre
+        self.alpha = _field(alpha, N, "alpha")
+        d = _lib.WedgAdjDesc()
+        d.N, d.qu1, d.qv1 = N, pre.qu1, pre.qv1
+        d.theta, d.tau, d.lambda_u, d.lambda_v = pre.theta, pre.tau, pre.lambda_u, pre.lambda_v
+        for i, side in enumerate(_lib.SIDES):
+            b = pre.bc[side]
+            d.bc[i].kind = _lib.KINDS[b["kind"]]
+            d.bc[i].gamma_abc, d.bc[i].eta_abc, d.bc[i].sigma = b["gamma_abc"], b["eta_abc"], b["sigma"]
+            d.bc[i].kb, d.bc[i].kappa = b["kb"], b["kappa"]
+        d.mass_tol = pre.mass_tol
+        d.src_omega, d.src_Ts = source.omega, source.Ts
+        keep = {}
+        for n in ("w", "P_u", "S_u", "Ebl_u", "Ebr_u", "E1bl_u", "E1br_u", "P_h_v", "W2d_h", "src_projE"):
+            keep[n] = np.asfortranarray(getattr(pre, n), dtype=np.float64)
+            setattr(d, n, _ptr(keep[n]))
+        d.alpha = _ptr(self.alpha)
+        h = C.c_void_p()
+        pol = {"global": _lib.CHEB_GLOBAL, "element": _lib.CHEB_ELEMENT}[cheb]
+        st = lib.wedg_adj_create(C.byref(d), pol, C.byref(h))
+        if st != _lib.OK:
+            raise WedgError(f"wedg_adj_create failed with status {st}")
+        self.h_ = h
+        self.c2 = np.ones((N, N), order="F")
+        self.beta = np.zeros((N, N), order="F")
+        self.target_steps = 0
+
+    def __del__(self):
+        h = getattr(self, "h_", None)
+        if h is not None and h.value:
+            lib.wedg_adj_destroy(h)
+            self.h_ = None
+
+    def _check(self, st, what):
+        if st == _lib.OK:
+            return
+        msg = {_lib.ERR_MASS: "mass matrix not positive definite: 1 - 2βv_h ≤ 0",
+               _lib.ERR_NONFINITE: "non-finite state (blow-up)",
+               _lib.ERR_BOUNDARY_TRACE: "boundary weight 1 - kβv_h ≤ 0 at a face point",
+               _lib.ERR_ARGUMENT: "invalid argument", _lib.ERR_HIP: "HIP runtime error"}.get(st, f"status {st}")
+        raise WedgError(f"{what}: {msg}")
+
+    def set_params(self, c2, beta):
+        """Per-element c² (> 0) and β, numbers or (N, N) arrays."""
+        self.c2 = _field(c2, self.N, "c2")
+        self.beta = _field(beta, self.N, "beta")
+        self._check(lib.wedg_adj_set_params(self.h_, _ptr(self.c2), _ptr(self.beta)), "set_params")
+
+    def set_target(self, target):
+        """The target traces g, (nsteps, N, q_u+1): [n, k, i] after step n + 1."""
+        g = np.ascontiguousarray(target, dtype=np.float64)
+        if g.ndim != 3 or g.shape[1:] != (self.N, self.qu1):
+            raise ValueError(f"target must be (nsteps, N, q_u+1) = (*, {self.N}, {self.qu1})")
+        self._check(lib.wedg_adj_set_target(self.h_, g.shape[0], _ptr(g)), "set_target")
+        self.target_steps = g.shape[0]
+
+    def traces(self, dt: float, nsteps: int):
+        """u_h at the bottom face points after every step, (nsteps, N, q_u+1)."""
+        tr = np.zeros((nsteps, self.N, self.qu1))
+        self._check(lib.wedg_adj_forward(self.h_, float(dt), int(nsteps), None, _ptr(tr)), "forward")
+        return tr
+
+    def misfit(self, dt: float, nsteps: int) -> float:
+        J = C.c_double(0.0)
+        self._check(lib.wedg_adj_forward(self.h_, float(dt), int(nsteps), C.byref(J), None), "forward")
+        return J.value
+
+    def gradient(self, dt: float, nsteps: int, snapshots: int = 32):
+        """(J, dJ/dc², dJ/dβ) with binomial checkpointing in `snapshots` state slots."""
+        J = C.c_double(0.0)
+        gc2 = np.zeros((self.N, self.N), order="F")
+        gb = np.zeros((self.N, self.N), order="F")
+        st = lib.wedg_adj_gradient(self.h_, float(dt), int(nsteps), int(snapshots), C.byref(J), _ptr(gc2), _ptr(gb))
+        self._check(st, "gradient")
+        return J.value, gc2, gb
+
+    def stats(self):
+        s = _lib.WedgAdjStats()
+        lib.wedg_adj_last_stats(self.h_, C.byref(s))
+        return SimpleNamespace(forward_steps=s.forward_steps, taped_steps=s.taped_steps,
+                               adjoint_steps=s.adjoint_steps, snapshots_taken=s.snapshots_taken,
+                               snapshots_used=s.snapshots_used)
+
+    def _state(self, u, v):
+        u = np.asfortranarray(u, dtype=np.float64)
+        v = np.asfortranarray(v, dtype=np.float64)
+        if u.shape != (self.N, self.N, self.qu1, self.qu1) or v.shape != (self.N, self.N, self.qv1, self.qv1):
+            raise ValueError("state shapes do not match the problem")
+        return u, v
+
+    def rhs_vjp(self, u, v, t, ub, vb):
+        """Cotangents of (u, v, c², β) from those of (u_t, v_t) of one right-hand side at time t."""
+        u, v = self._state(u, v)
+        ub, vb = self._state(ub, vb)
+        Ub = np.zeros_like(u, order="F"); Vb = np.zeros_like(v, order="F")
+        c2b = np.zeros((self.N, self.N), order="F"); bb = np.zeros((self.N, self.N), order="F")
+        st = lib.wedg_adj_rhs_vjp(self.h_, _ptr(u), _ptr(v), float(t), _ptr(ub), _ptr(vb), _ptr(Ub), _ptr(Vb),
+                                  _ptr(c2b), _ptr(bb))
+        self._check(st, "rhs_vjp")
+        return Ub, Vb, c2b, bb
+
+    def rhs_tl(self, u, v, t, du, dv, dc2, dbeta):
+        """(du_t, dv_t) of one right-hand side at time t along (du, dv, dc², dβ)."""
+        u, v = self._state(u, v)
+        du, dv = self._state(du, dv)
+        dc2 = _field(dc2, self.N, "dc2"); dbeta = _field(dbeta, self.N, "dbeta")
+        ou = np.zeros_like(u, order="F"); ov = np.zeros_like(v, order="F")
+        st = lib.wedg_adj_rhs_tl(self.h_, _ptr(u), _ptr(v), float(t), _ptr(du), _ptr(dv), _ptr(dc2), _ptr(dbeta),
+                                 _ptr(ou), _ptr(ov))
+        self._check(st, "rhs_tl")
+        return ou, ov
+
+
+def misfit_rrule(adj: SourceAdjoint, c2, beta, dt: float, nsteps: int, snapshots: int = 32):
+    """The value J(c², β) and its pullback J̄ -> (J̄ dJ/dc², J̄ dJ/dβ), as a
+    ChainRules ``rrule`` returns them (the gradient is computed once, here)."""
+    adj.set_params(c2, beta)
+    J, gc2, gb = adj.gradient(dt, nsteps, snapshots)
+
+    def pullback(Jbar=1.0):
+        return Jbar * gc2, Jbar * gb
+
+    return J, pullback

@@ -1,0 +1,13 @@
+// Adjoint, tangent-linear and misfit kernel instantiations for q_u = 2 (n_u = 3),
+// both pairings q_v = q_u - 1 and q_v = q_u.
+#include "wedg_adjoint.h"
+
+namespace wedg {
+
+hipError_t launch_adj_q2(const AdjArgs& a, int nv, int which, hipStream_t s) {
+    if (nv == 3) return launch_adj_nv<3, 3>(a, which, s);
+    if (nv == 2) return launch_adj_nv<3, 2>(a, which, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace wedg

@@ -1,0 +1,13 @@
+// Adjoint, tangent-linear and misfit kernel instantiations for q_u = 3 (n_u = 4),
+// both pairings q_v = q_u - 1 and q_v = q_u.
+#include "wedg_adjoint.h"
+
+namespace wedg {
+
+hipError_t launch_adj_q3(const AdjArgs& a, int nv, int which, hipStream_t s) {
+    if (nv == 4) return launch_adj_nv<4, 4>(a, which, s);
+    if (nv == 3) return launch_adj_nv<4, 3>(a, which, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace wedg

@@ -1,0 +1,13 @@
+// Adjoint, tangent-linear and misfit kernel instantiations for q_u = 4 (n_u = 5),
+// both pairings q_v = q_u - 1 and q_v = q_u.
+#include "wedg_adjoint.h"
+
+namespace wedg {
+
+hipError_t launch_adj_q4(const AdjArgs& a, int nv, int which, hipStream_t s) {
+    if (nv == 5) return launch_adj_nv<5, 5>(a, which, s);
+    if (nv == 4) return launch_adj_nv<5, 4>(a, which, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace wedg

@@ -1,0 +1,748 @@
+// Host side of the discrete adjoint (wedg_adj_* in include/wedg.h): source-mode
+// runs with per-element coefficients c², α, β, the boundary misfit of u_h, its
+// gradient with respect to c² and β by the adjoint of RK4 with binomial
+// checkpointing, and single tangent-linear / vector-Jacobian products of the
+// right-hand side.
+//
+// The forward steps run the forward stage kernels (k_prep, k_stage) on a
+// workspace of a problem built here from the per-element arrays: the same
+// tables precompute makes for coefficients that are constant on every element
+// (src/solver.jl:509-552, 716-770), with one-sided face values where
+// neighbours differ.
+//
+// The gradient runs two streams: the forward stream does the checkpointing
+// schedule (advances, snapshots, restores) and tapes the step to be reversed
+// (its stage states and stage derivatives) into one of two tape sets; the
+// adjoint stream seeds the misfit of the step's end state and runs the eight
+// adjoint kernels of the step. Events order a tape between its writer and its
+// reader, so the forward stream runs ahead of the adjoint stream by up to two
+// tapes and the two chains overlap on the GPU.
+#include "wedg_internal.h"
+#include "wedg_adjoint.h"
+
+#include <functional>
+#include <math.h>
+
+namespace {
+
+hipError_t launch_adj(const wedg::AdjArgs& a, int qu, int nv, int which, hipStream_t s) {
+    switch (qu) {
+        case 1: return wedg::launch_adj_q1(a, nv, which, s);
+        case 2: return wedg::launch_adj_q2(a, nv, which, s);
+        case 3: return wedg::launch_adj_q3(a, nv, which, s);
+        case 4: return wedg::launch_adj_q4(a, nv, which, s);
+        case 5: return wedg::launch_adj_q5(a, nv, which, s);
+        case 6: return wedg::launch_adj_q6(a, nv, which, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// binomial coefficient as a double (large arguments saturate harmlessly)
+double binom(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Optimal first split of n >= 2 steps reversed with s >= 1 free snapshot slots
+// besides the one holding the segment's first state (binomial checkpointing,
+// Griewank and Walther, ACM TOMS 26(1), 2000): with c = s + 1 snapshots and r
+// the least repetition number with C(c + r, r) >= n, the left part of m steps
+// is reversed with c snapshots and r - 1 repetitions, the right part with
+// c - 1 and r, and the total number of forward steps, r n - C(c + r, r - 1), is
+// reached for every m with C(c+r-2, r-2) <= m <= C(c+r-1, r-1) and
+// C(c+r-2, r-1) <= n - m <= C(c+r-1, r).
+long revolve_split(long n, int s) {
+    const int c = s + 1;
+    int r = 0;
+    while (binom(c + r, r) < (double)n) ++r;
+    double lo = std::max(1.0, std::max(binom(c + r - 2, r - 2), (double)n - binom(c + r - 1, r)));
+    double hi = std::min((double)(n - 1), std::min(binom(c + r - 1, r - 1), (double)n - binom(c + r - 2, r - 1)));
+    long m = (long)lo;
+    if (!(lo <= hi)) m = std::max(1L, std::min(n - 1, (long)hi));
+    return m;
+}
+
+}  // namespace
+
+struct AdjTape {
+    double *Yu[4] = {}, *Yv[4] = {}, *Ku[4] = {}, *Kv[4] = {};   // stage states (Y1 a copy of y_n) and F(Y_s)
+    double *ynu = nullptr, *ynv = nullptr;                        // the step's end state
+    hipEvent_t taped = nullptr, used = nullptr;                   // written (forward) / read (adjoint)
+    bool in_use = false;
+};
+
+struct wedg_adj {
+    wedg_problem* p = nullptr;
+    wedg_workspace* ws = nullptr;
+    int N = 0, E = 0, nu = 0, nv = 0, nh = 0, qu = 0, cheb_policy = 0;
+    double theta = 0, tau = 0, lu = 0, lv = 0, mass_tol = 1e-14, src_omega = 0, src_Ts = 0;
+    wedg_side bc[4];
+    std::vector<double> w, Pu, Su, Ebl, Ebr, E1bl, E1br, Ph, W2dh, src;   // host copies of the tables
+    std::vector<double> alpha, c2, beta;                                     // [E], e = j + N k
+    std::vector<double> lam, Q, K1;                                          // 1D stiffness
+    std::vector<void*> allocs;
+    double *dc2 = nullptr, *dal = nullptr, *dbe = nullptr, *atab = nullptr;
+    AdjTape tape[2];
+    double *ybu = nullptr, *ybv = nullptr, *accu = nullptr, *accv = nullptr, *kbu = nullptr, *kbv = nullptr;
+    double *pu = nullptr, *pv = nullptr, *slot = nullptr, *gc2 = nullptr, *gbe = nullptr;
+    double *tdc2 = nullptr, *tdbe = nullptr;   // tangent-linear directions of the coefficients
+    int* fail = nullptr;
+    double* snaps = nullptr; int nsnap_cap = 0;
+    double* dtgt = nullptr; long tgt_steps = 0;
+    double* djp = nullptr; double* dtr = nullptr; long jp_cap = 0;
+    double* pin = nullptr; size_t pin_n = 0;   // pinned staging of host data
+    hipStream_t sa = nullptr;                  // the adjoint stream
+    wedg_adj_stats stats{};
+    size_t nU() const { return (size_t)E * nu * nu; }
+    size_t nV() const { return (size_t)E * nv * nv; }
+};
+
+namespace {
+
+// host <-> device through the pinned staging buffer, on the workspace stream
+wedg_status h2d(wedg_adj* A, double* d, const double* h, size_t n) {
+    hipStream_t s = A->ws->stream;
+    for (size_t off = 0; off < n; off += A->pin_n) {
+        const size_t k = std::min(A->pin_n, n - off);
+        HIPOK(hipStreamSynchronize(s));
+        std::memcpy(A->pin, h + off, k * sizeof(double));
+        HIPOK(hipMemcpyAsync(d + off, A->pin, k * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    HIPOK(hipStreamSynchronize(s));
+    return WEDG_OK;
+}
+
+wedg_status d2h(wedg_adj* A, double* h, const double* d, size_t n) {
+    hipStream_t s = A->ws->stream;
+    for (size_t off = 0; off < n; off += A->pin_n) {
+        const size_t k = std::min(A->pin_n, n - off);
+        HIPOK(hipMemcpyAsync(A->pin, d + off, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPOK(hipStreamSynchronize(s));
+        std::memcpy(h + off, A->pin, k * sizeof(double));
+    }
+    return WEDG_OK;
+}
+
+double* adj_alloc(wedg_adj* A, size_t n, hipError_t* he) {
+    if (*he != hipSuccess) return nullptr;
+    double* d = dalloc<double>(n, he);
+    if (*he == hipSuccess) A->allocs.push_back(d);
+    return d;
+}
+
+// The forward problem of the current coefficients: the tables of precompute for
+// coefficients constant on every element (src/solver.jl:509-552, 647-662, 716-770),
+// built from the per-element arrays directly.
+wedg_status build_problem(wedg_adj* A, wedg_problem** out) {
+    *out = nullptr;
+    const int N = A->N, nu = A->nu, nv = A->nv, nh = A->nh, nq = nu;
+    const long E = A->E;
+    auto* p = new wedg_problem();
+    p->stage_bytes = (size_t)4 << 20;
+    if (hipHostMalloc(&p->stage, p->stage_bytes, hipHostMallocDefault) != hipSuccess) {
+        delete p;
+        return WEDG_ERR_HIP;
+    }
+    p->N = N; p->E = (int)E; p->nu = nu; p->nv = nv; p->nh = nh; p->qu = nu - 1;
+    p->omega = 0.0; p->theta = A->theta; p->tau = A->tau; p->lu = A->lu; p->lv = A->lv;
+    p->mass_tol = A->mass_tol; p->mass_cg = 0; p->mms_on = 0; p->src_on = 1;
+    p->src_omega = A->src_omega; p->src_Ts = A->src_Ts;
+    for (int s = 0; s < 4; ++s) p->bc[s] = A->bc[s];
+    p->lam = A->lam;
+    p->Q = A->Q;
+    hipError_t he = hipSuccess;
+#define UP(dst, src, n)                                                     \
+    do {                                                                    \
+        dst = p->upload(src, n, &he);                                       \
+        if (he != hipSuccess) { wedg_problem_destroy(p); return WEDG_ERR_HIP; } \
+    } while (0)
+    {
+        std::vector<double> t;
+        t.insert(t.end(), A->Pu.begin(), A->Pu.end());
+        t.insert(t.end(), A->Su.begin(), A->Su.end());
+        t.insert(t.end(), A->w.begin(), A->w.end());
+        t.insert(t.end(), A->Ebl.begin(), A->Ebl.end());
+        t.insert(t.end(), A->Ebr.begin(), A->Ebr.end());
+        t.insert(t.end(), A->E1bl.begin(), A->E1bl.end());
+        t.insert(t.end(), A->E1br.begin(), A->E1br.end());
+        t.insert(t.end(), A->Ph.begin(), A->Ph.end());
+        t.insert(t.end(), A->W2dh.begin(), A->W2dh.end());
+        t.insert(t.end(), A->Q.begin(), A->Q.end());
+        const double h = 1.0 / N, two_h = 2.0 / h, four_hh = 4.0 / (h * h);
+        for (int a = 0; a < nu; ++a) t.push_back(two_h * A->Ebl[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(two_h * A->Ebr[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(four_hh * A->E1bl[a]);
+        for (int a = 0; a < nu; ++a) t.push_back(four_hh * A->E1br[a]);
+        UP(p->tab, t.data(), t.size());
+    }
+    // classes of (c^2, alpha): Cw = c^2 w w^T, Aw = alpha w w^T and the spectral
+    // factors 1 / (c^2 (lam_i + lam_j) + [i = j = 0]) of the pinned stiffness
+    {
+        std::vector<int> cls;
+        std::vector<double> uniq;
+        dedup(E, 2, [&](long e, double* b) { b[0] = A->c2[e]; b[1] = A->alpha[e]; }, cls, uniq);
+        const int nc = (int)(uniq.size() / 2);
+        std::vector<double> cw((size_t)nc * nq * nq), aw((size_t)nc * nq * nq), dfac((size_t)nc * nu * nu);
+        for (int c = 0; c < nc; ++c) {
+            const double c2 = uniq[2 * c], al = uniq[2 * c + 1];
+            for (int ix = 0; ix < nq; ++ix)
+                for (int iy = 0; iy < nq; ++iy) {
+                    const double ww = A->w[iy] * A->w[ix];
+                    cw[(size_t)c * nq * nq + iy + nq * ix] = c2 * ww;
+                    aw[(size_t)c * nq * nq + iy + nq * ix] = al * ww;
+                }
+            for (int jj = 0; jj < nu; ++jj)
+                for (int i = 0; i < nu; ++i)
+                    dfac[(size_t)c * nu * nu + i + nu * jj] =
+                        1.0 / (c2 * (A->lam[i] + A->lam[jj]) + (i == 0 && jj == 0 ? 1.0 : 0.0));
+        }
+        p->ncls_ca = nc;
+        UP(p->Cw, cw.data(), cw.size());
+        UP(p->Aw, aw.data(), aw.size());
+        UP(p->Ainv, dfac.data(), dfac.size());
+        UP(p->cls_ca, cls.data(), cls.size());
+    }
+    {
+        std::vector<int> cls;
+        std::vector<double> uniq;
+        dedup(E, 1, [&](long e, double* b) { b[0] = A->beta[e]; }, cls, uniq);
+        const int nc = (int)uniq.size();
+        std::vector<double> bt((size_t)nc * nh * nh);
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < nh * nh; ++i) bt[(size_t)c * nh * nh + i] = uniq[c];
+        p->ncls_b = nc;
+        UP(p->beta, bt.data(), bt.size());
+        UP(p->cls_b, cls.data(), cls.size());
+    }
+    // face arrays [i][face]: horizontal faces jf + (N + 1) k, vertical j + N kf; the
+    // "+" element's values (the inside element's on the physical boundary) and the
+    // "-" element's; c on the boundary sqrt(c^2) (src/solver.jl:546-552, 716-751)
+    bool one_sided = false;
+    {
+        const size_t nf = (size_t)N * (N + 1);
+        auto el = [&](const std::vector<double>& f, long j, long k) { return f[j + (long)N * k]; };
+        std::vector<double> cfv(E);
+        for (long e = 0; e < E; ++e) cfv[e] = std::sqrt(A->c2[e]);
+        auto hplus = [&](const std::vector<double>& f, long jf, long k) { return el(f, jf < N ? jf : N - 1, k); };
+        auto hminus = [&](const std::vector<double>& f, long jf, long k) { return el(f, jf > 0 ? jf - 1 : 0, k); };
+        auto vplus = [&](const std::vector<double>& f, long j, long kf) { return el(f, j, kf < N ? kf : N - 1); };
+        auto vminus = [&](const std::vector<double>& f, long j, long kf) { return el(f, j, kf > 0 ? kf - 1 : 0); };
+        std::vector<double> arr[F_COUNT];
+        for (int a = 0; a < F_COUNT; ++a) arr[a].assign(nf, 0.0);
+        for (long k = 0; k < N; ++k)
+            for (long jf = 0; jf <= N; ++jf) {
+                const size_t f = jf + (size_t)(N + 1) * k;
+                arr[F_HC2][f] = hplus(A->c2, jf, k);
+                arr[F_HAF][f] = hplus(A->alpha, jf, k);
+                arr[F_HCF][f] = hplus(cfv, jf, k);
+                arr[F_HBF][f] = hplus(A->beta, jf, k);
+                arr[F_HC2M][f] = hminus(A->c2, jf, k);
+                arr[F_HAFM][f] = hminus(A->alpha, jf, k);
+                if (jf > 0 && jf < N &&
+                    (arr[F_HC2][f] != arr[F_HC2M][f] || arr[F_HAF][f] != arr[F_HAFM][f])) one_sided = true;
+            }
+        for (long kf = 0; kf <= N; ++kf)
+            for (long j = 0; j < N; ++j) {
+                const size_t f = j + (size_t)N * kf;
+                arr[F_VC2][f] = vplus(A->c2, j, kf);
+                arr[F_VAF][f] = vplus(A->alpha, j, kf);
+                arr[F_VCF][f] = vplus(cfv, j, kf);
+                arr[F_VBF][f] = vplus(A->beta, j, kf);
+                arr[F_VC2M][f] = vminus(A->c2, j, kf);
+                arr[F_VAFM][f] = vminus(A->alpha, j, kf);
+                if (kf > 0 && kf < N &&
+                    (arr[F_VC2][f] != arr[F_VC2M][f] || arr[F_VAF][f] != arr[F_VAFM][f])) one_sided = true;
+            }
+        bool faces_uni = !one_sided;
+        for (int a = 0; a < F_COUNT; ++a) {
+            if (all_equal(arr[a].data(), nf)) {
+                double v = arr[a][0];
+                UP(p->f[a].p, &v, 1);
+                p->f[a].uni = 1;
+                p->fu[a] = v;
+            } else {
+                std::vector<double> t(nf * nq);   // constant over the points of a face
+                for (int i = 0; i < nq; ++i) std::memcpy(&t[(size_t)i * nf], arr[a].data(), nf * sizeof(double));
+                UP(p->f[a].p, t.data(), t.size());
+                p->f[a].uni = 0;
+                faces_uni = false;
+            }
+        }
+        p->one_sided = one_sided ? 1 : 0;
+        p->mode = (p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) ? MODE_UNIFORM : MODE_SEPARABLE;
+    }
+    {
+        std::vector<double> z((size_t)nq * N, 0.0);   // zero boundary data (src/solver.jl:647-655)
+        for (int a = 0; a < 8; ++a) UP(p->bcd[a], z.data(), z.size());
+    }
+    {
+        const size_t n2 = (size_t)nv * nv;
+        std::vector<double> t(n2 * E);
+        bool zero = true;
+        for (long e = 0; e < E; ++e)
+            for (size_t m = 0; m < n2; ++m) {
+                const double v = A->src[m + n2 * e];
+                t[m * E + e] = v;
+                if (v != 0.0) zero = false;
+            }
+        if (!zero) {
+            UP(p->proj[4], t.data(), t.size());
+            p->has_proj[4] = 1;
+        }
+    }
+#undef UP
+    (void)hipHostFree(p->stage);
+    p->stage = nullptr;
+    *out = p;
+    return WEDG_OK;
+}
+
+AdjArgs adj_args(wedg_adj* A) {
+    AdjArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.N = A->N; a.E = A->E;
+    const double h = 1.0 / A->N;
+    a.two_h = 2.0 / h; a.four_hh = 4.0 / (h * h); a.hw = 0.5 * h;
+    a.theta = A->theta; a.tau = A->tau; a.lu = A->lu; a.lv = A->lv;
+    a.cheb_L = std::acosh(1.0 / A->mass_tol);
+    for (int s = 0; s < 4; ++s) {
+        a.kind[s] = A->bc[s].kind; a.gam[s] = A->bc[s].gamma_abc; a.eta[s] = A->bc[s].eta_abc;
+        a.sig[s] = A->bc[s].sigma; a.kb[s] = A->bc[s].kb; a.kap[s] = A->bc[s].kappa;
+    }
+    a.tab = A->p->tab; a.atab = A->atab;
+    a.c2 = A->dc2; a.al = A->dal; a.be = A->dbe;
+    a.slot = A->slot; a.pu = A->pu; a.pv = A->pv; a.gc2 = A->gc2; a.gbe = A->gbe;
+    a.ybu = A->ybu; a.ybv = A->ybv; a.accu = A->accu; a.accv = A->accv; a.nkbu = A->kbu; a.nkbv = A->kbv;
+    a.fail = A->fail;
+    a.kscale = 1.0;
+    return a;
+}
+
+// One RK4 step of the forward stream from the workspace state y_n (time t),
+// taped: the stage states Y1 (a copy of y_n) .. Y4, the stage derivatives
+// k1 .. k4 and the end state, as the forward steps compute them
+// (src/solver.jl:289-321); the workspace state is left unchanged.
+wedg_status tape_step(wedg_adj* A, AdjTape& tp, double t, double dt, long step) {
+    wedg_workspace* ws = A->ws;
+    HIPOK(hipMemcpyAsync(tp.Yu[0], ws->yu, A->nU() * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
+    HIPOK(hipMemcpyAsync(tp.Yv[0], ws->yv, A->nV() * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
+    const double hd = 0.5 * dt, th = t + 0.5 * dt, tf = t + dt;
+    const double ts[4] = {t, th, th, tf}, cs[4] = {hd, hd, dt, dt / 6.0}, tn[4] = {th, th, tf, tf};
+    wedg_status st;
+    for (int s = 0; s < 4; ++s) {
+        double* ou = s < 3 ? tp.Yu[s + 1] : tp.ynu;
+        double* ov = s < 3 ? tp.Yv[s + 1] : tp.ynv;
+        KArgs a = stage_args(ws, s + 1, tp.Yu[s], tp.Yv[s], tp.Yu[0], tp.Yv[0], ou, ov, ts[s], cs[s], step, s < 3,
+                             tn[s]);
+        a.recu = tp.Ku[s];
+        a.recv = tp.Kv[s];
+        if (s == 0 && (st = run_kernel(ws, a, 0)) != WEDG_OK) return st;
+        if ((st = run_kernel(ws, a, 1)) != WEDG_OK) return st;
+    }
+    return WEDG_OK;
+}
+
+// The adjoint of one taped step on the adjoint stream: the misfit of its end
+// state (target row n) into ybar, then stages 4..1 (k_adj_a, k_adj_b each).
+wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
+    const hipStream_t s = A->sa;
+    AdjArgs a = adj_args(A);
+    if (A->dtgt) {
+        a.yu = tp.ynu;
+        a.tgt = A->dtgt + (size_t)n * A->N * A->nu;
+        a.jp = A->djp + (size_t)n * A->N * A->nu;
+        a.ybu = A->ybu;
+        HIPOK(launch_adj(a, A->qu, A->nv, 3, s));
+        a.tgt = nullptr; a.jp = nullptr;
+    }
+    const double ca[5] = {0, 0, dt / 6.0, dt / 3.0, dt / 3.0};   // next cotangent: ca ybar + cb Ybar_s
+    const double cb[5] = {0, 0, 0.5 * dt, 0.5 * dt, dt};
+    for (int stg = 4; stg >= 1; --stg) {
+        a.stage = stg;
+        a.yu = tp.Yu[stg - 1]; a.yv = tp.Yv[stg - 1];
+        a.ku = tp.Ku[stg - 1]; a.kv = tp.Kv[stg - 1];
+        if (stg == 4) { a.kbu = A->ybu; a.kbv = A->ybv; a.kscale = dt / 6.0; }
+        else { a.kbu = A->kbu; a.kbv = A->kbv; a.kscale = 1.0; }
+        a.ca = ca[stg]; a.cb = cb[stg];
+        HIPOK(launch_adj(a, A->qu, A->nv, 0, s));
+        HIPOK(launch_adj(a, A->qu, A->nv, 1, s));
+    }
+    ++A->stats.adjoint_steps;
+    return WEDG_OK;
+}
+
+wedg_status zero_dev(hipStream_t s, double* d, size_t n) {
+    HIPOK(hipMemsetAsync(d, 0, n * sizeof(double), s));
+    return WEDG_OK;
+}
+
+wedg_status read_fail(wedg_adj* A) {
+    int f = 0;
+    HIPOK(hipStreamSynchronize(A->sa));
+    HIPOK(hipStreamSynchronize(A->ws->stream));
+    HIPOK(hipMemcpy(&f, A->fail, sizeof(int), hipMemcpyDeviceToHost));
+    return f ? WEDG_ERR_MASS : WEDG_OK;
+}
+
+wedg_status jp_capacity(wedg_adj* A, long nsteps) {
+    const long need = nsteps * A->N * A->nu;
+    if (need <= A->jp_cap) return WEDG_OK;
+    HIPOK(hipStreamSynchronize(A->ws->stream));
+    HIPOK(hipStreamSynchronize(A->sa));
+    if (A->djp) (void)hipFree(A->djp);
+    if (A->dtr) (void)hipFree(A->dtr);
+    A->djp = nullptr; A->dtr = nullptr; A->jp_cap = 0;
+    HIPOK(hipMalloc((void**)&A->djp, need * sizeof(double)));
+    HIPOK(hipMalloc((void**)&A->dtr, need * sizeof(double)));
+    A->jp_cap = need;
+    return WEDG_OK;
+}
+
+// J = sum over steps and points of the per-point terms, in step order
+wedg_status sum_misfit(wedg_adj* A, long nsteps, double* J) {
+    const size_t n = (size_t)nsteps * A->N * A->nu;
+    std::vector<double> h(n);
+    wedg_status s;
+    if ((s = d2h(A, h.data(), A->djp, n))) return s;
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) acc += h[i];
+    *J = acc;
+    return WEDG_OK;
+}
+
+}  // namespace
+
+extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, wedg_adj** out) {
+    if (!d || !out) return WEDG_ERR_ARGUMENT;
+    *out = nullptr;
+    const int N = d->N, nu = d->qu1, nv = d->qv1;
+    if (N < 2 || nu < 2 || nu > 7 || (nv != nu && nv != nu - 1)) return WEDG_ERR_ARGUMENT;
+    if (!d->w || !d->P_u || !d->S_u || !d->Ebl_u || !d->Ebr_u || !d->E1bl_u || !d->E1br_u || !d->P_h_v ||
+        !d->W2d_h || !d->src_projE || !d->alpha)
+        return WEDG_ERR_ARGUMENT;
+    if (cheb_policy != WEDG_CHEB_GLOBAL && cheb_policy != WEDG_CHEB_ELEMENT) return WEDG_ERR_ARGUMENT;
+    const int qv = nv - 1;
+    int nh = (3 * qv + 1) / 2 + 1;
+    if (nh < nu) nh = nu;
+    auto* A = new wedg_adj();
+    A->N = N; A->E = N * N; A->nu = nu; A->nv = nv; A->nh = nh; A->qu = nu - 1; A->cheb_policy = cheb_policy;
+    A->theta = d->theta; A->tau = d->tau; A->lu = d->lambda_u; A->lv = d->lambda_v;
+    A->mass_tol = d->mass_tol > 0 ? d->mass_tol : 1e-14;
+    A->src_omega = d->src_omega; A->src_Ts = d->src_Ts;
+    for (int s = 0; s < 4; ++s) A->bc[s] = d->bc[s];
+    const int nq = nu;
+    auto cp = [](const double* p, size_t n) { return std::vector<double>(p, p + n); };
+    A->w = cp(d->w, nq); A->Pu = cp(d->P_u, (size_t)nq * nu); A->Su = cp(d->S_u, (size_t)nq * nu);
+    A->Ebl = cp(d->Ebl_u, nu); A->Ebr = cp(d->Ebr_u, nu); A->E1bl = cp(d->E1bl_u, nu); A->E1br = cp(d->E1br_u, nu);
+    A->Ph = cp(d->P_h_v, (size_t)nh * nv); A->W2dh = cp(d->W2d_h, (size_t)nh * nh);
+    A->src = cp(d->src_projE, (size_t)nv * nv * A->E);
+    A->alpha = cp(d->alpha, A->E);
+    A->c2.assign(A->E, 1.0);
+    A->beta.assign(A->E, 0.0);
+    stiffness_spectrum(d->S_u, d->w, nu, A->Q, A->lam);
+    A->K1.assign((size_t)nu * nu, 0.0);
+    for (int a = 0; a < nu; ++a)
+        for (int b = 0; b < nu; ++b) {
+            double s = 0.0;
+            for (int p = 0; p < nq; ++p) s += d->w[p] * d->S_u[p + nq * a] * d->S_u[p + nq * b];
+            A->K1[a + (size_t)nu * b] = s;
+        }
+    auto fail = [&](wedg_status s) { wedg_adj_destroy(A); return s; };
+    wedg_status st;
+    if ((st = build_problem(A, &A->p))) return fail(st);
+    if ((st = wedg_workspace_create(A->p, cheb_policy, &A->ws))) return fail(st);
+    hipError_t he = hipStreamCreateWithFlags(&A->sa, hipStreamNonBlocking);
+    if (he != hipSuccess) return fail(WEDG_ERR_HIP);
+    A->pin_n = std::max<size_t>((size_t)1 << 18, (size_t)A->E);
+    if (hipHostMalloc((void**)&A->pin, A->pin_n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        A->pin = nullptr;
+        return fail(WEDG_ERR_HIP);
+    }
+    const size_t E = A->E, nU = A->nU(), nV = A->nV();
+    A->dc2 = adj_alloc(A, E, &he); A->dal = adj_alloc(A, E, &he); A->dbe = adj_alloc(A, E, &he);
+    A->tdc2 = adj_alloc(A, E, &he); A->tdbe = adj_alloc(A, E, &he);
+    A->atab = adj_alloc(A, (size_t)nq * nq + (size_t)nu * nu + nu, &he);
+    for (auto& tp : A->tape) {
+        for (int s = 0; s < 4; ++s) {
+            tp.Yu[s] = adj_alloc(A, nU, &he); tp.Yv[s] = adj_alloc(A, nV, &he);
+            tp.Ku[s] = adj_alloc(A, nU, &he); tp.Kv[s] = adj_alloc(A, nV, &he);
+        }
+        tp.ynu = adj_alloc(A, nU, &he); tp.ynv = adj_alloc(A, nV, &he);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.taped, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.used, hipEventDisableTiming);
+    }
+    A->ybu = adj_alloc(A, nU, &he); A->ybv = adj_alloc(A, nV, &he);
+    A->accu = adj_alloc(A, nU, &he); A->accv = adj_alloc(A, nV, &he);
+    A->kbu = adj_alloc(A, nU, &he); A->kbv = adj_alloc(A, nV, &he);
+    A->pu = adj_alloc(A, nU, &he); A->pv = adj_alloc(A, nV, &he);
+    A->slot = adj_alloc(A, (size_t)12 * nq * E, &he);
+    A->gc2 = adj_alloc(A, E, &he); A->gbe = adj_alloc(A, E, &he);
+    if (he == hipSuccess) {
+        void* f = nullptr;
+        he = hipMalloc(&f, 16);
+        if (he == hipSuccess) { A->allocs.push_back(f); A->fail = (int*)f; }
+    }
+    if (he != hipSuccess) return fail(WEDG_ERR_HIP);
+    std::vector<double> at;
+    for (int q = 0; q < nq; ++q)
+        for (int p = 0; p < nq; ++p) at.push_back(A->w[p] * A->w[q]);
+    at.insert(at.end(), A->K1.begin(), A->K1.end());
+    at.insert(at.end(), A->lam.begin(), A->lam.end());
+    if ((st = h2d(A, A->atab, at.data(), at.size()))) return fail(st);
+    if ((st = h2d(A, A->dal, A->alpha.data(), E))) return fail(st);
+    if ((st = wedg_adj_set_params(A, A->c2.data(), A->beta.data()))) return fail(st);
+    *out = A;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_destroy(wedg_adj* A) {
+    if (!A) return WEDG_OK;
+    if (A->sa) (void)hipStreamSynchronize(A->sa);
+    if (A->ws) (void)hipStreamSynchronize(A->ws->stream);
+    for (auto& tp : A->tape) {
+        if (tp.taped) (void)hipEventDestroy(tp.taped);
+        if (tp.used) (void)hipEventDestroy(tp.used);
+    }
+    for (void* a : A->allocs) (void)hipFree(a);
+    if (A->snaps) (void)hipFree(A->snaps);
+    if (A->dtgt) (void)hipFree(A->dtgt);
+    if (A->djp) (void)hipFree(A->djp);
+    if (A->dtr) (void)hipFree(A->dtr);
+    if (A->pin) (void)hipHostFree(A->pin);
+    if (A->sa) (void)hipStreamDestroy(A->sa);
+    if (A->ws) wedg_workspace_destroy(A->ws);
+    if (A->p) wedg_problem_destroy(A->p);
+    delete A;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_set_params(wedg_adj* A, const double* c2, const double* beta) {
+    if (!A || !c2 || !beta) return WEDG_ERR_ARGUMENT;
+    for (int e = 0; e < A->E; ++e)
+        if (!(c2[e] > 0.0) || !std::isfinite(c2[e]) || !std::isfinite(beta[e])) return WEDG_ERR_ARGUMENT;
+    if (A->ws) {
+        HIPOK(hipStreamSynchronize(A->ws->stream));
+        HIPOK(hipStreamSynchronize(A->sa));
+    }
+    A->c2.assign(c2, c2 + A->E);
+    A->beta.assign(beta, beta + A->E);
+    wedg_problem* np = nullptr;
+    wedg_status st = build_problem(A, &np);
+    if (st) return st;
+    if (A->ws) {
+        wedg_problem* old = A->ws->p;
+        A->ws->p = np;
+        if (old && old != A->p) wedg_problem_destroy(old);
+        if (A->p) wedg_problem_destroy(A->p);
+    }
+    A->p = np;
+    if ((st = h2d(A, A->dc2, A->c2.data(), A->E))) return st;
+    return h2d(A, A->dbe, A->beta.data(), A->E);
+}
+
+extern "C" wedg_status wedg_adj_set_target(wedg_adj* A, long nsteps, const double* target) {
+    if (!A || nsteps < 0 || (nsteps > 0 && !target)) return WEDG_ERR_ARGUMENT;
+    HIPOK(hipStreamSynchronize(A->ws->stream));
+    HIPOK(hipStreamSynchronize(A->sa));
+    if (A->dtgt) (void)hipFree(A->dtgt);
+    A->dtgt = nullptr;
+    A->tgt_steps = 0;
+    if (nsteps == 0) return WEDG_OK;
+    const size_t n = (size_t)nsteps * A->N * A->nu;
+    HIPOK(hipMalloc((void**)&A->dtgt, n * sizeof(double)));
+    A->tgt_steps = nsteps;
+    return h2d(A, A->dtgt, target, n);
+}
+
+extern "C" wedg_status wedg_adj_forward(wedg_adj* A, double dt, long nsteps, double* J, double* traces) {
+    if (!A || nsteps < 1 || !(dt > 0)) return WEDG_ERR_ARGUMENT;
+    if (J && A->tgt_steps < nsteps) return WEDG_ERR_ARGUMENT;
+    wedg_workspace* ws = A->ws;
+    wedg_status st;
+    if ((st = jp_capacity(A, nsteps))) return st;
+    if ((st = reset_err(ws))) return st;
+    HIPOK(hipMemsetAsync(ws->yu, 0, A->nU() * sizeof(double), ws->stream));
+    HIPOK(hipMemsetAsync(ws->yv, 0, A->nV() * sizeof(double), ws->stream));
+    AdjArgs a = adj_args(A);
+    a.yu = ws->yu;
+    double t = 0.0;
+    const size_t row = (size_t)A->N * A->nu;
+    for (long n = 0; n < nsteps; ++n) {
+        if ((st = rk4_device(ws, t, dt, n + 1, n == 0, n == nsteps - 1))) return st;
+        t += dt;
+        a.tgt = J ? A->dtgt + n * row : nullptr;
+        a.jp = J ? A->djp + n * row : nullptr;
+        a.tr = traces ? A->dtr + n * row : nullptr;
+        HIPOK(launch_adj(a, A->qu, A->nv, 3, ws->stream));
+    }
+    if ((st = collect(ws, 1))) return st;
+    if (J && (st = sum_misfit(A, nsteps, J))) return st;
+    if (traces && (st = d2h(A, traces, A->dtr, nsteps * row))) return st;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, int nsnap, double* J,
+                                         double* dJ_dc2, double* dJ_dbeta) {
+    if (!A || nsteps < 1 || !(dt > 0) || nsnap < 1 || !J || !dJ_dc2 || !dJ_dbeta) return WEDG_ERR_ARGUMENT;
+    if (A->tgt_steps < nsteps) return WEDG_ERR_ARGUMENT;
+    wedg_workspace* ws = A->ws;
+    const hipStream_t fs = ws->stream, as = A->sa;
+    wedg_status st;
+    if ((st = jp_capacity(A, nsteps))) return st;
+    const size_t nU = A->nU(), nV = A->nV(), sz = nU + nV;
+    if (nsnap > A->nsnap_cap) {
+        HIPOK(hipStreamSynchronize(fs));
+        if (A->snaps) (void)hipFree(A->snaps);
+        A->snaps = nullptr;
+        A->nsnap_cap = 0;
+        HIPOK(hipMalloc((void**)&A->snaps, (size_t)nsnap * sz * sizeof(double)));
+        A->nsnap_cap = nsnap;
+    }
+    std::memset(&A->stats, 0, sizeof(A->stats));
+    if ((st = reset_err(ws))) return st;
+    HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), fs));
+    HIPOK(hipMemsetAsync(ws->yu, 0, nU * sizeof(double), fs));
+    HIPOK(hipMemsetAsync(ws->yv, 0, nV * sizeof(double), fs));
+    if ((st = zero_dev(fs, A->gc2, A->E)) || (st = zero_dev(fs, A->gbe, A->E)) || (st = zero_dev(fs, A->ybu, nU)) ||
+        (st = zero_dev(fs, A->ybv, nV)))
+        return st;
+    HIPOK(hipStreamSynchronize(fs));
+    for (auto& tp : A->tape) tp.in_use = false;
+    // times after n steps, accumulated as the forward loop does (src/solver.jl:213-216)
+    std::vector<double> tt(nsteps + 1);
+    tt[0] = 0.0;
+    for (long n = 0; n < nsteps; ++n) tt[n + 1] = tt[n] + dt;
+    auto save = [&](int slot) -> wedg_status {
+        double* d = A->snaps + (size_t)slot * sz;
+        HIPOK(hipMemcpyAsync(d, ws->yu, nU * sizeof(double), hipMemcpyDeviceToDevice, fs));
+        HIPOK(hipMemcpyAsync(d + nU, ws->yv, nV * sizeof(double), hipMemcpyDeviceToDevice, fs));
+        ++A->stats.snapshots_taken;
+        return WEDG_OK;
+    };
+    auto restore = [&](int slot) -> wedg_status {
+        const double* d = A->snaps + (size_t)slot * sz;
+        HIPOK(hipMemcpyAsync(ws->yu, d, nU * sizeof(double), hipMemcpyDeviceToDevice, fs));
+        HIPOK(hipMemcpyAsync(ws->yv, d + nU, nV * sizeof(double), hipMemcpyDeviceToDevice, fs));
+        return WEDG_OK;
+    };
+    auto advance = [&](long a, long m) -> wedg_status {
+        wedg_status s2;
+        for (long i = a; i < m; ++i) {
+            if ((s2 = rk4_device(ws, tt[i], dt, i + 1, i == a, i == m - 1))) return s2;
+            ++A->stats.forward_steps;
+        }
+        return WEDG_OK;
+    };
+    // reverse step n (the workspace holds y_n): tape it on the forward stream, reverse it on the adjoint stream
+    auto leaf = [&](long n) -> wedg_status {
+        AdjTape& tp = A->tape[n & 1];
+        if (tp.in_use) HIPOK(hipStreamWaitEvent(fs, tp.used, 0));
+        wedg_status s2;
+        if ((s2 = tape_step(A, tp, tt[n], dt, n + 1))) return s2;
+        ++A->stats.taped_steps;
+        HIPOK(hipEventRecord(tp.taped, fs));
+        HIPOK(hipStreamWaitEvent(as, tp.taped, 0));
+        if ((s2 = adjoint_step(A, tp, n, dt))) return s2;
+        HIPOK(hipEventRecord(tp.used, as));
+        tp.in_use = true;
+        return WEDG_OK;
+    };
+    std::vector<int> free_slots;
+    for (int s = nsnap - 1; s >= 1; --s) free_slots.push_back(s);
+    int max_used = 1;
+    std::function<wedg_status(long, long, int)> reverse = [&](long a, long b, int slot_a) -> wedg_status {
+        wedg_status s2;
+        if (b - a == 1) return leaf(a);
+        const int sfree = (int)free_slots.size();
+        if (sfree == 0) {
+            for (long i = b - 1; i >= a; --i) {
+                if (i < b - 1 && (s2 = restore(slot_a))) return s2;
+                if ((s2 = advance(a, i))) return s2;
+                if ((s2 = leaf(i))) return s2;
+            }
+            return WEDG_OK;
+        }
+        const long m = a + revolve_split(b - a, sfree);
+        if ((s2 = advance(a, m))) return s2;
+        const int sm = free_slots.back();
+        free_slots.pop_back();
+        max_used = std::max(max_used, nsnap - (int)free_slots.size());
+        if ((s2 = save(sm))) return s2;
+        if ((s2 = reverse(m, b, sm))) return s2;
+        free_slots.push_back(sm);
+        if ((s2 = restore(slot_a))) return s2;
+        return reverse(a, m, slot_a);
+    };
+    if ((st = save(0))) return st;
+    if ((st = reverse(0, nsteps, 0))) return st;
+    A->stats.snapshots_used = max_used;
+    HIPOK(hipStreamSynchronize(as));
+    if ((st = collect(ws, 1))) return st;
+    if ((st = read_fail(A))) return st;
+    if ((st = sum_misfit(A, nsteps, J))) return st;
+    if ((st = d2h(A, dJ_dc2, A->gc2, A->E))) return st;
+    return d2h(A, dJ_dbeta, A->gbe, A->E);
+}
+
+extern "C" wedg_status wedg_adj_last_stats(const wedg_adj* A, wedg_adj_stats* st) {
+    if (!A || !st) return WEDG_ERR_ARGUMENT;
+    *st = A->stats;
+    return WEDG_OK;
+}
+
+// F(Y) of one state into tape set 0 (Y1 = the state, K1 = F(Y)).
+static wedg_status rhs_into_tape(wedg_adj* A, const double* u, const double* v, double t) {
+    AdjTape& tp = A->tape[0];
+    wedg_status st;
+    if ((st = h2d(A, tp.Yu[0], u, A->nU())) || (st = h2d(A, tp.Yv[0], v, A->nV()))) return st;
+    if ((st = reset_err(A->ws))) return st;
+    KArgs a = stage_args(A->ws, 0, tp.Yu[0], tp.Yv[0], tp.Yu[0], tp.Yv[0], tp.Ku[0], tp.Kv[0], t, 0.0, 0, false, t);
+    if ((st = run_kernel(A->ws, a, 0)) || (st = run_kernel(A->ws, a, 1))) return st;
+    return collect(A->ws, 0);
+}
+
+extern "C" wedg_status wedg_adj_rhs_vjp(wedg_adj* A, const double* u, const double* v, double t, const double* ub,
+                                        const double* vb, double* u_bar, double* v_bar, double* c2_bar,
+                                        double* beta_bar) {
+    if (!A || !u || !v || !ub || !vb || !u_bar || !v_bar || !c2_bar || !beta_bar) return WEDG_ERR_ARGUMENT;
+    wedg_status st;
+    if ((st = rhs_into_tape(A, u, v, t))) return st;
+    if ((st = h2d(A, A->kbu, ub, A->nU())) || (st = h2d(A, A->kbv, vb, A->nV()))) return st;
+    const hipStream_t s = A->ws->stream;
+    HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), s));
+    if ((st = zero_dev(s, A->gc2, A->E)) || (st = zero_dev(s, A->gbe, A->E))) return st;
+    AdjArgs a = adj_args(A);
+    AdjTape& tp = A->tape[0];
+    a.stage = 0;
+    a.yu = tp.Yu[0]; a.yv = tp.Yv[0]; a.ku = tp.Ku[0]; a.kv = tp.Kv[0];
+    a.kbu = A->kbu; a.kbv = A->kbv;
+    HIPOK(launch_adj(a, A->qu, A->nv, 0, s));
+    HIPOK(launch_adj(a, A->qu, A->nv, 1, s));
+    if ((st = read_fail(A))) return st;
+    if ((st = d2h(A, u_bar, A->accu, A->nU())) || (st = d2h(A, v_bar, A->accv, A->nV()))) return st;
+    if ((st = d2h(A, c2_bar, A->gc2, A->E))) return st;
+    return d2h(A, beta_bar, A->gbe, A->E);
+}
+
+extern "C" wedg_status wedg_adj_rhs_tl(wedg_adj* A, const double* u, const double* v, double t, const double* du,
+                                       const double* dv, const double* dc2, const double* dbeta, double* dut,
+                                       double* dvt) {
+    if (!A || !u || !v || !du || !dv || !dc2 || !dbeta || !dut || !dvt) return WEDG_ERR_ARGUMENT;
+    wedg_status st;
+    if ((st = rhs_into_tape(A, u, v, t))) return st;
+    AdjTape& tp = A->tape[0];
+    if ((st = h2d(A, tp.Yu[1], du, A->nU())) || (st = h2d(A, tp.Yv[1], dv, A->nV()))) return st;
+    if ((st = h2d(A, A->tdc2, dc2, A->E)) || (st = h2d(A, A->tdbe, dbeta, A->E))) return st;
+    const hipStream_t s = A->ws->stream;
+    HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), s));
+    AdjArgs a = adj_args(A);
+    a.yu = tp.Yu[0]; a.yv = tp.Yv[0]; a.ku = tp.Ku[0]; a.kv = tp.Kv[0];
+    a.du = tp.Yu[1]; a.dv = tp.Yv[1]; a.dc2 = A->tdc2; a.dbe = A->tdbe;
+    a.outu = tp.Yu[2]; a.outv = tp.Yv[2];
+    HIPOK(launch_adj(a, A->qu, A->nv, 2, s));
+    if ((st = read_fail(A))) return st;
+    if ((st = d2h(A, dut, tp.Yu[2], A->nU()))) return st;
+    return d2h(A, dvt, tp.Yv[2], A->nV());
+}

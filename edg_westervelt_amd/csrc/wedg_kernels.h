@@ -138,6 +138,7 @@ struct KArgs {
     DevErr* err;
     int* cnt;                               // Chebyshev count of every wave [2][waves], by stage parity
     unsigned* wstat;                        // per wave: sum [waves] and max [waves] of the counts it ran
+    double* recu; double* recv;             // when not null: the stage's derivative k = F(Y_s) is kept here
 };
 
 // ---------------------------------------------------------------------------
@@ -1164,6 +1165,11 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
         }
         {
             CKArgs& A = *kargs<D::BT>();
+            if (A.recu) {
+                double* ru = A.recu;
+#pragma unroll
+                for (int m = 0; m < nU; ++m) ru[(long)m * E + e] = ku[m];
+            }
             double yu0[nU], acu[nU];
             rk_load<nU>(A.stage, A.y0u, A.accu, E, e, yu0, acu);
             if (!rk_store<nU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, e))
@@ -1346,6 +1352,11 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
             double yv0[nV], acv[nV];
             const int stage = A.stage;
             const double c = A.cstage;
+            if (A.recv) {
+                double* rv = A.recv;
+#pragma unroll
+                for (int m = 0; m < nV; ++m) rv[(long)m * E + e] = X[m];
+            }
             rk_load<nV>(stage, A.y0v, A.accv, E, e, yv0, acv);
             if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e))
                 atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);

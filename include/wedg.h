@@ -257,6 +257,72 @@ wedg_status wedg_batch_advance_receivers(wedg_batch* b, const double* t0, const 
 wedg_status wedg_batch_configure(wedg_batch* b, int lanes, long chunk_steps, int schedule);
 wedg_status wedg_batch_config(const wedg_batch* b, int* lanes, long* chunk_steps, int* schedule);
 
+/* ------------------------------------------------------------------------
+ * Discrete adjoint of source-mode runs (no counterpart in the reference: the
+ * reference's hot path is the forward map edg_rhs! + rk4_step! in
+ * mode = :source, src/rhs.jl:490-550, src/solver.jl:289-321, 892-958, whose
+ * derivatives these entry points compute).
+ *
+ * The problem: mode = :source with coefficients constant on every element,
+ * c^2, alpha and beta given per element as (N, N) arrays (row j in y, column k
+ * in x, element e = j + N k), zero initial state, t += dt after every step
+ * (src/solver.jl:213-216, 937-939). The objective is the misfit of u_h on the
+ * bottom side y = 0 at its face quadrature points after every RK4 step,
+ *   J = 1/2 sum_{n=1..S} sum_k sum_i (h/2) w_i (u_h(x_ki, 0, t_n) - g_n[k, i])^2,
+ * with the target g ([n][k][i], q_u+1 points per element) set by
+ * wedg_adj_set_target. wedg_adj_gradient returns J and dJ/dc^2, dJ/dbeta per
+ * element by the discrete adjoint of RK4 and of every stage (faces, boundary
+ * fluxes with their sqrt(1 - k beta v) weights, volume terms, lifts, the
+ * u-solve and the mass solve, the latter as the exact solve M^-1 with the
+ * Chebyshev iteration to 1e-14), with binomial checkpointing of the RK4
+ * states in `snapshots` state slots (the initial state counts as one). */
+typedef struct wedg_adj_desc {
+    int N, qu1, qv1;
+    double theta, tau, lambda_u, lambda_v;     /* flux parameters (src/solver.jl:16-21) */
+    wedg_side bc[4];                           /* bot, top, lft, rgt (src/solver.jl:39-45) */
+    double mass_tol;                           /* 1e-14 */
+    double src_omega, src_Ts;                  /* GaussianSource omega, Ts (src/solver.jl:80-86) */
+    /* the 1D tables of wedg_pre */
+    const double *w, *P_u, *S_u, *Ebl_u, *Ebr_u, *E1bl_u, *E1br_u, *P_h_v, *W2d_h;
+    const double* src_projE;                   /* (q_v+1, q_v+1, N, N): P_h^T (W_h o g) P_h (src/solver.jl:656-660) */
+    const double* alpha;                       /* (N, N) per element */
+} wedg_adj_desc;
+
+typedef struct wedg_adj wedg_adj;
+
+/* Counts of the last wedg_adj_gradient: forward steps advanced (first sweep and
+ * recomputations), steps taped and reversed, snapshots written and the most
+ * slots held at once. */
+typedef struct wedg_adj_stats {
+    long forward_steps, taped_steps, adjoint_steps, snapshots_taken;
+    int snapshots_used, pad;
+} wedg_adj_stats;
+
+/* cheb_policy of the forward steps as in wedg_workspace_create. The
+ * coefficients start as c^2 = 1, beta = 0 until wedg_adj_set_params. */
+wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, wedg_adj** out);
+wedg_status wedg_adj_destroy(wedg_adj* a);
+/* c2 (> 0) and beta per element, (N, N) each; rebuilds the forward tables. */
+wedg_status wedg_adj_set_params(wedg_adj* a, const double* c2, const double* beta);
+/* The target g of the misfit for nsteps steps, [n][k][i] (nsteps x N x (q_u+1)). */
+wedg_status wedg_adj_set_target(wedg_adj* a, long nsteps, const double* target);
+/* nsteps RK4 steps of dt from the zero state: J (needs a target of at least
+ * nsteps rows; null: not formed) and the bottom traces u_h after every step
+ * into traces ([n][k][i], or null). */
+wedg_status wedg_adj_forward(wedg_adj* a, double dt, long nsteps, double* J, double* traces);
+/* J and its gradient, dJ_dc2 and dJ_dbeta (N, N) each. */
+wedg_status wedg_adj_gradient(wedg_adj* a, double dt, long nsteps, int snapshots, double* J, double* dJ_dc2,
+                              double* dJ_dbeta);
+wedg_status wedg_adj_last_stats(const wedg_adj* a, wedg_adj_stats* st);
+/* One right-hand side (u_t, v_t) = F(u, v, t; c^2, beta) differentiated: the
+ * vector-Jacobian product, from the cotangents (ub, vb) of (u_t, v_t) those of
+ * (u, v, c^2, beta); and the tangent-linear map, (du_t, dv_t) along
+ * (du, dv, dc2, dbeta). Host arrays in the state layouts. */
+wedg_status wedg_adj_rhs_vjp(wedg_adj* a, const double* u, const double* v, double t, const double* ub,
+                             const double* vb, double* u_bar, double* v_bar, double* c2_bar, double* beta_bar);
+wedg_status wedg_adj_rhs_tl(wedg_adj* a, const double* u, const double* v, double t, const double* du,
+                            const double* dv, const double* dc2, const double* dbeta, double* dut, double* dvt);
+
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);
 wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);
