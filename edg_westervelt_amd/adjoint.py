@@ -31,7 +31,7 @@ from ._lib import WedgError, lib
 from .coefficients import constant_coefficients
 from .precompute import GaussianSource, precompute
 
-__all__ = ["SourceAdjoint", "misfit_rrule"]
+__all__ = ["SourceAdjoint", "misfit_rrule", "beta_field_workload"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -192,3 +192,28 @@ def misfit_rrule(adj: SourceAdjoint, c2, beta, dt: float, nsteps: int, snapshots
         return Jbar * gc2, Jbar * gb
 
     return J, pullback
+
+
+def beta_field_workload(N: int = 64, q: int = 4, steps: int = 2000, seed: int = 1, cheb: str = "global",
+                        x0=(0.5, 0.7)):
+    """The receiver-misfit gradient workload: q_u = q_v = q on an N x N mesh in
+    source mode, a Gaussian source of width 0.05 at (0.5, 0.7) (amplitude 1,
+    omega 16 pi, Ts = 1), c = 1, alpha = 1e-5, beta per element
+    4 + 2 sin(2 pi x) cos(2 pi y) + N(0, 0.1) at the element centres (numpy
+    generator of `seed`), the bottom side absorbing_2beta and the other three
+    walls with kappa = 0.5, `steps` RK4 steps of dt = h / (10 q^2); the target
+    is the bottom trace of the same run with beta = 4. `x0` moves the source."""
+    import math
+    h = 1.0 / N
+    xc = (np.arange(N) + 0.5) * h
+    X, Y = xc[None, :], xc[:, None]                     # column k in x, row j in y
+    rng = np.random.default_rng(seed)
+    beta = 4.0 + 2.0 * np.sin(2 * math.pi * X) * np.cos(2 * math.pi * Y) + 0.1 * rng.standard_normal((N, N))
+    adj = SourceAdjoint(q, N, mixed=False, boundary=dict(bot="absorbing_2beta", top="wall", lft="wall", rgt="wall"),
+                        kappa=0.5, alpha=1e-5, source=GaussianSource(x0=tuple(x0), r0=0.05), cheb=cheb)
+    dt = h / (10 * q * q)
+    adj.set_params(1.0, 4.0)
+    adj.set_target(adj.traces(dt, steps))
+    c2 = np.ones((N, N), order="F")
+    adj.set_params(c2, beta)
+    return SimpleNamespace(adj=adj, c2=c2, beta=np.asfortranarray(beta), dt=dt, steps=steps, N=N, q=q)
