@@ -131,6 +131,40 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
                            probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()}))
 
 
+def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
+    """Wall time of the receiver-misfit gradient (forward sweep, recomputations,
+    taped steps and the adjoint of every stage, binomial checkpointing in
+    `snapshots` slots) against one forward solve of the same run with the misfit,
+    on edg_westervelt_amd.adjoint.beta_field_workload (q = 4, 64 x 64, 2000
+    steps, per-element beta, source at (0.5, 0.7))."""
+    from edg_westervelt_amd.adjoint import beta_field_workload
+    t = time.perf_counter()
+    wl = beta_field_workload(N=N, q=q, steps=steps)
+    t_build = time.perf_counter() - t
+    adj = wl.adj
+    adj.misfit(wl.dt, 20)                          # warm-up of both paths
+    adj.gradient(wl.dt, 20, snapshots)
+    print(f"# adjoint: forward solve of {steps} steps", file=sys.stderr, flush=True)
+    c0 = time.perf_counter()
+    J = adj.misfit(wl.dt, steps)
+    tf = time.perf_counter() - c0
+    print(f"# adjoint: gradient with {snapshots} snapshots", file=sys.stderr, flush=True)
+    c0 = time.perf_counter()
+    Jg, gc2, gb = adj.gradient(wl.dt, steps, snapshots)
+    tg = time.perf_counter() - c0
+    st = adj.stats()
+    dof = N * N * ((q + 1) ** 2 * 2)
+    return dict(adjoint_forward_ratio=tg / tf, forward_s=tf, gradient_s=tg, J=J, J_gradient_run=Jg,
+                forward_dof_stages_per_s=4 * steps * dof / tf, snapshots=snapshots,
+                forward_steps_recomputed=st.forward_steps, taped_steps=st.taped_steps,
+                snapshots_used=st.snapshots_used, build_s=round(t_build, 2),
+                host_enqueue_ms=st.enqueue_ms, device_ms=st.total_ms,
+                workload=f"edg_westervelt_amd.adjoint.beta_field_workload(N={N}, q={q}, steps={steps}): source "
+                         "mode, Gaussian source width 0.05 at (0.5, 0.7), c = 1, alpha = 1e-5, per-element beta "
+                         "4 + 2 sin(2 pi x) cos(2 pi y) + N(0, 0.1) (seed 1), bottom absorbing_2beta, walls with "
+                         "kappa = 0.5; J = bottom-side L2 misfit after every step against the beta = 4 run")
+
+
 def reference_runs():
     """Wall times (precompute, upload, run and results, on this machine) of the
     runs the reference times itself, as it states them: README.md:244-245,
@@ -220,6 +254,9 @@ def main():
                     help="the uneven-batch speedup: with the headline (on), not at all (off), or alone (only)")
     ap.add_argument("--batched-sequential", choices=("on", "off"), default="on",
                     help="off: time only the batched run (for a kernel trace of it)")
+    ap.add_argument("--adjoint", choices=("on", "off", "only"), default="on",
+                    help="the receiver-misfit gradient and its cost against one forward solve: with the headline "
+                         "(on), not at all (off), or alone (only)")
     ap.add_argument("--reference-runs", action="store_true",
                     help="only time the runs the reference times itself (README, data files) and exit")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -246,6 +283,11 @@ def main():
     if args.reference_runs:
         print(json.dumps(dict(metric="wall time of the reference's own timed runs", unit="s", n_gpus=1,
                               higher_is_better=False, dtype="f64", data="synthetic", runs=reference_runs())))
+        return
+    if args.adjoint == "only":
+        ad = bench_adjoint()
+        print(json.dumps(dict(metric="adjoint/forward cost ratio", value=ad["adjoint_forward_ratio"], unit="x",
+                              n_gpus=1, higher_is_better=False, dtype="f64", data="synthetic", adjoint=ad)))
         return
     if args.batched == "only":
         bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb)
@@ -348,6 +390,7 @@ def main():
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
     batched = bench_batched(cheb=args.cheb) if args.batched == "on" else None
+    adjoint = bench_adjoint() if args.adjoint == "on" else None
 
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
     # problem, a bounded sample of RK4 steps from the same initial state
@@ -377,7 +420,9 @@ def main():
                                 "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
-               batched=batched, final_state_finite=finite)
+               batched=batched, adjoint=adjoint,
+               adjoint_forward_ratio=adjoint["adjoint_forward_ratio"] if adjoint else None,
+               final_state_finite=finite)
     print(json.dumps(out))
 
 

@@ -10,4 +10,10 @@ hipError_t launch_adj_q1(const AdjArgs& a, int nv, int which, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adj2_q1(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s) {
+    if (nv == 2) return launch_adj2_nv<2, 2>(b, a, s);
+    if (nv == 1) return launch_adj2_nv<2, 1>(b, a, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -10,4 +10,10 @@ hipError_t launch_adj_q3(const AdjArgs& a, int nv, int which, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adj2_q3(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s) {
+    if (nv == 4) return launch_adj2_nv<4, 4>(b, a, s);
+    if (nv == 3) return launch_adj2_nv<4, 3>(b, a, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -10,4 +10,10 @@ hipError_t launch_adj_q4(const AdjArgs& a, int nv, int which, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adj2_q4(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s) {
+    if (nv == 5) return launch_adj2_nv<5, 5>(b, a, s);
+    if (nv == 4) return launch_adj2_nv<5, 4>(b, a, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -10,4 +10,10 @@ hipError_t launch_adj_q5(const AdjArgs& a, int nv, int which, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adj2_q5(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s) {
+    if (nv == 6) return launch_adj2_nv<6, 6>(b, a, s);
+    if (nv == 5) return launch_adj2_nv<6, 5>(b, a, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

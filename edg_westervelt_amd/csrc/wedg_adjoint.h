@@ -40,7 +40,8 @@ struct AdjArgs {
     const double* ku; const double* kv;                      // F(Y_s)
     const double* kbu; const double* kbv;                    // k_adj_a: cotangent of F(Y_s), times kscale
     double kscale, ca, cb, pad1;                             // k_adj_b: next cotangent = ca ybar + cb Ybar_s
-    double* slot;           // slot adjoints [side][A, D, F][i][E]
+    double* slot;           // slot adjoints [side][A, D, F][i][E], written by k_adj_a
+    const double* slotr;    // the slot adjoints k_adj_b reads (the stage's k_adj_a wrote them)
     double* pu; double* pv; // k_adj_a's share of the state cotangent [m][E]
     double* gc2; double* gbe;                                // dJ/dc2, dJ/dbeta [E], accumulated
     double* ybu; double* ybv;                                // ybar of the step; stage 1 replaces it
@@ -55,6 +56,26 @@ struct AdjArgs {
     double* jp;             // 1/2 w r^2 per point [k][i] (or null)
     int* fail;              // set to 1 when a mass matrix is not positive definite
 };
+
+// The kernels' AdjArgs read through the kernarg segment pointer in the constant
+// address space, opaque to the optimizer (as kargs() for the forward kernels):
+// each phase fetches it anew, so fields are loaded (scalar) where they are used
+// instead of being held in SGPRs across the whole kernel.
+// A kernel that runs two stages' parts (k_adj_ba) takes two AdjArgs; W selects one.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const WEDG_CONST_AS AdjArgs CAdjArgs;
+template <int W = 0>
+__device__ __forceinline__ CAdjArgs& adjargs() {
+    CAdjArgs* p = (CAdjArgs*)__builtin_amdgcn_kernarg_segment_ptr() + W;
+    asm volatile("" : "+s"(p));
+    return *p;
+}
+#else
+typedef const AdjArgs CAdjArgs;
+extern const AdjArgs* wedg_emu_adjargs;   // set by a host build that runs the kernels on the CPU
+template <int W = 0>
+__device__ __forceinline__ CAdjArgs& adjargs() { return wedg_emu_adjargs[W]; }
+#endif
 
 // ---------------------------------------------------------------------------
 // Small helpers.
@@ -259,7 +280,8 @@ __device__ __forceinline__ void side_nb(int s, int e, int j, int k, int N, int& 
 
 // Interior face point, one-sided coefficients, K = 0 (src/rhs.jl:683-696): the
 // slot values of the element on the side `plus` (true: "+").
-__device__ __forceinline__ void face_slots(const AdjArgs& a, bool plus, double vm, double vp, double dum,
+template <class KA>
+__device__ __forceinline__ void face_slots(const KA& a, bool plus, double vm, double vp, double dum,
                                            double dup, double dvm, double dvp, double c2m, double c2p,
                                            double afm, double afp, double& A, double& Dd, double& Ff) {
     const double th = a.theta, tau = a.tau, two_h = a.two_h;
@@ -277,7 +299,8 @@ __device__ __forceinline__ void face_slots(const AdjArgs& a, bool plus, double v
 }
 
 // Tangent of face_slots in the traces and c2 of both sides (alpha fixed).
-__device__ __forceinline__ void face_slots_tl(const AdjArgs& a, bool plus, double vm, double vp, double dum,
+template <class KA>
+__device__ __forceinline__ void face_slots_tl(const KA& a, bool plus, double vm, double vp, double dum,
                                               double dup, double dvm, double dvp, double c2m, double c2p,
                                               double afm, double afp, double tvm, double tvp, double tdum,
                                               double tdup, double tdvm, double tdvp, double tc2m, double tc2p,
@@ -302,7 +325,8 @@ __device__ __forceinline__ void face_slots_tl(const AdjArgs& a, bool plus, doubl
 // Adjoint of face_slots of both sides of a face point: from the slot adjoints of
 // the "+" (Abp, Dbp, Fbp) and "-" (Abm, Dbm, Fbm) elements, the adjoints of the
 // traces and c2 of the side `plus` (its own share).
-__device__ __forceinline__ void face_slots_T(const AdjArgs& a, bool plus, double vm, double vp, double dum,
+template <class KA>
+__device__ __forceinline__ void face_slots_T(const KA& a, bool plus, double vm, double vp, double dum,
                                              double dup, double dvm, double dvp, double c2m, double c2p,
                                              double afm, double afp, double Abp, double Dbp, double Fbp,
                                              double Abm, double Dbm, double Fbm, double& vb, double& dub,
@@ -334,7 +358,8 @@ __device__ __forceinline__ void face_slots_T(const AdjArgs& a, bool plus, double
 
 // Physical boundary point of side s with zero data (src/rhs.jl:74-96, 738-763):
 // the slots; returns false when the weight 1 - k beta v_h is not positive.
-__device__ __forceinline__ bool bnd_slots(const AdjArgs& a, int s, double vt, double du, double dv, double c2,
+template <class KA>
+__device__ __forceinline__ bool bnd_slots(const KA& a, int s, double vt, double du, double dv, double c2,
                                           double af, double bf, double& A, double& Dd, double& Ff) {
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
     const double cf = sqrt(c2);
@@ -358,7 +383,8 @@ __device__ __forceinline__ bool bnd_slots(const AdjArgs& a, int s, double vt, do
 }
 
 // Tangent of bnd_slots in (vt, du, dv, c2, bf).
-__device__ __forceinline__ void bnd_slots_tl(const AdjArgs& a, int s, double vt, double du, double dv, double c2,
+template <class KA>
+__device__ __forceinline__ void bnd_slots_tl(const KA& a, int s, double vt, double du, double dv, double c2,
                                              double af, double bf, double tvt, double tdu, double tdv, double tc2,
                                              double tbf, double& A, double& Dd, double& Ff) {
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
@@ -388,7 +414,8 @@ __device__ __forceinline__ void bnd_slots_tl(const AdjArgs& a, int s, double vt,
 }
 
 // Adjoint of bnd_slots: from (Ab, Db, Fb), the adjoints of (vt, du, dv, c2, bf).
-__device__ __forceinline__ void bnd_slots_T(const AdjArgs& a, int s, double vt, double du, double dv, double c2,
+template <class KA>
+__device__ __forceinline__ void bnd_slots_T(const KA& a, int s, double vt, double du, double dv, double c2,
                                             double af, double bf, double Ab, double Db, double Fb, double& vtb,
                                             double& dub, double& dvb, double& c2b, double& bfb) {
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
@@ -440,104 +467,169 @@ __device__ __forceinline__ void traces3(cdbl* T, const double (&U)[D::NU * D::NU
 }
 
 // ---------------------------------------------------------------------------
-// k_adj_a: the element's own part of the VJP of one stage.
-template <int NU, int NV>
-__device__ __forceinline__ void adj_a_body(const AdjArgs& a, int e, int lane, double* Gl) {
-    using D = Dims<NU, NV>;
-    constexpr int NQ = D::NQ, nU = NU * NU, nV = NV * NV;
-    const int E = a.E;
-    cdbl* T = ctab(a.tab);
-    cdbl* AT = ctab(a.atab);
-    const double c2e = a.c2[e], ale = a.al[e], bee = a.be[e];
-    // ---- mass system: lambda = M^-1 vbar_t; beta_bar and the v-terms of M
-    double lam[nV], Vb[nV];
-    double betab = 0.0;
-    {
-        double V[nV], R[nV];
-        ld<nV>(a.yv, E, e, V);
-        ld<nV>(a.kbv, E, e, R);
+// k_adj_a: the element's own part of the VJP of one stage. The phases are
+// ordered so that the two solves' results die early: lambda = M^-1 vbar_t
+// feeds the beta and v terms of M, the lift adjoints of A and D and the v-test
+// volume terms, then bbar = A^-1 ubar_t feeds the lift adjoints of F and the
+// u-test volume term.
+
+// the adjoints of slot q (0 A, 1 D: from lambda with the v rows; 2 F: from bbar
+// with the u rows) of the four sides (src/rhs.jl:434-473), into the slot buffer
+template <class D, int n, int q, class KA>
+__device__ __forceinline__ void lift_adjoint(const KA& a, cdbl* T, int e, const double (&Xb)[n * n]) {
+    constexpr int NQ = D::NQ;
+    auto side = [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr bool hz = s < 2, lo = (s == 0 || s == 2);
+        constexpr int oR = q == 0 ? (lo ? D::oTEl : D::oTEr) : q == 1 ? (lo ? D::oFE1l : D::oFE1r)
+                                                                    : (lo ? D::oE1l : D::oE1r);
+        constexpr double sg = q == 2 ? -1.0 : 1.0;   // b = ... - Fu
+        double Y[n];
 #pragma unroll
-        for (int m = 0; m < nV; ++m) R[m] *= a.kscale;
-        double th = 1.0, de = 1.0, s1 = 1.0;
-        const int n = mass_setup<D>(T, bee, V, Gl, lane, a.cheb_L, th, de, s1);
-        if (n == 0) {
-            atomicOr(a.fail, 1);
+        for (int x = 0; x < n; ++x) {
+            double sa = 0.0;
 #pragma unroll
-            for (int m = 0; m < nV; ++m) lam[m] = 0.0;
-        } else {
-            cheb_solve<D>(T + D::oPh, Gl, lane, n, th, de, s1, R, lam);
+            for (int y = 0; y < n; ++y) sa = fma(T[oR + y], hz ? Xb[y + n * x] : Xb[x + n * y], sa);
+            Y[x] = sg * sa;
         }
-        phase_barrier();
-        // beta_bar = sum 2 W v_q Lambda_q X_q; G' = 2 beta W Lambda_q into LDS
-        double X[nV];
-        ld<nV>(a.kv, E, e, X);
-        constexpr int NH = D::NH;
+        double S[NQ];
+        rows_sym<NQ, n, false>(T + D::oP, [&](int b) { return Y[b]; }, S);
+        double* sl = a.slot + (long)(s * 3 + q) * NQ * a.E + e;
 #pragma unroll
-        for (int p = 0; p < (NH + 1) / 2; ++p) {
-            const bool mid = (NH & 1) && p == NH / 2;
-            double vp_[NH], vq_[NH], xp[NH], xq[NH], lp[NH], lq[NH];
-            vq_rows<D>(T + D::oPh, p, mid, V, vp_, vq_);
-            vq_rows<D>(T + D::oPh, p, mid, X, xp, xq);
-            vq_rows<D>(T + D::oPh, p, mid, lam, lp, lq);
+        for (int i = 0; i < NQ; ++i) sl[(long)i * a.E] = T[D::oW + i] * S[i];
+    };
+    side(std::integral_constant<int, 0>());
+    side(std::integral_constant<int, 1>());
+    side(std::integral_constant<int, 2>());
+    side(std::integral_constant<int, 3>());
+}
+
+template <int NU, int NV, int W = 0>
+__device__ __forceinline__ void adj_a_body(int e, int lane, double* Gl) {
+    using D = Dims<NU, NV>;
+    constexpr int NQ = D::NQ, NH = D::NH, nU = NU * NU, nV = NV * NV;
+    const int E = adjargs<W>().E;
+    const double c2e = adjargs<W>().c2[e], ale = adjargs<W>().al[e], bee = adjargs<W>().be[e];
+    // the v cotangent accumulates in the lane's LDS slots once the mass weights
+    // are no longer needed (NH^2 >= NV^2 entries per lane); the u cotangent is
+    // stored as soon as it is formed
+    auto vacc = [&](int m) -> double& { return Gl[m * kWave + lane]; };
+    double betab = 0.0, c2b = 0.0;
+    {
+        // ---- mass system: lambda = M^-1 vbar_t (src/rhs.jl:886-939)
+        double lam[nV];
+        {
+            auto& a = adjargs<W>();
+            cdbl* T = ctab(a.tab);
+            double V[nV], R[nV];
+            ld<nV>(a.yv, E, e, V);
+            ld<nV>(a.kbv, E, e, R);
 #pragma unroll
-            for (int q = 0; q < NH; ++q) {
-                const double w1 = 2.0 * T[D::oWh + p + NH * q];
-                betab = fma(w1 * vp_[q], lp[q] * xp[q], betab);
-                Gl[(p * NH + q) * kWave + lane] = (w1 * bee) * lp[q];
-                if (!mid) {
-                    const int r = NH - 1 - p;
-                    const double w2 = 2.0 * T[D::oWh + r + NH * q];
-                    betab = fma(w2 * vq_[q], lq[q] * xq[q], betab);
-                    Gl[(r * NH + q) * kWave + lane] = (w2 * bee) * lq[q];
-                }
+            for (int m = 0; m < nV; ++m) R[m] *= a.kscale;
+            double th = 1.0, de = 1.0, s1 = 1.0;
+            const int n = mass_setup<D>(T, bee, V, Gl, lane, a.cheb_L, th, de, s1);
+            if (n == 0) {
+                atomicOr(a.fail, 1);
+#pragma unroll
+                for (int m = 0; m < nV; ++m) lam[m] = 0.0;
+            } else {
+                cheb_solve<D>(T + D::oPh, Gl, lane, n, th, de, s1, R, lam);
             }
         }
+        phase_barrier();
+        // ---- the beta and v terms of M: beta_bar = sum 2 W v_q Lambda_q X_q,
+        // vbar += K(G') X with G' = 2 beta W Lambda_q
+        {
+            auto& a = adjargs<W>();
+            cdbl* T = ctab(a.tab);
+            double V[nV], X[nV];
+            ld<nV>(a.yv, E, e, V);
+            ld<nV>(a.kv, E, e, X);
+#pragma unroll 1
+            for (int p = 0; p < (NH + 1) / 2; ++p) {
+                const bool mid = (NH & 1) && p == NH / 2;
+                double lp[NH], lq[NH], yp[NH], yq[NH], xp[NH], xq[NH];
+                vq_rows<D>(T + D::oPh, p, mid, lam, lp, lq);
+                vq_rows<D>(T + D::oPh, p, mid, V, yp, yq);
+                vq_rows<D>(T + D::oPh, p, mid, X, xp, xq);
+                const int r = NH - 1 - p;
 #pragma unroll
-        for (int m = 0; m < nV; ++m) Vb[m] = 0.0;
-        kapply<D>(T + D::oPh, Gl, lane, X, Vb);
-    }
-    phase_barrier();
-    // ---- u-equation: ut = A^-1 b; bbar = A^-1 ubar_t, c2bar -= bbar^T K2 ut
-    double bb[nU];
-    double c2b;
-    {
-        double ub[nU], Ut[nU];
-        ld<nU>(a.kbu, E, e, ub);
+                for (int q = 0; q < NH; ++q) {
+                    const double w1 = 2.0 * T[D::oWh + p + NH * q];
+                    betab = fma(w1 * yp[q], lp[q] * xp[q], betab);
+                    Gl[(p * NH + q) * kWave + lane] = (w1 * bee) * lp[q];
+                    if (!mid) {
+                        const double w2 = 2.0 * T[D::oWh + r + NH * q];
+                        betab = fma(w2 * yq[q], lq[q] * xq[q], betab);
+                        Gl[(r * NH + q) * kWave + lane] = (w2 * bee) * lq[q];
+                    }
+                }
+            }
+            double Vb[nV];
 #pragma unroll
-        for (int m = 0; m < nU; ++m) ub[m] *= a.kscale;
-        usolve<D>(T + D::oQ, AT + NQ * NQ + NU * NU, c2e, ub, bb);
-        ld<nU>(a.ku, E, e, Ut);
-        c2b = -stiff_dot<D>(AT + NQ * NQ, bb, Ut);
-    }
-    Vb[0] += bb[0];
-    phase_barrier();
-    // ---- volume terms (src/rhs.jl:317-333): Rvol = c2 Vol(U) + alpha Vol(V), VB = c2 Vol(V)
-    double Ub[nU];
-    {
-        double Rb[nV];
+            for (int m = 0; m < nV; ++m) Vb[m] = 0.0;
+            kapply<D>(T + D::oPh, Gl, lane, X, Vb);
 #pragma unroll
-        for (int m = 0; m < nV; ++m) Rb[m] = -a.four_hh * lam[m];
-        double Z[nU];
-#pragma unroll
-        for (int m = 0; m < nU; ++m) Z[m] = 0.0;
-        vol_pass<D, NV, NU>(a.tab, Rb, a.atab, Z);
-        double U[nU];
-        ld<nU>(a.yu, E, e, U);
-#pragma unroll
-        for (int m = 0; m < nU; ++m) {
-            Ub[m] = c2e * Z[m];
-            c2b = fma(Z[m], U[m], c2b);
+            for (int m = 0; m < nV; ++m) vacc(m) = Vb[m];
         }
         phase_barrier();
-        double Zv[nV];
+        lift_adjoint<D, NV, 0>(adjargs<W>(), ctab(adjargs<W>().tab), e, lam);
+        lift_adjoint<D, NV, 1>(adjargs<W>(), ctab(adjargs<W>().tab), e, lam);
+        phase_barrier();
+        // ---- v-test volume terms (src/rhs.jl:317-333): Rvol = c2 Vol(U) + alpha Vol(V),
+        // Rv = -(4/h^2) Rvol + ...: Ubar = c2 Vol^T(Rbar), c2bar += <Vol^T(Rbar), U>
+        const double s4 = -adjargs<W>().four_hh;
 #pragma unroll
-        for (int m = 0; m < nV; ++m) Zv[m] = 0.0;
-        vol_pass<D, NV, NV>(a.tab, Rb, a.atab, Zv);
+        for (int m = 0; m < nV; ++m) lam[m] *= s4;
+        {
+            auto& a = adjargs<W>();
+            double Z[nU];
 #pragma unroll
-        for (int m = 0; m < nV; ++m) Vb[m] = fma(ale, Zv[m], Vb[m]);
+            for (int m = 0; m < nU; ++m) Z[m] = 0.0;
+            vol_pass<D, NV, NU>(a.tab, lam, a.atab, Z);
+            double U[nU];
+            ld<nU>(a.yu, E, e, U);
+#pragma unroll
+            for (int m = 0; m < nU; ++m) {
+                c2b = fma(Z[m], U[m], c2b);
+                Z[m] *= c2e;
+            }
+            st<nU>(a.pu, E, e, Z);
+        }
+        phase_barrier();
+        {
+            auto& a = adjargs<W>();
+            double Zv[nV];
+#pragma unroll
+            for (int m = 0; m < nV; ++m) Zv[m] = 0.0;
+            vol_pass<D, NV, NV>(a.tab, lam, a.atab, Zv);
+#pragma unroll
+            for (int m = 0; m < nV; ++m) vacc(m) = fma(ale, Zv[m], vacc(m));
+        }
     }
     phase_barrier();
     {
+        // ---- u-equation (src/rhs.jl:361-382): ut = A^-1 b; bbar = A^-1 ubar_t,
+        // c2bar -= bbar^T K2 ut; b = VB - Fu + e0 V[0]
+        double bb[nU];
+        {
+            auto& a = adjargs<W>();
+            cdbl* T = ctab(a.tab);
+            cdbl* AT = ctab(a.atab);
+            double ub[nU], Ut[nU];
+            ld<nU>(a.kbu, E, e, ub);
+#pragma unroll
+            for (int m = 0; m < nU; ++m) ub[m] *= a.kscale;
+            usolve<D>(T + D::oQ, AT + NQ * NQ + NU * NU, c2e, ub, bb);
+            ld<nU>(a.ku, E, e, Ut);
+            c2b -= stiff_dot<D>(AT + NQ * NQ, bb, Ut);
+        }
+        vacc(0) += bb[0];
+        phase_barrier();
+        lift_adjoint<D, NU, 2>(adjargs<W>(), ctab(adjargs<W>().tab), e, bb);
+        phase_barrier();
+        auto& a = adjargs<W>();
+        // VB = c2 Vol(V) (u-test): Vbar += c2 Vol^T(bbar), c2bar += <Vol^T(bbar), V>
         double Zv[nV];
 #pragma unroll
         for (int m = 0; m < nV; ++m) Zv[m] = 0.0;
@@ -546,54 +638,13 @@ __device__ __forceinline__ void adj_a_body(const AdjArgs& a, int e, int lane, do
         ld<nV>(a.yv, E, e, V);
 #pragma unroll
         for (int m = 0; m < nV; ++m) {
-            Vb[m] = fma(c2e, Zv[m], Vb[m]);
-            c2b = fma(Zv[m], V[m], c2b);
+            const double z = Zv[m];
+            c2b = fma(z, V[m], c2b);
+            Zv[m] = fma(c2e, z, vacc(m));
         }
+        st<nV>(a.pv, E, e, Zv);
     }
-    phase_barrier();
-    // ---- lifts (src/rhs.jl:421-473) -> slot adjoints of the four sides
-    auto side = [&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        constexpr bool hz = s < 2, lo = (s == 0 || s == 2);
-        constexpr int oE = lo ? D::oTEl : D::oTEr, oE1 = lo ? D::oFE1l : D::oFE1r, oF1 = lo ? D::oE1l : D::oE1r;
-        double YA[NV], YD[NV], YF[NU];
-#pragma unroll
-        for (int x = 0; x < NV; ++x) {
-            double sa = 0.0, sd = 0.0;
-#pragma unroll
-            for (int y = 0; y < NV; ++y) {
-                const double l = hz ? lam[y + NV * x] : lam[x + NV * y];
-                sa = fma(T[oE + y], l, sa);
-                sd = fma(T[oE1 + y], l, sd);
-            }
-            YA[x] = sa; YD[x] = sd;
-        }
-#pragma unroll
-        for (int x = 0; x < NU; ++x) {
-            double sf = 0.0;
-#pragma unroll
-            for (int y = 0; y < NU; ++y) sf = fma(T[oF1 + y], hz ? bb[y + NU * x] : bb[x + NU * y], sf);
-            YF[x] = -sf;
-        }
-        double A[NQ], Dd[NQ], F[NQ];
-        rows_sym<NQ, NV, false>(T + D::oP, [&](int b) { return YA[b]; }, A);
-        rows_sym<NQ, NV, false>(T + D::oP, [&](int b) { return YD[b]; }, Dd);
-        rows_sym<NQ, NU, false>(T + D::oP, [&](int b) { return YF[b]; }, F);
-        double* sl = a.slot + (long)(s * 3) * NQ * E + e;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const double w = T[D::oW + i];
-            sl[(long)(0 * NQ + i) * E] = w * A[i];
-            sl[(long)(1 * NQ + i) * E] = w * Dd[i];
-            sl[(long)(2 * NQ + i) * E] = w * F[i];
-        }
-    };
-    side(std::integral_constant<int, 0>());
-    side(std::integral_constant<int, 1>());
-    side(std::integral_constant<int, 2>());
-    side(std::integral_constant<int, 3>());
-    st<nU>(a.pu, E, e, Ub);
-    st<nV>(a.pv, E, e, Vb);
+    auto& a = adjargs<W>();
     a.gc2[e] += c2b;
     a.gbe[e] += betab;
 }
@@ -601,8 +652,8 @@ __device__ __forceinline__ void adj_a_body(const AdjArgs& a, int e, int lane, do
 // ---------------------------------------------------------------------------
 // k_adj_b: the faces of the element (its share of each face's adjoint) and the
 // RK4 bookkeeping of the stage.
-template <class D, int s>
-__device__ __forceinline__ void adj_face(const AdjArgs& a, int e, int j, int k, const double (&U)[D::NU * D::NU],
+template <class D, int s, class KA>
+__device__ __forceinline__ void adj_face(const KA& a, int e, int j, int k, const double (&U)[D::NU * D::NU],
                                          const double (&V)[D::NV * D::NV], double c2e, double ale, double bee,
                                          double (&Ub)[D::NU * D::NU], double (&Vb)[D::NV * D::NV], double& c2b,
                                          double& betab) {
@@ -618,7 +669,7 @@ __device__ __forceinline__ void adj_face(const AdjArgs& a, int e, int j, int k, 
     traces3<D, s>(T, U, V, du, vt, dv);
     double Ab[NQ], Db[NQ], Fb[NQ];
     {
-        const double* sl = a.slot + (long)(s * 3) * NQ * E + e;
+        const double* sl = a.slotr + (long)(s * 3) * NQ * E + e;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             Ab[i] = sl[(long)(0 * NQ + i) * E];
@@ -642,7 +693,7 @@ __device__ __forceinline__ void adj_face(const AdjArgs& a, int e, int j, int k, 
         double ndu[NQ], nvt[NQ], ndv[NQ];
         traces3<D, ns>(T, nU_, nV_, ndu, nvt, ndv);
         const double c2n = a.c2[nb], aln = a.al[nb];
-        const double* sl = a.slot + (long)(ns * 3) * NQ * E + nb;
+        const double* sl = a.slotr + (long)(ns * 3) * NQ * E + nb;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             const double An = sl[(long)(0 * NQ + i) * E], Dn = sl[(long)(1 * NQ + i) * E], Fn = sl[(long)(2 * NQ + i) * E];
@@ -661,25 +712,30 @@ __device__ __forceinline__ void adj_face(const AdjArgs& a, int e, int j, int k, 
     side_trace_T<D, NV, true, s>(T, D::oE1r, dvb, Vb);
 }
 
-template <int NU, int NV>
-__device__ __forceinline__ void adj_b_body(const AdjArgs& a, int e) {
+template <int NU, int NV, int W = 0>
+__device__ __forceinline__ void adj_b_body(int e) {
     using D = Dims<NU, NV>;
     constexpr int nU = NU * NU, nV = NV * NV;
-    const int E = a.E, N = a.N, j = e % N, k = e / N;
-    const double c2e = a.c2[e], ale = a.al[e], bee = a.be[e];
+    const int E = adjargs<W>().E, N = adjargs<W>().N, j = e % N, k = e / N;
+    const double c2e = adjargs<W>().c2[e], ale = adjargs<W>().al[e], bee = adjargs<W>().be[e];
     double U[nU], V[nV], Ub[nU], Vb[nV];
-    ld<nU>(a.yu, E, e, U);
-    ld<nV>(a.yv, E, e, V);
-    ld<nU>(a.pu, E, e, Ub);
-    ld<nV>(a.pv, E, e, Vb);
+    {
+        auto& a = adjargs<W>();
+        ld<nU>(a.yu, E, e, U);
+        ld<nV>(a.yv, E, e, V);
+        ld<nU>(a.pu, E, e, Ub);
+        ld<nV>(a.pv, E, e, Vb);
+    }
     double c2b = 0.0, betab = 0.0;
-    adj_face<D, 0>(a, e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
+    adj_face<D, 0>(adjargs<W>(), e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
     phase_barrier();
-    adj_face<D, 1>(a, e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
+    adj_face<D, 1>(adjargs<W>(), e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
     phase_barrier();
-    adj_face<D, 2>(a, e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
+    adj_face<D, 2>(adjargs<W>(), e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
     phase_barrier();
-    adj_face<D, 3>(a, e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
+    adj_face<D, 3>(adjargs<W>(), e, j, k, U, V, c2e, ale, bee, Ub, Vb, c2b, betab);
+    phase_barrier();
+    auto& a = adjargs<W>();
     a.gc2[e] += c2b;
     a.gbe[e] += betab;
     // RK4 (src/solver.jl:289-321) backwards: ybar_n = ybar + sum_s Ybar_s; the
@@ -719,8 +775,8 @@ __device__ __forceinline__ void adj_b_body(const AdjArgs& a, int e) {
 // ---------------------------------------------------------------------------
 // Tangent-linear right-hand side: dF = F'(Y; c2, beta)(dY, dc2, dbeta) with
 // the forward derivative k = F(Y) given.
-template <class D, int s>
-__device__ __forceinline__ void tl_face(const AdjArgs& a, int e, int j, int k, const double (&U)[D::NU * D::NU],
+template <class D, int s, class KA>
+__device__ __forceinline__ void tl_face(const KA& a, int e, int j, int k, const double (&U)[D::NU * D::NU],
                                         const double (&V)[D::NV * D::NV], const double (&dU)[D::NU * D::NU],
                                         const double (&dV)[D::NV * D::NV], double c2e, double ale, double bee,
                                         double tc2e, double tbee, double (&Rv)[D::NV * D::NV],
@@ -792,9 +848,10 @@ __device__ __forceinline__ void tl_face(const AdjArgs& a, int e, int j, int k, c
 }
 
 template <int NU, int NV>
-__device__ __forceinline__ void tl_body(const AdjArgs& a, int e, int lane, double* Gl) {
+__device__ __forceinline__ void tl_body(int e, int lane, double* Gl) {
     using D = Dims<NU, NV>;
     constexpr int NH = D::NH, nU = NU * NU, nV = NV * NV;
+    auto& a = adjargs();
     const int E = a.E, N = a.N, j = e % N, k = e / N;
     cdbl* T = ctab(a.tab);
     cdbl* AT = ctab(a.atab);
@@ -894,7 +951,8 @@ __device__ __forceinline__ void tl_body(const AdjArgs& a, int e, int lane, doubl
 // of row j = 0 (e = N k): the trace, 1/2 (h/2) w_i (u_h - g)^2 per point, and
 // with ybu set the misfit's derivative added to the state cotangent.
 template <int NU>
-__device__ __forceinline__ void misfit_body(const AdjArgs& a, int k) {
+__device__ __forceinline__ void misfit_body(int k) {
+    auto& a = adjargs();
     using D = Dims<NU, NU>;
     constexpr int NQ = D::NQ, nU = NU * NU;
     const int E = a.E, e = a.N * k;
@@ -927,32 +985,52 @@ template <int NU, int NV>
 __global__ __launch_bounds__(kWave) void k_adj_a(const AdjArgs a) {
     using D = Dims<NU, NV>;
     __shared__ double Gl[D::NH * D::NH * kWave];
+    (void)a;
     const int e = block_first_element<false>() + (int)threadIdx.x;
-    if (e >= a.E) return;
-    adj_a_body<NU, NV>(a, e, threadIdx.x, Gl);
+    if (e >= adjargs().E) return;
+    adj_a_body<NU, NV>(e, threadIdx.x, Gl);
 }
 
 template <int NU, int NV>
 __global__ __launch_bounds__(kWave) void k_adj_b(const AdjArgs a) {
+    (void)a;
     const int e = block_first_element<false>() + (int)threadIdx.x;
-    if (e >= a.E) return;
-    adj_b_body<NU, NV>(a, e);
+    if (e >= adjargs().E) return;
+    adj_b_body<NU, NV>(e);
+}
+
+// k_adj_b of stage s, then k_adj_a of stage s - 1 for the same element: the
+// second part needs only the element's own cotangent from the first (the
+// neighbours' slot adjoints of stage s are read from slotr while stage s - 1's
+// are written to slot, a different buffer).
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_adj_ba(const AdjArgs b, const AdjArgs a) {
+    using D = Dims<NU, NV>;
+    __shared__ double Gl[D::NH * D::NH * kWave];
+    (void)a; (void)b;
+    const int e = block_first_element<false>() + (int)threadIdx.x;
+    if (e >= adjargs<0>().E) return;
+    adj_b_body<NU, NV, 0>(e);
+    phase_barrier();
+    adj_a_body<NU, NV, 1>(e, threadIdx.x, Gl);
 }
 
 template <int NU, int NV>
 __global__ __launch_bounds__(kWave) void k_tl(const AdjArgs a) {
     using D = Dims<NU, NV>;
     __shared__ double Gl[D::NH * D::NH * kWave];
+    (void)a;
     const int e = block_first_element<false>() + (int)threadIdx.x;
-    if (e >= a.E) return;
-    tl_body<NU, NV>(a, e, threadIdx.x, Gl);
+    if (e >= adjargs().E) return;
+    tl_body<NU, NV>(e, threadIdx.x, Gl);
 }
 
 template <int NU>
 __global__ __launch_bounds__(kWave) void k_misfit(const AdjArgs a) {
+    (void)a;
     const int k = blockIdx.x * kWave + (int)threadIdx.x;
-    if (k >= a.N) return;
-    misfit_body<NU>(a, k);
+    if (k >= adjargs().N) return;
+    misfit_body<NU>(k);
 }
 
 // launches of one (n_u, n_v) pair; which: 0 k_adj_a, 1 k_adj_b, 2 k_tl, 3 k_misfit
@@ -969,11 +1047,23 @@ static hipError_t launch_adj_nv(const AdjArgs& a, int which, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int NU, int NV>
+static hipError_t launch_adj2_nv(const AdjArgs& b, const AdjArgs& a, hipStream_t s) {
+    k_adj_ba<NU, NV><<<(a.E + kWave - 1) / kWave, kWave, 0, s>>>(b, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_adj_q1(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q1(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q2(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q2(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q3(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q3(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q4(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q4(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q5(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q5(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q6(const AdjArgs& a, int nv, int which, hipStream_t s);
+hipError_t launch_adj2_q6(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 
 }  // namespace wedg

@@ -10,20 +10,39 @@
 // (src/solver.jl:509-552, 716-770), with one-sided face values where
 // neighbours differ.
 //
-// The gradient runs two streams: the forward stream does the checkpointing
-// schedule (advances, snapshots, restores) and tapes the step to be reversed
-// (its stage states and stage derivatives) into one of two tape sets; the
-// adjoint stream seeds the misfit of the step's end state and runs the eight
-// adjoint kernels of the step. Events order a tape between its writer and its
-// reader, so the forward stream runs ahead of the adjoint stream by up to two
-// tapes and the two chains overlap on the GPU.
+// The gradient runs three streams, each with its own workspace: the forward
+// stream does the checkpointing schedule (advances, snapshots, restores) and
+// copies the state of each step to be reversed into one of kTapes tapes; the
+// tape stream re-runs that step keeping its stage states and stage
+// derivatives; the adjoint stream seeds the misfit of the step's end state and
+// runs the eight adjoint kernels of the step. Events order a tape between its
+// writers and its reader, so the forward stream runs ahead by up to kTapes
+// steps to be reversed and the three chains overlap on the GPU (a 64 x 64 mesh
+// fills a sixteenth of its SIMDs).
 #include "wedg_internal.h"
 #include "wedg_adjoint.h"
 
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <functional>
+#include <mutex>
+#include <thread>
 #include <math.h>
 
 namespace {
+
+hipError_t launch_adj2(const wedg::AdjArgs& b, const wedg::AdjArgs& a, int qu, int nv, hipStream_t s) {
+    switch (qu) {
+        case 1: return wedg::launch_adj2_q1(b, a, nv, s);
+        case 2: return wedg::launch_adj2_q2(b, a, nv, s);
+        case 3: return wedg::launch_adj2_q3(b, a, nv, s);
+        case 4: return wedg::launch_adj2_q4(b, a, nv, s);
+        case 5: return wedg::launch_adj2_q5(b, a, nv, s);
+        case 6: return wedg::launch_adj2_q6(b, a, nv, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_adj(const wedg::AdjArgs& a, int qu, int nv, int which, hipStream_t s) {
     switch (qu) {
@@ -64,13 +83,33 @@ long revolve_split(long n, int s) {
     return m;
 }
 
+// the state (u, v) copied in one launch
+__global__ void k_copy_state(const double* __restrict__ su, double* __restrict__ du, size_t nu,
+                             const double* __restrict__ sv, double* __restrict__ dv, size_t nv) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nu + nv; i += stride) {
+        if (i < nu) du[i] = su[i];
+        else dv[i - nu] = sv[i - nu];
+    }
+}
+
+hipError_t copy_state(const double* su, double* du, size_t nu, const double* sv, double* dv, size_t nv,
+                      hipStream_t s) {
+    const size_t n = nu + nv;
+    const int grid = (int)std::min<size_t>(1024, (n + 255) / 256);
+    k_copy_state<<<grid, 256, 0, s>>>(su, du, nu, sv, dv, nv);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+// tapes in flight between the forward and the adjoint stream
+constexpr int kTapes = 4;
 
 struct AdjTape {
     double *Yu[4] = {}, *Yv[4] = {}, *Ku[4] = {}, *Kv[4] = {};   // stage states (Y1 a copy of y_n) and F(Y_s)
     double *ynu = nullptr, *ynv = nullptr;                        // the step's end state
-    hipEvent_t taped = nullptr, used = nullptr;                   // written (forward) / read (adjoint)
-    bool in_use = false;
+    hipEvent_t copied = nullptr, taped = nullptr, used = nullptr; // y_n copied (forward), taped, reversed
 };
 
 struct wedg_adj {
@@ -84,16 +123,18 @@ struct wedg_adj {
     std::vector<double> lam, Q, K1;                                          // 1D stiffness
     std::vector<void*> allocs;
     double *dc2 = nullptr, *dal = nullptr, *dbe = nullptr, *atab = nullptr;
-    AdjTape tape[2];
+    AdjTape tape[kTapes];
     double *ybu = nullptr, *ybv = nullptr, *accu = nullptr, *accv = nullptr, *kbu = nullptr, *kbv = nullptr;
-    double *pu = nullptr, *pv = nullptr, *slot = nullptr, *gc2 = nullptr, *gbe = nullptr;
+    double *pu = nullptr, *pv = nullptr, *slot = nullptr, *slot2 = nullptr, *gc2 = nullptr, *gbe = nullptr;
     double *tdc2 = nullptr, *tdbe = nullptr;   // tangent-linear directions of the coefficients
     int* fail = nullptr;
     double* snaps = nullptr; int nsnap_cap = 0;
     double* dtgt = nullptr; long tgt_steps = 0;
     double* djp = nullptr; double* dtr = nullptr; long jp_cap = 0;
     double* pin = nullptr; size_t pin_n = 0;   // pinned staging of host data
-    hipStream_t sa = nullptr;                  // the adjoint stream
+    wedg_workspace* wt = nullptr;              // the tape stream's workspace
+    wedg_workspace* wa = nullptr;              // the adjoint stream's workspace
+    hipStream_t sa = nullptr;                  // the adjoint stream (wa's)
     wedg_adj_stats stats{};
     size_t nU() const { return (size_t)E * nu * nu; }
     size_t nV() const { return (size_t)E * nv * nv; }
@@ -313,21 +354,17 @@ AdjArgs adj_args(wedg_adj* A) {
     }
     a.tab = A->p->tab; a.atab = A->atab;
     a.c2 = A->dc2; a.al = A->dal; a.be = A->dbe;
-    a.slot = A->slot; a.pu = A->pu; a.pv = A->pv; a.gc2 = A->gc2; a.gbe = A->gbe;
+    a.slot = A->slot; a.slotr = A->slot; a.pu = A->pu; a.pv = A->pv; a.gc2 = A->gc2; a.gbe = A->gbe;
     a.ybu = A->ybu; a.ybv = A->ybv; a.accu = A->accu; a.accv = A->accv; a.nkbu = A->kbu; a.nkbv = A->kbv;
     a.fail = A->fail;
     a.kscale = 1.0;
     return a;
 }
 
-// One RK4 step of the forward stream from the workspace state y_n (time t),
-// taped: the stage states Y1 (a copy of y_n) .. Y4, the stage derivatives
-// k1 .. k4 and the end state, as the forward steps compute them
-// (src/solver.jl:289-321); the workspace state is left unchanged.
-wedg_status tape_step(wedg_adj* A, AdjTape& tp, double t, double dt, long step) {
-    wedg_workspace* ws = A->ws;
-    HIPOK(hipMemcpyAsync(tp.Yu[0], ws->yu, A->nU() * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
-    HIPOK(hipMemcpyAsync(tp.Yv[0], ws->yv, A->nV() * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
+// One RK4 step from the tape's Y1 = y_n (time t) on the stream of workspace ws,
+// taped: the stage states Y2 .. Y4, the stage derivatives k1 .. k4 and the end
+// state, as the forward steps compute them (src/solver.jl:289-321).
+wedg_status tape_step(wedg_workspace* ws, AdjTape& tp, double t, double dt, long step) {
     const double hd = 0.5 * dt, th = t + 0.5 * dt, tf = t + dt;
     const double ts[4] = {t, th, th, tf}, cs[4] = {hd, hd, dt, dt / 6.0}, tn[4] = {th, th, tf, tf};
     wedg_status st;
@@ -345,7 +382,9 @@ wedg_status tape_step(wedg_adj* A, AdjTape& tp, double t, double dt, long step) 
 }
 
 // The adjoint of one taped step on the adjoint stream: the misfit of its end
-// state (target row n) into ybar, then stages 4..1 (k_adj_a, k_adj_b each).
+// state (target row n) into ybar, then stages 4..1: k_adj_a of stage 4, three
+// launches of k_adj_b of stage s with k_adj_a of stage s - 1, and k_adj_b of
+// stage 1; the slot adjoints alternate between two buffers.
 wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
     const hipStream_t s = A->sa;
     AdjArgs a = adj_args(A);
@@ -359,16 +398,22 @@ wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
     }
     const double ca[5] = {0, 0, dt / 6.0, dt / 3.0, dt / 3.0};   // next cotangent: ca ybar + cb Ybar_s
     const double cb[5] = {0, 0, 0.5 * dt, 0.5 * dt, dt};
+    AdjArgs st[5];
     for (int stg = 4; stg >= 1; --stg) {
-        a.stage = stg;
-        a.yu = tp.Yu[stg - 1]; a.yv = tp.Yv[stg - 1];
-        a.ku = tp.Ku[stg - 1]; a.kv = tp.Kv[stg - 1];
-        if (stg == 4) { a.kbu = A->ybu; a.kbv = A->ybv; a.kscale = dt / 6.0; }
-        else { a.kbu = A->kbu; a.kbv = A->kbv; a.kscale = 1.0; }
-        a.ca = ca[stg]; a.cb = cb[stg];
-        HIPOK(launch_adj(a, A->qu, A->nv, 0, s));
-        HIPOK(launch_adj(a, A->qu, A->nv, 1, s));
+        AdjArgs& x = st[stg];
+        x = a;
+        x.stage = stg;
+        x.yu = tp.Yu[stg - 1]; x.yv = tp.Yv[stg - 1];
+        x.ku = tp.Ku[stg - 1]; x.kv = tp.Kv[stg - 1];
+        if (stg == 4) { x.kbu = A->ybu; x.kbv = A->ybv; x.kscale = dt / 6.0; }
+        else { x.kbu = A->kbu; x.kbv = A->kbv; x.kscale = 1.0; }
+        x.ca = ca[stg]; x.cb = cb[stg];
+        x.slot = (stg & 1) ? A->slot2 : A->slot;   // written by the stage's k_adj_a
+        x.slotr = x.slot;                          // read by its k_adj_b
     }
+    HIPOK(launch_adj(st[4], A->qu, A->nv, 0, s));
+    for (int stg = 4; stg >= 2; --stg) HIPOK(launch_adj2(st[stg], st[stg - 1], A->qu, A->nv, s));
+    HIPOK(launch_adj(st[1], A->qu, A->nv, 1, s));
     ++A->stats.adjoint_steps;
     return WEDG_OK;
 }
@@ -453,8 +498,10 @@ extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, 
     wedg_status st;
     if ((st = build_problem(A, &A->p))) return fail(st);
     if ((st = wedg_workspace_create(A->p, cheb_policy, &A->ws))) return fail(st);
-    hipError_t he = hipStreamCreateWithFlags(&A->sa, hipStreamNonBlocking);
-    if (he != hipSuccess) return fail(WEDG_ERR_HIP);
+    if ((st = wedg_workspace_create(A->p, cheb_policy, &A->wt))) return fail(st);
+    if ((st = wedg_workspace_create(A->p, cheb_policy, &A->wa))) return fail(st);
+    A->sa = A->wa->stream;
+    hipError_t he = hipSuccess;
     A->pin_n = std::max<size_t>((size_t)1 << 18, (size_t)A->E);
     if (hipHostMalloc((void**)&A->pin, A->pin_n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         A->pin = nullptr;
@@ -470,6 +517,7 @@ extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, 
             tp.Ku[s] = adj_alloc(A, nU, &he); tp.Kv[s] = adj_alloc(A, nV, &he);
         }
         tp.ynu = adj_alloc(A, nU, &he); tp.ynv = adj_alloc(A, nV, &he);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.copied, hipEventDisableTiming);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.taped, hipEventDisableTiming);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.used, hipEventDisableTiming);
     }
@@ -478,6 +526,7 @@ extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, 
     A->kbu = adj_alloc(A, nU, &he); A->kbv = adj_alloc(A, nV, &he);
     A->pu = adj_alloc(A, nU, &he); A->pv = adj_alloc(A, nV, &he);
     A->slot = adj_alloc(A, (size_t)12 * nq * E, &he);
+    A->slot2 = adj_alloc(A, (size_t)12 * nq * E, &he);
     A->gc2 = adj_alloc(A, E, &he); A->gbe = adj_alloc(A, E, &he);
     if (he == hipSuccess) {
         void* f = nullptr;
@@ -502,6 +551,7 @@ extern "C" wedg_status wedg_adj_destroy(wedg_adj* A) {
     if (A->sa) (void)hipStreamSynchronize(A->sa);
     if (A->ws) (void)hipStreamSynchronize(A->ws->stream);
     for (auto& tp : A->tape) {
+        if (tp.copied) (void)hipEventDestroy(tp.copied);
         if (tp.taped) (void)hipEventDestroy(tp.taped);
         if (tp.used) (void)hipEventDestroy(tp.used);
     }
@@ -511,7 +561,9 @@ extern "C" wedg_status wedg_adj_destroy(wedg_adj* A) {
     if (A->djp) (void)hipFree(A->djp);
     if (A->dtr) (void)hipFree(A->dtr);
     if (A->pin) (void)hipHostFree(A->pin);
-    if (A->sa) (void)hipStreamDestroy(A->sa);
+    if (A->wt) (void)hipStreamSynchronize(A->wt->stream);
+    if (A->wa) wedg_workspace_destroy(A->wa);
+    if (A->wt) wedg_workspace_destroy(A->wt);
     if (A->ws) wedg_workspace_destroy(A->ws);
     if (A->p) wedg_problem_destroy(A->p);
     delete A;
@@ -531,12 +583,10 @@ extern "C" wedg_status wedg_adj_set_params(wedg_adj* A, const double* c2, const 
     wedg_problem* np = nullptr;
     wedg_status st = build_problem(A, &np);
     if (st) return st;
-    if (A->ws) {
-        wedg_problem* old = A->ws->p;
-        A->ws->p = np;
-        if (old && old != A->p) wedg_problem_destroy(old);
-        if (A->p) wedg_problem_destroy(A->p);
-    }
+    if (A->ws) A->ws->p = np;
+    if (A->wa) A->wa->p = np;
+    if (A->wt) A->wt->p = np;
+    if (A->p) wedg_problem_destroy(A->p);
     A->p = np;
     if ((st = h2d(A, A->dc2, A->c2.data(), A->E))) return st;
     return h2d(A, A->dbe, A->beta.data(), A->E);
@@ -601,7 +651,7 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
         A->nsnap_cap = nsnap;
     }
     std::memset(&A->stats, 0, sizeof(A->stats));
-    if ((st = reset_err(ws))) return st;
+    if ((st = reset_err(ws)) || (st = reset_err(A->wt)) || (st = reset_err(A->wa))) return st;
     HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), fs));
     HIPOK(hipMemsetAsync(ws->yu, 0, nU * sizeof(double), fs));
     HIPOK(hipMemsetAsync(ws->yv, 0, nV * sizeof(double), fs));
@@ -609,44 +659,102 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
         (st = zero_dev(fs, A->ybv, nV)))
         return st;
     HIPOK(hipStreamSynchronize(fs));
-    for (auto& tp : A->tape) tp.in_use = false;
     // times after n steps, accumulated as the forward loop does (src/solver.jl:213-216)
     std::vector<double> tt(nsteps + 1);
     tt[0] = 0.0;
     for (long n = 0; n < nsteps; ++n) tt[n + 1] = tt[n] + dt;
-    auto save = [&](int slot) -> wedg_status {
-        double* d = A->snaps + (size_t)slot * sz;
-        HIPOK(hipMemcpyAsync(d, ws->yu, nU * sizeof(double), hipMemcpyDeviceToDevice, fs));
-        HIPOK(hipMemcpyAsync(d + nU, ws->yv, nV * sizeof(double), hipMemcpyDeviceToDevice, fs));
-        ++A->stats.snapshots_taken;
-        return WEDG_OK;
-    };
-    auto restore = [&](int slot) -> wedg_status {
-        const double* d = A->snaps + (size_t)slot * sz;
-        HIPOK(hipMemcpyAsync(ws->yu, d, nU * sizeof(double), hipMemcpyDeviceToDevice, fs));
-        HIPOK(hipMemcpyAsync(ws->yv, d + nU, nV * sizeof(double), hipMemcpyDeviceToDevice, fs));
-        return WEDG_OK;
-    };
-    auto advance = [&](long a, long m) -> wedg_status {
+    // The current state lives in a snapshot slot (after a restore) or in the
+    // workspace state; an advance reads it where it is and writes its last step
+    // into the workspace state or straight into the snapshot slot it is saved in.
+    auto snap_u = [&](int slot) { return A->snaps + (size_t)slot * sz; };
+    auto snap_v = [&](int slot) { return A->snaps + (size_t)slot * sz + nU; };
+    const double* cur_u = snap_u(0);
+    const double* cur_v = snap_v(0);
+    auto restore = [&](int slot) { cur_u = snap_u(slot); cur_v = snap_v(slot); };
+    auto advance = [&](long a, long m, int save_slot) -> wedg_status {
         wedg_status s2;
         for (long i = a; i < m; ++i) {
-            if ((s2 = rk4_device(ws, tt[i], dt, i + 1, i == a, i == m - 1))) return s2;
+            const bool into_slot = i == m - 1 && save_slot >= 0;
+            double* ou = into_slot ? snap_u(save_slot) : ws->yu;
+            double* ov = into_slot ? snap_v(save_slot) : ws->yv;
+            if ((s2 = rk4_step_io(ws, cur_u, cur_v, ou, ov, tt[i], dt, i + 1, i == a, i == m - 1))) return s2;
+            cur_u = ou;
+            cur_v = ov;
             ++A->stats.forward_steps;
         }
+        if (save_slot >= 0) ++A->stats.snapshots_taken;
         return WEDG_OK;
     };
-    // reverse step n (the workspace holds y_n): tape it on the forward stream, reverse it on the adjoint stream
-    auto leaf = [&](long n) -> wedg_status {
-        AdjTape& tp = A->tape[n & 1];
-        if (tp.in_use) HIPOK(hipStreamWaitEvent(fs, tp.used, 0));
+    // reverse step n (the current state is y_n): the forward stream copies y_n into a
+    // free tape, the tape stream re-runs the step keeping its stages, the adjoint
+    // stream reverses it
+    const hipStream_t ts = A->wt->stream;
+    //
+    // A second host thread issues the tape and adjoint streams' work from a queue
+    // of steps, so that a full forward queue, which blocks the thread issuing
+    // into it, never holds back the work the forward stream may be waiting for.
+    struct Job { long n; int tape; };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job> jobs;
+    bool closing = false;
+    long issued = 0;                     // jobs whose work the second thread has issued
+    wedg_status worker_status = WEDG_OK;
+    std::vector<long> tape_job(kTapes, -1);   // the job that last used each tape
+    int device = 0;
+    HIPOK(hipGetDevice(&device));
+    auto issue = [&](const Job& jb) -> wedg_status {
+        AdjTape& tp = A->tape[jb.tape];
+        HIPOK(hipStreamWaitEvent(ts, tp.copied, 0));
         wedg_status s2;
-        if ((s2 = tape_step(A, tp, tt[n], dt, n + 1))) return s2;
-        ++A->stats.taped_steps;
-        HIPOK(hipEventRecord(tp.taped, fs));
+        if ((s2 = tape_step(A->wt, tp, tt[jb.n], dt, jb.n + 1))) return s2;
+        HIPOK(hipEventRecord(tp.taped, ts));
         HIPOK(hipStreamWaitEvent(as, tp.taped, 0));
-        if ((s2 = adjoint_step(A, tp, n, dt))) return s2;
+        if ((s2 = adjoint_step(A, tp, jb.n, dt))) return s2;
         HIPOK(hipEventRecord(tp.used, as));
-        tp.in_use = true;
+        return WEDG_OK;
+    };
+    std::thread worker([&]() {
+        if (hipSetDevice(device) != hipSuccess) worker_status = WEDG_ERR_HIP;
+        for (;;) {
+            Job jb;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return closing || !jobs.empty(); });
+                if (jobs.empty()) return;
+                jb = jobs.front();
+                jobs.pop_front();
+            }
+            wedg_status s2 = worker_status == WEDG_OK ? issue(jb) : worker_status;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (s2 != WEDG_OK && worker_status == WEDG_OK) worker_status = s2;
+                ++issued;
+            }
+            cv.notify_all();
+        }
+    });
+    long pushed = 0;
+    auto leaf = [&](long n) -> wedg_status {
+        const int ti = (int)(n % kTapes);
+        AdjTape& tp = A->tape[ti];
+        if (tape_job[ti] >= 0) {
+            // the tape's last use must have been issued (its `used` recorded) before the wait
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return issued > tape_job[ti]; });
+            lk.unlock();
+            HIPOK(hipStreamWaitEvent(fs, tp.used, 0));
+        }
+        HIPOK(copy_state(cur_u, tp.Yu[0], nU, cur_v, tp.Yv[0], nV, fs));
+        HIPOK(hipEventRecord(tp.copied, fs));
+        ++A->stats.taped_steps;
+        tape_job[ti] = pushed;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            jobs.push_back(Job{n, ti});
+            ++pushed;
+        }
+        cv.notify_all();
         return WEDG_OK;
     };
     std::vector<int> free_slots;
@@ -658,28 +766,42 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
         const int sfree = (int)free_slots.size();
         if (sfree == 0) {
             for (long i = b - 1; i >= a; --i) {
-                if (i < b - 1 && (s2 = restore(slot_a))) return s2;
-                if ((s2 = advance(a, i))) return s2;
+                restore(slot_a);
+                if ((s2 = advance(a, i, -1))) return s2;
                 if ((s2 = leaf(i))) return s2;
             }
             return WEDG_OK;
         }
         const long m = a + revolve_split(b - a, sfree);
-        if ((s2 = advance(a, m))) return s2;
         const int sm = free_slots.back();
         free_slots.pop_back();
         max_used = std::max(max_used, nsnap - (int)free_slots.size());
-        if ((s2 = save(sm))) return s2;
+        if ((s2 = advance(a, m, sm))) return s2;
         if ((s2 = reverse(m, b, sm))) return s2;
         free_slots.push_back(sm);
-        if ((s2 = restore(slot_a))) return s2;
+        restore(slot_a);
         return reverse(a, m, slot_a);
     };
-    if ((st = save(0))) return st;
-    if ((st = reverse(0, nsteps, 0))) return st;
+    const auto c0 = std::chrono::steady_clock::now();
+    HIPOK(hipMemsetAsync(snap_u(0), 0, sz * sizeof(double), fs));   // y_0 = 0 in slot 0
+    ++A->stats.snapshots_taken;
+    st = reverse(0, nsteps, 0);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        closing = true;
+    }
+    cv.notify_all();
+    worker.join();
+    if (st) return st;
+    if (worker_status) return worker_status;
     A->stats.snapshots_used = max_used;
+    const auto c1 = std::chrono::steady_clock::now();
     HIPOK(hipStreamSynchronize(as));
-    if ((st = collect(ws, 1))) return st;
+    HIPOK(hipStreamSynchronize(fs));
+    const auto c2 = std::chrono::steady_clock::now();
+    A->stats.enqueue_ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
+    A->stats.total_ms = std::chrono::duration<double, std::milli>(c2 - c0).count();
+    if ((st = collect(ws, 1)) || (st = collect(A->wt, 1))) return st;
     if ((st = read_fail(A))) return st;
     if ((st = sum_misfit(A, nsteps, J))) return st;
     if ((st = d2h(A, dJ_dc2, A->gc2, A->E))) return st;

@@ -575,6 +575,27 @@ wedg_status wedg::rk4_device(wedg_workspace* ws, double t, double dt, long step,
     return WEDG_OK;
 }
 
+// One RK4 step from the state (inu, inv) into (outu, outv), the same stages as
+// rk4_device with the input and output anywhere (the output may be the input:
+// the last stage reads only its own element of it).
+wedg_status wedg::rk4_step_io(wedg_workspace* ws, const double* inu, const double* inv, double* outu, double* outv,
+                              double t, double dt, long step, bool first, bool last) {
+    const double hd = 0.5 * dt, th = t + 0.5 * dt, tf = t + dt;
+    wedg_status st;
+    for (int s = 1; s <= 4; ++s) {
+        KArgs a;
+        switch (s) {
+            case 1: a = stage_args(ws, 1, inu, inv, inu, inv, ws->su, ws->sv, t, hd, step, true, th); break;
+            case 2: a = stage_args(ws, 2, ws->su, ws->sv, inu, inv, ws->ou, ws->ov, th, hd, step, true, th); break;
+            case 3: a = stage_args(ws, 3, ws->ou, ws->ov, inu, inv, ws->su, ws->sv, th, dt, step, true, tf); break;
+            default: a = stage_args(ws, 4, ws->su, ws->sv, inu, inv, outu, outv, tf, dt / 6.0, step, !last, tf);
+        }
+        if (s == 1 && first && (st = run_kernel(ws, a, 0)) != WEDG_OK) return st;
+        if ((st = run_kernel(ws, a, 1)) != WEDG_OK) return st;
+    }
+    return WEDG_OK;
+}
+
 // Reads the failure record and maps it to a status (boundary before mass,
 // src/rhs.jl:530-531).
 wedg_status wedg::collect(wedg_workspace* ws, int nonfinite_is_error) {

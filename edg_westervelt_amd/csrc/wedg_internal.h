@@ -168,5 +168,7 @@ KArgs stage_args(wedg_workspace* ws, int stage, const double* yu, const double* 
                  const double* y0v, double* outu, double* outv, double t, double c, long step, bool has_next,
                  double t_next);
 wedg_status rk4_device(wedg_workspace* ws, double t, double dt, long step, bool first, bool last);
+wedg_status rk4_step_io(wedg_workspace* ws, const double* inu, const double* inv, double* outu, double* outv,
+                        double t, double dt, long step, bool first, bool last);
 
 }  // namespace wedg

@@ -296,6 +296,7 @@ typedef struct wedg_adj wedg_adj;
 typedef struct wedg_adj_stats {
     long forward_steps, taped_steps, adjoint_steps, snapshots_taken;
     int snapshots_used, pad;
+    double enqueue_ms, total_ms;   /* host time to issue the run, and to its end */
 } wedg_adj_stats;
 
 /* cheb_policy of the forward steps as in wedg_workspace_create. The
