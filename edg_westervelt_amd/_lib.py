@@ -18,7 +18,8 @@ LIB_PATH = os.environ.get("WEDG_LIBRARY") or os.path.join(_HERE, "libwedg.so")
 KINDS = {"absorbing_2beta": 0, "absorbing": 1, "dirichlet": 2, "wall": 3}
 SIDES = ("bot", "top", "lft", "rgt")
 
-OK, ERR_BOUNDARY_TRACE, ERR_BOUNDARY_DATA, ERR_MASS, ERR_DIMENSION, ERR_NONFINITE, ERR_HIP, ERR_ARGUMENT = range(8)
+(OK, ERR_BOUNDARY_TRACE, ERR_BOUNDARY_DATA, ERR_MASS, ERR_DIMENSION, ERR_NONFINITE, ERR_HIP, ERR_ARGUMENT,
+ ERR_SCRATCH) = range(9)
 CHEB_GLOBAL, CHEB_ELEMENT = 0, 1
 
 _dp = C.POINTER(C.c_double)
@@ -80,6 +81,7 @@ def _load():
         "wedg_workspace_destroy": [vp],
         "wedg_rhs": [vp, _dp, _dp, C.c_double, C.c_int, C.c_double, _dp, _dp],
         "wedg_rhs_device": [vp, vp, vp, C.c_double, vp, vp],
+        "wedg_rhs_noalloc": [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp, vp],
         "wedg_rk4_step": [vp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.POINTER(C.c_int)],
         "wedg_set_state": [vp, _dp, _dp],
         "wedg_get_state": [vp, _dp, _dp],

@@ -437,12 +437,12 @@ extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     return WEDG_OK;
 }
 
-KArgs wedg::base_args(wedg_workspace* ws) {
-    wedg_problem* p = ws->p;
+KArgs wedg::problem_args(const wedg_problem* p, int cheb_global, double* bsl, DevErr* err, int* cnt,
+                         unsigned* wstat) {
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.N = p->N; a.E = p->E;
-    a.cheb_global = p->mass_cg ? 0 : ws->cheb_global;   // no counts with conjugate gradients
+    a.cheb_global = p->mass_cg ? 0 : cheb_global;   // no counts with conjugate gradients
     a.mass_cg = p->mass_cg;
     a.mass_tol = p->mass_tol;
     a.one_sided = p->one_sided;
@@ -460,8 +460,12 @@ KArgs wedg::base_args(wedg_workspace* ws) {
     for (int i = 0; i < F_COUNT; ++i) { a.f[i] = p->f[i]; a.fu[i] = p->fu[i]; }
     a.fstride = p->N * (p->N + 1);
     for (int i = 0; i < 8; ++i) a.bcd[i] = p->bcd[i];
-    a.bsl = ws->bsl; a.err = ws->err; a.cnt = ws->cnt; a.wstat = ws->wstat;
+    a.bsl = bsl; a.err = err; a.cnt = cnt; a.wstat = wstat;
     return a;
+}
+
+KArgs wedg::base_args(wedg_workspace* ws) {
+    return problem_args(ws->p, ws->cheb_global, ws->bsl, ws->err, ws->cnt, ws->wstat);
 }
 
 // Time factors of the stage at time t and the projection coefficients, in the
@@ -631,28 +635,7 @@ wedg_status wedg::collect(wedg_workspace* ws, int nonfinite_is_error) {
         L.cg_unconverged = h.cg_unconverged;
         ws->cheb_wave_sum = sum;
     }
-    const int nq = ws->p->nu, N = ws->p->N;
-    if (h.fkey != ~0ull) {
-        const int kind = (int)((h.fkey >> 32) & 1u);
-        const long key = (long)(h.fkey & 0xffffffffull);
-        if (kind == 0) {   // a boundary weight (src/rhs.jl:112-116)
-            L.status = (key % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
-            L.key = key;
-            L.side = (int)((key - 1) / (2 * nq * N));
-            return (wedg_status)L.status;
-        }
-        L.status = WEDG_ERR_MASS;   // src/rhs.jl:916
-        L.element = key;
-        return WEDG_ERR_MASS;
-    }
-    if (h.nonfinite != INT_MAX) {
-        L.step = h.nonfinite;
-        if (nonfinite_is_error) {
-            L.status = WEDG_ERR_NONFINITE;
-            return WEDG_ERR_NONFINITE;
-        }
-    }
-    return WEDG_OK;
+    return (wedg_status)map_failure(h, ws->p->nu, ws->p->N, nonfinite_is_error, L);
 }
 
 extern "C" wedg_status wedg_rhs_device(wedg_workspace* ws, const double* u, const double* v, double t,

@@ -60,6 +60,32 @@ void dedup(long E, size_t len, F rec, std::vector<int>& cls, std::vector<double>
     }
 }
 
+// The failure record of a run mapped to the error it reports: the first
+// failure in serial order, a boundary weight before the mass matrix of the same
+// stage (src/rhs.jl:530-531); a non-finite step is an error only when asked.
+// Sets status, key, side, element and step of L; returns the status.
+__host__ __device__ inline int map_failure(const DevErr& h, int nq, int N, int nonfinite_is_error,
+                                           wedg_error_info& L) {
+    if (h.fkey != ~0ull) {
+        const int kind = (int)((h.fkey >> 32) & 1u);
+        const long key = (long)(h.fkey & 0xffffffffull);
+        if (kind == 0) {   // a boundary weight (src/rhs.jl:112-116)
+            L.status = (key % 2 == 1) ? WEDG_ERR_BOUNDARY_TRACE : WEDG_ERR_BOUNDARY_DATA;
+            L.key = key;
+            L.side = (int)((key - 1) / (2 * nq * N));
+            return L.status;
+        }
+        L.status = WEDG_ERR_MASS;   // src/rhs.jl:916
+        L.element = key;
+        return L.status;
+    }
+    if (h.nonfinite != INT_MAX) {
+        L.step = h.nonfinite;
+        if (nonfinite_is_error) L.status = WEDG_ERR_NONFINITE;
+    }
+    return L.status;
+}
+
 void sym_eigen(std::vector<double> A, int n, std::vector<double>& lam, std::vector<double>& V);
 void stiffness_spectrum(const double* S, const double* w, int nu, std::vector<double>& Q,
                         std::vector<double>& lam);
@@ -161,6 +187,7 @@ namespace wedg {
 hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s);
 wedg_status reset_err(wedg_workspace* ws);
 wedg_status collect(wedg_workspace* ws, int nonfinite_is_error);
+KArgs problem_args(const wedg_problem* p, int cheb_global, double* bsl, DevErr* err, int* cnt, unsigned* wstat);
 KArgs base_args(wedg_workspace* ws);
 void stage_time(const wedg_problem* p, double t, KArgs& a);
 wedg_status run_kernel(wedg_workspace* ws, const KArgs& a, int which);

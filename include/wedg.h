@@ -44,6 +44,7 @@ typedef enum wedg_status {
     WEDG_ERR_NONFINITE = 5,       /* non-finite state after a step (src/solver.jl:318, 940) */
     WEDG_ERR_HIP = 6,             /* HIP runtime error */
     WEDG_ERR_ARGUMENT = 7,        /* invalid argument or unsupported option */
+    WEDG_ERR_SCRATCH = 8,         /* the problem exceeds the static scratch of wedg_rhs_noalloc */
 } wedg_status;
 
 /* Boundary kinds, per side (src/solver.jl:37). */
@@ -122,6 +123,25 @@ wedg_status wedg_rhs(wedg_workspace* ws, const double* u, const double* v, doubl
  * workspace stream; errors of the evaluation are reported by wedg_sync. */
 wedg_status wedg_rhs_device(wedg_workspace* ws, const double* u, const double* v, double t,
                             double* ut, double* vt);
+
+/* wedg_rhs_device without a workspace, allocating nothing: no hipMalloc, no
+ * hipFree, no allocating library routine. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The stage
+ * kernels' scratch (boundary face slots, failure record, per-wave Chebyshev
+ * counts) is one static device array of WEDG_NOALLOC_SCRATCH_BYTES on every
+ * GPU the library runs on, held until the process ends; problems with
+ * N > WEDG_NOALLOC_MAX_N return WEDG_ERR_SCRATCH. u, v, ut, vt are device
+ * arrays of the current device, which must be the problem's. Asynchronous on
+ * `stream` (a hipStream_t; NULL is the null stream). When info is not NULL it
+ * is DEVICE memory that receives the call's failure record (status WEDG_OK,
+ * the largest Chebyshev count) once the stream reaches it. Every call on one
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+#define WEDG_NOALLOC_MAX_N 1024
+#define WEDG_NOALLOC_SCRATCH_BYTES 950528
+wedg_status wedg_rhs_noalloc(const wedg_problem* p, int cheb_policy, const double* u, const double* v,
+                             double t, double* ut, double* vt, wedg_error_info* info, void* stream);
 
 /* rk4_step!: one classic RK4 step in place on host arrays (src/solver.jl:289-321).
  * *finite receives all(ws.fin). */
