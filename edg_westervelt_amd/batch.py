@@ -65,6 +65,8 @@ class Batch:
         status = np.zeros(n, dtype=np.int32)
         st = lib.wedg_batch_advance(self.h, _ptr(t0), _ptr(dt), ns, int(stop_on_nonfinite), _ptr(tend),
                                     status.ctypes.data_as(C.POINTER(C.c_int)))
+        if st != _lib.OK and not np.any(status != _lib.OK):   # the call ended before any instance's record
+            raise WedgError(f"wedg_batch_advance failed with status {st}")
         if raise_errors:
             for i, w in enumerate(self.workspaces):
                 if status[i] != _lib.OK:
@@ -335,8 +337,10 @@ def run_source_many(runs, cheb: str = "global"):
         tend = np.zeros(n)
         status = np.zeros(n, dtype=np.int32)
         ns = (C.c_long * n)(*steps)
-        lib.wedg_batch_advance_receivers(b.h, _ptr(t0), _ptr(dts), ns, 1, _ptr(tend),
-                                         status.ctypes.data_as(C.POINTER(C.c_int)), ptrs)
+        st = lib.wedg_batch_advance_receivers(b.h, _ptr(t0), _ptr(dts), ns, 1, _ptr(tend),
+                                              status.ctypes.data_as(C.POINTER(C.c_int)), ptrs)
+        if st != _lib.OK and not np.any(status != _lib.OK):   # the call ended before any instance's record
+            raise WedgError(f"wedg_batch_advance_receivers failed with status {st}")
         for i, S in enumerate(setups):
             if status[i] == _lib.ERR_NONFINITE:
                 raise _blow_up(S)

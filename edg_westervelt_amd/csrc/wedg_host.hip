@@ -348,7 +348,9 @@ wedg_status wedg::reset_err(wedg_workspace* ws) {
     std::memset(&h, 0, sizeof(h));
     h.fkey = ~0ull;
     h.nonfinite = INT_MAX;
+    h.nf_serial = INT_MAX;
     ws->stages = 0;
+    ws->serial = 0;   // k_prep forms the count slot anew at the start of every call
     HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
     HIPOK(hipMemsetAsync(ws->wstat, 0, 2 * ws->nwaves() * sizeof(unsigned), ws->stream));
     return WEDG_OK;

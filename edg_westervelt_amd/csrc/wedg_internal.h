@@ -62,11 +62,15 @@ void dedup(long E, size_t len, F rec, std::vector<int>& cls, std::vector<double>
 
 // The failure record of a run mapped to the error it reports: the first
 // failure in serial order, a boundary weight before the mass matrix of the same
-// stage (src/rhs.jl:530-531); a non-finite step is an error only when asked.
-// Sets status, key, side, element and step of L; returns the status.
+// stage (src/rhs.jl:530-531). A non-finite state is an error only when asked
+// (the blow-up check of src/solver.jl:940, after the step): then it comes
+// before every failure of a later stage, which the device records as it runs
+// on from the non-finite state. Sets status, key, side, element and step of L;
+// returns the status.
 __host__ __device__ inline int map_failure(const DevErr& h, int nq, int N, int nonfinite_is_error,
                                            wedg_error_info& L) {
-    if (h.fkey != ~0ull) {
+    const bool blown = nonfinite_is_error && h.nonfinite != INT_MAX;
+    if (h.fkey != ~0ull && !(blown && (long long)(h.fkey >> 33) > (long long)h.nf_serial)) {
         const int kind = (int)((h.fkey >> 32) & 1u);
         const long key = (long)(h.fkey & 0xffffffffull);
         if (kind == 0) {   // a boundary weight (src/rhs.jl:112-116)

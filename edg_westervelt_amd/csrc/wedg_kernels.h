@@ -98,7 +98,8 @@ struct DevErr {
     unsigned long long fkey;
     int nonfinite;     // first RK4 step with a non-finite entry (INT_MAX = none)
     int cg_unconverged;   // element systems conjugate gradients left above the tolerance
-    int pad[4];
+    int nf_serial;     // serial of the stage that made the first non-finite state (INT_MAX = none)
+    int pad[3];
 };
 
 enum { F_HC2 = 0, F_HAF, F_HCF, F_HBF, F_VC2, F_VAF, F_VCF, F_VBF,
@@ -1172,8 +1173,10 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
             }
             double yu0[nU], acu[nU];
             rk_load<nU>(A.stage, A.y0u, A.accu, E, e, yu0, acu);
-            if (!rk_store<nU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, e))
+            if (!rk_store<nU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, e)) {
                 atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
+                atomicMin(&kargs<D::BT>()->err->nf_serial, kargs<D::BT>()->serial);
+            }
         }
 
         phase_barrier();
@@ -1358,8 +1361,10 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
                 for (int m = 0; m < nV; ++m) rv[(long)m * E + e] = X[m];
             }
             rk_load<nV>(stage, A.y0v, A.accv, E, e, yv0, acv);
-            if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e))
+            if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e)) {
                 atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
+                atomicMin(&kargs<D::BT>()->err->nf_serial, kargs<D::BT>()->serial);
+            }
             if (A.has_next) {
                 // the new state v, as rk_store wrote it; u is read back from outu
                 double Vn[nV];

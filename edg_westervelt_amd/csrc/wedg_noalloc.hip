@@ -67,6 +67,7 @@ __global__ void k_noalloc_clear(DevErr* err, unsigned* wstat, int n) {
         DevErr z = {};
         z.fkey = ~0ull;
         z.nonfinite = INT_MAX;
+        z.nf_serial = INT_MAX;
         *err = z;
     }
     for (int k = i; k < n; k += gridDim.x * blockDim.x) wstat[k] = 0u;

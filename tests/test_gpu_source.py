@@ -288,3 +288,81 @@ def test_norms_after_receivers_on_one_workspace():
     assert t3.value == t2.value
     assert errs(ws, t2.value) == errs(ref, t3.value)
     del ws
+
+
+@pytest.mark.parametrize("cheb", ["global", "element"])
+def test_source_blow_up_inside_a_record_interval(cheb):
+    """beta = 0 and a time step 200 times the stable one (dt_fac = 0.05): the
+    state overflows at a step inside a record interval (record_every = 20). The
+    run stops with the reference's "blow-up at t=..." (src/solver.jl:940), t
+    after that step as the C oracle's RK4 loop finds it, and not with the mass
+    failure the non-finite state gives in the next stage, which the device
+    records as it runs on to the end of the interval."""
+    from edg_westervelt_amd.solver import time_step
+    q, N, T, dtf = 2, 4, 400.0, 0.05
+    lin = w.constant_coefficients(c=1.0, alpha=1e-4, beta=0.0)
+    pre, _ = precompute(q, N, mixed=True, mode="source", coefficients=lin)
+    dt, ns = time_step(q, N, T, dtf)
+    u0 = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+    v0 = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+    _, _, _, ost = ob.rk4_steps(ob.OraclePre(pre, N), u0, v0, 0.0, dt, ns, cheb_mode=0 if cheb == "global" else 1,
+                                check=False)
+    assert ost.code == _lib.ERR_NONFINITE and ost.step % 20 != 0
+    tb = 0.0
+    for _ in range(ost.step):
+        tb += dt
+    say(f"case blow-up {cheb}: oracle step {ost.step}, t={tb!r}")
+    with pytest.raises(w.WedgError) as e:
+        w.solve(q, N, T, mixed=True, mode="source", coefficients=lin, dt_fac=dtf, cheb=cheb)
+    assert str(e.value) == f"blow-up at t={tb!r}"
+
+
+def _bottom_beta(f, b0=20.0, ell=0.04):
+    """beta = b0 (1 + f exp(-y / ell)): larger at the bottom face points than at
+    the volume points near them, so the boundary weight and the mass matrix
+    fail at different stages as f varies."""
+    from edg_westervelt_amd.coefficients import Coefficients
+
+    def z0(x, y):
+        return 0.0 * x
+
+    return Coefficients(lambda x, y: 1.0 + 0 * x, z0, z0, lambda x, y: 1e-3 + 0 * x, z0, z0,
+                        lambda x, y: b0 * (1 + f * np.exp(-y / ell)))
+
+
+# f: the failure the oracle's RK4 finds first (step, stage found on the CPU)
+MIDSTEP = [(0.2, "mass, step 8 stage 2"), (1.0, "boundary, step 5 stage 4"),
+           (0.6, "boundary and mass at the same stage, step 7 stage 2")]
+
+
+@pytest.mark.parametrize("cheb", ["global", "element"])
+@pytest.mark.parametrize("f,what", MIDSTEP)
+def test_failure_inside_an_advance(f, what, cheb):
+    """v grows from rest under the u0 forcing of :initial_value mode (a bump
+    on the bottom side) until a boundary weight or a mass matrix fails at a
+    stage 2-4 of a step after the first. wedg_advance reports the failure the
+    C oracle's rk4 loop reports: status, side, key, element; where both fail
+    at one stage, the boundary weight (src/rhs.jl:530-531)."""
+    from edg_westervelt_amd.batch import gaussian_profile
+    q, N = 3, 6
+    pre, _ = precompute(q, N, mixed=True, mode="initial_value", coefficients=_bottom_beta(f),
+                        boundary=dict(bot="absorbing_2beta", top="wall", lft="wall", rgt="wall"),
+                        initial_profile=gaussian_profile(0.5, 0.0, 0.12, 0.05))
+    dt, ns = 1.0 / (N * 10 * q * q), 20
+    u0 = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+    v0 = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+    _, _, _, ost = ob.rk4_steps(ob.OraclePre(pre, N), u0, v0, 0.0, dt, ns, cheb_mode=0 if cheb == "global" else 1,
+                                check=False)
+    say(f"case failure inside an advance f={f} ({what}) {cheb}: oracle code {ost.code}")
+    assert ost.code in (_lib.ERR_BOUNDARY_TRACE, _lib.ERR_MASS)
+    ws = w.Workspace(pre, N, cheb=cheb)
+    ws.dev.check(lib.wedg_set_state(ws.dev.h, u0.ctypes.data_as(C.POINTER(C.c_double)),
+                                    v0.ctypes.data_as(C.POINTER(C.c_double))))
+    tend = C.c_double(0.0)
+    st = lib.wedg_advance(ws.dev.h, 0.0, dt, ns, 0, C.byref(tend))
+    info = ws.dev.last_error()
+    assert st == ost.code
+    if st == _lib.ERR_MASS:
+        assert info.element == ost.elem
+    else:
+        assert (info.side, info.key) == (ost.side, ost.key)
