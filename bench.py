@@ -67,6 +67,17 @@ def build_problem(q, N):
     return pre, geo
 
 
+def cpu_threads():
+    """The host threads of the CPU baseline: every CPU this process may run on,
+    capped by OMP_NUM_THREADS where the machine sets it (a one-GPU share of a
+    shared host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
 def flops_per_element_stage(nu, nv, nh, nc):
     """Algorithmic FP64 flops of one element and stage: traces, volume, u-solve,
     lifts, Chebyshev set-up and nc - 1 Chebyshev steps (the plain sum-factorized
@@ -246,7 +257,7 @@ def main():
                          "limit near t = 0.131)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cheb", choices=("global", "element"), default="global")
-    ap.add_argument("--cpu-steps", type=int, default=64, help="RK4 steps of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=192, help="RK4 steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
@@ -398,7 +409,7 @@ def main():
     if args.cpu_steps > 0:
         from oracle import batched as ob
         ob.build() if not os.path.exists(ob._SO) else None
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         op = ob.OraclePre(pre, N)
         print(f"# cpu oracle {args.cpu_steps} steps on {threads} threads", file=sys.stderr, flush=True)
         c0 = time.perf_counter()
@@ -414,7 +425,8 @@ def main():
                data="synthetic",
                config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, c=1 alpha=1e-4 beta=2, "
                                     f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
-                           elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, full_run_steps=nsteps_full,
+                           elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, mean_cheb_steps=nc_timing,
+                           max_cheb_steps=info.cheb_steps, full_run_steps=nsteps_full,
                            timed_window=[t0, t0 + steps * dt],
                            note="the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "
                                 "timed window is the part it completes",
