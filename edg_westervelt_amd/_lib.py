@@ -119,6 +119,12 @@ def _load():
         "wedg_adj_last_stats": [vp, C.POINTER(WedgAdjStats)],
         "wedg_adj_rhs_vjp": [vp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
         "wedg_adj_rhs_tl": [vp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
+        "wedg_adj_batch_create": [C.POINTER(vp), C.c_int, C.POINTER(vp)],
+        "wedg_adj_batch_destroy": [vp],
+        "wedg_adj_batch_forward": [vp, _dp, C.POINTER(C.c_long), _dp, _ip],
+        "wedg_adj_batch_gradient": [vp, _dp, C.POINTER(C.c_long), C.c_int, _dp, C.POINTER(_dp), C.POINTER(_dp),
+                                    _ip],
+        "wedg_adj_batch_time": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

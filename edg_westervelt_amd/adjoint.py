@@ -31,7 +31,7 @@ from ._lib import WedgError, lib
 from .coefficients import constant_coefficients
 from .precompute import GaussianSource, precompute
 
-__all__ = ["SourceAdjoint", "misfit_rrule", "beta_field_workload"]
+__all__ = ["SourceAdjoint", "AdjointBatch", "misfit_rrule", "beta_field_workload", "steepening_batch_workload"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -217,3 +217,128 @@ def beta_field_workload(N: int = 64, q: int = 4, steps: int = 2000, seed: int = 
     c2 = np.ones((N, N), order="F")
     adj.set_params(c2, beta)
     return SimpleNamespace(adj=adj, c2=c2, beta=np.asfortranarray(beta), dt=dt, steps=steps, N=N, q=q)
+
+
+class AdjointBatch:
+    """Forward runs and gradients of several ``SourceAdjoint`` instances
+    together (``wedg_adj_batch_*``): lockstep rounds on one stream, one launch
+    per RK stage and per adjoint kernel for each group of instances with the
+    same degrees. Every instance gets exactly its own run's results. The
+    instances keep their coefficients and targets."""
+
+    def __init__(self, instances):
+        self.instances = list(instances)
+        n = len(self.instances)
+        if n < 1:
+            raise ValueError("an adjoint batch needs at least one instance")
+        arr = (C.c_void_p * n)(*[a.h_.value for a in self.instances])
+        h = C.c_void_p()
+        st = lib.wedg_adj_batch_create(arr, n, C.byref(h))
+        if st != _lib.OK:
+            raise WedgError(f"wedg_adj_batch_create failed with status {st}")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib.wedg_adj_batch_destroy(h)
+            self.h = None
+
+    def _steps(self, dts, nsteps):
+        n = len(self.instances)
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (n,)))
+        ns = (C.c_long * n)(*[int(k) for k in np.broadcast_to(np.asarray(nsteps), (n,))])
+        return dts, ns
+
+    def _raise(self, st, status, what):
+        for a, si in zip(self.instances, status):
+            if si != _lib.OK:
+                a._check(int(si), what)
+        if st != _lib.OK:
+            raise WedgError(f"{what} failed with status {st}")
+
+    def forward(self, dts, nsteps):
+        """J of every instance after nsteps[i] steps of dts[i]."""
+        n = len(self.instances)
+        dts, ns = self._steps(dts, nsteps)
+        J = np.zeros(n)
+        status = np.zeros(n, dtype=np.int32)
+        st = lib.wedg_adj_batch_forward(self.h, _ptr(dts), ns, _ptr(J), status.ctypes.data_as(C.POINTER(C.c_int)))
+        self._raise(st, status, "batch forward")
+        return J
+
+    def gradient(self, dts, nsteps, snapshots: int = 32):
+        """[(J, dJ/dc², dJ/dβ)] of every instance."""
+        n = len(self.instances)
+        dts, ns = self._steps(dts, nsteps)
+        J = np.zeros(n)
+        gc2 = [np.zeros((a.N, a.N), order="F") for a in self.instances]
+        gb = [np.zeros((a.N, a.N), order="F") for a in self.instances]
+        pc = (_dp * n)(*[_ptr(g) for g in gc2])
+        pb = (_dp * n)(*[_ptr(g) for g in gb])
+        status = np.zeros(n, dtype=np.int32)
+        st = lib.wedg_adj_batch_gradient(self.h, _ptr(dts), ns, int(snapshots), _ptr(J), pc, pb,
+                                         status.ctypes.data_as(C.POINTER(C.c_int)))
+        self._raise(st, status, "batch gradient")
+        return [(float(J[i]), gc2[i], gb[i]) for i in range(n)]
+
+    def time(self):
+        """(device ms, host issue ms) of the last batch call."""
+        d, e = C.c_double(0.0), C.c_double(0.0)
+        lib.wedg_adj_batch_time(self.h, C.byref(d), C.byref(e))
+        return d.value, e.value
+
+
+def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0=(0.5, 0.3), r0: float = 0.04,
+                              margin: float = 0.3, cheb: str = "global"):
+    """The batched-adjoint workload: `count` source-mode instances drawn from
+    numpy's generator of `seed`, each with N in {64, 96, 128}, q_u = q_v in
+    {5, 6}, c = 1, alpha log-uniform in [1e-6, 1e-5], beta0 uniform in [6, 8],
+    and each side absorbing (absorbing_2beta or absorbing) or a wall (kappa
+    0.5). The Gaussian source has width r0 at x0 (omega 16 pi, Ts = 1); its
+    amplitude is set from a linear (beta = 0) run of the instance's degree on
+    a 32 x 32 mesh over min(T, 0.6) so that 1 - 2 beta0 max|v| stays about
+    `margin` (with a 0.8 safety factor for the steepening). The run takes
+    round(T / dt) steps of dt = h / (10 q^2). The coefficients of the gradient
+    are beta = beta0 + 0.5 sin(2 pi x) cos(2 pi y) per element (c^2 = 1); the
+    target is the bottom trace of the run with beta = beta0."""
+    import math
+    from .solver import solve
+    rng = np.random.default_rng(seed)
+    kinds = ("absorbing_2beta", "absorbing", "wall")
+    out = []
+    vlin = {}
+    for _ in range(count):
+        N = int(rng.choice([64, 96, 128]))
+        q = int(rng.choice([5, 6]))
+        alpha = float(10 ** rng.uniform(-6, -5))
+        beta0 = float(rng.uniform(6, 8))
+        sides = {s: kinds[int(rng.integers(0, 3))] for s in ("bot", "top", "lft", "rgt")}
+        sides["bot"] = "absorbing_2beta"   # the receivers' side absorbs
+        if q not in vlin:
+            Tc = min(T, 0.6)
+            H = solve(q, 32, Tc, mixed=False, mode="source", boundary="absorbing",
+                      coefficients=_constant(alpha=1e-5, beta=0.0), source=GaussianSource(x0=tuple(x0), r0=r0),
+                      record_every=4)
+            vlin[q] = float(max(np.max(H.vmax), 1e-300))
+        amp = 0.8 * (1.0 - margin) / (2.0 * beta0 * vlin[q])
+        src = GaussianSource(x0=tuple(x0), r0=r0, amplitude=amp)
+        adj = SourceAdjoint(q, N, mixed=False, boundary=sides, kappa=0.5, alpha=alpha, source=src, cheb=cheb)
+        h = 1.0 / N
+        dt = h / (10 * q * q)
+        nsteps = max(1, int(round(T / dt)))
+        dt = T / nsteps
+        xc = (np.arange(N) + 0.5) * h
+        X, Y = xc[None, :], xc[:, None]
+        adj.set_params(1.0, beta0)
+        adj.set_target(adj.traces(dt, nsteps))
+        beta = np.asfortranarray(beta0 + 0.5 * np.sin(2 * math.pi * X) * np.cos(2 * math.pi * Y))
+        adj.set_params(1.0, beta)
+        out.append(SimpleNamespace(adj=adj, N=N, q=q, alpha=alpha, beta0=beta0, beta=beta, sides=sides,
+                                   amplitude=amp, dt=dt, steps=nsteps))
+    return out
+
+
+def _constant(alpha, beta):
+    from .coefficients import constant_coefficients as cc
+    return cc(c=1.0, alpha=alpha, beta=beta)

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q1(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q1(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 2) return launch_adjb_nv<2, 2>(b, grid, which, s);
+    if (nv == 1) return launch_adjb_nv<2, 1>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q2(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q2(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 3) return launch_adjb_nv<3, 3>(b, grid, which, s);
+    if (nv == 2) return launch_adjb_nv<3, 2>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q3(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q3(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 4) return launch_adjb_nv<4, 4>(b, grid, which, s);
+    if (nv == 3) return launch_adjb_nv<4, 3>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q4(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q4(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 5) return launch_adjb_nv<5, 5>(b, grid, which, s);
+    if (nv == 4) return launch_adjb_nv<5, 4>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q5(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q5(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 6) return launch_adjb_nv<6, 6>(b, grid, which, s);
+    if (nv == 5) return launch_adjb_nv<6, 5>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

@@ -16,4 +16,10 @@ hipError_t launch_adj2_q6(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_adjb_q6(const BatchArgs& b, int grid, int nv, int which, hipStream_t s) {
+    if (nv == 7) return launch_adjb_nv<7, 7>(b, grid, which, s);
+    if (nv == 6) return launch_adjb_nv<7, 6>(b, grid, which, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace wedg

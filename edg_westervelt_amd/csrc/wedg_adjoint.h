@@ -62,11 +62,15 @@ struct AdjArgs {
 // each phase fetches it anew, so fields are loaded (scalar) where they are used
 // instead of being held in SGPRs across the whole kernel.
 // A kernel that runs two stages' parts (k_adj_ba) takes two AdjArgs; W selects one.
+// W = 2, 3: a batched launch (a BatchArgs whose args point to AdjArgs pairs,
+// [instance][2]): the block's instance selects its pair, W - 2 the member.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const WEDG_CONST_AS AdjArgs CAdjArgs;
 template <int W = 0>
 __device__ __forceinline__ CAdjArgs& adjargs() {
-    CAdjArgs* p = (CAdjArgs*)__builtin_amdgcn_kernarg_segment_ptr() + W;
+    CAdjArgs* p;
+    if constexpr (W >= 2) p = (CAdjArgs*)(const void*)bargs()->args + 2 * block_instance() + (W - 2);
+    else p = (CAdjArgs*)__builtin_amdgcn_kernarg_segment_ptr() + W;
     asm volatile("" : "+s"(p));
     return *p;
 }
@@ -950,9 +954,9 @@ __device__ __forceinline__ void tl_body(int e, int lane, double* Gl) {
 // Misfit of u_h on the bottom side (y = 0) at the face points of the elements
 // of row j = 0 (e = N k): the trace, 1/2 (h/2) w_i (u_h - g)^2 per point, and
 // with ybu set the misfit's derivative added to the state cotangent.
-template <int NU>
+template <int NU, int W = 0>
 __device__ __forceinline__ void misfit_body(int k) {
-    auto& a = adjargs();
+    auto& a = adjargs<W>();
     using D = Dims<NU, NU>;
     constexpr int NQ = D::NQ, nU = NU * NU;
     const int E = a.E, e = a.N * k;
@@ -1033,6 +1037,61 @@ __global__ __launch_bounds__(kWave) void k_misfit(const AdjArgs a) {
     misfit_body<NU>(k);
 }
 
+// Batched forms: each block's instance and its place in the instance's range
+// come from the BatchArgs block map; args holds an AdjArgs pair per instance
+// (k_adj_a, k_adj_b and k_misfit use the first, k_adj_ba both).
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_adj_a_bt(const BatchArgs b) {
+    using D = Dims<NU, NV>;
+    __shared__ double Gl[D::NH * D::NH * kWave];
+    (void)b;
+    const int e = block_first_element<true>() + (int)threadIdx.x;
+    if (e >= adjargs<2>().E) return;
+    adj_a_body<NU, NV, 2>(e, threadIdx.x, Gl);
+}
+
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_adj_b_bt(const BatchArgs b) {
+    (void)b;
+    const int e = block_first_element<true>() + (int)threadIdx.x;
+    if (e >= adjargs<2>().E) return;
+    adj_b_body<NU, NV, 2>(e);
+}
+
+template <int NU, int NV>
+__global__ __launch_bounds__(kWave) void k_adj_ba_bt(const BatchArgs b) {
+    using D = Dims<NU, NV>;
+    __shared__ double Gl[D::NH * D::NH * kWave];
+    (void)b;
+    const int e = block_first_element<true>() + (int)threadIdx.x;
+    if (e >= adjargs<2>().E) return;
+    adj_b_body<NU, NV, 2>(e);
+    phase_barrier();
+    adj_a_body<NU, NV, 3>(e, threadIdx.x, Gl);
+}
+
+template <int NU>
+__global__ __launch_bounds__(kWave) void k_misfit_bt(const BatchArgs b) {
+    (void)b;
+    const int k = block_first_element<true>() + (int)threadIdx.x;
+    if (k >= adjargs<2>().N) return;
+    misfit_body<NU, 2>(k);
+}
+
+// batched launches of one (n_u, n_v) pair over `grid` blocks; which: 0 k_adj_a,
+// 1 k_adj_b, 2 k_adj_ba, 3 k_misfit (blocks over the bottom row's elements)
+template <int NU, int NV>
+static hipError_t launch_adjb_nv(const BatchArgs& b, int grid, int which, hipStream_t s) {
+    switch (which) {
+        case 0: k_adj_a_bt<NU, NV><<<grid, kWave, 0, s>>>(b); break;
+        case 1: k_adj_b_bt<NU, NV><<<grid, kWave, 0, s>>>(b); break;
+        case 2: k_adj_ba_bt<NU, NV><<<grid, kWave, 0, s>>>(b); break;
+        case 3: k_misfit_bt<NU><<<grid, kWave, 0, s>>>(b); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // launches of one (n_u, n_v) pair; which: 0 k_adj_a, 1 k_adj_b, 2 k_tl, 3 k_misfit
 template <int NU, int NV>
 static hipError_t launch_adj_nv(const AdjArgs& a, int which, hipStream_t s) {
@@ -1065,5 +1124,11 @@ hipError_t launch_adj_q5(const AdjArgs& a, int nv, int which, hipStream_t s);
 hipError_t launch_adj2_q5(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
 hipError_t launch_adj_q6(const AdjArgs& a, int nv, int which, hipStream_t s);
 hipError_t launch_adj2_q6(const AdjArgs& b, const AdjArgs& a, int nv, hipStream_t s);
+hipError_t launch_adjb_q1(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
+hipError_t launch_adjb_q2(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
+hipError_t launch_adjb_q3(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
+hipError_t launch_adjb_q4(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
+hipError_t launch_adjb_q5(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
+hipError_t launch_adjb_q6(const BatchArgs& b, int grid, int nv, int which, hipStream_t s);
 
 }  // namespace wedg

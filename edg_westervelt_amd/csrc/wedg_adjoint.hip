@@ -381,29 +381,25 @@ wedg_status tape_step(wedg_workspace* ws, AdjTape& tp, double t, double dt, long
     return WEDG_OK;
 }
 
-// The adjoint of one taped step on the adjoint stream: the misfit of its end
-// state (target row n) into ybar, then stages 4..1: k_adj_a of stage 4, three
-// launches of k_adj_b of stage s with k_adj_a of stage s - 1, and k_adj_b of
-// stage 1; the slot adjoints alternate between two buffers.
-wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
-    const hipStream_t s = A->sa;
+// The arguments of the adjoint of one taped step: the misfit of its end state
+// (target row n) into ybar (mis), and stages 4..1 (st[1..4]); Y1, the step's
+// start state, is (y1u, y1v). The slot adjoints alternate between two buffers.
+void adjoint_step_args(wedg_adj* A, const AdjTape& tp, long n, double dt, const double* y1u, const double* y1v,
+                       AdjArgs& mis, AdjArgs (&st)[5]) {
     AdjArgs a = adj_args(A);
-    if (A->dtgt) {
-        a.yu = tp.ynu;
-        a.tgt = A->dtgt + (size_t)n * A->N * A->nu;
-        a.jp = A->djp + (size_t)n * A->N * A->nu;
-        a.ybu = A->ybu;
-        HIPOK(launch_adj(a, A->qu, A->nv, 3, s));
-        a.tgt = nullptr; a.jp = nullptr;
-    }
+    mis = a;
+    mis.yu = tp.ynu;
+    mis.tgt = A->dtgt ? A->dtgt + (size_t)n * A->N * A->nu : nullptr;
+    mis.jp = A->dtgt ? A->djp + (size_t)n * A->N * A->nu : nullptr;
+    mis.ybu = A->ybu;
     const double ca[5] = {0, 0, dt / 6.0, dt / 3.0, dt / 3.0};   // next cotangent: ca ybar + cb Ybar_s
     const double cb[5] = {0, 0, 0.5 * dt, 0.5 * dt, dt};
-    AdjArgs st[5];
     for (int stg = 4; stg >= 1; --stg) {
         AdjArgs& x = st[stg];
         x = a;
         x.stage = stg;
-        x.yu = tp.Yu[stg - 1]; x.yv = tp.Yv[stg - 1];
+        x.yu = stg == 1 ? y1u : tp.Yu[stg - 1];
+        x.yv = stg == 1 ? y1v : tp.Yv[stg - 1];
         x.ku = tp.Ku[stg - 1]; x.kv = tp.Kv[stg - 1];
         if (stg == 4) { x.kbu = A->ybu; x.kbv = A->ybv; x.kscale = dt / 6.0; }
         else { x.kbu = A->kbu; x.kbv = A->kbv; x.kscale = 1.0; }
@@ -411,6 +407,17 @@ wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
         x.slot = (stg & 1) ? A->slot2 : A->slot;   // written by the stage's k_adj_a
         x.slotr = x.slot;                          // read by its k_adj_b
     }
+    st[0] = a;
+}
+
+// The adjoint of one taped step on the adjoint stream: k_misfit, then k_adj_a
+// of stage 4, three launches of k_adj_b of stage s with k_adj_a of stage s - 1,
+// and k_adj_b of stage 1.
+wedg_status adjoint_step(wedg_adj* A, AdjTape& tp, long n, double dt) {
+    const hipStream_t s = A->sa;
+    AdjArgs mis, st[5];
+    adjoint_step_args(A, tp, n, dt, tp.Yu[0], tp.Yv[0], mis, st);
+    if (A->dtgt) HIPOK(launch_adj(mis, A->qu, A->nv, 3, s));
     HIPOK(launch_adj(st[4], A->qu, A->nv, 0, s));
     for (int stg = 4; stg >= 2; --stg) HIPOK(launch_adj2(st[stg], st[stg - 1], A->qu, A->nv, s));
     HIPOK(launch_adj(st[1], A->qu, A->nv, 1, s));
@@ -867,4 +874,467 @@ extern "C" wedg_status wedg_adj_rhs_tl(wedg_adj* A, const double* u, const doubl
     if ((st = read_fail(A))) return st;
     if ((st = d2h(A, dut, tp.Yu[2], A->nU()))) return st;
     return d2h(A, dvt, tp.Yv[2], A->nV());
+}
+
+// ---------------------------------------------------------------------------
+// Batched gradients (wedg_adj_batch_*): several instances run their binomial
+// checkpointing schedules in lockstep rounds on one stream. In a round every
+// instance with work left takes its next unit, a forward step or the reversal
+// of a step (the step re-run with its stages taped, then its adjoint); per group
+// of instances with the same (q_u, q_v, kernel layout) a round is one k_prep and
+// four k_stage launches over every instance of the group, then k_misfit,
+// k_adj_a, three k_adj_ba and k_adj_b over the instances reversing a step. The
+// arguments of a round (stage and adjoint arguments, block maps) reach the
+// device in one copy through a ring of pinned buffers. Every instance computes
+// exactly what its own wedg_adj_gradient computes.
+
+struct wedg_adj_batch {
+    std::vector<wedg_adj*> inst;
+    hipStream_t stream = nullptr;
+    static constexpr int kRing = 8;
+    char* hbuf[kRing] = {};
+    char* dbuf[kRing] = {};
+    hipEvent_t done[kRing] = {};
+    bool used[kRing] = {};
+    size_t cap = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double device_ms = 0.0, enqueue_ms = 0.0;
+};
+
+namespace {
+
+hipError_t launch_fwd_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode) {
+    switch (qu) {
+        case 1: return launch_batch_q1(b, grid, nv, which, s, mode);
+        case 2: return launch_batch_q2(b, grid, nv, which, s, mode);
+        case 3: return launch_batch_q3(b, grid, nv, which, s, mode);
+        case 4: return launch_batch_q4(b, grid, nv, which, s, mode);
+        case 5: return launch_batch_q5(b, grid, nv, which, s, mode);
+        case 6: return launch_batch_q6(b, grid, nv, which, s, mode);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adj_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s) {
+    switch (qu) {
+        case 1: return launch_adjb_q1(b, grid, nv, which, s);
+        case 2: return launch_adjb_q2(b, grid, nv, which, s);
+        case 3: return launch_adjb_q3(b, grid, nv, which, s);
+        case 4: return launch_adjb_q4(b, grid, nv, which, s);
+        case 5: return launch_adjb_q5(b, grid, nv, which, s);
+        case 6: return launch_adjb_q6(b, grid, nv, which, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// One unit of an instance's schedule: the forward step n from state `in` into
+// state `out`, or (rev) the reversal of step n from state `in`. States: -1 the
+// working state of the instance's workspace, s >= 0 snapshot slot s.
+struct AdjUnit {
+    long n;
+    int rev, in, out;
+};
+
+// The binomial checkpointing schedule of wedg_adj_gradient as units, in the
+// order it issues them, with its counts.
+void revolve_units(long nsteps, int nsnap, std::vector<AdjUnit>& units, wedg_adj_stats& stats) {
+    units.clear();
+    int cur = 0;
+    std::vector<int> free_slots;
+    for (int s = nsnap - 1; s >= 1; --s) free_slots.push_back(s);
+    int max_used = 1;
+    auto advance = [&](long a, long m, int save) {
+        for (long i = a; i < m; ++i) {
+            const int out = (i == m - 1 && save >= 0) ? save : -1;
+            units.push_back({i, 0, cur, out});
+            cur = out;
+            ++stats.forward_steps;
+        }
+        if (save >= 0) ++stats.snapshots_taken;
+    };
+    auto leaf = [&](long n) {
+        units.push_back({n, 1, cur, -1});
+        ++stats.taped_steps;
+        ++stats.adjoint_steps;
+    };
+    std::function<void(long, long, int)> reverse = [&](long a, long b, int slot_a) {
+        if (b - a == 1) {
+            leaf(a);
+            return;
+        }
+        const int sfree = (int)free_slots.size();
+        if (sfree == 0) {
+            for (long i = b - 1; i >= a; --i) {
+                cur = slot_a;
+                advance(a, i, -1);
+                leaf(i);
+            }
+            return;
+        }
+        const long m = a + revolve_split(b - a, sfree);
+        const int sm = free_slots.back();
+        free_slots.pop_back();
+        max_used = std::max(max_used, nsnap - (int)free_slots.size());
+        advance(a, m, sm);
+        reverse(m, b, sm);
+        free_slots.push_back(sm);
+        cur = slot_a;
+        reverse(a, m, slot_a);
+    };
+    ++stats.snapshots_taken;   // y_0 in slot 0
+    reverse(0, nsteps, 0);
+    stats.snapshots_used = max_used;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Bytes of one round's arguments when every instance takes part in every launch.
+size_t round_bytes(const wedg_adj_batch* b) {
+    size_t n = 0, blocks = 0, mblocks = 0;
+    for (wedg_adj* A : b->inst) {
+        ++n;
+        blocks += (A->E + kWave - 1) / kWave;
+        mblocks += (A->N + kWave - 1) / kWave;
+    }
+    const size_t per_group_arrays = 4 + 6 + 6;
+    return 4 * align256(n * sizeof(KArgs)) + 6 * align256(2 * n * sizeof(AdjArgs)) +
+           2 * align256(blocks * sizeof(int)) + align256(mblocks * sizeof(int)) + 3 * align256(n * sizeof(int)) +
+           b->inst.size() * per_group_arrays * 256 + 4096;
+}
+
+// the group key of an instance: kernels are instantiated per (q_u, q_v, layout)
+long group_key(const wedg_adj* A) { return ((long)A->qu * 16 + A->nv) * 16 + A->p->mode; }
+
+wedg_status batch_prepare_instance(wedg_adj* A, long nsteps, int nsnap, bool gradient) {
+    wedg_workspace* ws = A->ws;
+    const hipStream_t s = ws->stream;
+    wedg_status st;
+    if ((st = jp_capacity(A, nsteps))) return st;
+    const size_t nU = A->nU(), nV = A->nV(), sz = nU + nV;
+    if (gradient && nsnap > A->nsnap_cap) {
+        HIPOK(hipStreamSynchronize(s));
+        if (A->snaps) (void)hipFree(A->snaps);
+        A->snaps = nullptr;
+        A->nsnap_cap = 0;
+        HIPOK(hipMalloc((void**)&A->snaps, (size_t)nsnap * sz * sizeof(double)));
+        A->nsnap_cap = nsnap;
+    }
+    std::memset(&A->stats, 0, sizeof(A->stats));
+    if ((st = reset_err(ws))) return st;
+    HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), s));
+    HIPOK(hipMemsetAsync(ws->yu, 0, nU * sizeof(double), s));
+    HIPOK(hipMemsetAsync(ws->yv, 0, nV * sizeof(double), s));
+    if (gradient) {
+        if ((st = zero_dev(s, A->gc2, A->E)) || (st = zero_dev(s, A->gbe, A->E)) || (st = zero_dev(s, A->ybu, nU)) ||
+            (st = zero_dev(s, A->ybv, nV)))
+            return st;
+        HIPOK(hipMemsetAsync(A->snaps, 0, sz * sizeof(double), s));   // y_0 = 0 in slot 0
+    }
+    HIPOK(hipStreamSynchronize(s));
+    return WEDG_OK;
+}
+
+// Runs the units of every instance in lockstep rounds (units[i] of instance i).
+// forward_misfit: every unit is a forward step and forms the misfit of its end
+// state; otherwise the misfit is formed by the reversals (as the gradient does).
+wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUnit>>& units, const double* dt,
+                         bool forward_misfit) {
+    const int n = (int)B->inst.size();
+    const hipStream_t s = B->stream;
+    // groups
+    std::vector<long> keys;
+    for (wedg_adj* A : B->inst) keys.push_back(group_key(A));
+    std::vector<long> gk = keys;
+    std::sort(gk.begin(), gk.end());
+    gk.erase(std::unique(gk.begin(), gk.end()), gk.end());
+    std::vector<std::vector<int>> gmem(gk.size());
+    for (int i = 0; i < n; ++i) gmem[std::lower_bound(gk.begin(), gk.end(), keys[i]) - gk.begin()].push_back(i);
+    // times after n steps, accumulated as the forward loop does (src/solver.jl:213-216)
+    std::vector<std::vector<double>> tt(n);
+    long rounds = 0;
+    for (int i = 0; i < n; ++i) {
+        long ns = 0;
+        for (const AdjUnit& u : units[i]) ns = std::max(ns, u.n + 1);
+        tt[i].assign(ns + 1, 0.0);
+        for (long k = 0; k < ns; ++k) tt[i][k + 1] = tt[i][k] + dt[i];
+        rounds = std::max(rounds, (long)units[i].size());
+    }
+    auto state_u = [&](wedg_adj* A, int id) -> double* {
+        return id < 0 ? A->ws->yu : A->snaps + (size_t)id * (A->nU() + A->nV());
+    };
+    auto state_v = [&](wedg_adj* A, int id) -> double* {
+        return id < 0 ? A->ws->yv : A->snaps + (size_t)id * (A->nU() + A->nV()) + A->nU();
+    };
+    HIPOK(hipEventRecord(B->t0, s));
+    const auto c0 = std::chrono::steady_clock::now();
+    for (long r = 0; r < rounds; ++r) {
+        const int ring = (int)(r % wedg_adj_batch::kRing);
+        if (B->used[ring]) HIPOK(hipEventSynchronize(B->done[ring]));
+        char* hb = B->hbuf[ring];
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off = align256(off + bytes);
+            return o;
+        };
+        struct Launch {
+            int qu, nv, mode, which, grid, adj;
+            size_t args, binst, bfirst;
+        };
+        std::vector<Launch> launches;
+        for (size_t g = 0; g < gk.size(); ++g) {
+            std::vector<int> F, R;
+            for (int i : gmem[g])
+                if (r < (long)units[i].size()) {
+                    F.push_back(i);
+                    if (units[i][r].rev) R.push_back(i);
+                }
+            if (F.empty()) continue;
+            wedg_adj* A0 = B->inst[F[0]];
+            const int qu = A0->qu, nv = A0->nv, mode = A0->p->mode;
+            // forward: block map and the four stages' arguments
+            int fblocks = 0;
+            for (int i : F) fblocks += (B->inst[i]->E + kWave - 1) / kWave;
+            const size_t o_bi = take(fblocks * sizeof(int)), o_bf = take(F.size() * sizeof(int));
+            {
+                int* bi = (int*)(hb + o_bi);
+                int* bf = (int*)(hb + o_bf);
+                int blk = 0;
+                for (size_t x = 0; x < F.size(); ++x) {
+                    const int nb = (B->inst[F[x]]->E + kWave - 1) / kWave;
+                    bf[x] = blk;
+                    for (int k = 0; k < nb; ++k) bi[blk++] = (int)x;
+                }
+            }
+            size_t o_st[4];
+            for (int sg = 0; sg < 4; ++sg) o_st[sg] = take(F.size() * sizeof(KArgs));
+            for (size_t x = 0; x < F.size(); ++x) {
+                wedg_adj* A = B->inst[F[x]];
+                wedg_workspace* ws = A->ws;
+                const AdjUnit& u = units[F[x]][r];
+                const double t = tt[F[x]][u.n], d = dt[F[x]], hd = 0.5 * d, th = t + hd, tf = t + d;
+                const double* iu = state_u(A, u.in);
+                const double* iv = state_v(A, u.in);
+                const long step = u.n + 1;
+                KArgs a[4];
+                if (!u.rev) {
+                    double* ou = state_u(A, u.out);
+                    double* ov = state_v(A, u.out);
+                    a[0] = stage_args(ws, 1, iu, iv, iu, iv, ws->su, ws->sv, t, hd, step, true, th);
+                    a[1] = stage_args(ws, 2, ws->su, ws->sv, iu, iv, ws->ou, ws->ov, th, hd, step, true, th);
+                    a[2] = stage_args(ws, 3, ws->ou, ws->ov, iu, iv, ws->su, ws->sv, th, d, step, true, tf);
+                    a[3] = stage_args(ws, 4, ws->su, ws->sv, iu, iv, ou, ov, tf, d / 6.0, step, false, tf);
+                } else {
+                    AdjTape& tp = A->tape[0];
+                    const double ts[4] = {t, th, th, tf}, cs[4] = {hd, hd, d, d / 6.0}, tn[4] = {th, th, tf, tf};
+                    for (int sg = 0; sg < 4; ++sg) {
+                        const double* yu = sg == 0 ? iu : tp.Yu[sg];
+                        const double* yv = sg == 0 ? iv : tp.Yv[sg];
+                        double* ou = sg < 3 ? tp.Yu[sg + 1] : tp.ynu;
+                        double* ov = sg < 3 ? tp.Yv[sg + 1] : tp.ynv;
+                        a[sg] = stage_args(ws, sg + 1, yu, yv, iu, iv, ou, ov, ts[sg], cs[sg], step, sg < 3, tn[sg]);
+                        a[sg].recu = tp.Ku[sg];
+                        a[sg].recv = tp.Kv[sg];
+                    }
+                }
+                for (int sg = 0; sg < 4; ++sg) std::memcpy(hb + o_st[sg] + x * sizeof(KArgs), &a[sg], sizeof(KArgs));
+                if (u.rev) ++A->stats.taped_steps;
+                else ++A->stats.forward_steps;
+            }
+            launches.push_back({qu, nv, mode, 0, fblocks, 0, o_st[0], o_bi, o_bf});   // k_prep
+            for (int sg = 0; sg < 4; ++sg) launches.push_back({qu, nv, mode, 1, fblocks, 0, o_st[sg], o_bi, o_bf});
+            // misfit of the forward steps' end states (forward runs)
+            std::vector<int> M = forward_misfit ? F : R;
+            if (!M.empty()) {
+                int mblocks = 0;
+                for (int i : M) mblocks += (B->inst[i]->N + kWave - 1) / kWave;
+                const size_t o_mi = take(mblocks * sizeof(int)), o_mf = take(M.size() * sizeof(int));
+                const size_t o_ma = take(2 * M.size() * sizeof(AdjArgs));
+                int* bi = (int*)(hb + o_mi);
+                int* bf = (int*)(hb + o_mf);
+                int blk = 0;
+                for (size_t x = 0; x < M.size(); ++x) {
+                    const int nb = (B->inst[M[x]]->N + kWave - 1) / kWave;
+                    bf[x] = blk;
+                    for (int k = 0; k < nb; ++k) bi[blk++] = (int)x;
+                }
+                for (size_t x = 0; x < M.size(); ++x) {
+                    wedg_adj* A = B->inst[M[x]];
+                    const AdjUnit& u = units[M[x]][r];
+                    AdjArgs mis = adj_args(A);
+                    const size_t row = (size_t)A->N * A->nu;
+                    if (forward_misfit) {
+                        mis.yu = state_u(A, u.out);
+                        mis.tgt = A->dtgt + (size_t)u.n * row;
+                        mis.jp = A->djp + (size_t)u.n * row;
+                        mis.ybu = nullptr;
+                    } else {
+                        AdjArgs st5[5];
+                        adjoint_step_args(A, A->tape[0], u.n, dt[M[x]], state_u(A, u.in), state_v(A, u.in), mis, st5);
+                    }
+                    std::memcpy(hb + o_ma + (2 * x) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
+                    std::memcpy(hb + o_ma + (2 * x + 1) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
+                }
+                launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf});
+            }
+            if (R.empty() || forward_misfit) continue;
+            // the adjoint of the reversed steps
+            int ablocks = 0;
+            for (int i : R) ablocks += (B->inst[i]->E + kWave - 1) / kWave;
+            const size_t o_ai = take(ablocks * sizeof(int)), o_af = take(R.size() * sizeof(int));
+            {
+                int* bi = (int*)(hb + o_ai);
+                int* bf = (int*)(hb + o_af);
+                int blk = 0;
+                for (size_t x = 0; x < R.size(); ++x) {
+                    const int nb = (B->inst[R[x]]->E + kWave - 1) / kWave;
+                    bf[x] = blk;
+                    for (int k = 0; k < nb; ++k) bi[blk++] = (int)x;
+                }
+            }
+            size_t o_ad[5];   // k_adj_a(4), k_adj_ba(4, 3), (3, 2), (2, 1), k_adj_b(1)
+            for (int k = 0; k < 5; ++k) o_ad[k] = take(2 * R.size() * sizeof(AdjArgs));
+            for (size_t x = 0; x < R.size(); ++x) {
+                wedg_adj* A = B->inst[R[x]];
+                const AdjUnit& u = units[R[x]][r];
+                AdjArgs mis, st5[5];
+                adjoint_step_args(A, A->tape[0], u.n, dt[R[x]], state_u(A, u.in), state_v(A, u.in), mis, st5);
+                const AdjArgs pairs[5][2] = {{st5[4], st5[4]}, {st5[4], st5[3]}, {st5[3], st5[2]}, {st5[2], st5[1]},
+                                             {st5[1], st5[1]}};
+                for (int k = 0; k < 5; ++k)
+                    std::memcpy(hb + o_ad[k] + 2 * x * sizeof(AdjArgs), pairs[k], 2 * sizeof(AdjArgs));
+                ++A->stats.adjoint_steps;
+            }
+            launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af});
+            for (int k = 1; k <= 3; ++k) launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af});
+            launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af});
+        }
+        if (off > B->cap) return WEDG_ERR_ARGUMENT;
+        char* db = B->dbuf[ring];
+        HIPOK(hipMemcpyAsync(db, hb, off, hipMemcpyHostToDevice, s));
+        for (const Launch& L : launches) {
+            BatchArgs ba;
+            ba.args = (const KArgs*)(const void*)(db + L.args);
+            ba.binst = (const int*)(db + L.binst);
+            ba.bfirst = (const int*)(db + L.bfirst);
+            if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, s));
+            else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, s, L.mode));
+        }
+        HIPOK(hipEventRecord(B->done[ring], s));
+        B->used[ring] = true;
+    }
+    const auto c1 = std::chrono::steady_clock::now();
+    HIPOK(hipEventRecord(B->t1, s));
+    HIPOK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    B->device_ms = hipEventElapsedTime(&ms, B->t0, B->t1) == hipSuccess ? ms : 0.0;
+    B->enqueue_ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
+    return WEDG_OK;
+}
+
+}  // namespace
+
+extern "C" wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj_batch** out) {
+    if (!a || n < 1 || !out) return WEDG_ERR_ARGUMENT;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!a[i]) return WEDG_ERR_ARGUMENT;
+        for (int k = 0; k < i; ++k)
+            if (a[k] == a[i]) return WEDG_ERR_ARGUMENT;
+    }
+    auto* B = new wedg_adj_batch();
+    B->inst.assign(a, a + n);
+    auto fail = [&](wedg_status s) {
+        wedg_adj_batch_destroy(B);
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return fail(WEDG_ERR_HIP);
+    if (hipEventCreate(&B->t0) != hipSuccess || hipEventCreate(&B->t1) != hipSuccess) return fail(WEDG_ERR_HIP);
+    B->cap = round_bytes(B);
+    for (int k = 0; k < wedg_adj_batch::kRing; ++k) {
+        if (hipHostMalloc((void**)&B->hbuf[k], B->cap, hipHostMallocDefault) != hipSuccess) return fail(WEDG_ERR_HIP);
+        if (hipMalloc((void**)&B->dbuf[k], B->cap) != hipSuccess) return fail(WEDG_ERR_HIP);
+        if (hipEventCreateWithFlags(&B->done[k], hipEventDisableTiming) != hipSuccess) return fail(WEDG_ERR_HIP);
+    }
+    *out = B;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_batch_destroy(wedg_adj_batch* B) {
+    if (!B) return WEDG_OK;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    for (int k = 0; k < wedg_adj_batch::kRing; ++k) {
+        if (B->hbuf[k]) (void)hipHostFree(B->hbuf[k]);
+        if (B->dbuf[k]) (void)hipFree(B->dbuf[k]);
+        if (B->done[k]) (void)hipEventDestroy(B->done[k]);
+    }
+    if (B->t0) (void)hipEventDestroy(B->t0);
+    if (B->t1) (void)hipEventDestroy(B->t1);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return WEDG_OK;
+}
+
+// status[i] (or the return value) reports instance i's failure; the others finish.
+static wedg_status batch_finish(wedg_adj_batch* B, const long* nsteps, double* J, int* status, bool gradient,
+                                double* const* dJ_dc2, double* const* dJ_dbeta) {
+    wedg_status first = WEDG_OK;
+    for (size_t i = 0; i < B->inst.size(); ++i) {
+        wedg_adj* A = B->inst[i];
+        wedg_status si = collect(A->ws, 1);
+        if (si == WEDG_OK && gradient) si = read_fail(A);
+        if (si == WEDG_OK) si = sum_misfit(A, nsteps[i], &J[i]);
+        if (si == WEDG_OK && gradient && dJ_dc2 && dJ_dc2[i]) si = d2h(A, dJ_dc2[i], A->gc2, A->E);
+        if (si == WEDG_OK && gradient && dJ_dbeta && dJ_dbeta[i]) si = d2h(A, dJ_dbeta[i], A->gbe, A->E);
+        A->stats.enqueue_ms = B->enqueue_ms;
+        A->stats.total_ms = B->device_ms;
+        if (status) status[i] = si;
+        if (si != WEDG_OK && first == WEDG_OK) first = si;
+    }
+    return first;
+}
+
+extern "C" wedg_status wedg_adj_batch_forward(wedg_adj_batch* B, const double* dt, const long* nsteps, double* J,
+                                              int* status) {
+    if (!B || !dt || !nsteps || !J) return WEDG_ERR_ARGUMENT;
+    const size_t n = B->inst.size();
+    for (size_t i = 0; i < n; ++i)
+        if (nsteps[i] < 1 || !(dt[i] > 0) || B->inst[i]->tgt_steps < nsteps[i]) return WEDG_ERR_ARGUMENT;
+    std::vector<std::vector<AdjUnit>> units(n);
+    wedg_status st;
+    for (size_t i = 0; i < n; ++i) {
+        if ((st = batch_prepare_instance(B->inst[i], nsteps[i], 0, false))) return st;
+        for (long k = 0; k < nsteps[i]; ++k) units[i].push_back({k, 0, -1, -1});
+    }
+    if ((st = batch_rounds(B, units, dt, true))) return st;
+    return batch_finish(B, nsteps, J, status, false, nullptr, nullptr);
+}
+
+extern "C" wedg_status wedg_adj_batch_gradient(wedg_adj_batch* B, const double* dt, const long* nsteps, int nsnap,
+                                               double* J, double* const* dJ_dc2, double* const* dJ_dbeta,
+                                               int* status) {
+    if (!B || !dt || !nsteps || !J || !dJ_dc2 || !dJ_dbeta || nsnap < 1) return WEDG_ERR_ARGUMENT;
+    const size_t n = B->inst.size();
+    for (size_t i = 0; i < n; ++i)
+        if (nsteps[i] < 1 || !(dt[i] > 0) || B->inst[i]->tgt_steps < nsteps[i] || !dJ_dc2[i] || !dJ_dbeta[i])
+            return WEDG_ERR_ARGUMENT;
+    std::vector<std::vector<AdjUnit>> units(n);
+    wedg_status st;
+    for (size_t i = 0; i < n; ++i) {
+        wedg_adj* A = B->inst[i];
+        if ((st = batch_prepare_instance(A, nsteps[i], nsnap, true))) return st;
+        wedg_adj_stats plan{};
+        revolve_units(nsteps[i], nsnap, units[i], plan);
+        A->stats.snapshots_taken = plan.snapshots_taken;
+        A->stats.snapshots_used = plan.snapshots_used;
+    }
+    if ((st = batch_rounds(B, units, dt, false))) return st;
+    return batch_finish(B, nsteps, J, status, true, dJ_dc2, dJ_dbeta);
+}
+
+extern "C" wedg_status wedg_adj_batch_time(const wedg_adj_batch* B, double* device_ms, double* enqueue_ms) {
+    if (!B || !device_ms || !enqueue_ms) return WEDG_ERR_ARGUMENT;
+    *device_ms = B->device_ms;
+    *enqueue_ms = B->enqueue_ms;
+    return WEDG_OK;
 }

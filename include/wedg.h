@@ -344,6 +344,26 @@ wedg_status wedg_adj_rhs_vjp(wedg_adj* a, const double* u, const double* v, doub
 wedg_status wedg_adj_rhs_tl(wedg_adj* a, const double* u, const double* v, double t, const double* du,
                             const double* dv, const double* dc2, const double* dbeta, double* dut, double* dvt);
 
+/* Several instances' forward runs and gradients together (no counterpart in
+ * the reference): each instance runs what wedg_adj_forward / wedg_adj_gradient
+ * run on it, with identical results, in lockstep rounds on one stream; per
+ * round, the instances with the same (q_u, q_v, kernel layout) share one
+ * launch for every RK stage and every adjoint kernel. Instances keep their
+ * coefficients and targets (wedg_adj_set_params, wedg_adj_set_target) and may
+ * not be used elsewhere while a batch call runs. status[i] (or null) receives
+ * instance i's status; the return value is the first failure. Per-instance
+ * counts: wedg_adj_last_stats. */
+typedef struct wedg_adj_batch wedg_adj_batch;
+wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj_batch** out);
+wedg_status wedg_adj_batch_destroy(wedg_adj_batch* b);
+/* J[i] after nsteps[i] steps of dt[i] from the zero state (a target of at least nsteps[i] rows each). */
+wedg_status wedg_adj_batch_forward(wedg_adj_batch* b, const double* dt, const long* nsteps, double* J, int* status);
+/* J[i] and dJ_dc2[i], dJ_dbeta[i] ((N_i, N_i) host arrays) with `snapshots` slots per instance. */
+wedg_status wedg_adj_batch_gradient(wedg_adj_batch* b, const double* dt, const long* nsteps, int snapshots, double* J,
+                                    double* const* dJ_dc2, double* const* dJ_dbeta, int* status);
+/* Device time of the last batch call (stream events) and the host time to issue it. */
+wedg_status wedg_adj_batch_time(const wedg_adj_batch* b, double* device_ms, double* enqueue_ms);
+
 /* Library and device facts: compiled architecture, device name, CU count. */
 const char* wedg_version(void);
 wedg_status wedg_device_info(char* name, size_t len, int* cu_count, double* hbm_gbs);

@@ -196,3 +196,65 @@ def test_full_workload_as_specified():
     assert J == wl.adj.misfit(wl.dt, wl.steps) and J > 0
     J2, gc2b, gbb = wl.adj.gradient(wl.dt, wl.steps, snapshots=32)
     assert J2 == J and np.array_equal(gc2b, gc2) and np.array_equal(gbb, gb)
+
+
+def _batch_cases():
+    """Instances of two degree groups, uneven meshes and step counts."""
+    from oracle import adjoint as A
+    out = []
+    for k, (q, mixed, N, ns) in enumerate([(3, False, 6, 30), (4, False, 4, 16), (3, True, 5, 12),
+                                           (3, False, 8, 21)]):
+        adj, pb, c2, beta, _, _ = setup(q, mixed, N, 20 + k)
+        dt = 2e-3 if k != 3 else 1.5e-3
+        tgt = A.misfit(pb, np.ones_like(c2), 3 * np.ones_like(beta), dt, ns, traces=True)[1]
+        adj.set_target(tgt)
+        out.append((adj, dt, ns))
+    return out
+
+
+@pytest.mark.parametrize("snaps", [4, 1])
+def test_batched_gradient_equals_each_instances_own(snaps):
+    """wedg_adj_batch_gradient over four instances (two degree groups, uneven N
+    and step counts) gives every instance bitwise its own wedg_adj_gradient,
+    with the same checkpointing counts; wedg_adj_batch_forward gives bitwise
+    each instance's misfit."""
+    from edg_westervelt_amd.adjoint import AdjointBatch
+    cases = _batch_cases()
+    own, counts = [], []
+    for adj, dt, ns in cases:
+        own.append(adj.gradient(dt, ns, snapshots=snaps))
+        s = adj.stats()
+        counts.append((s.forward_steps, s.taped_steps, s.adjoint_steps, s.snapshots_taken, s.snapshots_used))
+        assert s.forward_steps == _closed_form_advances(ns, snaps)
+    batch = AdjointBatch([c[0] for c in cases])
+    res = batch.gradient([c[1] for c in cases], [c[2] for c in cases], snapshots=snaps)
+    for (adj, dt, ns), (J, gc2, gb), (Jb, gc2b, gbb), cnt in zip(cases, own, res, counts):
+        assert Jb == J and J > 0
+        assert np.array_equal(gc2b, gc2) and np.array_equal(gbb, gb)
+        s = adj.stats()
+        assert (s.forward_steps, s.taped_steps, s.adjoint_steps, s.snapshots_taken, s.snapshots_used) == cnt
+    Jf = batch.forward([c[1] for c in cases], [c[2] for c in cases])
+    for (adj, dt, ns), jf in zip(cases, Jf):
+        assert jf == adj.misfit(dt, ns)
+    dev_ms, enq_ms = batch.time()
+    assert dev_ms > 0 and enq_ms > 0
+
+
+def test_batched_gradient_reports_an_instances_failure():
+    """An instance whose coefficients make a mass matrix indefinite during the
+    run fails with the single run's error; the others' results are unchanged."""
+    from edg_westervelt_amd.adjoint import AdjointBatch
+    from edg_westervelt_amd._lib import WedgError
+    cases = _batch_cases()[:2]
+    good_adj, dt_g, ns_g = cases[1]
+    ref = good_adj.gradient(dt_g, ns_g, snapshots=4)
+    bad_adj, dt_b, ns_b = cases[0]
+    bad_adj.set_params(1.0, 1e6)          # 1 - 2 beta v_h <= 0 once the source has made v
+    with pytest.raises(WedgError) as e_single:
+        bad_adj.gradient(dt_b, ns_b, snapshots=4)
+    batch = AdjointBatch([bad_adj, good_adj])
+    with pytest.raises(WedgError) as e_batch:
+        batch.gradient([dt_b, dt_g], [ns_b, ns_g], snapshots=4)
+    assert str(e_batch.value).split(": ", 1)[1] == str(e_single.value).split(": ", 1)[1]
+    J, gc2, gb = good_adj.gradient(dt_g, ns_g, snapshots=4)
+    assert J == ref[0] and np.array_equal(gc2, ref[1]) and np.array_equal(gb, ref[2])
