@@ -176,6 +176,52 @@ def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
                          "kappa = 0.5; J = bottom-side L2 misfit after every step against the beta = 4 run")
 
 
+def bench_batched_adjoint(T=0.02, count=16, seed=2, snapshots=32, sequential=True):
+    """The batched adjoint: `count` source-mode instances of
+    edg_westervelt_amd.adjoint.steepening_batch_workload (N in {64, 96, 128},
+    q in {5, 6}, beta about 6-8), forward and checkpointed gradient of every
+    instance together (one launch per RK stage and per adjoint kernel for each
+    degree group), against the instances' own gradients one after another.
+    The workload is defined to T = 1.0 (16,000-46,000 steps per instance); the
+    timed runs stop at T (the same steps for both legs)."""
+    from edg_westervelt_amd.adjoint import AdjointBatch, steepening_batch_workload
+    t = time.perf_counter()
+    wl = steepening_batch_workload(count=count, seed=seed, T=T)
+    t_build = time.perf_counter() - t
+    dts = [x.dt for x in wl]
+    ns = [x.steps for x in wl]
+    batch = AdjointBatch([x.adj for x in wl])
+    batch.forward(dts, [min(n, 8) for n in ns])               # warm-up
+    print(f"# batched adjoint: forward of {count} instances", file=sys.stderr, flush=True)
+    c0 = time.perf_counter()
+    Jf = batch.forward(dts, ns)
+    tf = time.perf_counter() - c0
+    print(f"# batched adjoint: gradients of {count} instances", file=sys.stderr, flush=True)
+    c0 = time.perf_counter()
+    res = batch.gradient(dts, ns, snapshots)
+    tg = time.perf_counter() - c0
+    dev_ms, enq_ms = batch.time()
+    same = all(r[0] == j for r, j in zip(res, Jf))
+    out = dict(batched_gradient_s=tg, batched_forward_s=tf, adjoint_forward_ratio=tg / tf, device_s=dev_ms / 1e3,
+               host_enqueue_s=enq_ms / 1e3, instances=count, snapshots=snapshots, T=T,
+               steps=ns, N=[x.N for x in wl], q=[x.q for x in wl], build_s=round(t_build, 2),
+               J_equal_forward=bool(same),
+               workload=f"edg_westervelt_amd.adjoint.steepening_batch_workload(count={count}, seed={seed}, T={T}): "
+                        "source mode, N in {64, 96, 128}, q_u = q_v in {5, 6}, c = 1, alpha log-uniform "
+                        "[1e-6, 1e-5], beta0 uniform [6, 8], source width 0.04 at (0.5, 0.3), amplitude for "
+                        "1 - 2 beta0 max|v| about 0.3; gradient at beta0 + 0.5 sin(2 pi x) cos(2 pi y), target the "
+                        "beta0 run; T shortened from the workload's 1.0")
+    if sequential:
+        print(f"# batched adjoint: the {count} gradients one after another", file=sys.stderr, flush=True)
+        c0 = time.perf_counter()
+        seq = [x.adj.gradient(x.dt, x.steps, snapshots) for x in wl]
+        ts = time.perf_counter() - c0
+        out.update(sequential_s=ts, speedup=ts / tg,
+                   bitwise_equal_to_sequential=bool(all(a[0] == b[0] and np.array_equal(a[1], b[1]) and
+                                                        np.array_equal(a[2], b[2]) for a, b in zip(res, seq))))
+    return out
+
+
 def reference_runs():
     """Wall times (precompute, upload, run and results, on this machine) of the
     runs the reference times itself, as it states them: README.md:244-245,
@@ -268,6 +314,11 @@ def main():
     ap.add_argument("--adjoint", choices=("on", "off", "only"), default="on",
                     help="the receiver-misfit gradient and its cost against one forward solve: with the headline "
                          "(on), not at all (off), or alone (only)")
+    ap.add_argument("--batched-adjoint", choices=("on", "off", "only"), default="on",
+                    help="the 16-instance batched adjoint against the instances' own gradients: with the headline "
+                         "(on), not at all (off), or alone (only)")
+    ap.add_argument("--batched-adjoint-T", type=float, default=0.02,
+                    help="end time of the batched-adjoint runs (the workload is defined to 1.0)")
     ap.add_argument("--reference-runs", action="store_true",
                     help="only time the runs the reference times itself (README, data files) and exit")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -299,6 +350,11 @@ def main():
         ad = bench_adjoint()
         print(json.dumps(dict(metric="adjoint/forward cost ratio", value=ad["adjoint_forward_ratio"], unit="x",
                               n_gpus=1, higher_is_better=False, dtype="f64", data="synthetic", adjoint=ad)))
+        return
+    if args.batched_adjoint == "only":
+        ba = bench_batched_adjoint(T=args.batched_adjoint_T)
+        print(json.dumps(dict(metric="batched adjoint speedup (16 instances)", value=ba.get("speedup"), unit="x",
+                              n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched_adjoint=ba)))
         return
     if args.batched == "only":
         bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb)
@@ -402,6 +458,7 @@ def main():
 
     batched = bench_batched(cheb=args.cheb) if args.batched == "on" else None
     adjoint = bench_adjoint() if args.adjoint == "on" else None
+    batched_adjoint = bench_batched_adjoint(T=args.batched_adjoint_T) if args.batched_adjoint == "on" else None
 
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
     # problem, a bounded sample of RK4 steps from the same initial state
@@ -432,7 +489,7 @@ def main():
                                 "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
-               batched=batched, adjoint=adjoint,
+               batched=batched, adjoint=adjoint, batched_adjoint=batched_adjoint,
                adjoint_forward_ratio=adjoint["adjoint_forward_ratio"] if adjoint else None,
                final_state_finite=finite)
     print(json.dumps(out))

@@ -294,20 +294,19 @@ def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0
     """The batched-adjoint workload: `count` source-mode instances drawn from
     numpy's generator of `seed`, each with N in {64, 96, 128}, q_u = q_v in
     {5, 6}, c = 1, alpha log-uniform in [1e-6, 1e-5], beta0 uniform in [6, 8],
-    and each side absorbing (absorbing_2beta or absorbing) or a wall (kappa
-    0.5). The Gaussian source has width r0 at x0 (omega 16 pi, Ts = 1); its
-    amplitude is set from a linear (beta = 0) run of the instance's degree on
-    a 32 x 32 mesh over min(T, 0.6) so that 1 - 2 beta0 max|v| stays about
-    `margin` (with a 0.8 safety factor for the steepening). The run takes
-    round(T / dt) steps of dt = h / (10 q^2). The coefficients of the gradient
-    are beta = beta0 + 0.5 sin(2 pi x) cos(2 pi y) per element (c^2 = 1); the
-    target is the bottom trace of the run with beta = beta0."""
+    the bottom side absorbing_2beta and each other side absorbing_2beta,
+    absorbing or a wall (kappa 0.5). The Gaussian source has width r0 at x0
+    (omega 16 pi, Ts = 1); its amplitude is the largest for which the run with
+    beta = beta0 on a 32 x 32 mesh of the same degree keeps
+    min(1 - 2 beta0 v_h) >= margin + 0.05 at its record steps (bisection), so
+    that the instance keeps about `margin`. The run takes round(T / dt) steps of
+    dt = h / (10 q^2). The gradient is taken at beta = beta0 + 0.5 sin(2 pi x)
+    cos(2 pi y) per element (c^2 = 1); the target is the bottom trace of the
+    run with beta = beta0."""
     import math
-    from .solver import solve
     rng = np.random.default_rng(seed)
     kinds = ("absorbing_2beta", "absorbing", "wall")
     out = []
-    vlin = {}
     for _ in range(count):
         N = int(rng.choice([64, 96, 128]))
         q = int(rng.choice([5, 6]))
@@ -315,28 +314,61 @@ def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0
         beta0 = float(rng.uniform(6, 8))
         sides = {s: kinds[int(rng.integers(0, 3))] for s in ("bot", "top", "lft", "rgt")}
         sides["bot"] = "absorbing_2beta"   # the receivers' side absorbs
-        if q not in vlin:
-            Tc = min(T, 0.6)
-            H = solve(q, 32, Tc, mixed=False, mode="source", boundary="absorbing",
-                      coefficients=_constant(alpha=1e-5, beta=0.0), source=GaussianSource(x0=tuple(x0), r0=r0),
-                      record_every=4)
-            vlin[q] = float(max(np.max(H.vmax), 1e-300))
-        amp = 0.8 * (1.0 - margin) / (2.0 * beta0 * vlin[q])
-        src = GaussianSource(x0=tuple(x0), r0=r0, amplitude=amp)
-        adj = SourceAdjoint(q, N, mixed=False, boundary=sides, kappa=0.5, alpha=alpha, source=src, cheb=cheb)
+        amp = _calibrated_amplitude(q, alpha, beta0, sides, T, x0, r0, margin + 0.05)
         h = 1.0 / N
         dt = h / (10 * q * q)
         nsteps = max(1, int(round(T / dt)))
         dt = T / nsteps
         xc = (np.arange(N) + 0.5) * h
         X, Y = xc[None, :], xc[:, None]
-        adj.set_params(1.0, beta0)
-        adj.set_target(adj.traces(dt, nsteps))
         beta = np.asfortranarray(beta0 + 0.5 * np.sin(2 * math.pi * X) * np.cos(2 * math.pi * Y))
-        adj.set_params(1.0, beta)
+        for _try in range(8):   # the finer mesh may steepen a little more than the calibration mesh
+            src = GaussianSource(x0=tuple(x0), r0=r0, amplitude=amp)
+            adj = SourceAdjoint(q, N, mixed=False, boundary=sides, kappa=0.5, alpha=alpha, source=src, cheb=cheb)
+            try:
+                adj.set_params(1.0, beta0)
+                adj.set_target(adj.traces(dt, nsteps))
+                adj.set_params(1.0, beta)
+                adj.misfit(dt, nsteps)
+                break
+            except WedgError:
+                amp *= 0.9
+        else:
+            raise WedgError("no source amplitude keeps the instance's mass matrices positive definite")
         out.append(SimpleNamespace(adj=adj, N=N, q=q, alpha=alpha, beta0=beta0, beta=beta, sides=sides,
                                    amplitude=amp, dt=dt, steps=nsteps))
     return out
+
+
+def _calibrated_amplitude(q, alpha, beta0, sides, T, x0, r0, floor, Ncal: int = 32, iters: int = 10):
+    """The largest source amplitude (bisection) for which the source-mode run
+    of degree q with beta = beta0 on an Ncal x Ncal mesh to T keeps
+    min(1 - 2 beta0 v_h) >= floor at its record steps."""
+    from .solver import solve
+
+    def ndmin(amp):
+        try:
+            H = solve(q, Ncal, T, mixed=False, mode="source", boundary=sides, kappa=0.5,
+                      coefficients=_constant(alpha, beta0), source=GaussianSource(x0=tuple(x0), r0=r0, amplitude=amp),
+                      record_every=4)
+            return float(np.min(H.ndmin))
+        except WedgError:
+            return -np.inf
+
+    H = solve(q, Ncal, T, mixed=False, mode="source", boundary=sides, kappa=0.5, coefficients=_constant(alpha, 0.0),
+              source=GaussianSource(x0=tuple(x0), r0=r0, amplitude=1.0), record_every=4)
+    vlin = float(max(np.max(H.vmax), 1e-300))
+    hi = (1.0 - floor) / (2.0 * beta0 * vlin)   # the linear estimate: the nonlinear peak is higher
+    if ndmin(hi) >= floor:
+        return hi
+    lo = 0.0
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        if ndmin(mid) >= floor:
+            lo = mid
+        else:
+            hi = mid
+    return lo
 
 
 def _constant(alpha, beta):
