@@ -222,6 +222,48 @@ def bench_batched_adjoint(T=0.02, count=16, seed=2, snapshots=32, sequential=Tru
     return out
 
 
+def bench_graph(chunk=16):
+    """Plain stage launches against HIP-graph chunks of `chunk` RK4 steps
+    (wedg_graph_enable) on meshes from launch-bound to the headline size, the
+    same steps from the same state (bitwise the same end states)."""
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import _lib
+    L = _lib.lib
+    P = C.POINTER(C.c_double)
+    out = []
+    for q, N, steps in ((3, 16, 4000), (4, 64, 1000), (5, 128, 400)):
+        pre, _ = w.precompute(q, N, mixed=False, flux="upwind", boundary="absorbing_2beta", mode="initial_value",
+                              initial_profile=gaussian_profile(),
+                              coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=0.0))
+        ws = w.Workspace(pre, N)
+        h = ws.dev.h
+        dof = N * N * (pre.qu1 ** 2 + pre.qv1 ** 2)
+        dt = 1.0 / N / (10 * q * q)
+        u = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
+        v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
+        res = {}
+        for k in (0, chunk, 0, chunk):                       # each twice; the second run is timed
+            ws.dev.check(L.wedg_graph_enable(h, k))
+            ws.dev.check(L.wedg_set_state(h, u.ctypes.data_as(P), v.ctypes.data_as(P)))
+            tend = C.c_double(0.0)
+            ws.dev.check(L.wedg_sync(h))
+            c0 = time.perf_counter()
+            ws.dev.check(L.wedg_advance(h, 0.0, dt, steps, 0, C.byref(tend)))
+            el = time.perf_counter() - c0
+            uo = np.zeros_like(u)
+            vo = np.zeros_like(v)
+            ws.dev.check(L.wedg_get_state(h, uo.ctypes.data_as(P), vo.ctypes.data_as(P)))
+            res[k] = (el, uo, vo)
+        same = np.array_equal(res[0][1], res[chunk][1]) and np.array_equal(res[0][2], res[chunk][2])
+        out.append(dict(q=q, N=N, steps=steps, plain_dof_stages_per_s=4 * steps * dof / res[0][0],
+                        graph_dof_stages_per_s=4 * steps * dof / res[chunk][0], speedup=res[0][0] / res[chunk][0],
+                        us_per_stage_plain=1e6 * res[0][0] / (4 * steps),
+                        us_per_stage_graph=1e6 * res[chunk][0] / (4 * steps), bitwise_equal=bool(same)))
+        print(f"# graph q={q} N={N}: {out[-1]['speedup']:.2f}x", file=sys.stderr, flush=True)
+    return dict(chunk_steps=chunk, runs=out, workload="initial_value Gaussian, c=1 alpha=1e-4 beta=0, upwind, "
+                                                        "absorbing_2beta (linear: no positivity limit over the steps)")
+
+
 def reference_runs():
     """Wall times (precompute, upload, run and results, on this machine) of the
     runs the reference times itself, as it states them: README.md:244-245,
@@ -319,6 +361,8 @@ def main():
                          "(on), not at all (off), or alone (only)")
     ap.add_argument("--batched-adjoint-T", type=float, default=0.02,
                     help="end time of the batched-adjoint runs (the workload is defined to 1.0)")
+    ap.add_argument("--graph", choices=("on", "off", "only"), default="on",
+                    help="plain stage launches against HIP-graph chunks on small to large meshes")
     ap.add_argument("--reference-runs", action="store_true",
                     help="only time the runs the reference times itself (README, data files) and exit")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -350,6 +394,10 @@ def main():
         ad = bench_adjoint()
         print(json.dumps(dict(metric="adjoint/forward cost ratio", value=ad["adjoint_forward_ratio"], unit="x",
                               n_gpus=1, higher_is_better=False, dtype="f64", data="synthetic", adjoint=ad)))
+        return
+    if args.graph == "only":
+        print(json.dumps(dict(metric="HIP graph speedup of RK4 chunks", value=None, unit="x", n_gpus=1,
+                              higher_is_better=True, dtype="f64", data="synthetic", graph=bench_graph())))
         return
     if args.batched_adjoint == "only":
         ba = bench_batched_adjoint(T=args.batched_adjoint_T)
@@ -459,6 +507,7 @@ def main():
     batched = bench_batched(cheb=args.cheb) if args.batched == "on" else None
     adjoint = bench_adjoint() if args.adjoint == "on" else None
     batched_adjoint = bench_batched_adjoint(T=args.batched_adjoint_T) if args.batched_adjoint == "on" else None
+    graph = bench_graph() if args.graph == "on" else None
 
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same
     # problem, a bounded sample of RK4 steps from the same initial state
@@ -489,7 +538,7 @@ def main():
                                 "timed window is the part it completes",
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
-               batched=batched, adjoint=adjoint, batched_adjoint=batched_adjoint,
+               batched=batched, adjoint=adjoint, batched_adjoint=batched_adjoint, graph=graph,
                adjoint_forward_ratio=adjoint["adjoint_forward_ratio"] if adjoint else None,
                final_state_finite=finite)
     print(json.dumps(out))

@@ -89,6 +89,7 @@ def _load():
         "wedg_advance": [vp, C.c_double, C.c_double, C.c_long, C.c_int, C.POINTER(C.c_double)],
         "wedg_advance_async": [vp, C.c_double, C.c_double, C.c_long, C.c_int, C.POINTER(C.c_double)],
         "wedg_sync": [vp],
+        "wedg_graph_enable": [vp, C.c_long],
         "wedg_last_error": [vp, C.POINTER(WedgErrorInfo)],
         "wedg_timing_enable": [vp, C.c_int],
         "wedg_timing_read": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)],

@@ -419,9 +419,12 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     return WEDG_OK;
 }
 
+static void graph_free(wedg_workspace* ws);
+
 extern "C" wedg_status wedg_workspace_destroy(wedg_workspace* ws) {
     if (!ws) return WEDG_OK;
     if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+    graph_free(ws);
     for (auto& u : ws->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     for (auto& e : ws->ev_pool) (void)hipEventDestroy(e);
     for (void* a : ws->allocs) (void)hipFree(a);
@@ -690,6 +693,101 @@ extern "C" wedg_status wedg_state_device(wedg_workspace* ws, double** u, double*
     return WEDG_OK;
 }
 
+static void graph_free(wedg_workspace* ws) {
+    for (int r = 0; r < wedg_workspace::kGraphRing; ++r) {
+        if (ws->gused[r]) (void)hipEventSynchronize(ws->gdone[r]);
+        if (ws->gexec[r]) (void)hipGraphExecDestroy(ws->gexec[r]);
+        if (ws->gdone[r]) (void)hipEventDestroy(ws->gdone[r]);
+        ws->gexec[r] = nullptr;
+        ws->gdone[r] = nullptr;
+        ws->gused[r] = false;
+    }
+    if (ws->graph) (void)hipGraphDestroy(ws->graph);
+    ws->graph = nullptr;
+    ws->graph_p = nullptr;
+    ws->gnodes.clear();
+    ws->gparams.clear();
+    ws->glaunch = 0;
+}
+
+// Captures the stage launches of args (one k_stage each) into the workspace's
+// graph: the kernel nodes in launch order and the executable copies.
+static wedg_status graph_capture(wedg_workspace* ws, const std::vector<KArgs>& args) {
+    HIPOK(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (const KArgs& a : args)
+        if (le == hipSuccess) le = launch(a, ws->p, 1, ws->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(ws->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return WEDG_ERR_HIP;
+    }
+    ws->graph = g;
+    ws->graph_p = ws->p;
+    // a chain (one stream): the root, then each node's one dependent
+    size_t nr = 0;
+    HIPOK(hipGraphGetRootNodes(g, nullptr, &nr));
+    if (nr != 1) return WEDG_ERR_HIP;
+    hipGraphNode_t node;
+    HIPOK(hipGraphGetRootNodes(g, &node, &nr));
+    for (;;) {
+        hipKernelNodeParams kp;
+        HIPOK(hipGraphKernelNodeGetParams(node, &kp));
+        ws->gnodes.push_back(node);
+        ws->gparams.push_back(kp);
+        size_t nd = 0;
+        HIPOK(hipGraphNodeGetDependentNodes(node, nullptr, &nd));
+        if (nd == 0) break;
+        if (nd != 1) return WEDG_ERR_HIP;
+        HIPOK(hipGraphNodeGetDependentNodes(node, &node, &nd));
+    }
+    if (ws->gnodes.size() != args.size()) return WEDG_ERR_HIP;
+    for (int r = 0; r < wedg_workspace::kGraphRing; ++r) {
+        HIPOK(hipGraphInstantiate(&ws->gexec[r], g, nullptr, nullptr, 0));
+        HIPOK(hipEventCreateWithFlags(&ws->gdone[r], hipEventDisableTiming));
+    }
+    return WEDG_OK;
+}
+
+// The stage launches of args through the graph: capture on first use (the
+// captured launches do not run), then each launch sets the arguments of an
+// executable copy whose previous launch has ended and launches it.
+static wedg_status graph_run(wedg_workspace* ws, const std::vector<KArgs>& args) {
+    if (ws->graph && (ws->graph_p != ws->p || ws->gnodes.size() != args.size())) graph_free(ws);
+    if (!ws->graph) {
+        const wedg_status st = graph_capture(ws, args);
+        if (st) {
+            graph_free(ws);
+            return st;
+        }
+    }
+    const int r = (int)(ws->glaunch % wedg_workspace::kGraphRing);
+    if (ws->gused[r]) HIPOK(hipEventSynchronize(ws->gdone[r]));
+    for (size_t i = 0; i < args.size(); ++i) {
+        hipKernelNodeParams kp = ws->gparams[i];
+        void* kargs_ptr[1] = {const_cast<KArgs*>(&args[i])};
+        kp.kernelParams = kargs_ptr;
+        kp.extra = nullptr;
+        HIPOK(hipGraphExecKernelNodeSetParams(ws->gexec[r], ws->gnodes[i], &kp));
+    }
+    HIPOK(hipGraphLaunch(ws->gexec[r], ws->stream));
+    HIPOK(hipEventRecord(ws->gdone[r], ws->stream));
+    ws->gused[r] = true;
+    ++ws->glaunch;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_graph_enable(wedg_workspace* ws, long chunk_steps) {
+    if (!ws || chunk_steps < 0 || chunk_steps > 4096) return WEDG_ERR_ARGUMENT;
+    if (chunk_steps != ws->graph_steps) {
+        HIPOK(hipStreamSynchronize(ws->stream));
+        graph_free(ws);
+    }
+    ws->graph_steps = chunk_steps;
+    return WEDG_OK;
+}
+
 extern "C" wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double dt, long nsteps,
                                           int stop_on_nonfinite, double* t_end) {
     if (!ws || nsteps < 0) return WEDG_ERR_ARGUMENT;
@@ -697,7 +795,27 @@ extern "C" wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double 
     wedg_status s;
     if ((s = reset_err(ws))) return s;
     double t = t0;
-    for (long it = 0; it < nsteps; ++it) {
+    long it = 0;
+    // whole chunks of graph_steps steps through the graph (the same arguments,
+    // in the same order, as the launches below)
+    const long K = ws->graph_steps;
+    if (K > 0 && !ws->timing) {
+        std::vector<KArgs> args((size_t)4 * K);
+        while (nsteps - it >= K) {
+            KArgs prep{};
+            const bool first = it == 0;
+            for (long k = 0; k < K; ++k, ++it) {
+                for (int st = 1; st <= 4; ++st) {
+                    args[(size_t)4 * k + st - 1] = rk4_args(ws, st, t, dt, it + 1, it == nsteps - 1);
+                    if (first && k == 0 && st == 1) prep = args[0];
+                }
+                t += dt;
+            }
+            if (first) HIPOK(launch(prep, ws->p, 0, ws->stream));
+            if ((s = graph_run(ws, args))) return s;
+        }
+    }
+    for (; it < nsteps; ++it) {
         if ((s = rk4_device(ws, t, dt, it + 1, it == 0, it == nsteps - 1))) return s;
         t += dt;
     }

@@ -181,6 +181,20 @@ struct wedg_workspace {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     double t_ms[4] = {0, 0, 0, 0};
     long t_n[4] = {0, 0, 0, 0};
+    // a HIP graph of the stage launches of graph_steps RK4 steps (wedg_graph_enable):
+    // the captured graph, its kernel nodes in launch order with their launch
+    // configuration, and a ring of executable copies whose arguments are set
+    // before each launch (a copy is reused once its previous launch has ended)
+    static constexpr int kGraphRing = 3;
+    long graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    const wedg_problem* graph_p = nullptr;
+    std::vector<hipGraphNode_t> gnodes;
+    std::vector<hipKernelNodeParams> gparams;
+    hipGraphExec_t gexec[kGraphRing] = {};
+    hipEvent_t gdone[kGraphRing] = {};
+    bool gused[kGraphRing] = {};
+    long glaunch = 0;
     size_t nU() const { return (size_t)p->E * p->nu * p->nu; }
     size_t nwaves() const { return ((size_t)p->E + kWave - 1) / kWave; }
     size_t nV() const { return (size_t)p->E * p->nv * p->nv; }

@@ -156,6 +156,12 @@ wedg_status wedg_set_state(wedg_workspace* ws, const double* u, const double* v)
 wedg_status wedg_get_state(wedg_workspace* ws, double* u, double* v);
 wedg_status wedg_advance(wedg_workspace* ws, double t0, double dt, long nsteps,
                          int stop_on_nonfinite, double* t_end);
+/* wedg_advance / wedg_advance_async run whole chunks of chunk_steps steps as
+ * one HIP graph: the chunk's stage launches are captured once and replayed
+ * with each chunk's arguments (the loop of solve, src/solver.jl:213-216, with
+ * RK4 fused into the stage kernels); the results are those of the plain
+ * launches, bit for bit. 0 (the default) launches every stage; 1..4096. */
+wedg_status wedg_graph_enable(wedg_workspace* ws, long chunk_steps);
 /* The same without waiting: the caller calls wedg_sync before reading results. */
 wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double dt, long nsteps,
                                int stop_on_nonfinite, double* t_end);
