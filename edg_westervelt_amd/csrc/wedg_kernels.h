@@ -215,6 +215,44 @@ __device__ __forceinline__ const double* vfresh(const double* p) {
     return p;
 }
 
+// Element-fastest arrays x[m][E] of doubles through buffer instructions: the
+// lane's byte offset 8 e sits in one VGPR for every mode m and the mode's
+// byte offset 8 m E in an SGPR, so an access needs no 64-bit address
+// arithmetic and no address registers of its own (arrays below 2 GiB).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef unsigned int wedg_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ersrc(const double* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ double eld(__amdgpu_buffer_rsrc_t r, int e, int m, int E) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8, m * E * 8, 0));
+}
+__device__ __forceinline__ void est(__amdgpu_buffer_rsrc_t r, int e, int m, int E, double x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wedg_u32x2, x), r, e * 8, m * E * 8, 0);
+}
+template <int n>
+__device__ __forceinline__ void eload(const double* p, int E, int e, double (&x)[n]) {
+    const __amdgpu_buffer_rsrc_t r = ersrc(p);
+#pragma unroll
+    for (int m = 0; m < n; ++m) x[m] = eld(r, e, m, E);
+}
+template <int n>
+__device__ __forceinline__ void estore(double* p, int E, int e, const double (&x)[n]) {
+    const __amdgpu_buffer_rsrc_t r = ersrc(p);
+#pragma unroll
+    for (int m = 0; m < n; ++m) est(r, e, m, E, x[m]);
+}
+#else
+template <int n>
+__device__ __forceinline__ void eload(const double* p, int E, int e, double (&x)[n]) {
+    for (int m = 0; m < n; ++m) x[m] = p[(long)m * E + e];
+}
+template <int n>
+__device__ __forceinline__ void estore(double* p, int E, int e, const double (&x)[n]) {
+    for (int m = 0; m < n; ++m) p[(long)m * E + e] = x[m];
+}
+#endif
+
 // value of face array x at flat index fi = i * (N (N + 1)) + face
 template <bool UNI, class KA>
 __device__ __forceinline__ double fval(const KA& a, int x, long fi) {
@@ -392,25 +430,43 @@ __device__ __forceinline__ void interior_point(const KA& a, bool plus, bool hz, 
 // Slot values of one point i of a physical boundary side s (src/rhs.jl:738-765);
 // returns the point's failing _boundary_key (src/rhs.jl:732) or 0. HZ: s is a
 // horizontal side (0 or 1, possibly different in different lanes).
+// The data of boundary point i of side s: c^2, alpha, c, beta at the point and
+// the boundary data (sp, spn). A side's points are all loaded before any of
+// its slots is stored: the stores may alias these arrays as far as the
+// compiler knows, and would otherwise hold each point's loads until the
+// previous point's stores.
+struct BPoint {
+    double c2f, af, cf, bf, sp, spn;
+};
+
 template <bool UNI, bool HZ, class KA>
-__device__ __forceinline__ int boundary_point(const KA& a, int s, long fi, int i, int m, int nq, double du,
+__device__ __forceinline__ BPoint boundary_data(const KA& a, int s, long fi, int i, int m, int nq) {
+    constexpr bool hz = HZ;
+    BPoint d;
+    d.c2f = fval<UNI>(a, hz ? F_HC2 : F_VC2, fi);
+    d.af = fval<UNI>(a, hz ? F_HAF : F_VAF, fi);
+    d.cf = fval<UNI>(a, hz ? F_HCF : F_VCF, fi);
+    d.bf = fval<UNI>(a, hz ? F_HBF : F_VBF, fi);
+    const double* bsp = HZ ? (s == 0 ? a.bcd[0] : a.bcd[2]) : a.bcd[2 * s];
+    const double* bspn = HZ ? (s == 0 ? a.bcd[1] : a.bcd[3]) : a.bcd[2 * s + 1];
+    d.sp = bsp[i + nq * m];
+    d.spn = bspn[i + nq * m];
+    return d;
+}
+
+template <bool UNI, bool HZ, class KA>
+__device__ __forceinline__ int boundary_point(const KA& a, int s, const BPoint& d, int i, int m, int nq, double du,
                                               double vt, double dv, double ct, double stt, double& A, double& Dd,
                                               double& Ff) {
-    constexpr bool hz = HZ;
     // the side's parameters: with HZ a choice between the two horizontal sides'
     // (uniform values), so that no lane-indexed load is needed
     auto sv = [&](const auto& arr) { return HZ ? (s == 0 ? arr[0] : arr[1]) : arr[s]; };
     const double sgn = (s == 0 || s == 2) ? -1.0 : 1.0;
-    const double c2f = fval<UNI>(a, hz ? F_HC2 : F_VC2, fi);
-    const double af = fval<UNI>(a, hz ? F_HAF : F_VAF, fi);
-    const double cf = fval<UNI>(a, hz ? F_HCF : F_VCF, fi);
-    const double bf = fval<UNI>(a, hz ? F_HBF : F_VBF, fi);
+    const double c2f = d.c2f, af = d.af, cf = d.cf, bf = d.bf;
     const double two_h = a.two_h, om = a.omega;
     const double gun = sgn * (two_h * du);
     const double gvn = sgn * (two_h * dv);
-    const double* bsp = HZ ? (s == 0 ? a.bcd[0] : a.bcd[2]) : a.bcd[2 * s];
-    const double* bspn = HZ ? (s == 0 ? a.bcd[1] : a.bcd[3]) : a.bcd[2 * s + 1];
-    const double spv = bsp[i + nq * m], spnv = bspn[i + nq * m];
+    const double spv = d.sp, spnv = d.spn;
     const double gu_ex = sgn * (ct * spnv - spnv);
     const double gv_ex = sgn * (-om * stt * spnv);
     const double v_ex = -om * stt * spv;
@@ -562,11 +618,13 @@ __device__ __forceinline__ void stage_boundary(const double (&U)[D::NU * D::NU],
             const int m = s < 2 ? k : j;
             const int fidx = s == 0 ? (N + 1) * k : s == 1 ? N + (N + 1) * k : s == 2 ? j : j + N * N;
             double* bs = A.bsl + (long)s * NQ * 3 * N + m;
+            BPoint pd[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) pd[i] = boundary_data<UNI, (s < 2)>(A, s, (long)i * A.fstride + fidx, i, m, NQ);
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
-                const long fi = (long)i * A.fstride + fidx;
                 double Av, Dd, Ff;
-                const int k2 = boundary_point<UNI, (s < 2)>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
+                const int k2 = boundary_point<UNI, (s < 2)>(A, s, pd[i], i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
                 if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
                 bs[(long)(i * 3 + 0) * N] = Av;
                 bs[(long)(i * 3 + 1) * N] = Dd;
@@ -587,11 +645,13 @@ __device__ __forceinline__ void stage_boundary(const double (&U)[D::NU * D::NU],
             const int m = k;
             const int fidx = pe ? N + (N + 1) * k : (N + 1) * k;
             double* bs = A.bsl + (long)s * NQ * 3 * N + m;
+            BPoint pd[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) pd[i] = boundary_data<UNI, true>(A, s, (long)i * A.fstride + fidx, i, m, NQ);
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
-                const long fi = (long)i * A.fstride + fidx;
                 double Av, Dd, Ff;
-                const int k2 = boundary_point<UNI, true>(A, s, fi, i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
+                const int k2 = boundary_point<UNI, true>(A, s, pd[i], i, m, NQ, du[i], vt[i], dv[i], ct, st, Av, Dd, Ff);
                 if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
                 bs[(long)(i * 3 + 0) * N] = Av;
                 bs[(long)(i * 3 + 1) * N] = Dd;
@@ -634,8 +694,7 @@ __global__ __launch_bounds__(kWave) void k_prep(const std::conditional_t<BT, Bat
     if (e < E) {
         const double* yv = kargs<D::BT>()->yv;
         double V[NV * NV];
-#pragma unroll
-        for (int m = 0; m < NV * NV; ++m) V[m] = yv[(long)m * E + e];
+        eload<NV * NV>(yv, E, e, V);
         n_e = stage_prep<D, UNI>(V, e, kargs<D::BT>()->serial, global);
     }
     if (global) wave_count<D::BT>(n_e, kargs<D::BT>()->slot, E);
@@ -771,15 +830,13 @@ __device__ __forceinline__ void face_side(int e, int j, int k, int N, int E, con
     {
         const double* yu = A.yu;
         double X[NU * NU];
-#pragma unroll
-        for (int mm = 0; mm < NU * NU; ++mm) X[mm] = yu[(long)mm * E + nb];
+        eload<NU * NU>(yu, E, nb, X);
         side_trace<D, NU, true, ns>(ctab(A.tab), D::oE1r, X, ndu);
     }
     {
         const double* yv = A.yv;
         double X[NV * NV];
-#pragma unroll
-        for (int mm = 0; mm < NV * NV; ++mm) X[mm] = yv[(long)mm * E + nb];
+        eload<NV * NV>(yv, E, nb, X);
         cdbl* T = ctab(A.tab);
         side_trace<D, NV, false, ns>(T, D::oEr, X, nvv);
         side_trace<D, NV, true, ns>(T, D::oE1r, X, ndv);
@@ -931,42 +988,36 @@ __device__ __forceinline__ void kd_mid(cdbl* Ph, const double (&d)[D::NV * D::NV
 template <int n>
 __device__ __forceinline__ void rk_load(int stage, const double* y0, const double* acc, int E, int e,
                                         double (&y)[n], double (&ac)[n]) {
-    if (stage >= 1) {
-#pragma unroll
-        for (int m = 0; m < n; ++m) y[m] = y0[(long)m * E + e];
-    }
-    if (stage >= 2) {
-#pragma unroll
-        for (int m = 0; m < n; ++m) ac[m] = acc[(long)m * E + e];
-    }
+    if (stage >= 1) eload<n>(y0, E, e, y);
+    if (stage >= 2) eload<n>(acc, E, e, ac);
 }
 
 template <int n>
 __device__ __forceinline__ bool rk_store(int stage, double c, const double (&kv)[n], const double (&y)[n],
                                          const double (&ac)[n], double* acc, double* out, int E, int e) {
     bool fin = true;
+    double o[n];
     if (stage == 0) {
-#pragma unroll
-        for (int m = 0; m < n; ++m) out[(long)m * E + e] = kv[m];
+        estore<n>(out, E, e, kv);
     } else if (stage == 1) {
 #pragma unroll
-        for (int m = 0; m < n; ++m) {
-            out[(long)m * E + e] = y[m] + c * kv[m];
-            acc[(long)m * E + e] = kv[m];
-        }
+        for (int m = 0; m < n; ++m) o[m] = y[m] + c * kv[m];
+        estore<n>(out, E, e, o);
+        estore<n>(acc, E, e, kv);
     } else if (stage < 4) {
 #pragma unroll
-        for (int m = 0; m < n; ++m) {
-            out[(long)m * E + e] = y[m] + c * kv[m];
-            acc[(long)m * E + e] = ac[m] + 2.0 * kv[m];
-        }
+        for (int m = 0; m < n; ++m) o[m] = y[m] + c * kv[m];
+        estore<n>(out, E, e, o);
+#pragma unroll
+        for (int m = 0; m < n; ++m) o[m] = ac[m] + 2.0 * kv[m];
+        estore<n>(acc, E, e, o);
     } else {
 #pragma unroll
         for (int m = 0; m < n; ++m) {
-            const double yn = y[m] + c * (ac[m] + kv[m]);
-            out[(long)m * E + e] = yn;
-            fin = fin && isfinite(yn);
+            o[m] = y[m] + c * (ac[m] + kv[m]);
+            fin = fin && isfinite(o[m]);
         }
+        estore<n>(out, E, e, o);
     }
     return fin;
 }
@@ -1031,10 +1082,8 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
         const int ec = e < E ? e : E - 1;
         const double* yu = kargs<D::BT>()->yu;
         const double* yv = kargs<D::BT>()->yv;
-#pragma unroll
-        for (int mm = 0; mm < nU; ++mm) U[mm] = yu[(long)mm * E + ec];
-#pragma unroll
-        for (int mm = 0; mm < nV; ++mm) V[mm] = yv[(long)mm * E + ec];
+        eload<nU>(yu, E, ec, U);
+        eload<nV>(yv, E, ec, V);
         if (global) {
             for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
             nglob = m;
@@ -1085,8 +1134,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
                 const double pc = kargs<D::BT>()->pc[t];
                 const double* pr = kargs<D::BT>()->proj[t];
                 double pv[nV];
-#pragma unroll
-                for (int m = 0; m < nV; ++m) pv[m] = pr[(long)m * E + e];
+                eload<nV>(pr, E, e, pv);
 #pragma unroll
                 for (int m = 0; m < nV; ++m) Rv[m] = Rv[m] + pc * pv[m];
             }
@@ -1168,8 +1216,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
             CKArgs& A = *kargs<D::BT>();
             if (A.recu) {
                 double* ru = A.recu;
-#pragma unroll
-                for (int m = 0; m < nU; ++m) ru[(long)m * E + e] = ku[m];
+                estore<nU>(ru, E, e, ku);
             }
             double yu0[nU], acu[nU];
             rk_load<nU>(A.stage, A.y0u, A.accu, E, e, yu0, acu);
@@ -1189,8 +1236,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
         {
             const double* yv = kargs<D::BT>()->yv;
             double V[nV];
-#pragma unroll
-            for (int m = 0; m < nV; ++m) V[m] = yv[(long)m * E + e];
+            eload<nV>(yv, E, e, V);
             const double* betav = kargs<D::BT>()->beta + (UNI ? 0 : (long)kargs<D::BT>()->cls_b[e] * NH * NH);
             double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
             bool nan = false;
@@ -1357,8 +1403,7 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
             const double c = A.cstage;
             if (A.recv) {
                 double* rv = A.recv;
-#pragma unroll
-                for (int m = 0; m < nV; ++m) rv[(long)m * E + e] = X[m];
+                estore<nV>(rv, E, e, X);
             }
             rk_load<nV>(stage, A.y0v, A.accv, E, e, yv0, acv);
             if (!rk_store<nV>(stage, c, X, yv0, acv, A.accv, A.outv, E, e)) {
