@@ -345,6 +345,8 @@ def main():
                          "limit near t = 0.131)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cheb", choices=("global", "element"), default="global")
+    ap.add_argument("--lanes", type=int, choices=(1, 2), default=1,
+                    help="lanes per element of the stage kernels of the headline run")
     ap.add_argument("--cpu-steps", type=int, default=192, help="RK4 steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
@@ -420,7 +422,7 @@ def main():
     nu, nv, nh = pre.qu1, pre.qv1, pre.q_high
     dof = E * (nu * nu + nv * nv)
 
-    ws = w.Workspace(pre, N, cheb=args.cheb)
+    ws = w.Workspace(pre, N, cheb=args.cheb, lanes=args.lanes)
     h = ws.dev.h
     L = _lib.lib
     u = np.zeros((N, N, nu, nu), order="F")
@@ -483,19 +485,21 @@ def main():
     # algorithmic flops per element and stage over its average duration
     f_total, f_traces = flops_per_element_stage(nu, nv, nh, nc_timing)
     st_s = fam["stage"]["us_avg"] * 1e-6
+    lanes = ws.dev.lanes
+    kname = "k_stage2" if lanes == 2 else "k_stage"
     achieved = E * f_total / st_s / 1e12
     roof = dict(bound="mfma", achieved=achieved, peak=FP64_PEAK_TFS, unit="TFLOP/s",
-                frac=achieved / FP64_PEAK_TFS, traffic=None, kernel="k_stage",
+                frac=achieved / FP64_PEAK_TFS, traffic=None, kernel=kname,
                 flops_per_launch=E * f_total, us_avg=fam["stage"]["us_avg"])
     bytes_stage = 8 * 17 * dof / 4
     roof_hbm = dict(bound="hbm", achieved=bytes_stage / st_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=bytes_stage / st_s / 1e9 / HBM_PEAK_GBS, kernel="k_stage",
+                    frac=bytes_stage / st_s / 1e9 / HBM_PEAK_GBS, kernel=kname,
                     bytes_per_launch=bytes_stage, note="compulsory bytes of the three-register RK4, 34 B per DOF")
     traffic_file = args.traffic or os.path.join(ROOT, "profiles", f"traffic_q{q}_n{N}.json")
     if os.path.exists(traffic_file):
         tj = json.load(open(traffic_file))
-        roof["traffic"] = tj.get("k_stage")
-        roof_hbm["traffic"] = tj.get("k_stage")
+        roof["traffic"] = tj.get(kname)
+        roof_hbm["traffic"] = tj.get(kname)
     stage_s = elapsed / (4 * steps)
     bytes_step = 8 * 17 * dof
     whole = dict(flops_per_stage=E * f_total, tflops=E * f_total / stage_s / 1e12,
@@ -531,7 +535,8 @@ def main():
                data="synthetic",
                config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, c=1 alpha=1e-4 beta=2, "
                                     f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
-                           elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, mean_cheb_steps=nc_timing,
+                           elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, lanes_per_element=lanes,
+                           mean_cheb_steps=nc_timing,
                            max_cheb_steps=info.cheb_steps, full_run_steps=nsteps_full,
                            timed_window=[t0, t0 + steps * dt],
                            note="the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "

@@ -90,6 +90,9 @@ def _load():
         "wedg_advance_async": [vp, C.c_double, C.c_double, C.c_long, C.c_int, C.POINTER(C.c_double)],
         "wedg_sync": [vp],
         "wedg_graph_enable": [vp, C.c_long],
+        "wedg_workspace_set_lanes_per_element": [vp, C.c_int],
+        "wedg_problem_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "wedg_workspace_lanes_per_element": [vp, C.POINTER(C.c_int)],
         "wedg_last_error": [vp, C.POINTER(WedgErrorInfo)],
         "wedg_timing_enable": [vp, C.c_int],
         "wedg_timing_read": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)],
@@ -207,6 +210,13 @@ class Problem:
         self.N = N
         self.pre = pre
 
+    def info(self) -> dict:
+        """The kernel layout of the upload: mode (0 general, 1 separable, 2
+        uniform, 3 conjugate gradients), coefficient classes, two-lane capable."""
+        m, c, t = C.c_int(0), C.c_int(0), C.c_int(0)
+        lib.wedg_problem_info(self.h, C.byref(m), C.byref(c), C.byref(t))
+        return dict(mode=m.value, classes=c.value, two_lane=bool(t.value))
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
@@ -217,7 +227,7 @@ class Problem:
 class DeviceWorkspace:
     """Device-resident work arrays and state (``wedg_workspace_create``)."""
 
-    def __init__(self, problem: Problem, cheb: str = "global"):
+    def __init__(self, problem: Problem, cheb: str = "global", lanes: int = 1):
         self.problem = problem
         pol = {"global": CHEB_GLOBAL, "element": CHEB_ELEMENT}[cheb]
         h = C.c_void_p()
@@ -225,6 +235,21 @@ class DeviceWorkspace:
         if st != OK:
             raise WedgError(f"wedg_workspace_create failed with status {st}")
         self.h = h
+        if lanes != 1:
+            self.set_lanes(lanes)
+
+    def set_lanes(self, lanes: int):
+        """Lanes per element of the stage kernels (1, or 2 where the problem allows it)."""
+        st = lib.wedg_workspace_set_lanes_per_element(self.h, int(lanes))
+        if st != OK:
+            raise WedgError(f"wedg_workspace_set_lanes_per_element failed with status {st}")
+
+    @property
+    def lanes(self) -> int:
+        """The lanes per element the workspace's calls use."""
+        n = C.c_int(0)
+        lib.wedg_workspace_lanes_per_element(self.h, C.byref(n))
+        return n.value
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -111,7 +111,8 @@ bool all_equal(const double* p, size_t n) {
 
 }  // namespace wedg
 
-hipError_t wedg::launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s) {
+hipError_t wedg::launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two) {
+    if (two) return launch2(a, p->nu, p->nv, which, s, p->mode);
     switch (p->qu) {
         case 1: return launch_q1(a, p->nv, which, s, p->mode);
         case 2: return launch_q2(a, p->nv, which, s, p->mode);
@@ -180,6 +181,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         p->lam = lam;
         p->Q = Q;
     }
+    std::vector<double> d0;   // the spectral u-solve factors of class 0
     // (Cw, Aw, Ainv) per element -> classes
     {
         const int npk = (nu * nu * (nu * nu + 1)) / 2;
@@ -251,6 +253,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
             if (!(dmax <= 1e-10 * amax)) separable = false;
         }
         const bool fd = separable;
+        if (fd) d0.assign(dfac.begin(), dfac.begin() + n);
         UP(p->Cw, cw.data(), cw.size());
         UP(p->Aw, aw.data(), aw.size());
         if (fd) UP(p->Ainv, dfac.data(), dfac.size());
@@ -296,6 +299,69 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         for (int a = 0; a < F_COUNT; ++a) faces_uni = faces_uni && p->f[a].uni;
         if (p->mode == MODE_SEPARABLE && p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) p->mode = MODE_UNIFORM;
     }
+    // the split tables of the two-lane kernels (wedg_stage2.h): per lane parity
+    // h the values T[p][2k + h] of the 1D tables and, for one class, its
+    // u-solve factors D[2k + h][jj]. The kernels read the quadrature weights,
+    // W_h and a one-class problem's coefficient records from row p in both
+    // lanes of a pair, so the problem is admitted only when those are
+    // mirror-symmetric (row NR-1-p equals row p to 1e-13 relative).
+    if (nu % 2 == 0 && nv == nu && nu <= 6 && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE)) {
+        auto mirror_ok = [](const double* a, int nr, int nc, long stride_c) {
+            for (int r = 0; r < nr; ++r)
+                for (int c = 0; c < nc; ++c) {
+                    const double x = a[r + stride_c * c], y = a[nr - 1 - r + stride_c * c];
+                    if (!(std::fabs(x - y) <= 1e-13 * std::max(std::fabs(x), std::fabs(y)))) return false;
+                }
+            return true;
+        };
+        bool sym = mirror_ok(pre->w, nq, 1, 0) && mirror_ok(pre->W2d_h, nh, nh, nh);
+        if (p->mode == MODE_UNIFORM) {
+            // the one class's records as uploaded: Cw, Aw (nq x nq), beta (nh x nh)
+            std::vector<double> rec((size_t)nq * nq);
+            for (int which = 0; which < 2 && sym; ++which) {
+                for (int ix = 0; ix < nq; ++ix)
+                    for (int iy = 0; iy < nq; ++iy)
+                        rec[iy + (size_t)nq * ix] = (which ? pre->Aw : pre->Cw)[iy + (long)nq * (0 + E * ix)];
+                sym = mirror_ok(rec.data(), nq, nq, nq);
+            }
+            std::vector<double> bq((size_t)nh * nh);
+            for (int q = 0; q < nh; ++q)
+                for (int pp = 0; pp < nh; ++pp) bq[pp + (size_t)nh * q] = pre->beta_hq[pp + (long)nh * (0 + E * q)];
+            sym = sym && mirror_ok(bq.data(), nh, nh, nh);
+        }
+        if (sym) {
+            const SplitLayout L = split_layout(nu, nh);
+            const double h = 1.0 / N, two_h = 2.0 / h, four_hh = 4.0 / (h * h);
+            std::vector<double> t2(2 * (size_t)L.n, 0.0);
+            for (int hp = 0; hp < 2; ++hp) {
+                double* s2 = t2.data() + (size_t)hp * L.n;
+                for (int pp = 0; pp < L.NQH; ++pp)
+                    for (int k = 0; k < L.RU; ++k) {
+                        s2[L.sP + pp * L.RU + k] = pre->P_u[pp + nq * (2 * k + hp)];
+                        s2[L.sS + pp * L.RU + k] = pre->S_u[pp + nq * (2 * k + hp)];
+                    }
+                for (int pp = 0; pp < L.NHH; ++pp) {
+                    const bool mid = (nh & 1) && pp == nh / 2;
+                    for (int k = 0; k < L.RU; ++k)
+                        s2[L.sPh + pp * L.RU + k] = (mid && hp == 1) ? 0.0 : pre->P_h_v[pp + nh * (2 * k + hp)];
+                }
+                for (int k = 0; k < L.RU; ++k) {
+                    const int a = 2 * k + hp;
+                    s2[L.sEr + k] = pre->Ebr_u[a];
+                    s2[L.sE1r + k] = pre->E1br_u[a];
+                    s2[L.sTEl + k] = two_h * pre->Ebl_u[a];
+                    s2[L.sTEr + k] = two_h * pre->Ebr_u[a];
+                    s2[L.sFE1l + k] = four_hh * pre->E1bl_u[a];
+                    s2[L.sFE1r + k] = four_hh * pre->E1br_u[a];
+                    s2[L.sE1l + k] = pre->E1bl_u[a];
+                    for (int k2 = 0; k2 < L.RU; ++k2) s2[L.sQ + k2 + L.RU * k] = p->Q[(2 * k2 + hp) + (size_t)nu * a];
+                    if (p->mode == MODE_UNIFORM)
+                        for (int jj = 0; jj < nu; ++jj) s2[L.sD + k + L.RU * jj] = d0[a + (size_t)nu * jj];
+                }
+            }
+            UP(p->tab2, t2.data(), t2.size());
+        }
+    }
     // boundary data (NQ, N), unchanged
     {
         const double* src[8] = {pre->bcB_sp, pre->bcB_spy, pre->bcT_sp, pre->bcT_spy,
@@ -333,6 +399,14 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
     return WEDG_OK;
 }
 
+extern "C" wedg_status wedg_problem_info(const wedg_problem* p, int* mode, int* classes, int* two_lane) {
+    if (!p) return WEDG_ERR_ARGUMENT;
+    if (mode) *mode = p->mode;
+    if (classes) *classes = p->ncls_ca;
+    if (two_lane) *two_lane = p->tab2 && !p->mass_cg && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE);
+    return WEDG_OK;
+}
+
 extern "C" wedg_status wedg_problem_destroy(wedg_problem* p) {
     if (!p) return WEDG_OK;
     for (void* a : p->allocs) (void)hipFree(a);
@@ -352,7 +426,7 @@ wedg_status wedg::reset_err(wedg_workspace* ws) {
     ws->stages = 0;
     ws->serial = 0;   // k_prep forms the count slot anew at the start of every call
     HIPOK(hipMemcpyAsync(ws->err, &h, sizeof(h), hipMemcpyHostToDevice, ws->stream));
-    HIPOK(hipMemsetAsync(ws->wstat, 0, 2 * ws->nwaves() * sizeof(unsigned), ws->stream));
+    HIPOK(hipMemsetAsync(ws->wstat, 0, 2 * ws->nwaves_cap() * sizeof(unsigned), ws->stream));
     return WEDG_OK;
 }
 
@@ -398,7 +472,7 @@ extern "C" wedg_status wedg_workspace_create(wedg_problem* p, int cheb_policy, w
     if (he == hipSuccess) he = hipHostMalloc((void**)&ws->hv, nv * sizeof(double), hipHostMallocDefault);
     if (he == hipSuccess) {
         void* c = nullptr;
-        const size_t nw = ws->nwaves();
+        const size_t nw = ws->nwaves_cap();   // slots for either stage kernel's waves
         he = hipMalloc(&c, 2 * nw * sizeof(int) + 2 * nw * sizeof(unsigned));
         if (he == hipSuccess) {
             ws->allocs.push_back(c);
@@ -460,6 +534,7 @@ KArgs wedg::problem_args(const wedg_problem* p, int cheb_global, double* bsl, De
         a.sig[s] = p->bc[s].sigma; a.kb[s] = p->bc[s].kb; a.kap[s] = p->bc[s].kappa;
     }
     a.tab = p->tab;
+    a.tab2 = p->tab2;
     a.cls_ca = p->cls_ca; a.Cw = p->Cw; a.Aw = p->Aw; a.Ainv = p->Ainv;
     a.cls_b = p->cls_b; a.beta = p->beta;
     for (int i = 0; i < F_COUNT; ++i) { a.f[i] = p->f[i]; a.fu[i] = p->fu[i]; }
@@ -513,7 +588,7 @@ wedg_status wedg::run_kernel(wedg_workspace* ws, const KArgs& a, int which) {
         e1 = ev_get(ws);
         HIPOK(hipEventRecord(e0, ws->stream));
     }
-    HIPOK(launch(a, ws->p, which, ws->stream));
+    HIPOK(launch(a, ws->p, which, ws->stream, ws->two()));
     if (ws->timing) {
         HIPOK(hipEventRecord(e1, ws->stream));
         ws->ev_used.push_back({which, {e0, e1}});
@@ -716,7 +791,7 @@ static wedg_status graph_capture(wedg_workspace* ws, const std::vector<KArgs>& a
     HIPOK(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (const KArgs& a : args)
-        if (le == hipSuccess) le = launch(a, ws->p, 1, ws->stream);
+        if (le == hipSuccess) le = launch(a, ws->p, 1, ws->stream, ws->two());
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(ws->stream, &g);
     if (le != hipSuccess || ce != hipSuccess || !g) {
@@ -775,6 +850,22 @@ static wedg_status graph_run(wedg_workspace* ws, const std::vector<KArgs>& args)
     HIPOK(hipEventRecord(ws->gdone[r], ws->stream));
     ws->gused[r] = true;
     ++ws->glaunch;
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_workspace_set_lanes_per_element(wedg_workspace* ws, int lanes) {
+    if (!ws || (lanes != 1 && lanes != 2)) return WEDG_ERR_ARGUMENT;
+    if (lanes != ws->lpe) {
+        HIPOK(hipStreamSynchronize(ws->stream));
+        graph_free(ws);   // a captured graph holds the other kernels
+        ws->lpe = lanes;
+    }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_workspace_lanes_per_element(const wedg_workspace* ws, int* lanes) {
+    if (!ws || !lanes) return WEDG_ERR_ARGUMENT;
+    *lanes = ws->two() ? 2 : 1;
     return WEDG_OK;
 }
 
@@ -1333,6 +1424,11 @@ extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_
     auto* b = new wedg_batch();
     b->ws.assign(ws, ws + n);
     for (int i = 0; i < n; ++i) {
+        // the batched kernels run one lane per element: so do the instances' own calls
+        if (ws[i]->lpe != 1) {
+            graph_free(ws[i]);
+            ws[i]->lpe = 1;
+        }
         const wedg_problem* p = ws[i]->p;
         auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BatchGroup& g) {
             return g.qu == p->qu && g.nv == p->nv && g.mode == p->mode;

@@ -6,6 +6,7 @@
 #include "../../include/wedg.h"
 #include "wedg_kernels.h"
 #include "wedg_launch.h"
+#include "wedg_stage2.h"
 
 #include <hip/hip_runtime.h>
 
@@ -111,6 +112,7 @@ struct wedg_problem {
     int ncls_ca = 0, ncls_b = 0;
     std::vector<void*> allocs;
     double* tab = nullptr;
+    double* tab2 = nullptr;   // split tables of the two-lane kernels (n_u = n_v even), or null
     int* cls_ca = nullptr;
     double *Cw = nullptr, *Aw = nullptr, *Ainv = nullptr;
     int* cls_b = nullptr;
@@ -195,14 +197,22 @@ struct wedg_workspace {
     hipEvent_t gdone[kGraphRing] = {};
     bool gused[kGraphRing] = {};
     long glaunch = 0;
+    // lanes per element of the stage kernels: 1 (k_stage) or 2 (k_stage2, where
+    // the problem allows it; wedg_workspace_set_lanes_per_element)
+    int lpe = 1;
+    bool two() const {
+        return lpe == 2 && p->tab2 && !p->mass_cg && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE);
+    }
     size_t nU() const { return (size_t)p->E * p->nu * p->nu; }
-    size_t nwaves() const { return ((size_t)p->E + kWave - 1) / kWave; }
+    // waves of the stage kernels this workspace runs (per-wave count slots and statistics)
+    size_t nwaves() const { return ((size_t)p->E + (two() ? kPairWave : kWave) - 1) / (two() ? kPairWave : kWave); }
+    size_t nwaves_cap() const { return ((size_t)p->E + kPairWave - 1) / kPairWave; }
     size_t nV() const { return (size_t)p->E * p->nv * p->nv; }
 };
 
 namespace wedg {
 
-hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s);
+hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two = false);
 wedg_status reset_err(wedg_workspace* ws);
 wedg_status collect(wedg_workspace* ws, int nonfinite_is_error);
 KArgs problem_args(const wedg_problem* p, int cheb_global, double* bsl, DevErr* err, int* cnt, unsigned* wstat);

@@ -140,6 +140,7 @@ struct KArgs {
     int* cnt;                               // Chebyshev count of every wave [2][waves], by stage parity
     unsigned* wstat;                        // per wave: sum [waves] and max [waves] of the counts it ran
     double* recu; double* recv;             // when not null: the stage's derivative k = F(Y_s) is kept here
+    const double* tab2;                     // the split tables of the two-lane kernels (wedg_stage2.h) or null
 };
 
 // ---------------------------------------------------------------------------

@@ -41,10 +41,14 @@ class Workspace:
     replaces the element blocks, and results do not depend on it. ``cheb``
     selects the Chebyshev step count: ``"global"`` (the reference: every element
     takes the largest count) or ``"element"`` (each element its own a-priori
-    count to the same 1e-14 residual bound).
+    count to the same 1e-14 residual bound). ``lanes`` is the lanes per element
+    of the device stage kernels: 1, or 2 (a pair of lanes per element: twice
+    the waves, half the registers per lane) where the problem allows it
+    (q_u = q_v odd, coefficients constant on each element, Chebyshev mass
+    solve); results agree to rounding.
     """
 
-    def __init__(self, pre, N: int, nblocks=None, cheb: str = "global"):
+    def __init__(self, pre, N: int, nblocks=None, cheb: str = "global", lanes: int = 1):
         if nblocks is not None and nblocks < 1:
             raise ValueError(f"nblocks must be ≥ 1 (got {nblocks})")
         if pre.Ainv_u.shape[1] != N * N:
@@ -55,7 +59,7 @@ class Workspace:
         self.nb = 1
         self.qu1, self.qv1, self.q_high = pre.qu1, pre.qv1, pre.q_high
         self.problem = _lib.Problem(pre, N)
-        self.dev = _lib.DeviceWorkspace(self.problem, cheb)
+        self.dev = _lib.DeviceWorkspace(self.problem, cheb, lanes)
         self.ut = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
         self.vt = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
         self.fin = [True]
@@ -91,14 +95,14 @@ def edg_rhs_(ws: Workspace, u, v, t: float, N: int, h: float, pre):
     return ws.ut, ws.vt
 
 
-def edg_rhs(u, v, t: float, N: int, h: float, pre, nblocks=None, cheb: str = "global"):
+def edg_rhs(u, v, t: float, N: int, h: float, pre, nblocks=None, cheb: str = "global", lanes: int = 1):
     """Allocating version of ``edg_rhs_`` (src/rhs.jl:560-571): copies of (u_t, v_t)."""
     u = _state(u); v = _state(v)
     nu, nv = pre.qu1, pre.qv1
     if (u.shape != (N, N, nu, nu) or v.shape != (N, N, nv, nv) or pre.Ainv_u.shape[1] != N * N):
         raise DimensionMismatch(f"state, workspace and pre do not match (N = {N}, q_u + 1 = {nu}, "
                                 f"q_v + 1 = {nv})")
-    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb)
+    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb, lanes=lanes)
     ut, vt = edg_rhs_(ws, u, v, t, N, h, pre)
     return ut.copy(order="F"), vt.copy(order="F")
 
@@ -117,9 +121,10 @@ def rk4_step_(u, v, t: float, dt: float, N: int, h: float, pre, ws: Workspace):
     return u, v
 
 
-def rk4_step(u, v, t: float, dt: float, N: int, h: float, pre, nblocks=None, cheb: str = "global"):
+def rk4_step(u, v, t: float, dt: float, N: int, h: float, pre, nblocks=None, cheb: str = "global",
+             lanes: int = 1):
     """Allocating version of ``rk4_step_`` (src/solver.jl:331-339)."""
-    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb)
+    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb, lanes=lanes)
     uu = np.array(_state(u), order="F", copy=True)
     vv = np.array(_state(v), order="F", copy=True)
     return rk4_step_(uu, vv, t, dt, N, h, pre, ws)
@@ -138,7 +143,7 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
           phase=(0.0, 0.0), coefficients=None, mode: str = "mms", kx: int = 1, ky: int = 1,
           dt_fac=10, mass_solver: str = "chebyshev", nblocks=None, initial_profile=None,
           return_state: bool = False, source: GaussianSource = None, cheb: str = "global",
-          record_every: int = 20, slice_y: float = 0.5,
+          lanes: int = 1, record_every: int = 20, slice_y: float = 0.5,
           receivers=((0.45, 0.5), (0.7, 0.5), (0.85, 0.5)), snap_times=()):
     """Integrate the semi-discrete scheme to time T on the device (src/solver.jl:167-228).
 
@@ -158,7 +163,7 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
                           mass_solver=mass_solver, initial_profile=initial_profile)
     h = 1.0 / N
     dt, nsteps = time_step(q, N, T, dt_fac)
-    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb)
+    ws = Workspace(pre, N, nblocks=nblocks, cheb=cheb, lanes=lanes)
     u = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
     v = np.zeros((N, N, pre.qv1, pre.qv1), order="F")
     st = lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v))

@@ -108,6 +108,13 @@ enum {
 /* Uploads the tables of one problem to the current device; replaces `pre`. */
 wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** out);
 wedg_status wedg_problem_destroy(wedg_problem* p);
+/* The kernel layout a problem was uploaded with (no counterpart in the
+ * reference: it reports what the upload found in precompute's arrays): mode
+ * 0 general (a stored u-stiffness inverse per class), 1 separable (c^2
+ * constant on each element: the spectral u-solve), 2 uniform (one class, face
+ * arrays constant), 3 general with conjugate gradients; the number of
+ * coefficient classes; whether the two-lane stage kernels can run it. */
+wedg_status wedg_problem_info(const wedg_problem* p, int* mode, int* classes, int* two_lane);
 
 /* Device-resident work arrays and state of one problem (src/rhs.jl:184-220).
  * cheb_policy is WEDG_CHEB_GLOBAL or WEDG_CHEB_ELEMENT. */
@@ -162,6 +169,19 @@ wedg_status wedg_advance(wedg_workspace* ws, double t0, double dt, long nsteps,
  * RK4 fused into the stage kernels); the results are those of the plain
  * launches, bit for bit. 0 (the default) launches every stage; 1..4096. */
 wedg_status wedg_graph_enable(wedg_workspace* ws, long chunk_steps);
+/* Lanes per element of the workspace's stage kernels (the Workspace of
+ * src/rhs.jl:184 has no counterpart: its element-block split,
+ * src/rhs.jl:222-252, is the CPU's form of the same choice). 1 (the default):
+ * one lane holds all of an element's arrays; 2: a pair of lanes shares each
+ * element, which doubles the waves and halves the registers per lane. The
+ * two-lane kernels cover n_u = n_v even (q_u = q_v odd) with the spectral
+ * u-solve (coefficients constant on each element) and Chebyshev iteration;
+ * other problems keep one lane whatever is set. Results agree with the
+ * one-lane kernels to rounding. A batch (wedg_batch_create) sets its
+ * workspaces back to 1. wedg_workspace_lanes_per_element reports the lanes
+ * the workspace's calls use. */
+wedg_status wedg_workspace_set_lanes_per_element(wedg_workspace* ws, int lanes);
+wedg_status wedg_workspace_lanes_per_element(const wedg_workspace* ws, int* lanes);
 /* The same without waiting: the caller calls wedg_sync before reading results. */
 wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double dt, long nsteps,
                                int stop_on_nonfinite, double* t_end);
