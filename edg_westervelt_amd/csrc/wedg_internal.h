@@ -6,7 +6,7 @@
 #include "../../include/wedg.h"
 #include "wedg_kernels.h"
 #include "wedg_launch.h"
-#include "wedg_stage2.h"
+#include "wedg_split.h"
 
 #include <hip/hip_runtime.h>
 
