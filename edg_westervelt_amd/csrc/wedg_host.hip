@@ -112,7 +112,17 @@ bool all_equal(const double* p, size_t n) {
 }  // namespace wedg
 
 hipError_t wedg::launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two) {
-    if (two) return launch2(a, p->nu, p->nv, which, s, p->mode);
+    if (two) {
+        switch (p->qu) {
+            case 1: return launch2_q1(a, p->nv, which, s, p->mode);
+            case 2: return launch2_q2(a, p->nv, which, s, p->mode);
+            case 3: return launch2_q3(a, p->nv, which, s, p->mode);
+            case 4: return launch2_q4(a, p->nv, which, s, p->mode);
+            case 5: return launch2_q5(a, p->nv, which, s, p->mode);
+            case 6: return launch2_q6(a, p->nv, which, s, p->mode);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (p->qu) {
         case 1: return launch_q1(a, p->nv, which, s, p->mode);
         case 2: return launch_q2(a, p->nv, which, s, p->mode);
@@ -300,12 +310,13 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         if (p->mode == MODE_SEPARABLE && p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) p->mode = MODE_UNIFORM;
     }
     // the split tables of the two-lane kernels (wedg_stage2.h): per lane parity
-    // h the values T[p][2k + h] of the 1D tables and, for one class, its
-    // u-solve factors D[2k + h][jj]. The kernels read the quadrature weights,
-    // W_h and a one-class problem's coefficient records from row p in both
-    // lanes of a pair, so the problem is admitted only when those are
-    // mirror-symmetric (row NR-1-p equals row p to 1e-13 relative).
-    if (nu % 2 == 0 && nv == nu && nu <= 6 && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE)) {
+    // h the values T[p][2k + h] of the 1D tables (0 at the dummy row 2k + h >= n
+    // of a field with n odd modes, and at the other parity of a middle point)
+    // and, for one class, its u-solve factors D[2k + h][jj]. The kernels read
+    // the quadrature weights, W_h and a one-class problem's coefficient records
+    // from row p in both lanes of a pair, so the problem is admitted only when
+    // those are mirror-symmetric (row NR-1-p equals row p to 1e-13 relative).
+    if (nu <= 7 && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE)) {
         auto mirror_ok = [](const double* a, int nr, int nc, long stride_c) {
             for (int r = 0; r < nr; ++r)
                 for (int c = 0; c < nc; ++c) {
@@ -330,33 +341,52 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
             sym = sym && mirror_ok(bq.data(), nh, nh, nh);
         }
         if (sym) {
-            const SplitLayout L = split_layout(nu, nh);
-            const double h = 1.0 / N, two_h = 2.0 / h, four_hh = 4.0 / (h * h);
+            const SplitLayout L = split_layout(nu, nv, nh);
             std::vector<double> t2(2 * (size_t)L.n, 0.0);
+            // T[pp][2k + hp] of a column-major (nr x ncol) table for a field of n
+            // modes; the middle point (nr odd) keeps only the parity `keep`
+            auto tv = [&](const double* T, int nr, int pp, int k, int hp, int n, int keep) {
+                const int a = 2 * k + hp;
+                if (a >= n) return 0.0;
+                if ((nr & 1) && pp == nr / 2 && (a & 1) != keep) return 0.0;
+                return T[pp + (size_t)nr * a];
+            };
             for (int hp = 0; hp < 2; ++hp) {
                 double* s2 = t2.data() + (size_t)hp * L.n;
-                for (int pp = 0; pp < L.NQH; ++pp)
+                for (int pp = 0; pp < L.NQS; ++pp) {
                     for (int k = 0; k < L.RU; ++k) {
-                        s2[L.sP + pp * L.RU + k] = pre->P_u[pp + nq * (2 * k + hp)];
-                        s2[L.sS + pp * L.RU + k] = pre->S_u[pp + nq * (2 * k + hp)];
+                        s2[L.sPu + pp * L.RU + k] = tv(pre->P_u, nq, pp, k, hp, nu, 0);   // values: even modes
+                        s2[L.sSu + pp * L.RU + k] = tv(pre->S_u, nq, pp, k, hp, nu, 1);   // derivatives: odd
                     }
-                for (int pp = 0; pp < L.NHH; ++pp) {
-                    const bool mid = (nh & 1) && pp == nh / 2;
-                    for (int k = 0; k < L.RU; ++k)
-                        s2[L.sPh + pp * L.RU + k] = (mid && hp == 1) ? 0.0 : pre->P_h_v[pp + nh * (2 * k + hp)];
+                    if (L.sPv != L.sPu)
+                        for (int k = 0; k < L.RV; ++k) {
+                            s2[L.sPv + pp * L.RV + k] = tv(pre->P_u, nq, pp, k, hp, nv, 0);
+                            s2[L.sSv + pp * L.RV + k] = tv(pre->S_u, nq, pp, k, hp, nv, 1);
+                        }
                 }
+                for (int pp = 0; pp < L.NHH; ++pp)
+                    for (int k = 0; k < L.RV; ++k) s2[L.sPh + pp * L.RV + k] = tv(pre->P_h_v, nh, pp, k, hp, nv, 0);
                 for (int k = 0; k < L.RU; ++k) {
                     const int a = 2 * k + hp;
-                    s2[L.sEr + k] = pre->Ebr_u[a];
-                    s2[L.sE1r + k] = pre->E1br_u[a];
-                    s2[L.sTEl + k] = two_h * pre->Ebl_u[a];
-                    s2[L.sTEr + k] = two_h * pre->Ebr_u[a];
-                    s2[L.sFE1l + k] = four_hh * pre->E1bl_u[a];
-                    s2[L.sFE1r + k] = four_hh * pre->E1br_u[a];
-                    s2[L.sE1l + k] = pre->E1bl_u[a];
-                    for (int k2 = 0; k2 < L.RU; ++k2) s2[L.sQ + k2 + L.RU * k] = p->Q[(2 * k2 + hp) + (size_t)nu * a];
+                    const bool ok = a < nu;
+                    s2[L.sE1lu + k] = ok ? pre->E1bl_u[a] : 0.0;
+                    s2[L.sE1ru + k] = ok ? pre->E1br_u[a] : 0.0;
+                    for (int k2 = 0; k2 < L.RU; ++k2) {
+                        const int a2 = 2 * k2 + hp;
+                        s2[L.sQ + k2 + L.RU * k] = (ok && a2 < nu) ? p->Q[a2 + (size_t)nu * a] : 0.0;
+                    }
                     if (p->mode == MODE_UNIFORM)
-                        for (int jj = 0; jj < nu; ++jj) s2[L.sD + k + L.RU * jj] = d0[a + (size_t)nu * jj];
+                        for (int jj = 0; jj < nu; ++jj) s2[L.sD + k + L.RU * jj] = ok ? d0[a + (size_t)nu * jj] : 0.0;
+                }
+                for (int k = 0; k < L.RV; ++k) {
+                    const int a = 2 * k + hp;
+                    const bool ok = a < nv;
+                    s2[L.sElv + k] = ok ? pre->Ebl_u[a] : 0.0;
+                    s2[L.sErv + k] = ok ? pre->Ebr_u[a] : 0.0;
+                    if (L.sE1lv != L.sE1lu) {
+                        s2[L.sE1lv + k] = ok ? pre->E1bl_u[a] : 0.0;
+                        s2[L.sE1rv + k] = ok ? pre->E1br_u[a] : 0.0;
+                    }
                 }
             }
             UP(p->tab2, t2.data(), t2.size());

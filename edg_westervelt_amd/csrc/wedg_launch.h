@@ -21,8 +21,13 @@ hipError_t launch_batch_q4(const BatchArgs& b, int grid, int nv, int which, hipS
 hipError_t launch_batch_q5(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
 hipError_t launch_batch_q6(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
 
-// the two-lane stage kernels (wedg_stage2.h): n_u = n_v in {2, 4, 6}, MODE_UNIFORM or MODE_SEPARABLE
-hipError_t launch2(const KArgs& a, int nu, int nv, int which, hipStream_t s, int mode);
+// the two-lane stage kernels (wedg_stage2.h; which as above, MODE_UNIFORM or MODE_SEPARABLE)
+hipError_t launch2_q1(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch2_q2(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch2_q3(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch2_q4(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch2_q5(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch2_q6(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 
 // the diagnostics and norm kernels (any argument may be null)
 hipError_t launch_diag_q1(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);

@@ -6,35 +6,48 @@
 
 namespace wedg {
 
-constexpr int kPairWave = 32;         // elements per wave of the two-lane kernels
-constexpr int kLds2 = 36;              // LDS doubles per lane: lift vectors, then mass-matrix weights
+constexpr int kPairWave = 32;   // elements per wave of the two-lane kernels
 
-// Offsets in the split table of one lane parity h (n_u = n_v = nu even, nh
-// high-order points): values T[p][2k + h] of the 1D tables for the lane's
-// parity, and the eigenvalue factors of a one-class spectral u-solve.
+// LDS doubles per lane of a two-lane stage kernel: the lift vectors of the
+// four sides (4 (2 RV + RU)), later the mass-matrix weights of the lane's rows.
+__host__ __device__ constexpr int lds2_per_lane(int nu) { return nu >= 7 ? 48 : 36; }
+
+// Offsets in the split table of one lane parity h (n_u = nu, n_v = nv in
+// {nu, nu - 1}, nq = nu points, nh high-order points). A lane holds the rows
+// a = 2k + h of a modal array, k < R = ceil(n / 2); a row a >= n (the parity-1
+// lane's last row when n is odd) is a dummy whose entries are 0 in every
+// table, so it stays 0. Point rows: p < nq / 2 (the lane's mirrored point),
+// then the middle point (nq odd), whose entries of the other parity are 0.
 struct SplitLayout {
-    int RU, NQH, NHH;
-    int sP, sS, sPh, sEr, sE1r, sTEl, sTEr, sFE1l, sFE1r, sE1l, sQ, sD, n;
+    int RU, RV, NQS, NHH;
+    int sPu, sSu, sPv, sSv, sPh;      // P_u, S_u, P_v, S_v (NQS x R), P_h (NHH x RV): [p * R + k]
+    int sE1lu, sE1ru;                 // E1bl, E1br at the u rows (u traces; the b lifts)
+    int sElv, sErv, sE1lv, sE1rv;     // Ebl, Ebr, E1bl, E1br at the v rows (v traces; the Rv lifts)
+    int sQ, sD;                       // Q[2k'+h][2k+h] ([k' + RU k]); one-class u-solve factors ([k + RU jj])
+    int n;
 };
 
-__host__ __device__ constexpr SplitLayout split_layout(int nu, int nh) {
+__host__ __device__ constexpr SplitLayout split_layout(int nu, int nv, int nh) {
     SplitLayout L{};
-    L.RU = nu / 2;
-    L.NQH = nu / 2;
+    L.RU = (nu + 1) / 2;
+    L.RV = (nv + 1) / 2;
+    L.NQS = (nu + 1) / 2;
     L.NHH = (nh + 1) / 2;
+    const bool same = nv == nu;   // the v tables are the u tables
     int o = 0;
-    L.sP = o; o += L.NQH * L.RU;     // P_u[p][2k+h], [p * RU + k]
-    L.sS = o; o += L.NQH * L.RU;     // S_u[p][2k+h]
-    L.sPh = o; o += L.NHH * L.RU;    // P_h[p][2k+h] (middle row: the odd columns are 0)
-    L.sEr = o; o += L.RU;            // Ebr[2k+h]
-    L.sE1r = o; o += L.RU;           // E1br[2k+h]
-    L.sTEl = o; o += L.RU;           // (2/h) Ebl[2k+h]
-    L.sTEr = o; o += L.RU;           // (2/h) Ebr
-    L.sFE1l = o; o += L.RU;          // (4/h^2) E1bl
-    L.sFE1r = o; o += L.RU;          // (4/h^2) E1br
-    L.sE1l = o; o += L.RU;           // E1bl
-    L.sQ = o; o += L.RU * L.RU;      // Q[2k'+h][2k+h], [k' + RU k]
-    L.sD = o; o += L.RU * nu;        // one-class u-solve factors D[2k+h][jj], [k + RU jj]
+    L.sPu = o; o += L.NQS * L.RU;
+    L.sSu = o; o += L.NQS * L.RU;
+    if (same) { L.sPv = L.sPu; L.sSv = L.sSu; }
+    else { L.sPv = o; o += L.NQS * L.RV; L.sSv = o; o += L.NQS * L.RV; }
+    L.sPh = o; o += L.NHH * L.RV;
+    L.sE1lu = o; o += L.RU;
+    L.sE1ru = o; o += L.RU;
+    L.sElv = o; o += L.RV;
+    L.sErv = o; o += L.RV;
+    if (same) { L.sE1lv = L.sE1lu; L.sE1rv = L.sE1ru; }
+    else { L.sE1lv = o; o += L.RV; L.sE1rv = o; o += L.RV; }
+    L.sQ = o; o += L.RU * L.RU;
+    L.sD = o; o += L.RU * nu;
     L.n = o;
     return L;
 }

@@ -3,18 +3,24 @@
 // The one-lane kernels (wedg_kernels.h) give each element one lane and keep
 // all of its small matrices in that lane's registers: at q = 5 that is the
 // whole register file, so a SIMD runs one wave, and with 64 elements per wave
-// a 256 x 256 mesh is exactly one wave per SIMD — every memory phase of the
-// stage stalls the SIMD. Here the lanes i and i + 32 of a wave share element
-// i: a wave holds 32 elements, the grid has twice the waves and each lane
-// half the registers, so two waves share a SIMD and one wave's memory phases
-// overlap the other's arithmetic. Each half-wave reads 32 consecutive
-// elements of one mode (256 contiguous bytes), so the element-fastest state
-// is read in whole cache lines as by the one-lane kernel.
+// a mesh of up to 256 x 256 elements has at most one wave per SIMD — every
+// memory phase of the stage stalls the SIMD, and a mesh that fills only part
+// of the chip takes a whole wave's latency per stage. Here the lanes i and
+// i + 32 of a wave share element i: a wave holds 32 elements, the grid has
+// twice the waves and each lane about half the registers, so a stage's
+// latency is about half a one-lane wave's work, and at q <= 5 two waves share
+// a SIMD. Each half-wave reads 32 consecutive elements of one mode (256
+// contiguous bytes), so the element-fastest state is read in whole cache
+// lines as by the one-lane kernel.
 //
-// How an element is shared (n_u = n_v = n even, n_q = n; h = lane / 32, the
-// lane's parity):
+// How an element is shared (n_q = n_u; n_v = n_u or n_u - 1; h = lane / 32,
+// the lane's parity):
 //  * a modal array X[a + n b] (a the first index) is split by the parity of
-//    a: lane h holds the rows a = 2k + h, k < n/2, as X[k + (n/2) b];
+//    a: lane h holds the rows a = 2k + h, k < R = ceil(n/2), as X[k + R b];
+//    with n odd the parity-1 lane's last row is a dummy: it is read as 0
+//    (its loads and stores fall outside the buffer's range and the hardware
+//    returns 0 and drops the store), and every table that adds into a row
+//    has 0 there, so it stays 0;
 //  * point values along a mirrored point index p (face points, rows of
 //    volume and mass-matrix points) are split by halves: lane 0 holds
 //    p < NR/2, lane 1 the mirror point NR - 1 - p in the same slot; with NR
@@ -26,9 +32,9 @@
 //    each keeps its own mirrored point: the 1D tables are values
 //    or derivatives of Legendre polynomials at symmetric points,
 //    T[NR-1-p][a] = (-1)^a T[p][a] (values), -(-1)^a T[p][a] (derivatives), so
-//    the even and odd partial sums give both mirrored points; the way back
-//    (points to modes) exchanges the point values and each lane accumulates
-//    the modes of its parity;
+//    the even and odd partial sums give both mirrored points, and only one
+//    parity reaches a middle point; the way back (points to modes) exchanges
+//    the point values and each lane accumulates the modes of its parity;
 //  * contractions over the second index run on each lane's own rows with the
 //    uniform tables (SGPR operands), as in the one-lane kernel;
 //  * point data of a mirrored row (quadrature weights, and the coefficient
@@ -48,6 +54,17 @@
 namespace wedg {
 
 static_assert(kPairWave * 2 == kWave, "a lane pair per element");
+
+// The per-lane sizes of one (n_u, n_v) pair.
+template <class D>
+struct Dims2 {
+    static constexpr int NU = D::NU, NV = D::NV, NQ = D::NQ, NH = D::NH;
+    static constexpr int RU = (NU + 1) / 2, RV = (NV + 1) / 2;   // rows per lane
+    static constexpr int NQS = (NQ + 1) / 2;                     // point slots per lane (pairs, middle)
+    static constexpr int nU = RU * NU, nV = RV * NV;             // a lane's share of a modal array
+    static constexpr SplitLayout SL = split_layout(NU, NV, NH);
+    static constexpr int NLDS = lds2_per_lane(NU);
+};
 
 // Both values of a lane pair: every lane gets the value of the pair's parity-0
 // lane (a) and of its parity-1 lane (b) (lanes i and i + 32; v_permlane32_swap
@@ -89,6 +106,10 @@ struct Lane2 {
     const double* t2;     // the lane's split table (LDS)
 };
 
+// A byte offset past the buffer range of the element-fastest arrays (their
+// records cover 0x7fffffff bytes): a load there returns 0, a store is dropped.
+constexpr unsigned kDummyOffset = 0x80000000u;
+
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ __forceinline__ double ld_rs(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
@@ -98,34 +119,39 @@ __device__ __forceinline__ void st_rs(__amdgpu_buffer_rsrc_t r, int voff, int so
 }
 #endif
 
-// rows 2k + h of the element's n x n modal array (element-fastest [m][E])
+// The lane's rows 2k + h of an element's n x n modal array (element-fastest
+// [m][E]; voff = (h E + e) * 8); the dummy row (n odd, parity 1) reads 0.
 template <int n>
-__device__ __forceinline__ void eload2(const double* p, int E, int voffE, double (&x)[(n / 2) * n]) {
-    constexpr int R = n / 2;
+__device__ __forceinline__ void eload2(const double* p, int E, int voff, int h, double (&x)[((n + 1) / 2) * n]) {
+    constexpr int R = (n + 1) / 2;
+    const int vlast = ((n & 1) && h) ? (int)kDummyOffset : voff;
 #if defined(__HIP_DEVICE_COMPILE__)
     const __amdgpu_buffer_rsrc_t r = ersrc(p);
 #pragma unroll
     for (int b = 0; b < n; ++b)
 #pragma unroll
-        for (int k = 0; k < R; ++k) x[k + R * b] = ld_rs(r, voffE, (2 * k + n * b) * E * 8);
+        for (int k = 0; k < R; ++k) x[k + R * b] = ld_rs(r, k == R - 1 ? vlast : voff, (2 * k + n * b) * E * 8);
 #else
     for (int b = 0; b < n; ++b)
-        for (int k = 0; k < R; ++k) x[k + R * b] = p[(long)(2 * k + n * b) * E + voffE / 8];
+        for (int k = 0; k < R; ++k)
+            x[k + R * b] = (k == R - 1 && vlast != voff) ? 0.0 : p[(long)(2 * k + n * b) * E + voff / 8];
 #endif
 }
 
 template <int n>
-__device__ __forceinline__ void estore2(double* p, int E, int voffE, const double (&x)[(n / 2) * n]) {
-    constexpr int R = n / 2;
+__device__ __forceinline__ void estore2(double* p, int E, int voff, int h, const double (&x)[((n + 1) / 2) * n]) {
+    constexpr int R = (n + 1) / 2;
+    const int vlast = ((n & 1) && h) ? (int)kDummyOffset : voff;
 #if defined(__HIP_DEVICE_COMPILE__)
     const __amdgpu_buffer_rsrc_t r = ersrc(p);
 #pragma unroll
     for (int b = 0; b < n; ++b)
 #pragma unroll
-        for (int k = 0; k < R; ++k) st_rs(r, voffE, (2 * k + n * b) * E * 8, x[k + R * b]);
+        for (int k = 0; k < R; ++k) st_rs(r, k == R - 1 ? vlast : voff, (2 * k + n * b) * E * 8, x[k + R * b]);
 #else
     for (int b = 0; b < n; ++b)
-        for (int k = 0; k < R; ++k) p[(long)(2 * k + n * b) * E + voffE / 8] = x[k + R * b];
+        for (int k = 0; k < R; ++k)
+            if (!(k == R - 1 && vlast != voff)) p[(long)(2 * k + n * b) * E + voff / 8] = x[k + R * b];
 #endif
 }
 
@@ -153,7 +179,7 @@ __device__ __forceinline__ double comb_back(Pair x, double sg, bool der) {
     return der ? fma(-sg, x.b, x.a) : fma(sg, x.b, x.a);
 }
 
-// Point values (own mirrored points p < NQ/2) of the vector x over the second
+// Point values (own mirrored points p < NR/2) of the vector x over the second
 // (unsplit) index with the uniform table T (NR x n, column-major): lane 0 the
 // point p, lane 1 the point NR-1-p.
 template <int NR, int n, class X>
@@ -166,19 +192,28 @@ __device__ __forceinline__ double pt_unsplit(cdbl* T, int p, X x, double sg) {
     return fma(sg, od, ev);
 }
 
+// The point index of the lane's point slot p.
+template <int NQ>
+__device__ __forceinline__ int point_of(int p, int h) {
+    return p < NQ / 2 ? (h ? NQ - 1 - p : p) : NQ / 2;
+}
+
 // ---------------------------------------------------------------------------
-// Traces at the lane's own points of one side (0 B, 1 T: the first index at
-// its ends; 2 L, 3 R: the second index), o[p] for p < NQ/2 (src/rhs.jl:384-410).
-// DER: the derivative row E1 instead of the value row E.
-template <class D, int n, bool DER, int sd>
-__device__ __forceinline__ void side_trace2(const KArgs& A, const Lane2& L, const double (&X)[(n / 2) * n],
-                                            double (&o)[D::NQ / 2]) {
-    constexpr int NQ = D::NQ, R = n / 2;
-    constexpr auto SL = split_layout(D::NU, D::NH);
+// Traces at the lane's point slots of one side (0 B, 1 T: the first index at
+// its ends; 2 L, 3 R: the second index), o[p] for p < NQS (src/rhs.jl:384-410).
+// VF: the field is v (n_v modes), else u; DER: the derivative row E1.
+template <class D, bool VF, bool DER, int sd>
+__device__ __forceinline__ void side_trace2(const KArgs& A, const Lane2& L,
+                                            const double (&X)[(VF ? Dims2<D>::nV : Dims2<D>::nU)],
+                                            double (&o)[Dims2<D>::NQS]) {
+    using Q2 = Dims2<D>;
+    constexpr int NQ = D::NQ, n = VF ? D::NV : D::NU, R = (n + 1) / 2;
+    constexpr SplitLayout SL = Q2::SL;
+    static_assert(VF || DER, "the u traces are derivatives");
     cdbl* T = ctab(A.tab);
     if (sd < 2) {
         // sv[b] = sum_a E[a] X[a, b]: the lane's parity, then the pair's sum
-        constexpr int oE = DER ? SL.sE1r : SL.sEr;
+        constexpr int oE = DER ? (VF ? SL.sE1rv : SL.sE1ru) : SL.sErv;
         double sv[n];
 #pragma unroll
         for (int b = 0; b < n; ++b) {
@@ -191,9 +226,11 @@ __device__ __forceinline__ void side_trace2(const KArgs& A, const Lane2& L, cons
         }
 #pragma unroll
         for (int p = 0; p < NQ / 2; ++p) o[p] = pt_unsplit<NQ, n>(T + D::oP, p, [&](int b) { return sv[b]; }, L.sg);
+        if (NQ & 1) o[NQ / 2] = row_mid<NQ, n, false>(T + D::oP, [&](int b) { return sv[b]; });
     } else {
         // sv[a] = sum_b E[b] X[a, b] on the lane's rows, then the points over a
         constexpr int oEr = DER ? D::oE1r : D::oEr;
+        constexpr int oP = VF ? SL.sPv : SL.sPu;
         double sv[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -205,39 +242,46 @@ __device__ __forceinline__ void side_trace2(const KArgs& A, const Lane2& L, cons
             sv[k] = sd == 3 ? ev + od : (DER ? od - ev : ev - od);
         }
 #pragma unroll
-        for (int p = 0; p < NQ / 2; ++p) {
+        for (int p = 0; p < Q2::NQS; ++p) {
             double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < R; ++k) s = fma(L.t2[SL.sP + p * SL.RU + k], sv[k], s);
-            o[p] = comb_fwd(xboth(s), L.sg, false);
+            for (int k = 0; k < R; ++k) s = fma(L.t2[oP + p * R + k], sv[k], s);
+            const Pair x = xboth(s);
+            o[p] = p < NQ / 2 ? comb_fwd(x, L.sg, false) : x.a + x.b;   // the middle point: even modes only
         }
     }
 }
 
-// Modes 2k + h of sum_i P[i, m] y[i] from the lane's own point values y (the
-// lift vectors, src/rhs.jl:434-441).
-template <class D, int R>
-__device__ __forceinline__ void lift_vec2(const Lane2& L, const double (&y)[D::NQ / 2], double (&z)[R]) {
-    constexpr auto SL = split_layout(D::NU, D::NH);
-    double c[D::NQ / 2];
+// Modes 2k + h of sum_i P[i, m] y[i] (m < n of the field, VF: v) from the
+// lane's point slots y (the lift vectors, src/rhs.jl:434-441).
+template <class D, bool VF>
+__device__ __forceinline__ void lift_vec2(const Lane2& L, const double (&y)[Dims2<D>::NQS],
+                                          double (&z)[VF ? Dims2<D>::RV : Dims2<D>::RU]) {
+    using Q2 = Dims2<D>;
+    constexpr int NQ = D::NQ, R = VF ? Q2::RV : Q2::RU;
+    constexpr int oP = VF ? Q2::SL.sPv : Q2::SL.sPu;
+    double c[Q2::NQS];
 #pragma unroll
-    for (int p = 0; p < D::NQ / 2; ++p) c[p] = comb_back(xboth(y[p]), L.sg, false);
+    for (int p = 0; p < NQ / 2; ++p) c[p] = comb_back(xboth(y[p]), L.sg, false);
+    if (NQ & 1) c[NQ / 2] = y[NQ / 2];   // the middle point: the table is 0 at the odd modes
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int p = 0; p < D::NQ / 2; ++p) s = fma(L.t2[SL.sP + p * SL.RU + k], c[p], s);
+        for (int p = 0; p < Q2::NQS; ++p) s = fma(L.t2[oP + p * R + k], c[p], s);
         z[k] = s;
     }
 }
 
-// The slots of the element's physical-boundary sides at the lane's own points
-// (src/rhs.jl:738-765), with the weight checks (kind 0, the stage's serial).
+// The slots of the element's physical-boundary sides at the lane's point
+// slots (src/rhs.jl:738-765), with the weight checks (kind 0, the stage's
+// serial). The middle point (n_q odd) is formed by both lanes alike.
 template <class D, bool UNI>
-__device__ __forceinline__ void stage_boundary2(const Lane2& L, const double (&U)[D::NU * D::NU / 2],
-                                                const double (&V)[D::NV * D::NV / 2], int e, int serial,
-                                                double ct, double st) {
-    constexpr int NQ = D::NQ, NU = D::NU, NV = D::NV, NQH = NQ / 2;
+__device__ __forceinline__ void stage_boundary2(const Lane2& L, const double (&U)[Dims2<D>::nU],
+                                                const double (&V)[Dims2<D>::nV], int e, int serial, double ct,
+                                                double st) {
+    using Q2 = Dims2<D>;
+    constexpr int NQ = D::NQ, NQS = Q2::NQS;
     CKArgs& B = *kargs<false>();
     const int N = B.N, j = e % N, k = e / N;
     int bkey = 0;
@@ -246,22 +290,22 @@ __device__ __forceinline__ void stage_boundary2(const Lane2& L, const double (&U
         const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
         if (!bnd) return;
         CKArgs& A = *kargs<false>();
-        double du[NQH], vt[NQH], dv[NQH];
-        side_trace2<D, NU, true, s>(A, L, U, du);
-        side_trace2<D, NV, false, s>(A, L, V, vt);
-        side_trace2<D, NV, true, s>(A, L, V, dv);
+        double du[NQS], vt[NQS], dv[NQS];
+        side_trace2<D, false, true, s>(A, L, U, du);
+        side_trace2<D, true, false, s>(A, L, V, vt);
+        side_trace2<D, true, true, s>(A, L, V, dv);
         const int m = s < 2 ? k : j;
         const int fidx = s == 0 ? (N + 1) * k : s == 1 ? N + (N + 1) * k : s == 2 ? j : j + N * N;
         double* bs = A.bsl + (long)s * NQ * 3 * N + m;
-        BPoint pd[NQH];
+        BPoint pd[NQS];
 #pragma unroll
-        for (int p = 0; p < NQH; ++p) {
-            const int i = L.h ? NQ - 1 - p : p;
+        for (int p = 0; p < NQS; ++p) {
+            const int i = point_of<NQ>(p, L.h);
             pd[p] = boundary_data<UNI, (s < 2)>(A, s, (long)i * A.fstride + fidx, i, m, NQ);
         }
 #pragma unroll
-        for (int p = 0; p < NQH; ++p) {
-            const int i = L.h ? NQ - 1 - p : p;
+        for (int p = 0; p < NQS; ++p) {
+            const int i = point_of<NQ>(p, L.h);
             double Av, Dd, Ff;
             const int k2 = boundary_point<UNI, (s < 2)>(A, s, pd[p], i, m, NQ, du[p], vt[p], dv[p], ct, st, Av, Dd, Ff);
             if (k2) bkey = bkey == 0 ? k2 : min(bkey, k2);
@@ -277,16 +321,17 @@ __device__ __forceinline__ void stage_boundary2(const Lane2& L, const double (&U
     if (bkey) atomicMin(&B.err->fkey, fail_key(serial, 0, bkey));
 }
 
-// Face slots of side s at the lane's own points (interior fluxes
+// Face slots of side s at the lane's point slots (interior fluxes
 // src/rhs.jl:620-700 with the neighbour's facing traces recomputed from its
 // state; on a physical boundary the slots stage_boundary2 formed), reduced to
-// the lane's share of the side's lift vectors Y_A, Y_D, Y_F (the modes of its
-// parity) in its LDS slots.
+// the lane's share of the side's lift vectors Y_A, Y_D (v modes), Y_F (u
+// modes) of its parity, into its LDS slots.
 template <class D, bool UNI, int s>
 __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, int N, int E,
-                                           const double (&U)[D::NU * D::NU / 2], const double (&V)[D::NV * D::NV / 2],
+                                           const double (&U)[Dims2<D>::nU], const double (&V)[Dims2<D>::nV],
                                            double* Ys, int lane) {
-    constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU, NQH = NQ / 2, RU = NU / 2, RV = NV / 2;
+    using Q2 = Dims2<D>;
+    constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU, NQS = Q2::NQS, RU = Q2::RU, RV = Q2::RV;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int ns = s ^ 1;
     CKArgs& A = *kargs<false>();
@@ -297,29 +342,29 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
     else if (s == 2) { fidx = j + N * k;           nb = e - N; bnd = k == 0;     m = j; }
     else             { fidx = j + N * (k + 1);     nb = e + N; bnd = k == N - 1; m = j; }
     if (bnd) nb = e;
-    double odu[NQH], ov[NQH], odv[NQH], ndu[NQH], nvv[NQH], ndv[NQH];
-    side_trace2<D, NU, true, s>(A, L, U, odu);
-    side_trace2<D, NV, false, s>(A, L, V, ov);
-    side_trace2<D, NV, true, s>(A, L, V, odv);
+    double odu[NQS], ov[NQS], odv[NQS], ndu[NQS], nvv[NQS], ndv[NQS];
+    side_trace2<D, false, true, s>(A, L, U, odu);
+    side_trace2<D, true, false, s>(A, L, V, ov);
+    side_trace2<D, true, true, s>(A, L, V, odv);
     const int voffN = (L.h * E + nb) * 8;
     {
-        double X[NU * NU / 2];
-        eload2<NU>(A.yu, E, voffN, X);
-        side_trace2<D, NU, true, ns>(A, L, X, ndu);
+        double X[Q2::nU];
+        eload2<NU>(A.yu, E, voffN, L.h, X);
+        side_trace2<D, false, true, ns>(A, L, X, ndu);
     }
     {
-        double X[NV * NV / 2];
-        eload2<NV>(A.yv, E, voffN, X);
-        side_trace2<D, NV, false, ns>(A, L, X, nvv);
-        side_trace2<D, NV, true, ns>(A, L, X, ndv);
+        double X[Q2::nV];
+        eload2<NV>(A.yv, E, voffN, L.h, X);
+        side_trace2<D, true, false, ns>(A, L, X, nvv);
+        side_trace2<D, true, true, ns>(A, L, X, ndv);
     }
-    double wA[NQH], wD[NQH], wF[NQH];
+    double wA[NQS], wD[NQS], wF[NQS];
     {
         CKArgs& I = *kargs<false>();
         cdbl* Tw = ctab(I.tab) + D::oW;   // w[NQ-1-p] = w[p] (admitted to 1e-13)
 #pragma unroll
-        for (int p = 0; p < NQH; ++p) {
-            const int i = L.h ? NQ - 1 - p : p;
+        for (int p = 0; p < NQS; ++p) {
+            const int i = point_of<NQ>(p, L.h);
             const long fi = (long)i * I.fstride + fidx;
             double Av, Dd, Ff;
             if (plus) interior_point<UNI>(I, true, hz, fi, nvv[p], ov[p], ndu[p], odu[p], ndv[p], odv[p], Av, Dd, Ff);
@@ -334,17 +379,17 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
         cdbl* Tw = ctab(B.tab) + D::oW;
         const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
 #pragma unroll
-        for (int p = 0; p < NQH; ++p) {
-            const int i = L.h ? NQ - 1 - p : p;
+        for (int p = 0; p < NQS; ++p) {
+            const int i = point_of<NQ>(p, L.h);
             wA[p] = Tw[p] * bs[(long)(i * 3 + 0) * N];
             wD[p] = Tw[p] * bs[(long)(i * 3 + 1) * N];
             wF[p] = Tw[p] * bs[(long)(i * 3 + 2) * N];
         }
     }
     double YA[RV], YD[RV], YF[RU];
-    lift_vec2<D, RV>(L, wA, YA);
-    lift_vec2<D, RV>(L, wD, YD);
-    lift_vec2<D, RU>(L, wF, YF);
+    lift_vec2<D, true>(L, wA, YA);
+    lift_vec2<D, true>(L, wD, YD);
+    lift_vec2<D, false>(L, wF, YF);
     constexpr int y0 = s * (2 * RV + RU);
 #pragma unroll
     for (int b = 0; b < RV; ++b) {
@@ -360,16 +405,25 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
 // modes of both parities from the pair's LDS slots), the factors transposed on
 // a vertical side (the lane's own modes).
 template <class D, int s>
-__device__ __forceinline__ void apply_lifts2(const Lane2& L, const double* Ys, int lane,
-                                             double (&Rv)[D::NV * D::NV / 2], double (&bv)[D::NU * D::NU / 2]) {
-    constexpr int NV = D::NV, NU = D::NU, RV = NV / 2, RU = NU / 2;
+__device__ __forceinline__ void apply_lifts2(const Lane2& L, const double* Ys, int lane, double (&Rv)[Dims2<D>::nV],
+                                             double (&bv)[Dims2<D>::nU]) {
+    using Q2 = Dims2<D>;
+    constexpr int NV = D::NV, NU = D::NU, RV = Q2::RV, RU = Q2::RU;
+    constexpr SplitLayout SL = Q2::SL;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int y0 = s * (2 * RV + RU);
-    constexpr auto SL = split_layout(D::NU, D::NH);
     CKArgs& A = *kargs<false>();
     if (hz) {
         const int l0 = lane & 31, l1 = l0 + 32;
-        constexpr int oE = plus ? SL.sTEl : SL.sTEr, oE1 = plus ? SL.sFE1l : SL.sFE1r, oF1 = plus ? SL.sE1l : SL.sE1r;
+        const double two_h = A.two_h, four_hh = A.four_hh;
+        constexpr int oE = plus ? SL.sElv : SL.sErv, oE1 = plus ? SL.sE1lv : SL.sE1rv;
+        constexpr int oF1 = plus ? SL.sE1lu : SL.sE1ru;
+        double te[RV], te1[RV];
+#pragma unroll
+        for (int k = 0; k < RV; ++k) {
+            te[k] = two_h * L.t2[oE + k];       // (2/h) E (src/rhs.jl:449-463)
+            te1[k] = four_hh * L.t2[oE1 + k];   // (4/h^2) E1
+        }
 #pragma unroll
         for (int b = 0; b < NV; ++b) {
             const int src = (b & 1) ? l1 : l0;
@@ -377,8 +431,8 @@ __device__ __forceinline__ void apply_lifts2(const Lane2& L, const double* Ys, i
 #pragma unroll
             for (int k = 0; k < RV; ++k) {
                 double r = Rv[k + RV * b];
-                r = fma(L.t2[oE + k], ya, r);
-                r = fma(L.t2[oE1 + k], yd, r);
+                r = fma(te[k], ya, r);
+                r = fma(te1[k], yd, r);
                 Rv[k + RV * b] = r;
             }
         }
@@ -419,22 +473,30 @@ __device__ __forceinline__ void apply_lifts2(const Lane2& L, const double* Ys, i
 // ---------------------------------------------------------------------------
 // Volume terms (src/rhs.jl:317-333) on the lane's share:
 //   out += S^T (W o (S X P^T)) P + P^T (W o (P X S^T)) S
-// one mirrored pair of point rows (p, NQ-1-p) at a time, lane 0 row p and
-// lane 1 row NQ-1-p. DX: the x-term (S on the first index).
-template <class D, int n, int NO, bool DX>
-__device__ __forceinline__ void vol_term2(const KArgs& A, const Lane2& L, int p, const double (&X)[(n / 2) * n],
-                                          const double (&w)[D::NQ], double (&out)[(NO / 2) * NO]) {
-    constexpr int NQ = D::NQ, R = n / 2, RO = NO / 2;
-    constexpr auto SL = split_layout(D::NU, D::NH);
-    const int oA = (DX ? SL.sS : SL.sP) + p * SL.RU;    // the lane's half of row p of A
-    cdbl* B = ctab(A.tab) + (DX ? D::oP : D::oS);        // the second index, uniform
+// for X of field VI (n modes) and out of field VO (NO modes), one mirrored
+// pair of point rows (p, NQ-1-p) at a time, lane 0 row p and lane 1 row
+// NQ-1-p; the middle row (p = NQ/2, NQ odd) in both lanes. DX: the x-term (S on
+// the first index).
+template <class D, bool VI, bool VO, bool DX>
+__device__ __forceinline__ void vol_term2(const KArgs& A, const Lane2& L, int p,
+                                          const double (&X)[(VI ? Dims2<D>::nV : Dims2<D>::nU)], const double (&w)[D::NQ],
+                                          double (&out)[(VO ? Dims2<D>::nV : Dims2<D>::nU)]) {
+    using Q2 = Dims2<D>;
+    constexpr int NQ = D::NQ, n = VI ? D::NV : D::NU, R = (n + 1) / 2;
+    constexpr int NO = VO ? D::NV : D::NU, RO = (NO + 1) / 2;
+    constexpr SplitLayout SL = Q2::SL;
+    const bool mid = (NQ & 1) && p == NQ / 2;
+    const int oI = (DX ? (VI ? SL.sSv : SL.sSu) : (VI ? SL.sPv : SL.sPu)) + p * R;    // row p of A, input rows
+    const int oO = (DX ? (VO ? SL.sSv : SL.sSu) : (VO ? SL.sPv : SL.sPu)) + p * RO;   // output rows
+    cdbl* B = ctab(A.tab) + (DX ? D::oP : D::oS);   // the second index, uniform
     double t[n];
 #pragma unroll
     for (int b = 0; b < n; ++b) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) s = fma(L.t2[oA + k], X[k + R * b], s);
-        t[b] = comb_fwd(xboth(s), L.sg, DX);
+        for (int k = 0; k < R; ++k) s = fma(L.t2[oI + k], X[k + R * b], s);
+        const Pair x = xboth(s);
+        t[b] = mid ? x.a + x.b : comb_fwd(x, L.sg, DX);   // the middle row: one parity is 0
     }
     double z[NO];
     {
@@ -446,35 +508,35 @@ __device__ __forceinline__ void vol_term2(const KArgs& A, const Lane2& L, int p,
     }
 #pragma unroll
     for (int b = 0; b < NO; ++b) {
-        const double c = comb_back(xboth(z[b]), L.sg, DX);
+        const double c = mid ? z[b] : comb_back(xboth(z[b]), L.sg, DX);
 #pragma unroll
-        for (int k = 0; k < RO; ++k) out[k + RO * b] = fma(L.t2[oA + k], c, out[k + RO * b]);
+        for (int k = 0; k < RO; ++k) out[k + RO * b] = fma(L.t2[oO + k], c, out[k + RO * b]);
     }
 }
 
 // The lane's weight row W[prow + NQ q]: for one class (UNI) row p in both
 // lanes (uniform, the record is mirror-symmetric), otherwise the lane's own
 // mirrored row prow of its element's class.
-template <class D, bool UNI, int n, int NO>
-__device__ __forceinline__ void vol_pass2(const Lane2& L, const double (&X)[(n / 2) * n], const double* Wsrc,
-                                          double (&out)[(NO / 2) * NO]) {
+template <class D, bool UNI, bool VI, bool VO>
+__device__ __forceinline__ void vol_pass2(const Lane2& L, const double (&X)[(VI ? Dims2<D>::nV : Dims2<D>::nU)],
+                                          const double* Wsrc, double (&out)[(VO ? Dims2<D>::nV : Dims2<D>::nU)]) {
     constexpr int NQ = D::NQ;
 #pragma unroll 1
-    for (int p = 0; p < NQ / 2; ++p) {
+    for (int p = 0; p < (NQ + 1) / 2; ++p) {
         double w[NQ];
         if (UNI) {
             cdbl* Wr = ctab(Wsrc) + p;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) w[q] = Wr[NQ * q];
         } else {
-            const int prow = L.h ? NQ - 1 - p : p;
+            const int prow = point_of<NQ>(p, L.h);
             const double* Wr = vfresh(Wsrc + prow);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) w[q] = Wr[NQ * q];
         }
-        vol_term2<D, n, NO, true>(*kargs<false>(), L, p, X, w, out);
+        vol_term2<D, VI, VO, true>(*kargs<false>(), L, p, X, w, out);
         __builtin_amdgcn_sched_barrier(0);
-        vol_term2<D, n, NO, false>(*kargs<false>(), L, p, X, w, out);
+        vol_term2<D, VI, VO, false>(*kargs<false>(), L, p, X, w, out);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -486,10 +548,11 @@ __device__ __forceinline__ void vol_pass2(const Lane2& L, const double (&X)[(n /
 // f(p, q, g, Gv) with g = 2 beta v_q and Gv = g W_h. W_h and a one-class beta
 // record are read from row p by both lanes (mirror-symmetric).
 template <class D, bool UNI, class F>
-__device__ __forceinline__ void mass_rows2(const KArgs& A, const Lane2& L, const double (&V)[D::NV * D::NV / 2],
+__device__ __forceinline__ void mass_rows2(const KArgs& A, const Lane2& L, const double (&V)[Dims2<D>::nV],
                                            int cls_b, F f) {
-    constexpr int NV = D::NV, NH = D::NH, RV = NV / 2;
-    constexpr auto SL = split_layout(D::NU, D::NH);
+    using Q2 = Dims2<D>;
+    constexpr int NV = D::NV, NH = D::NH, RV = Q2::RV;
+    constexpr SplitLayout SL = Q2::SL;
     const double* bq = A.beta + (UNI ? 0 : (long)cls_b * NH * NH);
 #pragma unroll
     for (int p = 0; p < SL.NHH; ++p) {
@@ -500,7 +563,7 @@ __device__ __forceinline__ void mass_rows2(const KArgs& A, const Lane2& L, const
         for (int b = 0; b < NV; ++b) {
             double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < RV; ++k) s = fma(L.t2[SL.sPh + p * SL.RU + k], V[k + RV * b], s);
+            for (int k = 0; k < RV; ++k) s = fma(L.t2[SL.sPh + p * RV + k], V[k + RV * b], s);
             t[b] = comb_fwd(xboth(s), L.sg, false);
         }
         cdbl* T = ctab(A.tab);
@@ -519,7 +582,7 @@ __device__ __forceinline__ void mass_rows2(const KArgs& A, const Lane2& L, const
 // The element's a-priori Chebyshev count of the state V (0: 1 - 2 beta v_h <= 0
 // somewhere, or not a number), the pair's extremes combined.
 template <class D, bool UNI>
-__device__ __forceinline__ int cheb_count2(const KArgs& A, const Lane2& L, const double (&V)[D::NV * D::NV / 2],
+__device__ __forceinline__ int cheb_count2(const KArgs& A, const Lane2& L, const double (&V)[Dims2<D>::nV],
                                            int cls_b, double& th, double& de, double& s1) {
     double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
     bool nan = false;
@@ -528,8 +591,8 @@ __device__ __forceinline__ int cheb_count2(const KArgs& A, const Lane2& L, const
         gmax = fmax(gmax, g);
         nan = nan || (g != g);
     });
-    const Pair a = xboth(gmin), b = xboth(gmax);
-    return cheb_interval(fmin(a.a, a.b), fmax(b.a, b.b), xany(nan ? 1 : 0) != 0, A.cheb_L, th, de, s1);
+    const Pair mn = xboth(gmin), mx = xboth(gmax);
+    return cheb_interval(fmin(mn.a, mn.b), fmax(mx.a, mx.b), xany(nan ? 1 : 0) != 0, A.cheb_L, th, de, s1);
 }
 
 // Writes the wave's largest count into the per-wave slot (waves of 32 elements).
@@ -546,7 +609,7 @@ __device__ __forceinline__ void wave_count2(int n_e, int slot, int E) {
 // visible to the wave after its own stores).
 template <class D>
 __device__ __forceinline__ Lane2 lane2_setup(int ec, int E, double* T2s) {
-    constexpr auto SL = split_layout(D::NU, D::NH);
+    constexpr SplitLayout SL = Dims2<D>::SL;
     const int lane = threadIdx.x;
     const double* t2 = kargs<false>()->tab2;
     for (int i = lane; i < 2 * SL.n; i += kWave) T2s[i] = t2[i];
@@ -563,17 +626,16 @@ template <int NU, int NV, int MODE>
 __global__ __launch_bounds__(kWave) void k_prep2(const KArgs args) {
     (void)args;
     using D = Dims<NU, NV>;
-    static_assert(NU == NV && NU % 2 == 0, "two lanes per element: n_u = n_v even");
+    using Q2 = Dims2<D>;
     constexpr bool UNI = MODE == MODE_UNIFORM;
-    constexpr auto SL = split_layout(NU, D::NH);
-    __shared__ double T2s[2 * SL.n];
+    __shared__ double T2s[2 * Q2::SL.n];
     const int E = kargs<false>()->E;
     const int e = block_first_element2() + (threadIdx.x & 31);
     const int ec = e < E ? e : E - 1;
     const bool global = kargs<false>()->cheb_global != 0;
     const Lane2 L = lane2_setup<D>(ec, E, T2s);
-    double V[NV * NV / 2];
-    eload2<NV>(kargs<false>()->yv, E, L.voffE, V);
+    double V[Q2::nV];
+    eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, V);
     int n_e = 1;
     if (global) {
         double th, de, s1;
@@ -591,13 +653,14 @@ __global__ __launch_bounds__(kWave) void k_prep2(const KArgs args) {
 // RK4 stage update of the lane's rows of an nn x nn modal array
 // (src/solver.jl:300-321), as rk_store.
 template <int nn>
-__device__ __forceinline__ bool rk_store2(int stage, double c, const double (&kv)[(nn / 2) * nn],
-                                          const double (&y)[(nn / 2) * nn], const double (&ac)[(nn / 2) * nn],
-                                          double* acc, double* out, int E, int voffE) {
-    constexpr int n = (nn / 2) * nn;
+__device__ __forceinline__ bool rk_store2(int stage, double c, const double (&kv)[((nn + 1) / 2) * nn],
+                                          const double (&y)[((nn + 1) / 2) * nn],
+                                          const double (&ac)[((nn + 1) / 2) * nn], double* acc, double* out, int E,
+                                          int voffE, int h) {
+    constexpr int n = ((nn + 1) / 2) * nn;
     bool fin = true;
     double o[n];
-    auto st = [&](double* p, const double (&x)[n]) { estore2<nn>(p, E, voffE, x); };
+    auto st = [&](double* p, const double (&x)[n]) { estore2<nn>(p, E, voffE, h, x); };
     if (stage == 0) {
         st(out, kv);
     } else if (stage == 1) {
@@ -624,22 +687,25 @@ __device__ __forceinline__ bool rk_store2(int stage, double c, const double (&kv
 }
 
 // One RK stage, two lanes per element (k_stage with the element's work shared
-// by the pair; MODE_UNIFORM or MODE_SEPARABLE: the spectral u-solve).
+// by the pair; MODE_UNIFORM or MODE_SEPARABLE: the spectral u-solve). Two
+// waves per SIMD up to n_u = 6, one at n_u = 7.
 template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void k_stage2(const KArgs args) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ? 1 : 2))) void k_stage2(
+    const KArgs args) {
     (void)args;
     using D = Dims<NU, NV>;
-    static_assert(NU == NV && NU % 2 == 0, "two lanes per element: n_u = n_v even");
+    using Q2 = Dims2<D>;
     static_assert(MODE == MODE_UNIFORM || MODE == MODE_SEPARABLE, "spectral u-solve modes only");
-    constexpr int NQ = D::NQ, NH = D::NH, RU = NU / 2, RV = NV / 2;
+    constexpr int NQ = D::NQ, NH = D::NH, RU = Q2::RU, RV = Q2::RV;
     constexpr bool UNI = MODE == MODE_UNIFORM;
-    constexpr int nU = RU * NU, nV = RV * NV;   // the lane's share of a modal array
-    constexpr auto SL = split_layout(NU, NH);
-    constexpr int NP = NH / 2;                   // mirrored pairs of high-order rows
-    constexpr int PL = kLds2 / NH < NP ? kLds2 / NH : NP;   // pair rows of G in LDS
+    constexpr int nU = Q2::nU, nV = Q2::nV;
+    constexpr SplitLayout SL = Q2::SL;
+    constexpr int NLDS = Q2::NLDS;
+    constexpr int NP = NH / 2;                            // mirrored pairs of high-order rows
+    constexpr int PL = NLDS / NH < NP ? NLDS / NH : NP;   // pair rows of G in LDS
     constexpr int PR = NP - PL;
-    static_assert(4 * (2 * RV + RU) <= kLds2, "lift vectors fit the LDS slots");
-    __shared__ double Gs[kLds2 * kWave];
+    static_assert(4 * (2 * RV + RU) <= NLDS, "lift vectors fit the LDS slots");
+    __shared__ double Gs[NLDS * kWave];
     __shared__ double T2s[2 * SL.n];
     if (skip_stage(*kargs<false>())) return;
     const int E = kargs<false>()->E, N = kargs<false>()->N;
@@ -664,8 +730,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                 for (int u = 0; u < 8; ++u) m = max(m, t[u]);
             }
         }
-        eload2<NU>(kargs<false>()->yu, E, L.voffE, U);
-        eload2<NV>(kargs<false>()->yv, E, L.voffE, V);
+        eload2<NU>(kargs<false>()->yu, E, L.voffE, L.h, U);
+        eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, V);
         if (global) {
             for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
             nglob = m;
@@ -697,10 +763,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
 #pragma unroll
     for (int m = 0; m < nU; ++m) bv[m] = 0.0;
     const double v0 = L.h == 0 ? V[0] : 0.0;   // the (0, 0) mode lives in lane 0
-    vol_pass2<D, UNI, NV, NU>(L, V, Wc, bv);
-    vol_pass2<D, UNI, NV, NV>(L, V, Wa, Rv);
+    vol_pass2<D, UNI, true, false>(L, V, Wc, bv);
+    vol_pass2<D, UNI, true, true>(L, V, Wa, Rv);
     phase_barrier();
-    vol_pass2<D, UNI, NU, NV>(L, U, Wc, Rv);
+    vol_pass2<D, UNI, false, true>(L, U, Wc, Rv);
     phase_barrier();
     {
         const double s = -kargs<false>()->four_hh;
@@ -710,7 +776,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
         for (int t = 0; t < np; ++t) {
             const double pc = kargs<false>()->pc[t];
             double pv[nV];
-            eload2<NV>(kargs<false>()->proj[t], E, L.voffE, pv);
+            eload2<NV>(kargs<false>()->proj[t], E, L.voffE, L.h, pv);
 #pragma unroll
             for (int m = 0; m < nV; ++m) Rv[m] = Rv[m] + pc * pv[m];
         }
@@ -739,8 +805,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                 Y1[kk + RU * b] = s;
             }
         // the eigenvalue factors D[i][jj] of the lane's rows i = 2kk + h: the
-        // split table's copy for one class, the class record otherwise
-        const double* Dv = vfresh(kargs<false>()->Ainv + cc * NU * NU + L.h);
+        // split table's copy for one class, the class record otherwise (the
+        // dummy row reads row n_u - 1 of the record and multiplies 0)
+        const double* Dv = vfresh(kargs<false>()->Ainv + cc * NU * NU);
 #pragma unroll
         for (int kk = 0; kk < RU; ++kk)
 #pragma unroll
@@ -748,7 +815,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                 double s = 0.0;
 #pragma unroll
                 for (int b = (jj & 1); b < NU; b += 2) s = fma(Y1[kk + RU * b], Q[b + NU * jj], s);
-                bv[kk + RU * jj] = s * (UNI ? L.t2[SL.sD + kk + RU * jj] : Dv[2 * kk + NU * jj]);
+                const int di = min(2 * kk + L.h, NU - 1);
+                bv[kk + RU * jj] = s * (UNI ? L.t2[SL.sD + kk + RU * jj] : Dv[di + NU * jj]);
             }
 #pragma unroll
         for (int jj = 0; jj < NU; ++jj)
@@ -771,11 +839,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
     }
     {
         CKArgs& A = *kargs<false>();
-        if (live && A.recu) estore2<NU>(A.recu, E, L.voffE, ku);
+        if (live && A.recu) estore2<NU>(A.recu, E, L.voffE, L.h, ku);
         double yu0[nU], acu[nU];
-        if (A.stage >= 1) eload2<NU>(A.y0u, E, L.voffE, yu0);
-        if (A.stage >= 2) eload2<NU>(A.accu, E, L.voffE, acu);
-        if (live && !rk_store2<NU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, L.voffE)) {
+        if (A.stage >= 1) eload2<NU>(A.y0u, E, L.voffE, L.h, yu0);
+        if (A.stage >= 2) eload2<NU>(A.accu, E, L.voffE, L.h, acu);
+        if (live && !rk_store2<NU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, L.voffE, L.h)) {
             atomicMin(&kargs<false>()->err->nonfinite, kargs<false>()->step);
             atomicMin(&kargs<false>()->err->nf_serial, kargs<false>()->serial);
         }
@@ -790,7 +858,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
     const int cb = UNI ? 0 : kargs<false>()->cls_b[ec];
     {
         double Vr[nV];
-        eload2<NV>(kargs<false>()->yv, E, L.voffE, Vr);
+        eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, Vr);
         double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
         bool nan = false;
         mass_rows2<D, UNI>(*kargs<false>(), L, Vr, cb, [&](int p, int q, double g, double Gv) {
@@ -801,9 +869,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
             else if (p < PL) Gs[(p * NH + q) * kWave + lane] = Gv;
             else Gt[(p - PL) * NH + q] = Gv;
         });
-        const Pair ga = xboth(gmin), gb = xboth(gmax);
-        n_e = cheb_interval(fmin(ga.a, ga.b), fmax(gb.a, gb.b), xany(nan ? 1 : 0) != 0, kargs<false>()->cheb_L,
-                            th, de, s1);
+        const Pair mn = xboth(gmin), mx = xboth(gmax);
+        n_e = cheb_interval(fmin(mn.a, mn.b), fmax(mx.a, mx.b), xany(nan ? 1 : 0) != 0, kargs<false>()->cheb_L, th,
+                            de, s1);
     }
     phase_barrier();
     double X[nV];
@@ -831,26 +899,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
 #pragma unroll
             for (int p = 0; p < SL.NHH; ++p) {
                 const bool mid = (NH & 1) && p == NH / 2;
-                const double* tph = L.t2 + SL.sPh + p * SL.RU;
-                double t[NV];
+                const double* tph = L.t2 + SL.sPh + p * RV;
+                double t[D::NV];
 #pragma unroll
-                for (int b = 0; b < NV; ++b) {
+                for (int b = 0; b < D::NV; ++b) {
                     double s = 0.0;
 #pragma unroll
                     for (int kk = 0; kk < RV; ++kk) s = fma(tph[kk], d[kk + RV * b], s);
                     t[b] = comb_fwd(xboth(s), L.sg, false);
                 }
                 double y[NH];
-                rows_sym<NH, NV, false>(Ph, [&](int b) { return t[b]; }, y);
+                rows_sym<NH, D::NV, false>(Ph, [&](int b) { return t[b]; }, y);
 #pragma unroll
                 for (int q = 0; q < NH; ++q) {
                     const double G = mid ? Gm[q] : (p < PL ? Gs[(p * NH + q) * kWave + lane] : Gt[(p - PL) * NH + q]);
                     y[q] *= G;
                 }
-                double z[NV];
-                cols_sym<NH, NV, false>(Ph, y, z);
+                double z[D::NV];
+                cols_sym<NH, D::NV, false>(Ph, y, z);
 #pragma unroll
-                for (int b = 0; b < NV; ++b) {
+                for (int b = 0; b < D::NV; ++b) {
                     const double c = mid ? z[b] : comb_back(xboth(z[b]), L.sg, false);
 #pragma unroll
                     for (int kk = 0; kk < RV; ++kk) r[kk + RV * b] = fma(tph[kk], c, r[kk + RV * b]);
@@ -869,10 +937,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
         double yv0[nV], acv[nV];
         const int stage = A.stage;
         const double c = A.cstage;
-        if (live && A.recv) estore2<NV>(A.recv, E, L.voffE, X);
-        if (stage >= 1) eload2<NV>(A.y0v, E, L.voffE, yv0);
-        if (stage >= 2) eload2<NV>(A.accv, E, L.voffE, acv);
-        if (live && !rk_store2<NV>(stage, c, X, yv0, acv, A.accv, A.outv, E, L.voffE)) {
+        if (live && A.recv) estore2<NV>(A.recv, E, L.voffE, L.h, X);
+        if (stage >= 1) eload2<NV>(A.y0v, E, L.voffE, L.h, yv0);
+        if (stage >= 2) eload2<NV>(A.accv, E, L.voffE, L.h, acv);
+        if (live && !rk_store2<NV>(stage, c, X, yv0, acv, A.accv, A.outv, E, L.voffE, L.h)) {
             atomicMin(&kargs<false>()->err->nonfinite, kargs<false>()->step);
             atomicMin(&kargs<false>()->err->nf_serial, kargs<false>()->serial);
         }

@@ -44,8 +44,8 @@ class Workspace:
     count to the same 1e-14 residual bound). ``lanes`` is the lanes per element
     of the device stage kernels: 1, or 2 (a pair of lanes per element: twice
     the waves, half the registers per lane) where the problem allows it
-    (q_u = q_v odd, coefficients constant on each element, Chebyshev mass
-    solve); results agree to rounding.
+    (q_u <= 6, coefficients constant on each element, Chebyshev mass solve);
+    results agree to rounding.
     """
 
     def __init__(self, pre, N: int, nblocks=None, cheb: str = "global", lanes: int = 1):
@@ -152,7 +152,8 @@ def solve(q: int, N: int, T: float, *, mixed: bool = True, flux: str = "upwind",
     time loop runs ``nsteps`` RK4 steps on the device-resident state with
     t += dt after every step, as the reference's loop. :source mode returns
     the time series of ``_run_source`` (src/solver.jl:872-958) whatever
-    ``return_state`` says, as the reference does.
+    ``return_state`` says, as the reference does. ``cheb`` and ``lanes``
+    choose the device kernels as for ``Workspace``.
     """
     mode = str(mode).lstrip(":")
     if mode not in ("mms", "initial_value", "source"):

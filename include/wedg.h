@@ -174,9 +174,10 @@ wedg_status wedg_graph_enable(wedg_workspace* ws, long chunk_steps);
  * src/rhs.jl:222-252, is the CPU's form of the same choice). 1 (the default):
  * one lane holds all of an element's arrays; 2: a pair of lanes shares each
  * element, which doubles the waves and halves the registers per lane. The
- * two-lane kernels cover n_u = n_v even (q_u = q_v odd) with the spectral
- * u-solve (coefficients constant on each element) and Chebyshev iteration;
- * other problems keep one lane whatever is set. Results agree with the
+ * two-lane kernels cover every q_u <= 6 and both pairings (n_v = n_u,
+ * n_u - 1) with the spectral u-solve (coefficients constant on each element)
+ * and Chebyshev iteration; other problems keep one lane whatever is set
+ * (wedg_problem_info says which). Results agree with the
  * one-lane kernels to rounding. A batch (wedg_batch_create) sets its
  * workspaces back to 1. wedg_workspace_lanes_per_element reports the lanes
  * the workspace's calls use. */

@@ -1,9 +1,11 @@
 """The two-lane stage kernels (a pair of lanes per element, wedg_stage2.h)
 against the CPU oracle and against the one-lane kernels, through the C ABI.
 
-They cover q_u = q_v odd with coefficients constant on each element (the
-spectral u-solve) and the Chebyshev mass solve; every test checks that the
-workspace really runs them (``lanes == 2``). Tolerance: 1e-12 relative per
+They cover every degree q_u = 1..6 with both pairings (q_v = q_u, q_u - 1),
+coefficients constant on each element (the spectral u-solve) and the
+Chebyshev mass solve; every test checks that the workspace really runs them
+(``lanes == 2``). Odd mode counts leave the parity-1 lane a dummy row, and odd
+point counts a middle point held by both lanes: both are covered. Tolerance: 1e-12 relative per
 right-hand side against the oracle, as the one-lane parity tests; the two
 device kernels group their sums differently, so they agree to rounding, not
 bit for bit.
@@ -48,16 +50,17 @@ def random_element_coefficients(rng, N):
 @pytest.mark.parametrize("mode", ["mms", "initial_value", "source"])
 @pytest.mark.parametrize("boundary", ["absorbing_2beta", "absorbing", "dirichlet", "wall", "MIXED", "MIXED2"])
 def test_two_lane_rhs_matrix(mode, boundary):
-    """Every flux, q in {1, 3, 5}, meshes from one element to 7 x 7 (element
-    counts that leave part of the last wave empty), constant coefficients."""
+    """Every flux, q = 1..6, both pairings, meshes from one element to 7 x 7
+    (element counts that leave part of the last wave empty), constant
+    coefficients."""
     bnd = {"MIXED": MIXED, "MIXED2": MIXED2}.get(boundary, boundary)
     rng = np.random.default_rng(21)
     worst = 0.0
-    for q in (1, 3, 5):
-        for N in (1, 2, 4, 7):
-            for flux in ("upwind", "central", "alternating"):
-                say(f"case two-lane rhs mode={mode} boundary={boundary} q={q} N={N} flux={flux}")
-                pre, _ = precompute(q, N, flux=flux, mixed=False, mode=mode, boundary=bnd, sigma=0.3,
+    for q in (1, 2, 3, 4, 5, 6):
+        for N in (1, 2, 7):
+            for flux, mixed in (("upwind", False), ("central", True), ("alternating", q % 2 == 0)):
+                say(f"case two-lane rhs mode={mode} boundary={boundary} q={q} N={N} flux={flux} mixed={mixed}")
+                pre, _ = precompute(q, N, flux=flux, mixed=mixed, mode=mode, boundary=bnd, sigma=0.3,
                                     sigma_dirichlet=0.6, kappa=0.7, phase=(0.4, 0.3),
                                     coefficients=constant_coefficients(c=0.9, alpha=3e-3, beta=0.02))
                 u, v = rand_state(rng, N, pre)
@@ -77,11 +80,12 @@ def test_two_lane_jumping_coefficients(mode, cheb):
     jumps fall on element faces)."""
     rng = np.random.default_rng(22)
     worst = 0.0
-    for q in (1, 3, 5):
+    for q in (1, 2, 3, 4, 5, 6):
         for N in (4, 8):
             for co in (random_element_coefficients(rng, N), quadrant_coefficients(0.5, 0.5)):
-                say(f"case two-lane jumps mode={mode} cheb={cheb} q={q} N={N}")
-                pre, _ = precompute(q, N, mixed=False, mode=mode, boundary=MIXED, sigma=0.3,
+                mixed = (q + N) % 2 == 1
+                say(f"case two-lane jumps mode={mode} cheb={cheb} q={q} N={N} mixed={mixed}")
+                pre, _ = precompute(q, N, mixed=mixed, mode=mode, boundary=MIXED, sigma=0.3,
                                     sigma_dirichlet=0.6, kappa=0.7, coefficients=co)
                 u, v = rand_state(rng, N, pre)
                 a = oracle_rhs(pre, N, u, v, 0.021, cheb)
@@ -96,9 +100,10 @@ def test_two_lane_larger_meshes():
     (grids of a multiple of 8 waves), against the oracle and the one-lane kernel."""
     rng = np.random.default_rng(23)
     worst = worst1 = 0.0
-    for q, N in ((5, 16), (3, 32), (5, 40), (1, 64)):
-        say(f"case two-lane large q={q} N={N}")
-        pre, _ = precompute(q, N, mixed=False, mode="initial_value", boundary=MIXED2, sigma=0.2, kappa=0.5,
+    for q, N, mixed in ((5, 16, False), (3, 32, False), (5, 40, True), (1, 64, False), (4, 24, False),
+                        (6, 16, False), (6, 24, True), (2, 40, True)):
+        say(f"case two-lane large q={q} N={N} mixed={mixed}")
+        pre, _ = precompute(q, N, mixed=mixed, mode="initial_value", boundary=MIXED2, sigma=0.2, kappa=0.5,
                             coefficients=random_element_coefficients(rng, N))
         u, v = rand_state(rng, N, pre, su=1e-2, sv=1e-2)
         a = oracle_rhs(pre, N, u, v, 0.0)
@@ -115,9 +120,9 @@ def test_two_lane_rk4_steps(cheb):
     """Several RK4 steps (every stage kind, the epilogue's next-stage counts)
     against the oracle's RK4 and the one-lane kernel."""
     rng = np.random.default_rng(24)
-    for q, N in ((3, 6), (5, 9)):
-        say(f"case two-lane rk4 cheb={cheb} q={q} N={N}")
-        pre, _ = precompute(q, N, mixed=False, mode="source", boundary=MIXED, sigma=0.3, sigma_dirichlet=0.6,
+    for q, N, mixed in ((3, 6, False), (5, 9, False), (4, 7, False), (6, 5, True), (2, 6, True)):
+        say(f"case two-lane rk4 cheb={cheb} q={q} N={N} mixed={mixed}")
+        pre, _ = precompute(q, N, mixed=mixed, mode="source", boundary=MIXED, sigma=0.3, sigma_dirichlet=0.6,
                             kappa=0.7, coefficients=random_element_coefficients(rng, N))
         u, v = rand_state(rng, N, pre, su=1e-3, sv=1e-2)
         dt, nst = 2e-4, 5
