@@ -93,8 +93,9 @@ def flops_per_element_stage(nu, nv, nh, nc):
     return base + (nc - 1) * step, traces
 
 
-def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
-    """Aggregate DOF·stages/s of the uneven batch: together vs one after the other."""
+def bench_batched(count=64, seed=0, sequential=True, cheb="global", lanes_per_element=(1, 2)):
+    """Aggregate DOF·stages/s of the uneven batch: together vs one after the
+    other, both with the lanes per element the tuner chose for each group."""
     import edg_westervelt_amd.batch as B
     from edg_westervelt_amd._lib import lib
     t = time.perf_counter()
@@ -103,8 +104,10 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
     work = sum(4 * b.nsteps * b.dof for b in built)
     bt = B.Batch([b.ws for b in built])
     bt.advance(0.0, [b.dt for b in built], [1] * count)          # warm-up step
-    # launch parameters by the autotuner (32-step probes of every stream count and chunk)
-    tune_ms, choice = bt.tune(0.0, [b.dt for b in built], probe_steps=32)
+    # launch parameters by the autotuner (32-step probes: lanes per element of
+    # each kernel group, then every stream count and chunk)
+    tune_ms, choice = bt.tune(0.0, [b.dt for b in built], probe_steps=32, lanes_per_element=lanes_per_element)
+    lpe = bt.lanes_per_element()
     for b in built:                                              # back to t = 0
         B.set_zero_state(b.ws)
     print(f"# batched: {count} instances together", file=sys.stderr, flush=True)
@@ -131,7 +134,8 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
     groups = len({(b.inst.q, b.inst.mixed, b.ws.problem.pre.one_sided) for b in built})
     if not sequential:
         return dict(value=work / wb, unit="DOF·stages/s", wall_s=wb, device_s=dev_ms / 1e3, dof_stages=work,
-                    instances=count, tuner=dict(lanes=choice[0], chunk_steps=choice[1]))
+                    instances=count, tuner=dict(lanes=choice[0], chunk_steps=choice[1],
+                                                two_lane_instances=sum(1 for x in lpe if x == 2)))
     return dict(value=work / wb, unit="DOF·stages/s", sequential_value=work / seq, speedup=seq / wb,
                 wall_s=wb, device_s=dev_ms / 1e3, sequential_s=seq, longest_instance_s=longest,
                 dof_stages=work, instances=count, groups=groups, bitwise_equal_to_sequential=bool(same),
@@ -139,7 +143,11 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global"):
                          "initial_value Gaussian, piecewise-constant c, alpha, beta on 4x4 blocks",
                 cheb_policy=cheb, build_s=round(t_build, 2),
                 tuner=dict(choice=dict(lanes=choice[0], chunk_steps=choice[1]),
-                           probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()}))
+                           probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()},
+                           lanes_per_element={f"q{k[0]}/{k[1]} mode {k[2]}": v for k, v in bt.lane_choice.items()},
+                           lanes_per_element_probe_ms={f"q{k[0]}/{k[1]} mode {k[2]} x{k[3]}": round(v, 3)
+                                                       for k, v in bt.lane_times.items()},
+                           two_lane_instances=sum(1 for x in lpe if x == 2)))
 
 
 def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
@@ -355,6 +363,8 @@ def main():
                     help="the uneven-batch speedup: with the headline (on), not at all (off), or alone (only)")
     ap.add_argument("--batched-sequential", choices=("on", "off"), default="on",
                     help="off: time only the batched run (for a kernel trace of it)")
+    ap.add_argument("--batched-lanes", choices=("tune", "1", "2"), default="tune",
+                    help="lanes per element of the batch's instances: the tuner's choice per kernel group, or fixed")
     ap.add_argument("--adjoint", choices=("on", "off", "only"), default="on",
                     help="the receiver-misfit gradient and its cost against one forward solve: with the headline "
                          "(on), not at all (off), or alone (only)")
@@ -374,6 +384,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+    blpe = (1, 2) if args.batched_lanes == "tune" else (int(args.batched_lanes),)
 
     import edg_westervelt_amd as w
     from edg_westervelt_amd import _lib
@@ -407,7 +418,7 @@ def main():
                               n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched_adjoint=ba)))
         return
     if args.batched == "only":
-        bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb)
+        bb = bench_batched(sequential=args.batched_sequential == "on", cheb=args.cheb, lanes_per_element=blpe)
         print(json.dumps(dict(metric="batched speedup (uneven instances)", value=bb.get("speedup"), unit="x",
                               n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched=bb)))
         return
@@ -508,7 +519,7 @@ def main():
                  hbm_frac_model=bytes_step / 4 / stage_s / 1e9 / HBM_PEAK_GBS,
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
-    batched = bench_batched(cheb=args.cheb) if args.batched == "on" else None
+    batched = bench_batched(cheb=args.cheb, lanes_per_element=blpe) if args.batched == "on" else None
     adjoint = bench_adjoint() if args.adjoint == "on" else None
     batched_adjoint = bench_batched_adjoint(T=args.batched_adjoint_T) if args.batched_adjoint == "on" else None
     graph = bench_graph() if args.graph == "on" else None

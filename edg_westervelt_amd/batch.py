@@ -16,7 +16,9 @@ source-mode ``solve`` (time series, receivers, snapshots) for a list of runs.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
+import os
 from types import SimpleNamespace
 
 import numpy as np
@@ -33,10 +35,16 @@ __all__ = ["Batch", "piecewise_constant", "block_coefficients", "gaussian_profil
 
 class Batch:
     """Workspaces advanced together (``wedg_batch_create``); each keeps its own
-    state, which ``wedg_get_state`` / ``Workspace`` calls read as usual."""
+    state, which ``wedg_get_state`` / ``Workspace`` calls read as usual, and
+    its own lanes per element (``Workspace(..., lanes=)``), which the batch's
+    kernels use too."""
 
     def __init__(self, workspaces):
         self.workspaces = list(workspaces)
+        self.h = None
+        self._create()
+
+    def _create(self):
         n = len(self.workspaces)
         arr = (C.c_void_p * n)(*[w.dev.h.value for w in self.workspaces])
         h = C.c_void_p()
@@ -45,11 +53,43 @@ class Batch:
             raise WedgError(f"wedg_batch_create failed with status {st}")
         self.h = h
 
-    def __del__(self):
+    def _destroy(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
             lib.wedg_batch_destroy(h)
             self.h = None
+
+    def __del__(self):
+        self._destroy()
+
+    def kernel_groups(self):
+        """{(q_u, q_v, mode): [instance indexes]}: the instances that share
+        kernels whatever their lanes per element."""
+        out = {}
+        for i, w in enumerate(self.workspaces):
+            key = (w.qu1 - 1, w.qv1 - 1, w.problem.info()["mode"])
+            out.setdefault(key, []).append(i)
+        return out
+
+    def set_lanes_per_element(self, lanes):
+        """Lanes per element (1 or 2; one value, or one per instance) of the
+        instances' kernels, the batch's and their own calls'; instances whose
+        problem has no two-lane kernels keep one lane. The batch is formed
+        again with its launch parameters."""
+        n = len(self.workspaces)
+        per = [int(lanes)] * n if np.ndim(lanes) == 0 else [int(x) for x in lanes]
+        if len(per) != n:
+            raise ValueError("one lane count per instance")
+        cfg = self.config()
+        self._destroy()
+        for w, l in zip(self.workspaces, per):
+            w.dev.set_lanes(l)
+        self._create()
+        self.configure(*cfg)
+
+    def lanes_per_element(self):
+        """The lanes per element each instance runs with."""
+        return [w.dev.lanes for w in self.workspaces]
 
     def advance(self, t0, dt, nsteps, stop_on_nonfinite: bool = False, raise_errors: bool = True):
         """nsteps[i] RK4 steps of dt[i] from t0[i] for instance i. Returns the
@@ -90,15 +130,51 @@ class Batch:
         lib.wedg_batch_config(self.h, C.byref(lanes), C.byref(chunk), C.byref(sched))
         return lanes.value, chunk.value, sched.value
 
-    def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8), chunks=(0, 4), repeats: int = 2):
+    def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8), chunks=(0, 4), repeats: int = 2,
+             lanes_per_element=None):
         """Autotuner of the launch parameters: times ``probe_steps`` steps of the
         batch (device time between the batch's events, best of ``repeats``) for
         every (lanes, chunk) pair, keeps the fastest and puts every state back
-        as it was. Returns {(lanes, chunk): ms} and the choice."""
+        as it was. Returns {(lanes, chunk): ms} and the choice.
+        ``lanes_per_element``: None keeps every instance's lanes per element;
+        one value sets it for all; with more than one option the tuner first
+        chooses, for each kernel group with two-lane kernels, the option whose
+        probe of the group alone is fastest (``lane_times``, ``lane_choice``).
+        Results then agree with one-lane runs to rounding, and bitwise with
+        the instances' own runs after the choice."""
         saved = [get_state(w) for w in self.workspaces]
         n = len(self.workspaces)
         times = {}
+        self.lane_times, self.lane_choice = {}, {}
         try:
+            opts = [] if lanes_per_element is None else sorted({int(x) for x in np.atleast_1d(lanes_per_element)})
+            if len(opts) == 1:
+                self.set_lanes_per_element(opts[0])
+            elif len(opts) > 1:
+                per = self.lanes_per_element()
+                for key, members in self.kernel_groups().items():
+                    if not self.workspaces[members[0]].problem.info()["two_lane"]:
+                        continue
+                    steps = [0] * n
+                    for i in members:
+                        steps[i] = probe_steps
+                    for lpe in opts:
+                        for i in members:
+                            per[i] = lpe
+                        self.set_lanes_per_element(per)
+                        best = math.inf
+                        for _ in range(repeats):
+                            for i in members:
+                                w = self.workspaces[i]
+                                w.dev.check(lib.wedg_set_state(w.dev.h, _ptr(saved[i][0]), _ptr(saved[i][1])))
+                            self.advance(t0, dt, steps)
+                            best = min(best, self.device_ms())
+                        self.lane_times[key + (lpe,)] = best
+                    pick = min(opts, key=lambda l: self.lane_times[key + (l,)])
+                    self.lane_choice[key] = pick
+                    for i in members:
+                        per[i] = pick
+                self.set_lanes_per_element(per)
             for nl in lanes:
                 for ch in chunks:
                     self.configure(nl, ch)
@@ -163,7 +239,23 @@ KINDS = ("absorbing_2beta", "absorbing", "dirichlet", "wall")
 FLUXES = ("upwind", "central", "alternating")
 
 
-def uneven_instances(seed: int = 0, count: int = 64, T: float = 0.25):
+AMPLITUDE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "uneven_amplitudes.json")
+
+
+def _calibrated_amplitudes(seed: int, T: float):
+    """The amplitudes of calibrate_amplitudes for (seed, T), or None."""
+    try:
+        with open(AMPLITUDE_FILE) as f:
+            tab = json.load(f)
+    except FileNotFoundError:
+        return None
+    for e in tab["runs"]:
+        if e["seed"] == seed and e["T"] == T:
+            return e["amp"]
+    return None
+
+
+def uneven_instances(seed: int = 0, count: int = 64, T: float = 0.25, amplitude: str = "auto"):
     """The uneven batched workload, drawn with numpy's default_rng(seed), per
     instance in this order: N in {16, 24, 32, 48, 64}; q_u in {2..6}; q_v =
     q_u - 1 or q_u; the flux (upwind, central, alternating); per 4 x 4 block c
@@ -171,11 +263,22 @@ def uneven_instances(seed: int = 0, count: int = 64, T: float = 0.25):
     Gaussian's centre in [0.3, 0.7]^2 and width in U[0.03, 0.08]; each side's
     kind from (absorbing_2beta, absorbing, dirichlet, wall); sigma in U[-1, 1]
     (Dirichlet sides take |sigma|, which must be >= 0) and kappa in U[0, 1].
-    The amplitude keeps 1 - 2 beta |v| >= 0.5 with a margin of 4 on the
-    free-space bound |v| <= c_max |grad u0|_max = c_max amp / (s sqrt(e))
-    (reflections from walls, Dirichlet sides and coefficient jumps raise |v|
-    above it): amp = min(0.05, s sqrt(e) / (16 beta_max c_max)). Every run
-    goes to T."""
+    Every run goes to T.
+
+    The amplitude (``amp_rule`` of each instance): ``"calibrated"`` takes the
+    amplitudes of ``calibrate_amplitudes`` stored in uneven_amplitudes.json,
+    which bring 1 - 2 beta_max max|v_h| over the whole run to 0.51 (max|v_h|
+    over the points of the diagnostics, sampled every 16 steps; >= 0.5);
+    ``"bound"`` keeps 1 - 2 beta |v| >= 0.5 with a margin of 4 on the
+    free-space bound |v| <= c_max |grad u0|_max = c_max amp / (s sqrt(e)):
+    amp = min(0.05, s sqrt(e) / (16 beta_max c_max)), the calibration's
+    starting point; ``"auto"`` is "calibrated" for the (seed, T) and instances
+    the file holds, else "bound"."""
+    if amplitude not in ("auto", "calibrated", "bound"):
+        raise ValueError(f"unknown amplitude rule {amplitude} (expected auto, calibrated or bound)")
+    cal = None if amplitude == "bound" else _calibrated_amplitudes(seed, T)
+    if amplitude == "calibrated" and (cal is None or len(cal) < count):
+        raise ValueError(f"no calibrated amplitudes for seed {seed}, T = {T}, {count} instances")
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
@@ -192,8 +295,11 @@ def uneven_instances(seed: int = 0, count: int = 64, T: float = 0.25):
         sigma = float(rng.uniform(-1.0, 1.0))
         kappa = float(rng.uniform(0.0, 1.0))
         amp = min(0.05, s * math.sqrt(math.e) / (16.0 * float(beta.max()) * float(c.max())))
+        rule = "bound"
+        if cal is not None and i < len(cal):
+            amp, rule = float(cal[i]), "calibrated"
         out.append(SimpleNamespace(index=i, N=N, q=q, mixed=mixed, flux=flux, c=c, alpha=alpha, beta=beta,
-                                   x0=float(x0), y0=float(y0), s=s, amp=amp,
+                                   x0=float(x0), y0=float(y0), s=s, amp=amp, amp_rule=rule,
                                    boundary=dict(zip(("bot", "top", "lft", "rgt"), kinds)),
                                    sigma=sigma, sigma_dirichlet=abs(sigma), kappa=kappa, T=T))
     return out
@@ -211,6 +317,75 @@ def build_instance(inst, dt_fac=10, cheb: str = "global"):
     set_zero_state(ws)
     return SimpleNamespace(inst=inst, pre=pre, geo=geo, ws=ws, dt=dt, nsteps=nsteps,
                            dof=inst.N * inst.N * (pre.qu1 ** 2 + pre.qv1 ** 2))
+
+
+def run_vmax(built, sample_every: int = 16):
+    """Each built instance's run from the zero state in one batch, with
+    max|v_h| at the points of the diagnostics (``wedg_diagnostics``) after
+    every ``sample_every`` of its steps and after its last: returns the largest
+    value per instance (inf where the run failed) and the statuses."""
+    from .solver import _set_diag
+    n = len(built)
+    for x in built:
+        _set_diag(x.ws, x.geo)
+        set_zero_state(x.ws)
+    b = Batch([x.ws for x in built])
+    done = np.zeros(n, dtype=np.int64)
+    t = np.zeros(n)
+    vm = np.zeros(n)
+    status = np.zeros(n, dtype=np.int32)
+    bufs = [SimpleNamespace(E=np.zeros(x.ws.N ** 2), vmax=np.zeros(x.ws.N ** 2), nd=np.zeros(x.ws.N ** 2),
+                            eta=np.zeros(x.ws.qv1), sl=np.zeros(x.ws.N * x.ws.qu1)) for x in built]
+    while True:
+        steps = [min(sample_every, x.nsteps - int(d)) if st == _lib.OK else 0
+                 for x, d, st in zip(built, done, status)]
+        if not any(steps):
+            break
+        tend, stat = b.advance(t, [x.dt for x in built], steps, raise_errors=False)
+        for i, x in enumerate(built):
+            if steps[i] == 0:
+                continue
+            if stat[i] != _lib.OK:
+                status[i] = stat[i]
+                vm[i] = math.inf
+                continue
+            t[i] = tend[i]
+            done[i] += steps[i]
+            f = bufs[i]
+            x.ws.dev.check(lib.wedg_diagnostics(x.ws.dev.h, _ptr(f.E), _ptr(f.vmax), _ptr(f.nd), 0, _ptr(f.eta),
+                                                _ptr(f.sl)))
+            vm[i] = max(vm[i], float(f.vmax.max()))
+    return vm, status
+
+
+def calibrate_amplitudes(seed: int = 0, count: int = 64, T: float = 0.25, target: float = 0.49,
+                         updates: int = 3, sample_every: int = 16, log=None):
+    """Amplitudes of the uneven workload that bring d = 2 beta_max max|v_h|
+    over the whole run (run_vmax) to ``target``, so that 1 - 2 beta_max
+    max|v| >= 0.5: from the "bound" amplitudes, ``updates`` rescalings amp <-
+    amp target / d (a run that fails takes 0.7 amp; the cap 0.05 stays), then
+    runs that confirm d <= 0.5 for every instance, rescaling any that exceed
+    it. Returns the amplitudes and the confirmed d."""
+    insts = uneven_instances(seed=seed, count=count, T=T, amplitude="bound")
+    amp = np.array([x.amp for x in insts])
+    bmax = np.array([float(x.beta.max()) for x in insts])
+    d = None
+    for it in range(updates + 4):
+        for x, a in zip(insts, amp):
+            x.amp = float(a)
+        built = [build_instance(x) for x in insts]
+        vm, st = run_vmax(built, sample_every)
+        d = 2.0 * bmax * vm
+        if log:
+            log(f"calibration pass {it}: d in [{d.min():.4f}, {d.max():.4f}], failed {int((st != 0).sum())}")
+        del built
+        if it >= updates and np.all(d <= 0.5):
+            break
+        scale = np.where(np.isfinite(d), target / np.maximum(d, 1e-300), 0.7)
+        if it >= updates:   # confirmation: only the instances above the bound move
+            scale = np.where(d > 0.5, np.minimum(scale, 0.98), 1.0)
+        amp = np.minimum(0.05, amp * scale)
+    return amp, d
 
 
 def set_zero_state(ws: Workspace):
@@ -353,3 +528,16 @@ def run_source_many(runs, cheb: str = "global"):
             S.done += k
             _source_record(S, S.done)
     return [_source_result(S) for S in setups]
+
+
+if __name__ == "__main__":
+    # python -m edg_westervelt_amd.batch: calibrates the uneven workload's
+    # amplitudes on the device and writes uneven_amplitudes.json
+    import sys
+    amp, d = calibrate_amplitudes(log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = dict(rule="amp such that d = 2 beta_max max|v_h| over the run <= 0.5 (target 0.49), max|v_h| at the "
+                    "diagnostics points after every 16 steps; from the bound amplitudes, 3 rescalings amp <- "
+                    "amp 0.49 / d, then confirmation runs",
+               runs=[dict(seed=0, T=0.25, amp=[float(a) for a in amp], d=[round(float(x), 5) for x in d])])
+    with open(AMPLITUDE_FILE if len(sys.argv) < 2 else sys.argv[1], "w") as f:
+        json.dump(out, f, indent=1)

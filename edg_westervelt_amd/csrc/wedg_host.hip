@@ -1328,7 +1328,19 @@ extern "C" wedg_status wedg_difference_norms(wedg_workspace* ws, const double* u
 // step count and failure record, and gets exactly the stage arguments (and so
 // the results) its own wedg_advance would give it.
 
-static hipError_t launch_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode) {
+static hipError_t launch_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode,
+                               bool two) {
+    if (two) {
+        switch (qu) {
+            case 1: return launch_batch2_q1(b, grid, nv, which, s, mode);
+            case 2: return launch_batch2_q2(b, grid, nv, which, s, mode);
+            case 3: return launch_batch2_q3(b, grid, nv, which, s, mode);
+            case 4: return launch_batch2_q4(b, grid, nv, which, s, mode);
+            case 5: return launch_batch2_q5(b, grid, nv, which, s, mode);
+            case 6: return launch_batch2_q6(b, grid, nv, which, s, mode);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (qu) {
         case 1: return launch_batch_q1(b, grid, nv, which, s, mode);
         case 2: return launch_batch_q2(b, grid, nv, which, s, mode);
@@ -1342,6 +1354,7 @@ static hipError_t launch_batch(const BatchArgs& b, int grid, int qu, int nv, int
 
 struct BatchGroup {
     int qu = 0, nv = 0, mode = 0;
+    bool two = false;             // the two-lane kernels (every instance's lanes per element is 2)
     int lane = 0;                 // the stream that runs the group
     std::vector<int> inst;        // the group's workspaces (indexes into wedg_batch::ws)
     std::vector<int> order;       // inst by decreasing step count: the active ones are a prefix
@@ -1454,19 +1467,17 @@ extern "C" wedg_status wedg_batch_create(wedg_workspace* const* ws, int n, wedg_
     auto* b = new wedg_batch();
     b->ws.assign(ws, ws + n);
     for (int i = 0; i < n; ++i) {
-        // the batched kernels run one lane per element: so do the instances' own calls
-        if (ws[i]->lpe != 1) {
-            graph_free(ws[i]);
-            ws[i]->lpe = 1;
-        }
+        // a group shares the kernels of its instances' own calls: the same
+        // (n_u, n_v, mode) and lanes per element
         const wedg_problem* p = ws[i]->p;
+        const bool two = ws[i]->two();
         auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BatchGroup& g) {
-            return g.qu == p->qu && g.nv == p->nv && g.mode == p->mode;
+            return g.qu == p->qu && g.nv == p->nv && g.mode == p->mode && g.two == two;
         });
         if (it == b->groups.end()) {
             b->groups.emplace_back();
             it = b->groups.end() - 1;
-            it->qu = p->qu; it->nv = p->nv; it->mode = p->mode;
+            it->qu = p->qu; it->nv = p->nv; it->mode = p->mode; it->two = two;
         }
         it->inst.push_back(i);
     }
@@ -1506,6 +1517,10 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
         }
     for (const auto& l : b->lanes)
         if (!l.stream || !l.dargs[0] || !l.dargs[1] || !l.hargs[0] || !l.hargs[1]) return WEDG_ERR_HIP;
+    // the block maps and kernels follow the lanes per element of batch_create
+    for (const auto& g : b->groups)
+        for (int i : g.inst)
+            if (b->ws[i]->two() != g.two) return WEDG_ERR_ARGUMENT;
     wedg_status s;
     for (int i = 0; i < n; ++i) {
         if ((s = reset_err(b->ws[i]))) return s;
@@ -1535,12 +1550,13 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
     }
     // groups to lanes, largest first, each to the least loaded lane; the weight
     // of a group is its cost per element and stage (the stage time of a wave,
-    // one element per lane) times its largest step count
+    // one element per lane, half that with two) times its largest step count
     {
         std::vector<std::pair<double, int>> w;
         for (size_t g = 0; g < b->groups.size(); ++g) {
             const wedg_problem* p0 = b->ws[b->groups[g].order[0]]->p;
-            w.push_back({stage_cost(p0->nu, p0->nv, p0->nh) * (double)nsteps[b->groups[g].order[0]], (int)g});
+            const double pair = b->groups[g].two ? 0.5 : 1.0;   // two lanes share an element's work
+            w.push_back({stage_cost(p0->nu, p0->nv, p0->nh) * pair * (double)nsteps[b->groups[g].order[0]], (int)g});
         }
         std::stable_sort(w.begin(), w.end(), [](auto& x, auto& y) { return x.first > y.first; });
         std::vector<double> load(b->lanes.size(), 0.0);
@@ -1619,7 +1635,7 @@ static wedg_status batch_run(wedg_batch* b, const double* t0, const double* dt, 
                 ba.args = l.dargs[buf] + x.off;
                 ba.binst = g.dmap;
                 ba.bfirst = g.dmap + g.bfirst.back();
-                HIPOK(launch_batch(ba, x.grid, g.qu, g.nv, x.which, l.stream, g.mode));
+                HIPOK(launch_batch(ba, x.grid, g.qu, g.nv, x.which, l.stream, g.mode, g.two));
             }
         }
     }

@@ -1,0 +1,14 @@
+// Batched two-lane stage kernel instantiations for q_u = 3 (n_u = 4), both pairings q_v = q_u - 1 and q_v = q_u.
+#include "wedg_k2_inst.h"
+#include "wedg_launch.h"
+
+namespace wedg {
+
+hipError_t launch_batch2_q3(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode) {
+    if (mode != MODE_UNIFORM && mode != MODE_SEPARABLE) return hipErrorInvalidValue;
+    if (nv == 4) return launch_batch2_nv<4, 4>(b, grid, which, s, mode);
+    if (nv == 3) return launch_batch2_nv<4, 3>(b, grid, which, s, mode);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace wedg

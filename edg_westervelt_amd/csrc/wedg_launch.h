@@ -28,6 +28,12 @@ hipError_t launch2_q3(const KArgs& a, int nv, int which, hipStream_t s, int mode
 hipError_t launch2_q4(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 hipError_t launch2_q5(const KArgs& a, int nv, int which, hipStream_t s, int mode);
 hipError_t launch2_q6(const KArgs& a, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q1(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q2(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q3(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q4(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q5(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
+hipError_t launch_batch2_q6(const BatchArgs& b, int grid, int nv, int which, hipStream_t s, int mode);
 
 // the diagnostics and norm kernels (any argument may be null)
 hipError_t launch_diag_q1(const DiagArgs* d, const RecvArgs* r, const NormArgs* n, int nv, hipStream_t s);

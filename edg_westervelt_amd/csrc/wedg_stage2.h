@@ -156,11 +156,17 @@ __device__ __forceinline__ void estore2(double* p, int E, int voff, int h, const
 }
 
 // First element of the block (one wave of 32 elements), blocks dealt to the
-// XCDs in contiguous bands as in block_first_element.
+// XCDs in contiguous bands as in block_first_element; in a batched launch, the
+// block's place in its instance's range.
+template <bool BT>
 __device__ __forceinline__ int block_first_element2() {
-    const int b = blockIdx.x, nb = gridDim.x;
-    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-    return lb * kPairWave;
+    if constexpr (BT) {
+        return ((int)blockIdx.x - ((cint*)bargs()->bfirst)[block_instance()]) * kPairWave;
+    } else {
+        const int b = blockIdx.x, nb = gridDim.x;
+        const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+        return lb * kPairWave;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -282,14 +288,14 @@ __device__ __forceinline__ void stage_boundary2(const Lane2& L, const double (&U
                                                 double st) {
     using Q2 = Dims2<D>;
     constexpr int NQ = D::NQ, NQS = Q2::NQS;
-    CKArgs& B = *kargs<false>();
+    CKArgs& B = *kargs<D::BT>();
     const int N = B.N, j = e % N, k = e / N;
     int bkey = 0;
     auto side = [&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const bool bnd = s == 0 ? j == 0 : s == 1 ? j == N - 1 : s == 2 ? k == 0 : k == N - 1;
         if (!bnd) return;
-        CKArgs& A = *kargs<false>();
+        CKArgs& A = *kargs<D::BT>();
         double du[NQS], vt[NQS], dv[NQS];
         side_trace2<D, false, true, s>(A, L, U, du);
         side_trace2<D, true, false, s>(A, L, V, vt);
@@ -334,7 +340,7 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
     constexpr int NQ = D::NQ, NV = D::NV, NU = D::NU, NQS = Q2::NQS, RU = Q2::RU, RV = Q2::RV;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int ns = s ^ 1;
-    CKArgs& A = *kargs<false>();
+    CKArgs& A = *kargs<D::BT>();
     int fidx, nb, m;
     bool bnd;
     if (s == 0)      { fidx = j + (N + 1) * k;     nb = e - 1; bnd = j == 0;     m = k; }
@@ -360,7 +366,7 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
     }
     double wA[NQS], wD[NQS], wF[NQS];
     {
-        CKArgs& I = *kargs<false>();
+        CKArgs& I = *kargs<D::BT>();
         cdbl* Tw = ctab(I.tab) + D::oW;   // w[NQ-1-p] = w[p] (admitted to 1e-13)
 #pragma unroll
         for (int p = 0; p < NQS; ++p) {
@@ -375,7 +381,7 @@ __device__ __forceinline__ void face_side2(const Lane2& L, int e, int j, int k, 
         }
     }
     if (bnd) {
-        CKArgs& B = *kargs<false>();
+        CKArgs& B = *kargs<D::BT>();
         cdbl* Tw = ctab(B.tab) + D::oW;
         const double* bs = B.bsl + (long)s * NQ * 3 * N + m;
 #pragma unroll
@@ -412,7 +418,7 @@ __device__ __forceinline__ void apply_lifts2(const Lane2& L, const double* Ys, i
     constexpr SplitLayout SL = Q2::SL;
     constexpr bool hz = s < 2, plus = (s == 0 || s == 2);
     constexpr int y0 = s * (2 * RV + RU);
-    CKArgs& A = *kargs<false>();
+    CKArgs& A = *kargs<D::BT>();
     if (hz) {
         const int l0 = lane & 31, l1 = l0 + 32;
         const double two_h = A.two_h, four_hh = A.four_hh;
@@ -534,9 +540,9 @@ __device__ __forceinline__ void vol_pass2(const Lane2& L, const double (&X)[(VI 
 #pragma unroll
             for (int q = 0; q < NQ; ++q) w[q] = Wr[NQ * q];
         }
-        vol_term2<D, VI, VO, true>(*kargs<false>(), L, p, X, w, out);
+        vol_term2<D, VI, VO, true>(*kargs<D::BT>(), L, p, X, w, out);
         __builtin_amdgcn_sched_barrier(0);
-        vol_term2<D, VI, VO, false>(*kargs<false>(), L, p, X, w, out);
+        vol_term2<D, VI, VO, false>(*kargs<D::BT>(), L, p, X, w, out);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -596,11 +602,12 @@ __device__ __forceinline__ int cheb_count2(const KArgs& A, const Lane2& L, const
 }
 
 // Writes the wave's largest count into the per-wave slot (waves of 32 elements).
+template <class D>
 __device__ __forceinline__ void wave_count2(int n_e, int slot, int E) {
     for (int off = 32; off > 0; off >>= 1) n_e = max(n_e, __shfl_xor(n_e, off));
     if ((threadIdx.x & 63) == 0) {
         const int nw = (E + kPairWave - 1) / kPairWave;
-        kargs<false>()->cnt[slot * nw + block_first_element2() / kPairWave] = n_e;
+        kargs<D::BT>()->cnt[slot * nw + block_first_element2<D::BT>() / kPairWave] = n_e;
     }
 }
 
@@ -611,7 +618,7 @@ template <class D>
 __device__ __forceinline__ Lane2 lane2_setup(int ec, int E, double* T2s) {
     constexpr SplitLayout SL = Dims2<D>::SL;
     const int lane = threadIdx.x;
-    const double* t2 = kargs<false>()->tab2;
+    const double* t2 = kargs<D::BT>()->tab2;
     for (int i = lane; i < 2 * SL.n; i += kWave) T2s[i] = t2[i];
     Lane2 L;
     L.h = lane >> 5;
@@ -622,31 +629,31 @@ __device__ __forceinline__ Lane2 lane2_setup(int ec, int E, double* T2s) {
 }
 
 // The first stage of a call: the Chebyshev counts of its input state.
-template <int NU, int NV, int MODE>
-__global__ __launch_bounds__(kWave) void k_prep2(const KArgs args) {
+template <int NU, int NV, int MODE, bool BT = false>
+__global__ __launch_bounds__(kWave) void k_prep2(const std::conditional_t<BT, BatchArgs, KArgs> args) {
     (void)args;
-    using D = Dims<NU, NV>;
+    using D = Dims<NU, NV, BT>;
     using Q2 = Dims2<D>;
     constexpr bool UNI = MODE == MODE_UNIFORM;
     __shared__ double T2s[2 * Q2::SL.n];
-    const int E = kargs<false>()->E;
-    const int e = block_first_element2() + (threadIdx.x & 31);
+    const int E = kargs<D::BT>()->E;
+    const int e = block_first_element2<D::BT>() + (threadIdx.x & 31);
     const int ec = e < E ? e : E - 1;
-    const bool global = kargs<false>()->cheb_global != 0;
+    const bool global = kargs<D::BT>()->cheb_global != 0;
     const Lane2 L = lane2_setup<D>(ec, E, T2s);
     double V[Q2::nV];
-    eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, V);
+    eload2<NV>(kargs<D::BT>()->yv, E, L.voffE, L.h, V);
     int n_e = 1;
     if (global) {
         double th, de, s1;
-        const int cb = UNI ? 0 : kargs<false>()->cls_b[ec];
-        n_e = cheb_count2<D, UNI>(*kargs<false>(), L, V, cb, th, de, s1);
+        const int cb = UNI ? 0 : kargs<D::BT>()->cls_b[ec];
+        n_e = cheb_count2<D, UNI>(*kargs<D::BT>(), L, V, cb, th, de, s1);
         if (n_e == 0) {
-            if (e < E && L.h == 0) atomicMin(&kargs<false>()->err->fkey, fail_key(kargs<false>()->serial, 1, e + 1));
+            if (e < E && L.h == 0) atomicMin(&kargs<D::BT>()->err->fkey, fail_key(kargs<D::BT>()->serial, 1, e + 1));
             n_e = 1;
         }
         if (e >= E) n_e = 1;
-        wave_count2(n_e, kargs<false>()->slot, E);
+        wave_count2<D>(n_e, kargs<D::BT>()->slot, E);
     }
 }
 
@@ -689,11 +696,11 @@ __device__ __forceinline__ bool rk_store2(int stage, double c, const double (&kv
 // One RK stage, two lanes per element (k_stage with the element's work shared
 // by the pair; MODE_UNIFORM or MODE_SEPARABLE: the spectral u-solve). Two
 // waves per SIMD up to n_u = 6, one at n_u = 7.
-template <int NU, int NV, int MODE>
+template <int NU, int NV, int MODE, bool BT = false>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ? 1 : 2))) void k_stage2(
-    const KArgs args) {
+    const std::conditional_t<BT, BatchArgs, KArgs> args) {
     (void)args;
-    using D = Dims<NU, NV>;
+    using D = Dims<NU, NV, BT>;
     using Q2 = Dims2<D>;
     static_assert(MODE == MODE_UNIFORM || MODE == MODE_SEPARABLE, "spectral u-solve modes only");
     constexpr int NQ = D::NQ, NH = D::NH, RU = Q2::RU, RV = Q2::RV;
@@ -707,11 +714,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     static_assert(4 * (2 * RV + RU) <= NLDS, "lift vectors fit the LDS slots");
     __shared__ double Gs[NLDS * kWave];
     __shared__ double T2s[2 * SL.n];
-    if (skip_stage(*kargs<false>())) return;
-    const int E = kargs<false>()->E, N = kargs<false>()->N;
-    const bool global = kargs<false>()->cheb_global != 0;
+    if (skip_stage(*kargs<D::BT>())) return;
+    const int E = kargs<D::BT>()->E, N = kargs<D::BT>()->N;
+    const bool global = kargs<D::BT>()->cheb_global != 0;
     const int lane = threadIdx.x;
-    const int e = block_first_element2() + (lane & 31);
+    const int e = block_first_element2<D::BT>() + (lane & 31);
     const bool live = e < E;
     const int ec = live ? e : E - 1;
     const int nw = (E + kPairWave - 1) / kPairWave;
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     {
         int m = 1;
         if (global) {
-            const int* wc = kargs<false>()->cnt + kargs<false>()->slot * nw;
+            const int* wc = kargs<D::BT>()->cnt + kargs<D::BT>()->slot * nw;
             for (int i0 = lane; i0 < nw; i0 += 8 * kWave) {
                 int t[8];
 #pragma unroll
@@ -730,8 +737,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
                 for (int u = 0; u < 8; ++u) m = max(m, t[u]);
             }
         }
-        eload2<NU>(kargs<false>()->yu, E, L.voffE, L.h, U);
-        eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, V);
+        eload2<NU>(kargs<D::BT>()->yu, E, L.voffE, L.h, U);
+        eload2<NV>(kargs<D::BT>()->yv, E, L.voffE, L.h, V);
         if (global) {
             for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
             nglob = m;
@@ -739,12 +746,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     }
     const int j = ec % N, k = ec / N;
     if (live && (j == 0 || j == N - 1 || k == 0 || k == N - 1)) {
-        CKArgs& A = *kargs<false>();
+        CKArgs& A = *kargs<D::BT>();
         stage_boundary2<D, UNI>(L, U, V, ec, A.serial, A.ct, A.st);
     }
-    const long cc = UNI ? 0 : (long)kargs<false>()->cls_ca[ec];
-    const double* Wc = kargs<false>()->Cw + cc * NQ * NQ;
-    const double* Wa = kargs<false>()->Aw + cc * NQ * NQ;
+    const long cc = UNI ? 0 : (long)kargs<D::BT>()->cls_ca[ec];
+    const double* Wc = kargs<D::BT>()->Cw + cc * NQ * NQ;
+    const double* Wa = kargs<D::BT>()->Aw + cc * NQ * NQ;
 
     // ---- face slots and the lane's share of the lift vectors, into LDS
     face_side2<D, UNI, 0>(L, ec, j, k, N, E, U, V, Gs, lane);
@@ -769,14 +776,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     vol_pass2<D, UNI, false, true>(L, U, Wc, Rv);
     phase_barrier();
     {
-        const double s = -kargs<false>()->four_hh;
+        const double s = -kargs<D::BT>()->four_hh;
 #pragma unroll
         for (int m = 0; m < nV; ++m) Rv[m] = s * Rv[m];
-        const int np = kargs<false>()->nproj;
+        const int np = kargs<D::BT>()->nproj;
         for (int t = 0; t < np; ++t) {
-            const double pc = kargs<false>()->pc[t];
+            const double pc = kargs<D::BT>()->pc[t];
             double pv[nV];
-            eload2<NV>(kargs<false>()->proj[t], E, L.voffE, L.h, pv);
+            eload2<NV>(kargs<D::BT>()->proj[t], E, L.voffE, L.h, pv);
 #pragma unroll
             for (int m = 0; m < nV; ++m) Rv[m] = Rv[m] + pc * pv[m];
         }
@@ -791,7 +798,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     // ---- u-equation by the spectral solve (parity blocks: no exchange)
     double ku[nU];
     {
-        cdbl* Q = ctab(kargs<false>()->tab) + D::oQ;
+        cdbl* Q = ctab(kargs<D::BT>()->tab) + D::oQ;
         const double* qs = L.t2 + SL.sQ;
         double Y1[nU];
         // Y1[i][b] = sum_{aa of i's parity} Q[aa][i] b[aa][b]
@@ -807,7 +814,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
         // the eigenvalue factors D[i][jj] of the lane's rows i = 2kk + h: the
         // split table's copy for one class, the class record otherwise (the
         // dummy row reads row n_u - 1 of the record and multiplies 0)
-        const double* Dv = vfresh(kargs<false>()->Ainv + cc * NU * NU);
+        const double* Dv = vfresh(kargs<D::BT>()->Ainv + cc * NU * NU);
 #pragma unroll
         for (int kk = 0; kk < RU; ++kk)
 #pragma unroll
@@ -838,14 +845,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
             }
     }
     {
-        CKArgs& A = *kargs<false>();
+        CKArgs& A = *kargs<D::BT>();
         if (live && A.recu) estore2<NU>(A.recu, E, L.voffE, L.h, ku);
         double yu0[nU], acu[nU];
         if (A.stage >= 1) eload2<NU>(A.y0u, E, L.voffE, L.h, yu0);
         if (A.stage >= 2) eload2<NU>(A.accu, E, L.voffE, L.h, acu);
         if (live && !rk_store2<NU>(A.stage, A.cstage, ku, yu0, acu, A.accu, A.outu, E, L.voffE, L.h)) {
-            atomicMin(&kargs<false>()->err->nonfinite, kargs<false>()->step);
-            atomicMin(&kargs<false>()->err->nf_serial, kargs<false>()->serial);
+            atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
+            atomicMin(&kargs<D::BT>()->err->nf_serial, kargs<D::BT>()->serial);
         }
     }
     phase_barrier();
@@ -855,13 +862,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     double Gm[NH];
     double th = 1.0, de = 1.0, s1 = 1.0;
     int n_e;
-    const int cb = UNI ? 0 : kargs<false>()->cls_b[ec];
+    const int cb = UNI ? 0 : kargs<D::BT>()->cls_b[ec];
     {
         double Vr[nV];
-        eload2<NV>(kargs<false>()->yv, E, L.voffE, L.h, Vr);
+        eload2<NV>(kargs<D::BT>()->yv, E, L.voffE, L.h, Vr);
         double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
         bool nan = false;
-        mass_rows2<D, UNI>(*kargs<false>(), L, Vr, cb, [&](int p, int q, double g, double Gv) {
+        mass_rows2<D, UNI>(*kargs<D::BT>(), L, Vr, cb, [&](int p, int q, double g, double Gv) {
             gmin = fmin(gmin, g);
             gmax = fmax(gmax, g);
             nan = nan || (g != g);
@@ -870,7 +877,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
             else Gt[(p - PL) * NH + q] = Gv;
         });
         const Pair mn = xboth(gmin), mx = xboth(gmax);
-        n_e = cheb_interval(fmin(mn.a, mn.b), fmax(mx.a, mx.b), xany(nan ? 1 : 0) != 0, kargs<false>()->cheb_L, th,
+        n_e = cheb_interval(fmin(mn.a, mn.b), fmax(mx.a, mx.b), xany(nan ? 1 : 0) != 0, kargs<D::BT>()->cheb_L, th,
                             de, s1);
     }
     phase_barrier();
@@ -880,7 +887,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     int ncheb = 0;
     if (n_e == 0) {
         if (!global && live && L.h == 0)
-            atomicMin(&kargs<false>()->err->fkey, fail_key(kargs<false>()->serial, 1, e + 1));
+            atomicMin(&kargs<D::BT>()->err->fkey, fail_key(kargs<D::BT>()->serial, 1, e + 1));
     } else {
         const int n = global ? nglob : n_e;
         ncheb = n;
@@ -888,7 +895,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
 #pragma unroll
         for (int m = 0; m < nV; ++m) { r[m] = Rv[m]; d[m] = r[m] / th; }
         double rho = 1.0 / s1;
-        cdbl* Ph = ctab(kargs<false>()->tab) + D::oPh;
+        cdbl* Ph = ctab(kargs<D::BT>()->tab) + D::oPh;
         for (int step = 1; step <= n; ++step) {
 #pragma unroll
             for (int m = 0; m < nV; ++m) X[m] += d[m];
@@ -933,7 +940,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
     }
     int n_next = 1;
     {
-        CKArgs& A = *kargs<false>();
+        CKArgs& A = *kargs<D::BT>();
         double yv0[nV], acv[nV];
         const int stage = A.stage;
         const double c = A.cstage;
@@ -941,8 +948,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
         if (stage >= 1) eload2<NV>(A.y0v, E, L.voffE, L.h, yv0);
         if (stage >= 2) eload2<NV>(A.accv, E, L.voffE, L.h, acv);
         if (live && !rk_store2<NV>(stage, c, X, yv0, acv, A.accv, A.outv, E, L.voffE, L.h)) {
-            atomicMin(&kargs<false>()->err->nonfinite, kargs<false>()->step);
-            atomicMin(&kargs<false>()->err->nf_serial, kargs<false>()->serial);
+            atomicMin(&kargs<D::BT>()->err->nonfinite, kargs<D::BT>()->step);
+            atomicMin(&kargs<D::BT>()->err->nf_serial, kargs<D::BT>()->serial);
         }
         if (A.has_next) {
             double Vn[nV];
@@ -950,21 +957,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NU >= 7 ?
             for (int m = 0; m < nV; ++m) Vn[m] = stage < 4 ? yv0[m] + c * X[m] : yv0[m] + c * (acv[m] + X[m]);
             if (global) {
                 double t1, t2, t3;
-                n_next = cheb_count2<D, UNI>(*kargs<false>(), L, Vn, cb, t1, t2, t3);
+                n_next = cheb_count2<D, UNI>(*kargs<D::BT>(), L, Vn, cb, t1, t2, t3);
                 if (n_next == 0) {
                     if (live && L.h == 0)
-                        atomicMin(&kargs<false>()->err->fkey, fail_key(kargs<false>()->serial + 1, 1, e + 1));
+                        atomicMin(&kargs<D::BT>()->err->fkey, fail_key(kargs<D::BT>()->serial + 1, 1, e + 1));
                     n_next = 1;
                 }
                 if (!live) n_next = 1;
             }
         }
     }
-    if (global && kargs<false>()->has_next) wave_count2(n_next, kargs<false>()->slot ^ 1, E);
+    if (global && kargs<D::BT>()->has_next) wave_count2<D>(n_next, kargs<D::BT>()->slot ^ 1, E);
     for (int off = 32; off > 0; off >>= 1) ncheb = max(ncheb, __shfl_xor(ncheb, off));
     if (lane == 0 && ncheb > 0) {
-        unsigned* ws = kargs<false>()->wstat;
-        const int w = block_first_element2() / kPairWave;
+        unsigned* ws = kargs<D::BT>()->wstat;
+        const int w = block_first_element2<D::BT>() / kPairWave;
         ws[w] += (unsigned)ncheb;
         ws[nw + w] = max(ws[nw + w], (unsigned)ncheb);
     }

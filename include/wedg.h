@@ -178,9 +178,10 @@ wedg_status wedg_graph_enable(wedg_workspace* ws, long chunk_steps);
  * n_u - 1) with the spectral u-solve (coefficients constant on each element)
  * and Chebyshev iteration; other problems keep one lane whatever is set
  * (wedg_problem_info says which). Results agree with the
- * one-lane kernels to rounding. A batch (wedg_batch_create) sets its
- * workspaces back to 1. wedg_workspace_lanes_per_element reports the lanes
- * the workspace's calls use. */
+ * one-lane kernels to rounding. A batch runs each workspace with the lanes
+ * it had at wedg_batch_create; changing them while the batch exists makes
+ * its advance fail with WEDG_ERR_ARGUMENT. wedg_workspace_lanes_per_element
+ * reports the lanes the workspace's calls use. */
 wedg_status wedg_workspace_set_lanes_per_element(wedg_workspace* ws, int lanes);
 wedg_status wedg_workspace_lanes_per_element(const wedg_workspace* ws, int* lanes);
 /* The same without waiting: the caller calls wedg_sync before reading results. */
@@ -268,8 +269,8 @@ wedg_status wedg_probe_copy(size_t n, int repeats, double* ms);
  * of solve, src/solver.jl:213-216); a batch advances n independent workspaces
  * (uneven N, q_u, q_v, coefficients, boundary kinds, time steps and step
  * counts) together: one kernel launch per RK stage for each group of
- * instances with the same (q_u, q_v) and coefficient layout, on the batch's
- * own stream. Instance i runs nsteps[i] steps of dt[i] from t0[i] exactly as
+ * instances with the same (q_u, q_v), coefficient layout and lanes per
+ * element (wedg_workspace_set_lanes_per_element), on the batch's streams. Instance i runs nsteps[i] steps of dt[i] from t0[i] exactly as
  * wedg_advance(ws[i], t0[i], dt[i], nsteps[i], ...) would (same stage
  * arguments, bitwise the same state); status[i] and t_end[i] (either may be
  * null) receive its status and final time, wedg_last_error(ws[i]) its failure

@@ -144,6 +144,108 @@ def test_batch_full_workload_two_steps():
         assert np.array_equal(u, ref[i][0]) and np.array_equal(v, ref[i][1]), describe(b)
 
 
+def test_batch_two_lanes_bitwise_equals_separate_runs():
+    """Instances with two lanes per element next to one-lane instances of the
+    same (q_u, q_v): two groups per pair, every instance bitwise as its own run
+    (which uses its lanes too), within the RK4 tolerance of the oracle, over
+    two streams, chunks of three steps and a second advance."""
+    insts = B.uneven_instances(seed=0, count=16)
+    built = [B.build_instance(x) for x in insts]
+    lanes = [1 if i % 3 == 1 else 2 for i in range(len(built))]
+    steps = [3, 7, 0, 5, 1, 9, 4, 2, 6, 8, 3, 5, 4, 6, 2, 7]
+    ref = []
+    for b, n, l in zip(built, steps, lanes):
+        b.ws.dev.set_lanes(l)
+        assert b.ws.dev.lanes == l, describe(b)   # the uneven workload has two-lane kernels throughout
+        say("solo", describe(b), "lanes", l, "steps", n)
+        ref.append(solo(b, n))
+    for b in built:
+        B.set_zero_state(b.ws)
+    bt = B.Batch([b.ws for b in built])
+    assert bt.lanes_per_element() == lanes
+    bt.configure(2, 3, 0)
+    say("batch of", len(built), "with lanes", lanes)
+    tend, status = bt.advance(0.0, [b.dt for b in built], steps)
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        assert status[i] == _lib.OK and tend[i] == ref[i][2]
+        assert np.array_equal(u, ref[i][0]) and np.array_equal(v, ref[i][1]), describe(b)
+    for i in (3, 12, 14):   # two-lane instances of q_u 6, 2 and 4 (seed 0)
+        b = built[i]
+        say("oracle", describe(b))
+        u, v = B.get_state(b.ws)
+        z_u = np.zeros_like(u, order="F")
+        z_v = np.zeros_like(v, order="F")
+        uo, vo, _, _ = ob.rk4_steps(ob.OraclePre(b.pre, b.inst.N), z_u, z_v, 0.0, b.dt, steps[i])
+        assert np.abs(vo).max() > 0
+        assert rel(uo, u) < 1e-11 and rel(vo, v) < 1e-11, describe(b)
+    tend2, _ = bt.advance(tend, [b.dt for b in built], [2] * len(built))
+    for i, b in enumerate(built):
+        u, v = B.get_state(b.ws)
+        B.set_zero_state(b.ws)
+        uu, vv, t, _, _ = solo(b, steps[i] + 2)
+        assert np.array_equal(u, uu) and np.array_equal(v, vv), describe(b)
+        assert tend2[i] == t
+    # the lanes are fixed while the batch exists; set_lanes_per_element forms it again
+    built[1].ws.dev.set_lanes(2)
+    with pytest.raises(_lib.WedgError, match=f"wedg_batch_advance failed with status {_lib.ERR_ARGUMENT}"):
+        bt.advance(0.0, [b.dt for b in built], [1] * len(built), raise_errors=False)
+    bt.set_lanes_per_element(2)
+    assert bt.lanes_per_element() == [2] * len(built) and bt.config()[:2] == (2, 3)
+    for b in built:
+        B.set_zero_state(b.ws)
+    bt.advance(0.0, [b.dt for b in built], [1] * len(built))
+
+
+def test_batch_two_lanes_failure_and_tuner():
+    """A failing two-lane instance reports its own run's error; the lanes
+    tuner chooses per kernel group, restores every state, and the tuned batch
+    is bitwise the instances' own runs with the chosen lanes."""
+    insts = B.uneven_instances(seed=0, count=8)
+    built = [B.build_instance(x) for x in insts]
+    for b in built:
+        b.ws.dev.set_lanes(2)
+    bad = 3
+    v0 = np.zeros((built[bad].inst.N, built[bad].inst.N, built[bad].ws.qv1, built[bad].ws.qv1), order="F")
+    v0[:, :, 0, 0] = 2.0   # v_h = 1
+    ref = [solo(b, 3, v0 if i == bad else None) for i, b in enumerate(built)]
+    assert ref[bad][3] != _lib.OK
+    for b in built:
+        B.set_zero_state(b.ws)
+    u = np.zeros((built[bad].ws.N, built[bad].ws.N, built[bad].ws.qu1, built[bad].ws.qu1), order="F")
+    built[bad].ws.dev.check(lib.wedg_set_state(built[bad].ws.dev.h, u.ctypes.data_as(C.POINTER(C.c_double)),
+                                               v0.ctypes.data_as(C.POINTER(C.c_double))))
+    bt = B.Batch([b.ws for b in built])
+    _, status = bt.advance(0.0, [b.dt for b in built], [3] * len(built), raise_errors=False)
+    for i, b in enumerate(built):
+        info = b.ws.dev.last_error()
+        assert status[i] == ref[i][3]
+        assert (info.status, info.side, info.key, info.element) == ref[i][4]
+        if i != bad:
+            uu, vv = B.get_state(b.ws)
+            assert np.array_equal(uu, ref[i][0]) and np.array_equal(vv, ref[i][1]), describe(b)
+    for b in built:
+        B.set_zero_state(b.ws)
+    times, choice = bt.tune(0.0, [b.dt for b in built], probe_steps=4, lanes=(1, 2), chunks=(0,), repeats=1,
+                            lanes_per_element=(1, 2))
+    say("lane tuner", bt.lane_times, bt.lane_choice, "streams", times, choice)
+    groups = bt.kernel_groups()
+    assert set(bt.lane_choice) == set(groups)
+    assert set(bt.lane_times) == {k + (l,) for k in groups for l in (1, 2)}
+    lpe = bt.lanes_per_element()
+    for key, members in groups.items():
+        assert all(lpe[i] == bt.lane_choice[key] for i in members)
+    for b in built:
+        u, v = B.get_state(b.ws)
+        assert not u.any() and not v.any()
+    steps = [5, 3, 6, 2, 4, 5, 1, 3]
+    tend, status = bt.advance(0.0, [b.dt for b in built], steps)
+    got = [B.get_state(b.ws) for b in built]
+    for i, b in enumerate(built):
+        uu, vv, t, _, _ = solo(b, steps[i])
+        assert np.array_equal(got[i][0], uu) and np.array_equal(got[i][1], vv) and tend[i] == t, describe(b)
+
+
 def test_batch_rejects_a_workspace_twice():
     b = B.build_instance(B.uneven_instances(seed=0, count=1)[0])
     arr = (C.c_void_p * 2)(b.ws.dev.h.value, b.ws.dev.h.value)
