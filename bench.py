@@ -141,6 +141,7 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global", lanes_per_el
                 dof_stages=work, instances=count, groups=groups, bitwise_equal_to_sequential=bool(same),
                 workload=f"edg_westervelt_amd.batch.uneven_instances(seed={seed}, count={count}), T = 0.25, "
                          "initial_value Gaussian, piecewise-constant c, alpha, beta on 4x4 blocks",
+                amplitude_rule=sorted({b.inst.amp_rule for b in built}),
                 cheb_policy=cheb, build_s=round(t_build, 2),
                 tuner=dict(choice=dict(lanes=choice[0], chunk_steps=choice[1]),
                            probe_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune_ms.items()},

@@ -25,11 +25,37 @@ def test_uneven_instances_are_reproducible_and_in_range():
         assert 0.3 <= x.x0 <= 0.7 and 0.3 <= x.y0 <= 0.7 and 0.03 <= x.s <= 0.08
         assert -1 <= x.sigma <= 1 and x.sigma_dirichlet == abs(x.sigma) and 0 <= x.kappa <= 1
         assert set(x.boundary.values()) <= set(B.KINDS)
-        # 1 - 2 beta |v| >= 0.5 for |v| up to 4 c_max amp / (s sqrt(e))
-        vmax = 4 * x.c.max() * x.amp / (x.s * math.sqrt(math.e))
-        assert 1 - 2 * x.beta.max() * vmax >= 0.5 - 1e-15 and x.amp <= 0.05
     # every (q_u, pairing) group of the workload occurs
     assert len({(x.q, x.mixed) for x in a}) == 10
+
+
+def test_uneven_amplitudes():
+    """The "bound" rule: 1 - 2 beta |v| >= 0.5 for |v| up to 4 c_max amp /
+    (s sqrt(e)). The calibrated amplitudes of seed 0 (the default): the stored
+    d = 2 beta_max max|v_h| of the calibration's confirming run lies in
+    [0.48, 0.5], above the bound rule's amplitudes (whose margin of 4 the
+    calibration spends); the rest of each instance is unchanged."""
+    bound = B.uneven_instances(seed=0, count=64, amplitude="bound")
+    for x in bound:
+        assert x.amp_rule == "bound"
+        vmax = 4 * x.c.max() * x.amp / (x.s * math.sqrt(math.e))
+        assert 1 - 2 * x.beta.max() * vmax >= 0.5 - 1e-15 and x.amp <= 0.05
+    cal = B.uneven_instances(seed=0, count=64)
+    with open(B.AMPLITUDE_FILE) as f:
+        import json
+        tab = json.load(f)["runs"][0]
+    assert (tab["seed"], tab["T"], len(tab["amp"])) == (0, 0.25, 64)
+    for x, y, d in zip(cal, bound, tab["d"]):
+        assert x.amp_rule == "calibrated" and 0.48 <= d <= 0.5
+        assert y.amp < x.amp <= 0.05
+        assert (x.N, x.q, x.s, x.x0) == (y.N, y.q, y.s, y.x0) and np.array_equal(x.beta, y.beta)
+    assert B.uneven_instances(seed=1, count=2)[0].amp_rule == "bound"   # no calibration stored
+    with pytest.raises(ValueError):
+        B.uneven_instances(seed=1, count=2, amplitude="calibrated")
+    with pytest.raises(ValueError):
+        B.uneven_instances(seed=0, count=65, amplitude="calibrated")
+    with pytest.raises(ValueError):
+        B.uneven_instances(seed=0, count=2, amplitude="margin")
 
 
 def test_piecewise_constant_blocks():

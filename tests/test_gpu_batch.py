@@ -246,6 +246,25 @@ def test_batch_two_lanes_failure_and_tuner():
         assert np.array_equal(got[i][0], uu) and np.array_equal(got[i][1], vv) and tend[i] == t, describe(b)
 
 
+def test_calibrated_amplitudes_keep_the_weight_bound():
+    """The 64 instances with their calibrated amplitudes, whole runs to T:
+    2 beta_max max|v_h| (after every 16 steps) stays in [0.48, 0.5], so
+    1 - 2 beta max|v| >= 0.5 as the workload states, and matches the stored
+    calibration to 1e-6 (the runs are deterministic)."""
+    import json
+    insts = B.uneven_instances(seed=0, count=64)
+    assert all(x.amp_rule == "calibrated" for x in insts)
+    built = [B.build_instance(x) for x in insts]
+    vm, status = B.run_vmax(built, 16)
+    assert (status == _lib.OK).all()
+    d = 2.0 * np.array([x.beta.max() for x in insts]) * vm
+    say("d range", d.min(), d.max())
+    with open(B.AMPLITUDE_FILE) as f:
+        stored = np.array(json.load(f)["runs"][0]["d"])
+    assert ((0.48 <= d) & (d <= 0.5)).all()
+    assert np.abs(d - stored).max() < 1e-4
+
+
 def test_batch_rejects_a_workspace_twice():
     b = B.build_instance(B.uneven_instances(seed=0, count=1)[0])
     arr = (C.c_void_p * 2)(b.ws.dev.h.value, b.ws.dev.h.value)
