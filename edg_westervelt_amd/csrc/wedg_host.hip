@@ -162,7 +162,7 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
     hipError_t he = hipSuccess;
 #define UP(dst, src, n)                                  \
     do {                                                 \
-        dst = p->upload(src, n, &he);                    \
+        (dst) = p->upload(src, n, &he);                  \
         if (he != hipSuccess) { wedg_problem_destroy(p); return WEDG_ERR_HIP; } \
     } while (0)
 

@@ -11,7 +11,10 @@ template <int NU, int NV>
 static hipError_t launch_nv(const KArgs& a, int which, hipStream_t s, int mode) {
     const int grid = (a.E + kWave - 1) / kWave;
     if (which == 0) {
+        // the problem's own record layout: class records (uniform, separable)
+        // or per-element element-fastest ones (general, conjugate gradients)
         if (mode == MODE_UNIFORM) k_prep<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
+        else if (mode == MODE_SEPARABLE) k_prep<NU, NV, MODE_SEPARABLE><<<grid, kWave, 0, s>>>(a);
         else k_prep<NU, NV, MODE_GENERAL><<<grid, kWave, 0, s>>>(a);
     } else if (which == 1) {
         if (mode == MODE_UNIFORM) k_stage<NU, NV, MODE_UNIFORM><<<grid, kWave, 0, s>>>(a);
@@ -30,6 +33,7 @@ template <int NU, int NV>
 static hipError_t launch_batch_nv(const BatchArgs& b, int grid, int which, hipStream_t s, int mode) {
     if (which == 0) {
         if (mode == MODE_UNIFORM) k_prep<NU, NV, MODE_UNIFORM, true><<<grid, kWave, 0, s>>>(b);
+        else if (mode == MODE_SEPARABLE) k_prep<NU, NV, MODE_SEPARABLE, true><<<grid, kWave, 0, s>>>(b);
         else k_prep<NU, NV, MODE_GENERAL, true><<<grid, kWave, 0, s>>>(b);
     } else if (which == 1) {
         if (mode == MODE_UNIFORM) k_stage<NU, NV, MODE_UNIFORM, true><<<grid, kWave, 0, s>>>(b);
