@@ -59,11 +59,16 @@ def gaussian_profile(s=0.05, amp=0.05):
                 u0_yy=lambda x, y: ((y - 0.5) ** 2 / s ** 4 - 1 / s ** 2) * u0(x, y))
 
 
-def build_problem(q, N):
+COEFFICIENTS = {"constant": "c=1 alpha=1e-4 beta=2",
+                "paper": "paper_coefficients (smooth c, alpha, beta: one u-stiffness inverse per element)"}
+
+
+def build_problem(q, N, coefficients="constant"):
     import edg_westervelt_amd as w
+    coef = (w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0) if coefficients == "constant"
+            else w.paper_coefficients())
     pre, geo = w.precompute(q, N, mixed=False, flux="upwind", boundary="absorbing_2beta",
-                            mode="initial_value", initial_profile=gaussian_profile(),
-                            coefficients=w.constant_coefficients(c=1.0, alpha=1e-4, beta=2.0))
+                            mode="initial_value", initial_profile=gaussian_profile(), coefficients=coef)
     return pre, geo
 
 
@@ -359,6 +364,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=192, help="RK4 steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch (rocprofv3 --pmc)")
     ap.add_argument("--q", type=int, default=5)
+    ap.add_argument("--coefficients", choices=tuple(COEFFICIENTS), default="constant",
+                    help="constant (the headline) or the smooth paper_coefficients (per-element records)")
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--batched", choices=("on", "off", "only"), default="on",
                     help="the uneven-batch speedup: with the headline (on), not at all (off), or alone (only)")
@@ -426,7 +433,7 @@ def main():
 
     q, N, T = args.q, args.N, 0.5
     t_pre = time.perf_counter()
-    pre, geo = build_problem(q, N)
+    pre, geo = build_problem(q, N, args.coefficients)
     t_pre = time.perf_counter() - t_pre
     dt, nsteps_full = time_step(q, N, T)
     steps = args.steps
@@ -504,10 +511,17 @@ def main():
                 frac=achieved / FP64_PEAK_TFS, traffic=None, kernel=kname,
                 flops_per_launch=E * f_total, us_avg=fam["stage"]["us_avg"])
     bytes_stage = 8 * 17 * dof / 4
+    note = "compulsory bytes of the three-register RK4, 34 B per DOF"
+    pmode = ws.problem.info()["mode"]
+    if pmode in (0, 3):   # per-element records: Cw, Aw, the packed u-stiffness inverse, beta at the points
+        npk = (nu * nu * (nu * nu + 1)) // 2
+        bytes_stage += 8 * E * (2 * nu * nu + npk + nh * nh)
+        note += f" + the per-element records, {8 * (2 * nu * nu + npk + nh * nh)} B per element"
     roof_hbm = dict(bound="hbm", achieved=bytes_stage / st_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=bytes_stage / st_s / 1e9 / HBM_PEAK_GBS, kernel=kname,
-                    bytes_per_launch=bytes_stage, note="compulsory bytes of the three-register RK4, 34 B per DOF")
-    traffic_file = args.traffic or os.path.join(ROOT, "profiles", f"traffic_q{q}_n{N}.json")
+                    bytes_per_launch=bytes_stage, note=note)
+    sfx = "" if args.coefficients == "constant" else "_" + args.coefficients
+    traffic_file = args.traffic or os.path.join(ROOT, "profiles", f"traffic_q{q}_n{N}{sfx}.json")
     if os.path.exists(traffic_file):
         tj = json.load(open(traffic_file))
         roof["traffic"] = tj.get(kname)
@@ -545,14 +559,16 @@ def main():
     out = dict(metric=METRIC, value=value, unit="DOF·stages/s", n_gpus=1, steps=steps, warmup=args.warmup,
                ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
                data="synthetic",
-               config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, c=1 alpha=1e-4 beta=2, "
+               config=dict(workload=f"q_u=q_v={q}, {N}x{N} mesh, initial_value Gaussian, "
+                                    f"{COEFFICIENTS[args.coefficients]}, "
                                     f"upwind, absorbing_2beta, Chebyshev 1e-14, RK4 dt=h/(10 q^2)",
+                           kernel_mode=["general", "separable", "uniform", "general_cg"][pmode],
                            elements=E, dof=dof, dt=dt, cheb_policy=args.cheb, lanes_per_element=lanes,
                            mean_cheb_steps=nc_timing,
                            max_cheb_steps=info.cheb_steps, full_run_steps=nsteps_full,
                            timed_window=[t0, t0 + steps * dt],
-                           note="the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "
-                                "timed window is the part it completes",
+                           note=("the run stops at the positivity limit 1 - 2 beta v_h <= 0 near t = 0.131; the "
+                                 "timed window is the part it completes") if args.coefficients == "constant" else None,
                            precompute_s=round(t_pre, 2)),
                roofline=roof, roofline_hbm=roof_hbm, whole_step=whole, kernels_us=fam, cpu_baseline=cpu,
                batched=batched, adjoint=adjoint, batched_adjoint=batched_adjoint, graph=graph,
