@@ -932,7 +932,7 @@ extern "C" wedg_status wedg_advance_async(wedg_workspace* ws, double t0, double 
                 }
                 t += dt;
             }
-            if (first) HIPOK(launch(prep, ws->p, 0, ws->stream));
+            if (first) HIPOK(launch(prep, ws->p, 0, ws->stream, ws->two()));
             if ((s = graph_run(ws, args))) return s;
         }
     }

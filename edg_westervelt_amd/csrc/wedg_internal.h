@@ -212,7 +212,7 @@ struct wedg_workspace {
 
 namespace wedg {
 
-hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two = false);
+hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two);   // two: the two-lane kernels
 wedg_status reset_err(wedg_workspace* ws);
 wedg_status collect(wedg_workspace* ws, int nonfinite_is_error);
 KArgs problem_args(const wedg_problem* p, int cheb_global, double* bsl, DevErr* err, int* cnt, unsigned* wstat);

@@ -128,8 +128,8 @@ extern "C" wedg_status wedg_rhs_noalloc(const wedg_problem* p, int cheb_policy, 
     a.outu = ut; a.outv = vt;
     a.has_next = 0;
     wedg_problem* pm = const_cast<wedg_problem*>(p);
-    HIPOK(launch(a, pm, 0, st));
-    HIPOK(launch(a, pm, 1, st));
+    HIPOK(launch(a, pm, 0, st, false));   // one lane per element: the scratch holds 64-element waves
+    HIPOK(launch(a, pm, 1, st, false));
     if (info) {
         k_noalloc_report<<<1, 256, 0, st>>>(err, wstat, nw, p->nu, p->N, info);
         HIPOK(hipGetLastError());

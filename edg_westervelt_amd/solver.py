@@ -45,10 +45,13 @@ class Workspace:
     of the device stage kernels: 1, or 2 (a pair of lanes per element: twice
     the waves, half the registers per lane) where the problem allows it
     (q_u <= 6, coefficients constant on each element, Chebyshev mass solve);
-    results agree to rounding.
+    results agree to rounding. ``"auto"`` takes two lanes where they were
+    measured faster: two-lane problems with q_u >= 3 whose one-lane grid
+    leaves SIMDs idle (fewer than 65,536 elements) or with q_u = 6, where
+    one lane spills registers.
     """
 
-    def __init__(self, pre, N: int, nblocks=None, cheb: str = "global", lanes: int = 1):
+    def __init__(self, pre, N: int, nblocks=None, cheb: str = "global", lanes=1):
         if nblocks is not None and nblocks < 1:
             raise ValueError(f"nblocks must be ≥ 1 (got {nblocks})")
         if pre.Ainv_u.shape[1] != N * N:
@@ -58,7 +61,12 @@ class Workspace:
         self.N = N
         self.nb = 1
         self.qu1, self.qv1, self.q_high = pre.qu1, pre.qv1, pre.q_high
+        if lanes not in (1, 2, "auto"):
+            raise ValueError(f"lanes must be 1, 2 or 'auto' (got {lanes!r})")
         self.problem = _lib.Problem(pre, N)
+        if lanes == "auto":
+            q_u = pre.qu1 - 1
+            lanes = 2 if (self.problem.info()["two_lane"] and q_u >= 3 and (N * N < 65536 or q_u == 6)) else 1
         self.dev = _lib.DeviceWorkspace(self.problem, cheb, lanes)
         self.ut = np.zeros((N, N, pre.qu1, pre.qu1), order="F")
         self.vt = np.zeros((N, N, pre.qv1, pre.qv1), order="F")

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -58,3 +60,13 @@ def test_entry_points_reject_null_handles():
         assert st == expect, (name, st)
         checked += 1
     assert checked >= 30
+
+
+def test_workspace_rejects_unknown_lane_counts():
+    """The lane count is checked before anything reaches the device."""
+    import edg_westervelt_amd as w
+    from edg_westervelt_amd import precompute
+    pre, _ = precompute(2, 2, mixed=False, mode="initial_value")
+    for bad in (0, 3, "two"):
+        with pytest.raises(ValueError, match="lanes"):
+            w.Workspace(pre, 2, lanes=bad)

@@ -226,3 +226,73 @@ def test_two_lane_errors_in_serial_order():
             msgs.append(str(ex.value))
         say(f"case two-lane failure {boundary}: {msgs[1]}")
         assert msgs[0] == msgs[1] and expect in msgs[1]
+
+
+def test_two_lane_graph_chunks_equal_plain_launches():
+    """RK4 chunks replayed as HIP graphs (wedg_graph_enable) with the two-lane
+    kernels: bit for bit the plain two-lane launches, over three replays with
+    plain work of the same workspace in between."""
+    import ctypes as C
+    from edg_westervelt_amd import _lib
+    from edg_westervelt_amd._lib import lib
+    from edg_westervelt_amd.solver import _ptr
+    co = constant_coefficients(c=1.0, alpha=1e-4, beta=1e-3)
+    for q, N, mixed, mode, chunk, n in ((5, 12, False, "source", 7, 30), (4, 9, True, "initial_value", 4, 17),
+                                        (6, 8, False, "source", 16, 16)):
+        say(f"case two-lane graph q={q} N={N} mixed={mixed} {mode} chunk={chunk}")
+        pre, _ = precompute(q, N, mixed=mixed, mode=mode, boundary=MIXED, sigma=0.3, sigma_dirichlet=0.6,
+                            kappa=0.7, coefficients=co)
+        u, v = rand_state(np.random.default_rng(70 + q), N, pre, su=1e-2, sv=2e-3)
+        ws = w.Workspace(pre, N, lanes=2)
+        assert ws.dev.lanes == 2
+
+        def run(k):
+            assert lib.wedg_graph_enable(ws.dev.h, k) == _lib.OK
+            uu, vv = np.array(u, order="F", copy=True), np.array(v, order="F", copy=True)
+            ws.dev.check(lib.wedg_set_state(ws.dev.h, _ptr(uu), _ptr(vv)))
+            t = C.c_double(0.0)
+            ws.dev.check(lib.wedg_advance(ws.dev.h, 0.01, 1e-4, n, 0, C.byref(t)))
+            ws.dev.check(lib.wedg_get_state(ws.dev.h, _ptr(uu), _ptr(vv)))
+            return uu, vv, t.value
+
+        ref = run(0)
+        for rep in range(3):
+            got = run(chunk)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]) and got[2] == ref[2], rep
+            w.edg_rhs_(ws, u, v, 0.3, N, 1.0 / N, pre)
+        assert ws.dev.lanes == 2
+
+
+def test_two_lane_source_mode_solve():
+    """solve(mode = :source) with two lanes per element against one lane:
+    receivers after every step, the record-step diagnostics, the slice and
+    the snapshots agree to 1e-11 relative (the kernels sum in different
+    orders); q_u = 4 and 6, the steepening coefficients (beta = 8)."""
+    for q, N, mixed in ((4, 16, True), (6, 8, False)):
+        say(f"case two-lane source solve q={q} N={N}")
+        kw = dict(mixed=mixed, mode="source", boundary={"bot": "absorbing_2beta", "top": "wall",
+                                                        "lft": "absorbing", "rgt": "absorbing_2beta"},
+                  kappa=0.5, coefficients=constant_coefficients(c=1.0, alpha=1e-6, beta=8.0),
+                  source=w.GaussianSource(x0=(0.3, 0.55), r0=0.1, amplitude=0.5),
+                  record_every=5, slice_y=0.3, receivers=[(0.45, 0.5), (0.3, 0.55)], snap_times=(0.01,))
+        A = w.solve(q, N, 0.03, lanes=1, **kw)
+        B = w.solve(q, N, 0.03, lanes=2, **kw)
+        assert list(A.t) == list(B.t) and np.abs(A.recv).max() > 0
+        assert rel(A.recv, B.recv) < 1e-11 and rel(A.slices, B.slices) < 1e-11
+        for k in ("energy", "vmax", "ndmin"):
+            assert rel(getattr(A, k), getattr(B, k)) < 1e-11, k
+        assert len(A.snaps) == len(B.snaps) == 1 and rel(A.snaps[0][1], B.snaps[0][1]) < 1e-11
+
+
+def test_auto_lanes():
+    """Workspace(lanes="auto"): two lanes for two-lane problems with q_u >= 3
+    on meshes that leave SIMDs idle with one lane, and always at q_u = 6;
+    one lane otherwise (q_u <= 2, the full-size q_u = 5 mesh, the general
+    layout)."""
+    co = constant_coefficients(c=1.0, alpha=1e-4, beta=0.5)
+    for q, N, coef, expect in ((5, 16, co, 2), (3, 24, co, 2), (2, 16, co, 1), (6, 8, co, 2), (5, 256, co, 1),
+                               (5, 8, None, 1)):
+        pre, _ = precompute(q, N, mixed=False, mode="initial_value", coefficients=coef)
+        ws = w.Workspace(pre, N, lanes="auto")
+        say(f"case auto lanes q={q} N={N} general={coef is None}: {ws.dev.lanes}")
+        assert ws.dev.lanes == expect
