@@ -156,15 +156,16 @@ def bench_batched(count=64, seed=0, sequential=True, cheb="global", lanes_per_el
                            two_lane_instances=sum(1 for x in lpe if x == 2)))
 
 
-def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
+def bench_adjoint(steps=2000, snapshots=32, N=64, q=4, lanes=1):
     """Wall time of the receiver-misfit gradient (forward sweep, recomputations,
     taped steps and the adjoint of every stage, binomial checkpointing in
     `snapshots` slots) against one forward solve of the same run with the misfit,
     on edg_westervelt_amd.adjoint.beta_field_workload (q = 4, 64 x 64, 2000
-    steps, per-element beta, source at (0.5, 0.7))."""
+    steps, per-element beta, source at (0.5, 0.7)); `lanes`: the forward
+    stage kernels' lanes per element (both legs)."""
     from edg_westervelt_amd.adjoint import beta_field_workload
     t = time.perf_counter()
-    wl = beta_field_workload(N=N, q=q, steps=steps)
+    wl = beta_field_workload(N=N, q=q, steps=steps, lanes=lanes)
     t_build = time.perf_counter() - t
     adj = wl.adj
     adj.misfit(wl.dt, 20)                          # warm-up of both paths
@@ -182,7 +183,7 @@ def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
     return dict(adjoint_forward_ratio=tg / tf, forward_s=tf, gradient_s=tg, J=J, J_gradient_run=Jg,
                 forward_dof_stages_per_s=4 * steps * dof / tf, snapshots=snapshots,
                 forward_steps_recomputed=st.forward_steps, taped_steps=st.taped_steps,
-                snapshots_used=st.snapshots_used, build_s=round(t_build, 2),
+                snapshots_used=st.snapshots_used, build_s=round(t_build, 2), forward_lanes_per_element=adj.lanes,
                 host_enqueue_ms=st.enqueue_ms, device_ms=st.total_ms,
                 workload=f"edg_westervelt_amd.adjoint.beta_field_workload(N={N}, q={q}, steps={steps}): source "
                          "mode, Gaussian source width 0.05 at (0.5, 0.7), c = 1, alpha = 1e-5, per-element beta "
@@ -190,17 +191,18 @@ def bench_adjoint(steps=2000, snapshots=32, N=64, q=4):
                          "kappa = 0.5; J = bottom-side L2 misfit after every step against the beta = 4 run")
 
 
-def bench_batched_adjoint(T=0.02, count=16, seed=2, snapshots=32, sequential=True):
+def bench_batched_adjoint(T=0.02, count=16, seed=2, snapshots=32, sequential=True, lanes=1):
     """The batched adjoint: `count` source-mode instances of
     edg_westervelt_amd.adjoint.steepening_batch_workload (N in {64, 96, 128},
     q in {5, 6}, beta about 6-8), forward and checkpointed gradient of every
     instance together (one launch per RK stage and per adjoint kernel for each
     degree group), against the instances' own gradients one after another.
     The workload is defined to T = 1.0 (16,000-46,000 steps per instance); the
-    timed runs stop at T (the same steps for both legs)."""
+    timed runs stop at T (the same steps for both legs). `lanes`: the lanes
+    per element of the forward stage kernels of every instance (both legs)."""
     from edg_westervelt_amd.adjoint import AdjointBatch, steepening_batch_workload
     t = time.perf_counter()
-    wl = steepening_batch_workload(count=count, seed=seed, T=T)
+    wl = steepening_batch_workload(count=count, seed=seed, T=T, lanes=lanes)
     t_build = time.perf_counter() - t
     dts = [x.dt for x in wl]
     ns = [x.steps for x in wl]
@@ -219,7 +221,7 @@ def bench_batched_adjoint(T=0.02, count=16, seed=2, snapshots=32, sequential=Tru
     out = dict(batched_gradient_s=tg, batched_forward_s=tf, adjoint_forward_ratio=tg / tf, device_s=dev_ms / 1e3,
                host_enqueue_s=enq_ms / 1e3, instances=count, snapshots=snapshots, T=T,
                steps=ns, N=[x.N for x in wl], q=[x.q for x in wl], build_s=round(t_build, 2),
-               J_equal_forward=bool(same),
+               J_equal_forward=bool(same), forward_lanes_per_element=[x.adj.lanes for x in wl],
                workload=f"edg_westervelt_amd.adjoint.steepening_batch_workload(count={count}, seed={seed}, T={T}): "
                         "source mode, N in {64, 96, 128}, q_u = q_v in {5, 6}, c = 1, alpha log-uniform "
                         "[1e-6, 1e-5], beta0 uniform [6, 8], source width 0.04 at (0.5, 0.3), amplitude for "
@@ -379,6 +381,9 @@ def main():
     ap.add_argument("--batched-adjoint", choices=("on", "off", "only"), default="on",
                     help="the 16-instance batched adjoint against the instances' own gradients: with the headline "
                          "(on), not at all (off), or alone (only)")
+    ap.add_argument("--adjoint-lanes", type=int, choices=(1, 2), default=2,
+                    help="lanes per element of the forward stage kernels in the adjoint legs (2: the two-lane "
+                         "kernels, which the adjoint workloads admit)")
     ap.add_argument("--batched-adjoint-T", type=float, default=0.02,
                     help="end time of the batched-adjoint runs (the workload is defined to 1.0)")
     ap.add_argument("--graph", choices=("on", "off", "only"), default="on",
@@ -412,7 +417,7 @@ def main():
                               higher_is_better=False, dtype="f64", data="synthetic", runs=reference_runs())))
         return
     if args.adjoint == "only":
-        ad = bench_adjoint()
+        ad = bench_adjoint(lanes=args.adjoint_lanes)
         print(json.dumps(dict(metric="adjoint/forward cost ratio", value=ad["adjoint_forward_ratio"], unit="x",
                               n_gpus=1, higher_is_better=False, dtype="f64", data="synthetic", adjoint=ad)))
         return
@@ -421,7 +426,7 @@ def main():
                               higher_is_better=True, dtype="f64", data="synthetic", graph=bench_graph())))
         return
     if args.batched_adjoint == "only":
-        ba = bench_batched_adjoint(T=args.batched_adjoint_T)
+        ba = bench_batched_adjoint(T=args.batched_adjoint_T, lanes=args.adjoint_lanes)
         print(json.dumps(dict(metric="batched adjoint speedup (16 instances)", value=ba.get("speedup"), unit="x",
                               n_gpus=1, higher_is_better=True, dtype="f64", data="synthetic", batched_adjoint=ba)))
         return
@@ -535,8 +540,9 @@ def main():
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
     batched = bench_batched(cheb=args.cheb, lanes_per_element=blpe) if args.batched == "on" else None
-    adjoint = bench_adjoint() if args.adjoint == "on" else None
-    batched_adjoint = bench_batched_adjoint(T=args.batched_adjoint_T) if args.batched_adjoint == "on" else None
+    adjoint = bench_adjoint(lanes=args.adjoint_lanes) if args.adjoint == "on" else None
+    batched_adjoint = (bench_batched_adjoint(T=args.batched_adjoint_T, lanes=args.adjoint_lanes)
+                       if args.batched_adjoint == "on" else None)
     graph = bench_graph() if args.graph == "on" else None
 
     # CPU baseline: the oracle (C/OpenMP restatement of the reference) on the same

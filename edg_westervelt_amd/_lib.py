@@ -93,6 +93,8 @@ def _load():
         "wedg_workspace_set_lanes_per_element": [vp, C.c_int],
         "wedg_problem_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "wedg_workspace_lanes_per_element": [vp, C.POINTER(C.c_int)],
+        "wedg_adj_set_lanes_per_element": [vp, C.c_int],
+        "wedg_adj_lanes_per_element": [vp, C.POINTER(C.c_int)],
         "wedg_last_error": [vp, C.POINTER(WedgErrorInfo)],
         "wedg_timing_enable": [vp, C.c_int],
         "wedg_timing_read": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)],

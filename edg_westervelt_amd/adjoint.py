@@ -56,11 +56,13 @@ class SourceAdjoint:
     Arguments as ``solve`` (src/solver.jl:167-188) where they apply; ``alpha``
     is a number or an (N, N) array and stays fixed, c² and β are set with
     ``set_params``. ``cheb`` is the Chebyshev step-count policy of the forward
-    steps ("global": the reference's; "element")."""
+    steps ("global": the reference's; "element"). ``lanes`` is the lanes per
+    element of the forward stage kernels (``Workspace``): 1, or 2 where the
+    two-lane kernels admit the problem; the adjoint kernels keep one."""
 
     def __init__(self, q: int, N: int, *, mixed: bool = False, flux: str = "upwind", flux_params=None,
                  sigma: float = 0.0, boundary="absorbing_2beta", sigma_dirichlet=None, kappa: float = 0.0,
-                 alpha=1e-5, source: GaussianSource = None, cheb: str = "global"):
+                 alpha=1e-5, source: GaussianSource = None, cheb: str = "global", lanes: int = 1):
         source = source or GaussianSource()
         if sigma_dirichlet is None:
             sigma_dirichlet = sigma
@@ -95,6 +97,19 @@ class SourceAdjoint:
         self.c2 = np.ones((N, N), order="F")
         self.beta = np.zeros((N, N), order="F")
         self.target_steps = 0
+        if lanes != 1:
+            self.set_lanes(lanes)
+
+    def set_lanes(self, lanes: int):
+        """Lanes per element of the forward stage kernels (1 or 2)."""
+        self._check(lib.wedg_adj_set_lanes_per_element(self.h_, int(lanes)), "set_lanes")
+
+    @property
+    def lanes(self) -> int:
+        """The lanes per element the forward steps use (2 only where admitted)."""
+        n = C.c_int(0)
+        self._check(lib.wedg_adj_lanes_per_element(self.h_, C.byref(n)), "lanes")
+        return n.value
 
     def __del__(self):
         h = getattr(self, "h_", None)
@@ -195,14 +210,15 @@ def misfit_rrule(adj: SourceAdjoint, c2, beta, dt: float, nsteps: int, snapshots
 
 
 def beta_field_workload(N: int = 64, q: int = 4, steps: int = 2000, seed: int = 1, cheb: str = "global",
-                        x0=(0.5, 0.7)):
+                        x0=(0.5, 0.7), lanes: int = 1):
     """The receiver-misfit gradient workload: q_u = q_v = q on an N x N mesh in
     source mode, a Gaussian source of width 0.05 at (0.5, 0.7) (amplitude 1,
     omega 16 pi, Ts = 1), c = 1, alpha = 1e-5, beta per element
     4 + 2 sin(2 pi x) cos(2 pi y) + N(0, 0.1) at the element centres (numpy
     generator of `seed`), the bottom side absorbing_2beta and the other three
     walls with kappa = 0.5, `steps` RK4 steps of dt = h / (10 q^2); the target
-    is the bottom trace of the same run with beta = 4. `x0` moves the source."""
+    is the bottom trace of the same run with beta = 4. `x0` moves the source;
+    `lanes`: the forward kernels' lanes per element."""
     import math
     h = 1.0 / N
     xc = (np.arange(N) + 0.5) * h
@@ -210,7 +226,7 @@ def beta_field_workload(N: int = 64, q: int = 4, steps: int = 2000, seed: int = 
     rng = np.random.default_rng(seed)
     beta = 4.0 + 2.0 * np.sin(2 * math.pi * X) * np.cos(2 * math.pi * Y) + 0.1 * rng.standard_normal((N, N))
     adj = SourceAdjoint(q, N, mixed=False, boundary=dict(bot="absorbing_2beta", top="wall", lft="wall", rgt="wall"),
-                        kappa=0.5, alpha=1e-5, source=GaussianSource(x0=tuple(x0), r0=0.05), cheb=cheb)
+                        kappa=0.5, alpha=1e-5, source=GaussianSource(x0=tuple(x0), r0=0.05), cheb=cheb, lanes=lanes)
     dt = h / (10 * q * q)
     adj.set_params(1.0, 4.0)
     adj.set_target(adj.traces(dt, steps))
@@ -290,7 +306,7 @@ class AdjointBatch:
 
 
 def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0=(0.5, 0.3), r0: float = 0.04,
-                              margin: float = 0.3, cheb: str = "global"):
+                              margin: float = 0.3, cheb: str = "global", lanes: int = 1):
     """The batched-adjoint workload: `count` source-mode instances drawn from
     numpy's generator of `seed`, each with N in {64, 96, 128}, q_u = q_v in
     {5, 6}, c = 1, alpha log-uniform in [1e-6, 1e-5], beta0 uniform in [6, 8],
@@ -302,7 +318,7 @@ def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0
     that the instance keeps about `margin`. The run takes round(T / dt) steps of
     dt = h / (10 q^2). The gradient is taken at beta = beta0 + 0.5 sin(2 pi x)
     cos(2 pi y) per element (c^2 = 1); the target is the bottom trace of the
-    run with beta = beta0."""
+    run with beta = beta0. `lanes`: the forward kernels' lanes per element."""
     import math
     rng = np.random.default_rng(seed)
     kinds = ("absorbing_2beta", "absorbing", "wall")
@@ -324,7 +340,8 @@ def steepening_batch_workload(count: int = 16, seed: int = 2, T: float = 1.0, x0
         beta = np.asfortranarray(beta0 + 0.5 * np.sin(2 * math.pi * X) * np.cos(2 * math.pi * Y))
         for _try in range(8):   # the finer mesh may steepen a little more than the calibration mesh
             src = GaussianSource(x0=tuple(x0), r0=r0, amplitude=amp)
-            adj = SourceAdjoint(q, N, mixed=False, boundary=sides, kappa=0.5, alpha=alpha, source=src, cheb=cheb)
+            adj = SourceAdjoint(q, N, mixed=False, boundary=sides, kappa=0.5, alpha=alpha, source=src, cheb=cheb,
+                                lanes=lanes)
             try:
                 adj.set_params(1.0, beta0)
                 adj.set_target(adj.traces(dt, nsteps))

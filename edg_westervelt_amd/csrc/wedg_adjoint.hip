@@ -314,6 +314,25 @@ wedg_status build_problem(wedg_adj* A, wedg_problem** out) {
         p->one_sided = one_sided ? 1 : 0;
         p->mode = (p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) ? MODE_UNIFORM : MODE_SEPARABLE;
     }
+    // the split tables of the two-lane stage kernels (the forward steps may run them)
+    {
+        SplitSrc src{A->w.data(), A->W2dh.data(), A->Pu.data(), A->Su.data(), A->Ph.data(), A->Ebl.data(),
+                     A->Ebr.data(), A->E1bl.data(), A->E1br.data(), nullptr, nullptr, nullptr, nullptr};
+        std::vector<double> rc((size_t)nq * nq), ra((size_t)nq * nq), rb((size_t)nh * nh, A->beta[0]),
+            d0((size_t)nu * nu);
+        if (p->mode == MODE_UNIFORM) {
+            for (int ix = 0; ix < nq; ++ix)
+                for (int iy = 0; iy < nq; ++iy) {
+                    rc[iy + (size_t)nq * ix] = A->c2[0] * A->w[iy] * A->w[ix];
+                    ra[iy + (size_t)nq * ix] = A->alpha[0] * A->w[iy] * A->w[ix];
+                }
+            for (int jj = 0; jj < nu; ++jj)
+                for (int i = 0; i < nu; ++i)
+                    d0[i + (size_t)nu * jj] = 1.0 / (A->c2[0] * (A->lam[i] + A->lam[jj]) + (i == 0 && jj == 0 ? 1.0 : 0.0));
+            src.cw = rc.data(); src.aw = ra.data(); src.beta = rb.data(); src.d0 = d0.data();
+        }
+        if (build_split_tables(p, src) != hipSuccess) { wedg_problem_destroy(p); return WEDG_ERR_HIP; }
+    }
     {
         std::vector<double> z((size_t)nq * N, 0.0);   // zero boundary data (src/solver.jl:647-655)
         for (int a = 0; a < 8; ++a) UP(p->bcd[a], z.data(), z.size());
@@ -551,6 +570,21 @@ extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, 
     if ((st = wedg_adj_set_params(A, A->c2.data(), A->beta.data()))) return fail(st);
     *out = A;
     return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_set_lanes_per_element(wedg_adj* A, int lanes) {
+    if (!A || (lanes != 1 && lanes != 2)) return WEDG_ERR_ARGUMENT;
+    // every workspace that runs forward stages: the forward, tape and adjoint streams'
+    for (wedg_workspace* w : {A->ws, A->wt, A->wa}) {
+        const wedg_status s = wedg_workspace_set_lanes_per_element(w, lanes);
+        if (s) return s;
+    }
+    return WEDG_OK;
+}
+
+extern "C" wedg_status wedg_adj_lanes_per_element(const wedg_adj* A, int* lanes) {
+    if (!A || !lanes) return WEDG_ERR_ARGUMENT;
+    return wedg_workspace_lanes_per_element(A->ws, lanes);
 }
 
 extern "C" wedg_status wedg_adj_destroy(wedg_adj* A) {
@@ -903,7 +937,19 @@ struct wedg_adj_batch {
 
 namespace {
 
-hipError_t launch_fwd_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode) {
+hipError_t launch_fwd_batch(const BatchArgs& b, int grid, int qu, int nv, int which, hipStream_t s, int mode,
+                            bool two) {
+    if (two) {
+        switch (qu) {
+            case 1: return launch_batch2_q1(b, grid, nv, which, s, mode);
+            case 2: return launch_batch2_q2(b, grid, nv, which, s, mode);
+            case 3: return launch_batch2_q3(b, grid, nv, which, s, mode);
+            case 4: return launch_batch2_q4(b, grid, nv, which, s, mode);
+            case 5: return launch_batch2_q5(b, grid, nv, which, s, mode);
+            case 6: return launch_batch2_q6(b, grid, nv, which, s, mode);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (qu) {
         case 1: return launch_batch_q1(b, grid, nv, which, s, mode);
         case 2: return launch_batch_q2(b, grid, nv, which, s, mode);
@@ -993,7 +1039,7 @@ size_t round_bytes(const wedg_adj_batch* b) {
     size_t n = 0, blocks = 0, mblocks = 0;
     for (wedg_adj* A : b->inst) {
         ++n;
-        blocks += (A->E + kWave - 1) / kWave;
+        blocks += (A->E + kPairWave - 1) / kPairWave;   // the forward launches' most (two lanes)
         mblocks += (A->N + kWave - 1) / kWave;
     }
     const size_t per_group_arrays = 4 + 6 + 6;
@@ -1003,7 +1049,14 @@ size_t round_bytes(const wedg_adj_batch* b) {
 }
 
 // the group key of an instance: kernels are instantiated per (q_u, q_v, layout)
-long group_key(const wedg_adj* A) { return ((long)A->qu * 16 + A->nv) * 16 + A->p->mode; }
+long group_key(const wedg_adj* A) { return (((long)A->qu * 16 + A->nv) * 16 + A->p->mode) * 2 + (A->ws->two() ? 1 : 0); }
+
+// Blocks (waves) of an instance's forward stage launch: 64 elements per wave,
+// 32 with two lanes per element.
+int fwd_blocks(const wedg_adj* A) {
+    const int w = A->ws->two() ? kPairWave : kWave;
+    return (A->E + w - 1) / w;
+}
 
 wedg_status batch_prepare_instance(wedg_adj* A, long nsteps, int nsnap, bool gradient) {
     wedg_workspace* ws = A->ws;
@@ -1080,6 +1133,7 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
         struct Launch {
             int qu, nv, mode, which, grid, adj;
             size_t args, binst, bfirst;
+            bool two;   // forward launches: the two-lane kernels
         };
         std::vector<Launch> launches;
         for (size_t g = 0; g < gk.size(); ++g) {
@@ -1093,15 +1147,16 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
             wedg_adj* A0 = B->inst[F[0]];
             const int qu = A0->qu, nv = A0->nv, mode = A0->p->mode;
             // forward: block map and the four stages' arguments
+            const bool two = A0->ws->two();
             int fblocks = 0;
-            for (int i : F) fblocks += (B->inst[i]->E + kWave - 1) / kWave;
+            for (int i : F) fblocks += fwd_blocks(B->inst[i]);
             const size_t o_bi = take(fblocks * sizeof(int)), o_bf = take(F.size() * sizeof(int));
             {
                 int* bi = (int*)(hb + o_bi);
                 int* bf = (int*)(hb + o_bf);
                 int blk = 0;
                 for (size_t x = 0; x < F.size(); ++x) {
-                    const int nb = (B->inst[F[x]]->E + kWave - 1) / kWave;
+                    const int nb = fwd_blocks(B->inst[F[x]]);
                     bf[x] = blk;
                     for (int k = 0; k < nb; ++k) bi[blk++] = (int)x;
                 }
@@ -1141,8 +1196,9 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                 if (u.rev) ++A->stats.taped_steps;
                 else ++A->stats.forward_steps;
             }
-            launches.push_back({qu, nv, mode, 0, fblocks, 0, o_st[0], o_bi, o_bf});   // k_prep
-            for (int sg = 0; sg < 4; ++sg) launches.push_back({qu, nv, mode, 1, fblocks, 0, o_st[sg], o_bi, o_bf});
+            launches.push_back({qu, nv, mode, 0, fblocks, 0, o_st[0], o_bi, o_bf, two});   // k_prep
+            for (int sg = 0; sg < 4; ++sg)
+                launches.push_back({qu, nv, mode, 1, fblocks, 0, o_st[sg], o_bi, o_bf, two});
             // misfit of the forward steps' end states (forward runs)
             std::vector<int> M = forward_misfit ? F : R;
             if (!M.empty()) {
@@ -1175,7 +1231,7 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                     std::memcpy(hb + o_ma + (2 * x) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
                     std::memcpy(hb + o_ma + (2 * x + 1) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
                 }
-                launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf});
+                launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf, false});
             }
             if (R.empty() || forward_misfit) continue;
             // the adjoint of the reversed steps
@@ -1205,9 +1261,9 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                     std::memcpy(hb + o_ad[k] + 2 * x * sizeof(AdjArgs), pairs[k], 2 * sizeof(AdjArgs));
                 ++A->stats.adjoint_steps;
             }
-            launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af});
-            for (int k = 1; k <= 3; ++k) launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af});
-            launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af});
+            launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af, false});
+            for (int k = 1; k <= 3; ++k) launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af, false});
+            launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af, false});
         }
         if (off > B->cap) return WEDG_ERR_ARGUMENT;
         char* db = B->dbuf[ring];
@@ -1218,7 +1274,7 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
             ba.binst = (const int*)(db + L.binst);
             ba.bfirst = (const int*)(db + L.bfirst);
             if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, s));
-            else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, s, L.mode));
+            else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, s, L.mode, L.two));
         }
         HIPOK(hipEventRecord(B->done[ring], s));
         B->used[ring] = true;

@@ -134,6 +134,84 @@ hipError_t wedg::launch(const KArgs& a, wedg_problem* p, int which, hipStream_t 
     }
 }
 
+// The split tables of the two-lane kernels (wedg_stage2.h) into p->tab2: per
+// lane parity h the values T[p][2k + h] of the 1D tables (0 at the dummy row
+// 2k + h >= n of a field with n odd modes, and at the other parity of a middle
+// point) and, for one class, its u-solve factors D[2k + h][jj]. The kernels
+// read the quadrature weights, W_h and a one-class problem's coefficient
+// records from row p in both lanes of a pair, so the problem is admitted only
+// when those are mirror-symmetric (row NR-1-p equals row p to 1e-13
+// relative); otherwise p->tab2 stays null and the problem runs one lane.
+hipError_t wedg::build_split_tables(wedg_problem* p, const SplitSrc& src) {
+    const int nu = p->nu, nv = p->nv, nh = p->nh, nq = nu;
+    if (nu > 7 || !(p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE)) return hipSuccess;
+    auto mirror_ok = [](const double* a, int nr, int nc, long stride_c) {
+        for (int r = 0; r < nr; ++r)
+            for (int c = 0; c < nc; ++c) {
+                const double x = a[r + stride_c * c], y = a[nr - 1 - r + stride_c * c];
+                if (!(std::fabs(x - y) <= 1e-13 * std::max(std::fabs(x), std::fabs(y)))) return false;
+            }
+        return true;
+    };
+    bool sym = mirror_ok(src.w, nq, 1, 0) && mirror_ok(src.W2d_h, nh, nh, nh);
+    if (p->mode == MODE_UNIFORM)
+        sym = sym && src.cw && src.aw && src.beta && src.d0 && mirror_ok(src.cw, nq, nq, nq) &&
+              mirror_ok(src.aw, nq, nq, nq) && mirror_ok(src.beta, nh, nh, nh);
+    if (!sym) return hipSuccess;
+    const SplitLayout L = split_layout(nu, nv, nh);
+    std::vector<double> t2(2 * (size_t)L.n, 0.0);
+    // T[pp][2k + hp] of a column-major (nr x ncol) table for a field of n
+    // modes; the middle point (nr odd) keeps only the parity `keep`
+    auto tv = [&](const double* T, int nr, int pp, int k, int hp, int n, int keep) {
+        const int a = 2 * k + hp;
+        if (a >= n) return 0.0;
+        if ((nr & 1) && pp == nr / 2 && (a & 1) != keep) return 0.0;
+        return T[pp + (size_t)nr * a];
+    };
+    for (int hp = 0; hp < 2; ++hp) {
+        double* s2 = t2.data() + (size_t)hp * L.n;
+        for (int pp = 0; pp < L.NQS; ++pp) {
+            for (int k = 0; k < L.RU; ++k) {
+                s2[L.sPu + pp * L.RU + k] = tv(src.P_u, nq, pp, k, hp, nu, 0);   // values: even modes
+                s2[L.sSu + pp * L.RU + k] = tv(src.S_u, nq, pp, k, hp, nu, 1);   // derivatives: odd
+            }
+            if (L.sPv != L.sPu)
+                for (int k = 0; k < L.RV; ++k) {
+                    s2[L.sPv + pp * L.RV + k] = tv(src.P_u, nq, pp, k, hp, nv, 0);
+                    s2[L.sSv + pp * L.RV + k] = tv(src.S_u, nq, pp, k, hp, nv, 1);
+                }
+        }
+        for (int pp = 0; pp < L.NHH; ++pp)
+            for (int k = 0; k < L.RV; ++k) s2[L.sPh + pp * L.RV + k] = tv(src.P_h_v, nh, pp, k, hp, nv, 0);
+        for (int k = 0; k < L.RU; ++k) {
+            const int a = 2 * k + hp;
+            const bool ok = a < nu;
+            s2[L.sE1lu + k] = ok ? src.E1bl[a] : 0.0;
+            s2[L.sE1ru + k] = ok ? src.E1br[a] : 0.0;
+            for (int k2 = 0; k2 < L.RU; ++k2) {
+                const int a2 = 2 * k2 + hp;
+                s2[L.sQ + k2 + L.RU * k] = (ok && a2 < nu) ? p->Q[a2 + (size_t)nu * a] : 0.0;
+            }
+            if (p->mode == MODE_UNIFORM)
+                for (int jj = 0; jj < nu; ++jj) s2[L.sD + k + L.RU * jj] = ok ? src.d0[a + (size_t)nu * jj] : 0.0;
+        }
+        for (int k = 0; k < L.RV; ++k) {
+            const int a = 2 * k + hp;
+            const bool ok = a < nv;
+            s2[L.sElv + k] = ok ? src.Ebl[a] : 0.0;
+            s2[L.sErv + k] = ok ? src.Ebr[a] : 0.0;
+            if (L.sE1lv != L.sE1lu) {
+                s2[L.sE1lv + k] = ok ? src.E1bl[a] : 0.0;
+                s2[L.sE1rv + k] = ok ? src.E1br[a] : 0.0;
+            }
+        }
+    }
+    hipError_t he = hipSuccess;
+    double* d = p->upload(t2.data(), t2.size(), &he);
+    if (he == hipSuccess) p->tab2 = d;
+    return he;
+}
+
 extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** out) {
     if (!pre || !out) return WEDG_ERR_ARGUMENT;
     *out = nullptr;
@@ -309,88 +387,23 @@ extern "C" wedg_status wedg_problem_create(const wedg_pre* pre, wedg_problem** o
         for (int a = 0; a < F_COUNT; ++a) faces_uni = faces_uni && p->f[a].uni;
         if (p->mode == MODE_SEPARABLE && p->ncls_ca == 1 && p->ncls_b == 1 && faces_uni) p->mode = MODE_UNIFORM;
     }
-    // the split tables of the two-lane kernels (wedg_stage2.h): per lane parity
-    // h the values T[p][2k + h] of the 1D tables (0 at the dummy row 2k + h >= n
-    // of a field with n odd modes, and at the other parity of a middle point)
-    // and, for one class, its u-solve factors D[2k + h][jj]. The kernels read
-    // the quadrature weights, W_h and a one-class problem's coefficient records
-    // from row p in both lanes of a pair, so the problem is admitted only when
-    // those are mirror-symmetric (row NR-1-p equals row p to 1e-13 relative).
-    if (nu <= 7 && (p->mode == MODE_UNIFORM || p->mode == MODE_SEPARABLE)) {
-        auto mirror_ok = [](const double* a, int nr, int nc, long stride_c) {
-            for (int r = 0; r < nr; ++r)
-                for (int c = 0; c < nc; ++c) {
-                    const double x = a[r + stride_c * c], y = a[nr - 1 - r + stride_c * c];
-                    if (!(std::fabs(x - y) <= 1e-13 * std::max(std::fabs(x), std::fabs(y)))) return false;
-                }
-            return true;
-        };
-        bool sym = mirror_ok(pre->w, nq, 1, 0) && mirror_ok(pre->W2d_h, nh, nh, nh);
+    // the split tables of the two-lane kernels; a one-class problem's records
+    // as uploaded (Cw, Aw: nq x nq, [iy + nq ix]; beta: nh x nh, [pp + nh q])
+    {
+        SplitSrc src{pre->w, pre->W2d_h, pre->P_u, pre->S_u, pre->P_h_v, pre->Ebl_u, pre->Ebr_u, pre->E1bl_u,
+                     pre->E1br_u, nullptr, nullptr, nullptr, d0.empty() ? nullptr : d0.data()};
+        std::vector<double> rc((size_t)nq * nq), ra((size_t)nq * nq), rb((size_t)nh * nh);
         if (p->mode == MODE_UNIFORM) {
-            // the one class's records as uploaded: Cw, Aw (nq x nq), beta (nh x nh)
-            std::vector<double> rec((size_t)nq * nq);
-            for (int which = 0; which < 2 && sym; ++which) {
-                for (int ix = 0; ix < nq; ++ix)
-                    for (int iy = 0; iy < nq; ++iy)
-                        rec[iy + (size_t)nq * ix] = (which ? pre->Aw : pre->Cw)[iy + (long)nq * (0 + E * ix)];
-                sym = mirror_ok(rec.data(), nq, nq, nq);
-            }
-            std::vector<double> bq((size_t)nh * nh);
+            for (int ix = 0; ix < nq; ++ix)
+                for (int iy = 0; iy < nq; ++iy) {
+                    rc[iy + (size_t)nq * ix] = pre->Cw[iy + (long)nq * (0 + E * ix)];
+                    ra[iy + (size_t)nq * ix] = pre->Aw[iy + (long)nq * (0 + E * ix)];
+                }
             for (int q = 0; q < nh; ++q)
-                for (int pp = 0; pp < nh; ++pp) bq[pp + (size_t)nh * q] = pre->beta_hq[pp + (long)nh * (0 + E * q)];
-            sym = sym && mirror_ok(bq.data(), nh, nh, nh);
+                for (int pp = 0; pp < nh; ++pp) rb[pp + (size_t)nh * q] = pre->beta_hq[pp + (long)nh * (0 + E * q)];
+            src.cw = rc.data(); src.aw = ra.data(); src.beta = rb.data();
         }
-        if (sym) {
-            const SplitLayout L = split_layout(nu, nv, nh);
-            std::vector<double> t2(2 * (size_t)L.n, 0.0);
-            // T[pp][2k + hp] of a column-major (nr x ncol) table for a field of n
-            // modes; the middle point (nr odd) keeps only the parity `keep`
-            auto tv = [&](const double* T, int nr, int pp, int k, int hp, int n, int keep) {
-                const int a = 2 * k + hp;
-                if (a >= n) return 0.0;
-                if ((nr & 1) && pp == nr / 2 && (a & 1) != keep) return 0.0;
-                return T[pp + (size_t)nr * a];
-            };
-            for (int hp = 0; hp < 2; ++hp) {
-                double* s2 = t2.data() + (size_t)hp * L.n;
-                for (int pp = 0; pp < L.NQS; ++pp) {
-                    for (int k = 0; k < L.RU; ++k) {
-                        s2[L.sPu + pp * L.RU + k] = tv(pre->P_u, nq, pp, k, hp, nu, 0);   // values: even modes
-                        s2[L.sSu + pp * L.RU + k] = tv(pre->S_u, nq, pp, k, hp, nu, 1);   // derivatives: odd
-                    }
-                    if (L.sPv != L.sPu)
-                        for (int k = 0; k < L.RV; ++k) {
-                            s2[L.sPv + pp * L.RV + k] = tv(pre->P_u, nq, pp, k, hp, nv, 0);
-                            s2[L.sSv + pp * L.RV + k] = tv(pre->S_u, nq, pp, k, hp, nv, 1);
-                        }
-                }
-                for (int pp = 0; pp < L.NHH; ++pp)
-                    for (int k = 0; k < L.RV; ++k) s2[L.sPh + pp * L.RV + k] = tv(pre->P_h_v, nh, pp, k, hp, nv, 0);
-                for (int k = 0; k < L.RU; ++k) {
-                    const int a = 2 * k + hp;
-                    const bool ok = a < nu;
-                    s2[L.sE1lu + k] = ok ? pre->E1bl_u[a] : 0.0;
-                    s2[L.sE1ru + k] = ok ? pre->E1br_u[a] : 0.0;
-                    for (int k2 = 0; k2 < L.RU; ++k2) {
-                        const int a2 = 2 * k2 + hp;
-                        s2[L.sQ + k2 + L.RU * k] = (ok && a2 < nu) ? p->Q[a2 + (size_t)nu * a] : 0.0;
-                    }
-                    if (p->mode == MODE_UNIFORM)
-                        for (int jj = 0; jj < nu; ++jj) s2[L.sD + k + L.RU * jj] = ok ? d0[a + (size_t)nu * jj] : 0.0;
-                }
-                for (int k = 0; k < L.RV; ++k) {
-                    const int a = 2 * k + hp;
-                    const bool ok = a < nv;
-                    s2[L.sElv + k] = ok ? pre->Ebl_u[a] : 0.0;
-                    s2[L.sErv + k] = ok ? pre->Ebr_u[a] : 0.0;
-                    if (L.sE1lv != L.sE1lu) {
-                        s2[L.sE1lv + k] = ok ? pre->E1bl_u[a] : 0.0;
-                        s2[L.sE1rv + k] = ok ? pre->E1br_u[a] : 0.0;
-                    }
-                }
-            }
-            UP(p->tab2, t2.data(), t2.size());
-        }
+        if (build_split_tables(p, src) != hipSuccess) { wedg_problem_destroy(p); return WEDG_ERR_HIP; }
     }
     // boundary data (NQ, N), unchanged
     {

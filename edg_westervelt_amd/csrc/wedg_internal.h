@@ -213,6 +213,19 @@ struct wedg_workspace {
 namespace wedg {
 
 hipError_t launch(const KArgs& a, wedg_problem* p, int which, hipStream_t s, bool two);   // two: the two-lane kernels
+
+// The host tables the split tables of the two-lane kernels are made from:
+// column-major 1D tables as precompute forms them (w: nq, W2d_h: nh x nh,
+// P_u/S_u: nq x nu, P_h_v: nh x nv, E rows: nu) and, for a one-class problem,
+// its records (Cw, Aw: nq x nq [iy + nq ix]; beta: nh x nh [p + nh q]) and
+// u-solve factors d0 (nu x nu); null where the problem has classes.
+struct SplitSrc {
+    const double *w, *W2d_h, *P_u, *S_u, *P_h_v, *Ebl, *Ebr, *E1bl, *E1br;
+    const double *cw, *aw, *beta, *d0;
+};
+// Uploads p->tab2 when the two-lane kernels admit the problem (p->mode, p->Q
+// and the sizes already set); leaves it null otherwise.
+hipError_t build_split_tables(wedg_problem* p, const SplitSrc& src);
 wedg_status reset_err(wedg_workspace* ws);
 wedg_status collect(wedg_workspace* ws, int nonfinite_is_error);
 KArgs problem_args(const wedg_problem* p, int cheb_global, double* bsl, DevErr* err, int* cnt, unsigned* wstat);

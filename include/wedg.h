@@ -363,6 +363,14 @@ wedg_status wedg_adj_forward(wedg_adj* a, double dt, long nsteps, double* J, dou
 wedg_status wedg_adj_gradient(wedg_adj* a, double dt, long nsteps, int snapshots, double* J, double* dJ_dc2,
                               double* dJ_dbeta);
 wedg_status wedg_adj_last_stats(const wedg_adj* a, wedg_adj_stats* st);
+/* Lanes per element of the instance's forward stage kernels (its forward
+ * runs, checkpoint recomputations and taped steps alike; the adjoint kernels
+ * keep one lane), as wedg_workspace_set_lanes_per_element: 1 (the default) or
+ * 2 where the two-lane kernels admit the problem. No counterpart in the
+ * reference, whose gradient follows its CPU forward. Results agree with one
+ * lane to rounding; recomputed states stay bitwise the forward's. */
+wedg_status wedg_adj_set_lanes_per_element(wedg_adj* a, int lanes);
+wedg_status wedg_adj_lanes_per_element(const wedg_adj* a, int* lanes);
 /* One right-hand side (u_t, v_t) = F(u, v, t; c^2, beta) differentiated: the
  * vector-Jacobian product, from the cotangents (ub, vb) of (u_t, v_t) those of
  * (u, v, c^2, beta); and the tangent-linear map, (du_t, dv_t) along
@@ -375,8 +383,8 @@ wedg_status wedg_adj_rhs_tl(wedg_adj* a, const double* u, const double* v, doubl
 /* Several instances' forward runs and gradients together (no counterpart in
  * the reference): each instance runs what wedg_adj_forward / wedg_adj_gradient
  * run on it, with identical results, in lockstep rounds on one stream; per
- * round, the instances with the same (q_u, q_v, kernel layout) share one
- * launch for every RK stage and every adjoint kernel. Instances keep their
+ * round, the instances with the same (q_u, q_v, kernel layout, lanes per
+ * element) share one launch for every RK stage and every adjoint kernel. Instances keep their
  * coefficients and targets (wedg_adj_set_params, wedg_adj_set_target) and may
  * not be used elsewhere while a batch call runs. status[i] (or null) receives
  * instance i's status; the return value is the first failure. Per-instance

@@ -23,7 +23,7 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def setup(q, mixed, N, seed):
+def setup(q, mixed, N, seed, lanes=1):
     import edg_westervelt_amd as w
     from edg_westervelt_amd.adjoint import SourceAdjoint
     from oracle import adjoint as A
@@ -34,7 +34,7 @@ def setup(q, mixed, N, seed):
     beta = 2 + rng.random((N, N))
     x0, r0, amp, om, Ts = SRC
     adj = SourceAdjoint(q, N, mixed=mixed, flux_params=FLUX, sigma=0.3, boundary=SIDES, sigma_dirichlet=0.6, kappa=0.5,
-                        alpha=al, source=w.GaussianSource(x0=x0, r0=r0, amplitude=amp, omega=om, Ts=Ts))
+                        alpha=al, source=w.GaussianSource(x0=x0, r0=r0, amplitude=amp, omega=om, Ts=Ts), lanes=lanes)
     adj.set_params(c2, beta)
     sides = [A.side_params(SIDES[s], 0.3, 0.6, 0.5) for s in ("bot", "top", "lft", "rgt")]
     pb = A.Problem(q, qv, N, flux=tuple(FLUX.values()), sides=sides, alpha=al, source=SRC)
@@ -115,6 +115,32 @@ def test_gradient_against_oracle_and_checkpointing(q, mixed, N, ns):
         J2, gc2b, gbb = adj.gradient(dt, ns, snapshots=snaps)
         assert J2 == J and np.array_equal(gc2b, gc2) and np.array_equal(gbb, gb)
         assert adj.stats().forward_steps == _closed_form_advances(ns, snaps)
+
+
+@pytest.mark.parametrize("q,mixed,N,ns", [(3, False, 6, 30), (5, False, 5, 12), (6, True, 4, 10), (4, True, 5, 14)])
+def test_gradient_with_two_lane_forward(q, mixed, N, ns):
+    """The gradient with the forward steps on the two-lane stage kernels
+    (wedg_adj_set_lanes_per_element): against the oracle's adjoint to 1e-9,
+    against the one-lane gradient to 1e-11, and bitwise the same for every
+    checkpointing schedule (the recomputed states are the forward's)."""
+    from oracle import adjoint as A
+    adj, pb, c2, beta, _, _ = setup(q, mixed, N, 3, lanes=2)
+    assert adj.lanes == 2
+    dt = 2e-3
+    tgt = A.misfit(pb, np.ones_like(c2), 3 * np.ones_like(beta), dt, ns, traces=True)[1]
+    adj.set_target(tgt)
+    J, gc2, gb = adj.gradient(dt, ns, snapshots=4)
+    Jo, oc2, ob = A.gradient(pb, c2, beta, dt, ns, tgt)
+    assert abs(J - Jo) <= 1e-9 * Jo
+    assert rel(gc2, oc2) < 1e-9 and rel(gb, ob) < 1e-9
+    for snaps in (1, ns + 1):
+        J2, gc2b, gbb = adj.gradient(dt, ns, snapshots=snaps)
+        assert J2 == J and np.array_equal(gc2b, gc2) and np.array_equal(gbb, gb)
+    assert adj.misfit(dt, ns) == J
+    adj.set_lanes(1)
+    assert adj.lanes == 1
+    J1, gc21, gb1 = adj.gradient(dt, ns, snapshots=4)
+    assert abs(J1 - J) <= 1e-11 * J and rel(gc2, gc21) < 1e-11 and rel(gb, gb1) < 1e-11
 
 
 def test_gradient_against_central_differences_of_the_c_oracle(oracle_built):
@@ -198,13 +224,14 @@ def test_full_workload_as_specified():
     assert J2 == J and np.array_equal(gc2b, gc2) and np.array_equal(gbb, gb)
 
 
-def _batch_cases():
+def _batch_cases(lanes=(1, 1, 1, 1)):
     """Instances of two degree groups, uneven meshes and step counts."""
     from oracle import adjoint as A
     out = []
     for k, (q, mixed, N, ns) in enumerate([(3, False, 6, 30), (4, False, 4, 16), (3, True, 5, 12),
                                            (3, False, 8, 21)]):
-        adj, pb, c2, beta, _, _ = setup(q, mixed, N, 20 + k)
+        adj, pb, c2, beta, _, _ = setup(q, mixed, N, 20 + k, lanes=lanes[k])
+        assert adj.lanes == lanes[k]
         dt = 2e-3 if k != 3 else 1.5e-3
         tgt = A.misfit(pb, np.ones_like(c2), 3 * np.ones_like(beta), dt, ns, traces=True)[1]
         adj.set_target(tgt)
@@ -212,14 +239,15 @@ def _batch_cases():
     return out
 
 
-@pytest.mark.parametrize("snaps", [4, 1])
-def test_batched_gradient_equals_each_instances_own(snaps):
+@pytest.mark.parametrize("snaps,lanes", [(4, (1, 1, 1, 1)), (1, (1, 1, 1, 1)), (4, (2, 1, 2, 2)), (1, (2, 2, 1, 2))])
+def test_batched_gradient_equals_each_instances_own(snaps, lanes):
     """wedg_adj_batch_gradient over four instances (two degree groups, uneven N
-    and step counts) gives every instance bitwise its own wedg_adj_gradient,
-    with the same checkpointing counts; wedg_adj_batch_forward gives bitwise
-    each instance's misfit."""
+    and step counts, forward kernels with one or two lanes per element: the
+    groups split by lanes) gives every instance bitwise its own
+    wedg_adj_gradient, with the same checkpointing counts;
+    wedg_adj_batch_forward gives bitwise each instance's misfit."""
     from edg_westervelt_amd.adjoint import AdjointBatch
-    cases = _batch_cases()
+    cases = _batch_cases(lanes)
     own, counts = [], []
     for adj, dt, ns in cases:
         own.append(adj.gradient(dt, ns, snapshots=snaps))
