@@ -183,7 +183,7 @@ def bench_adjoint(steps=2000, snapshots=32, N=64, q=4, lanes=1):
     return dict(adjoint_forward_ratio=tg / tf, forward_s=tf, gradient_s=tg, J=J, J_gradient_run=Jg,
                 forward_dof_stages_per_s=4 * steps * dof / tf, snapshots=snapshots,
                 forward_steps_recomputed=st.forward_steps, taped_steps=st.taped_steps,
-                snapshots_used=st.snapshots_used, build_s=round(t_build, 2), forward_lanes_per_element=adj.lanes,
+                snapshots_used=st.snapshots_used, tapes=st.tapes, build_s=round(t_build, 2), forward_lanes_per_element=adj.lanes,
                 host_enqueue_ms=st.enqueue_ms, device_ms=st.total_ms,
                 workload=f"edg_westervelt_amd.adjoint.beta_field_workload(N={N}, q={q}, steps={steps}): source "
                          "mode, Gaussian source width 0.05 at (0.5, 0.7), c = 1, alpha = 1e-5, per-element beta "

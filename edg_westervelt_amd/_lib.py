@@ -59,7 +59,7 @@ class WedgAdjDesc(C.Structure):
 
 class WedgAdjStats(C.Structure):
     _fields_ = [("forward_steps", C.c_long), ("taped_steps", C.c_long), ("adjoint_steps", C.c_long),
-                ("snapshots_taken", C.c_long), ("snapshots_used", C.c_int), ("pad", C.c_int),
+                ("snapshots_taken", C.c_long), ("snapshots_used", C.c_int), ("tapes", C.c_int),
                 ("enqueue_ms", C.c_double), ("total_ms", C.c_double)]
 
 

@@ -165,7 +165,8 @@ class SourceAdjoint:
         lib.wedg_adj_last_stats(self.h_, C.byref(s))
         return SimpleNamespace(forward_steps=s.forward_steps, taped_steps=s.taped_steps,
                                adjoint_steps=s.adjoint_steps, snapshots_taken=s.snapshots_taken,
-                               snapshots_used=s.snapshots_used, enqueue_ms=s.enqueue_ms, total_ms=s.total_ms)
+                               snapshots_used=s.snapshots_used, tapes=s.tapes, enqueue_ms=s.enqueue_ms,
+                               total_ms=s.total_ms)
 
     def _state(self, u, v):
         u = np.asfortranarray(u, dtype=np.float64)

@@ -12,13 +12,17 @@
 //
 // The gradient runs three streams, each with its own workspace: the forward
 // stream does the checkpointing schedule (advances, snapshots, restores) and
-// copies the state of each step to be reversed into one of kTapes tapes; the
+// copies the state of each step to be reversed into one of its tapes; the
 // tape stream re-runs that step keeping its stage states and stage
 // derivatives; the adjoint stream seeds the misfit of the step's end state and
 // runs the eight adjoint kernels of the step. Events order a tape between its
-// writers and its reader, so the forward stream runs ahead by up to kTapes
-// steps to be reversed and the three chains overlap on the GPU (a 64 x 64 mesh
-// fills a sixteenth of its SIMDs).
+// writers and its reader, so the forward stream runs ahead by as many steps to
+// be reversed as there are tapes and the three chains overlap on the GPU (a
+// 64 x 64 mesh fills a sixteenth of its SIMDs). Binomial checkpointing
+// alternates stretches of many recomputed forward steps per reversal with
+// stretches of few; the tapes let the adjoint chain work off in the first kind
+// what it falls behind by in the second, so a gradient takes more tapes when
+// they are small (up to kMaxTapes within kTapeBudget bytes, at least kMinTapes).
 #include "wedg_internal.h"
 #include "wedg_adjoint.h"
 
@@ -103,8 +107,11 @@ hipError_t copy_state(const double* su, double* du, size_t nu, const double* sv,
 
 }  // namespace
 
-// tapes in flight between the forward and the adjoint stream
-constexpr int kTapes = 4;
+// tapes in flight between the forward and the adjoint stream: as many as fit
+// in kTapeBudget bytes, between kMinTapes and kMaxTapes (a tape holds nine
+// states: Y1..Y4, F(Y1)..F(Y4) and the end state)
+constexpr int kMinTapes = 4, kMaxTapes = 32;
+constexpr size_t kTapeBudget = (size_t)2 << 30;
 
 struct AdjTape {
     double *Yu[4] = {}, *Yv[4] = {}, *Ku[4] = {}, *Kv[4] = {};   // stage states (Y1 a copy of y_n) and F(Y_s)
@@ -123,7 +130,7 @@ struct wedg_adj {
     std::vector<double> lam, Q, K1;                                          // 1D stiffness
     std::vector<void*> allocs;
     double *dc2 = nullptr, *dal = nullptr, *dbe = nullptr, *atab = nullptr;
-    AdjTape tape[kTapes];
+    std::vector<AdjTape> tape;   // tape[0] from creation (also the single products' scratch); more per gradient
     double *ybu = nullptr, *ybv = nullptr, *accu = nullptr, *accv = nullptr, *kbu = nullptr, *kbv = nullptr;
     double *pu = nullptr, *pv = nullptr, *slot = nullptr, *slot2 = nullptr, *gc2 = nullptr, *gbe = nullptr;
     double *tdc2 = nullptr, *tdbe = nullptr;   // tangent-linear directions of the coefficients
@@ -171,6 +178,34 @@ double* adj_alloc(wedg_adj* A, size_t n, hipError_t* he) {
     double* d = dalloc<double>(n, he);
     if (*he == hipSuccess) A->allocs.push_back(d);
     return d;
+}
+
+// Tapes up to n (the ones there are kept): device buffers and their three events.
+wedg_status ensure_tapes(wedg_adj* A, int n) {
+    const size_t nU = A->nU(), nV = A->nV();
+    while ((int)A->tape.size() < n) {
+        A->tape.emplace_back();
+        AdjTape& tp = A->tape.back();
+        hipError_t he = hipSuccess;
+        for (int s = 0; s < 4; ++s) {
+            tp.Yu[s] = adj_alloc(A, nU, &he); tp.Yv[s] = adj_alloc(A, nV, &he);
+            tp.Ku[s] = adj_alloc(A, nU, &he); tp.Kv[s] = adj_alloc(A, nV, &he);
+        }
+        tp.ynu = adj_alloc(A, nU, &he); tp.ynv = adj_alloc(A, nV, &he);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.copied, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.taped, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.used, hipEventDisableTiming);
+        if (he != hipSuccess) return WEDG_ERR_HIP;
+    }
+    return WEDG_OK;
+}
+
+// The tapes of a gradient of nsteps steps (at most one per step).
+int tapes_for(const wedg_adj* A, long nsteps) {
+    const size_t bytes = (size_t)9 * (A->nU() + A->nV()) * sizeof(double);
+    const long fit = (long)(kTapeBudget / std::max<size_t>(bytes, 1));
+    const long n = std::min<long>(kMaxTapes, std::max<long>(kMinTapes, fit));
+    return (int)std::max(1L, std::min(n, nsteps));
 }
 
 // The forward problem of the current coefficients: the tables of precompute for
@@ -537,16 +572,7 @@ extern "C" wedg_status wedg_adj_create(const wedg_adj_desc* d, int cheb_policy, 
     A->dc2 = adj_alloc(A, E, &he); A->dal = adj_alloc(A, E, &he); A->dbe = adj_alloc(A, E, &he);
     A->tdc2 = adj_alloc(A, E, &he); A->tdbe = adj_alloc(A, E, &he);
     A->atab = adj_alloc(A, (size_t)nq * nq + (size_t)nu * nu + nu, &he);
-    for (auto& tp : A->tape) {
-        for (int s = 0; s < 4; ++s) {
-            tp.Yu[s] = adj_alloc(A, nU, &he); tp.Yv[s] = adj_alloc(A, nV, &he);
-            tp.Ku[s] = adj_alloc(A, nU, &he); tp.Kv[s] = adj_alloc(A, nV, &he);
-        }
-        tp.ynu = adj_alloc(A, nU, &he); tp.ynv = adj_alloc(A, nV, &he);
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.copied, hipEventDisableTiming);
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.taped, hipEventDisableTiming);
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.used, hipEventDisableTiming);
-    }
+    if (ensure_tapes(A, 1) != WEDG_OK) return fail(WEDG_ERR_HIP);
     A->ybu = adj_alloc(A, nU, &he); A->ybv = adj_alloc(A, nV, &he);
     A->accu = adj_alloc(A, nU, &he); A->accv = adj_alloc(A, nV, &he);
     A->kbu = adj_alloc(A, nU, &he); A->kbv = adj_alloc(A, nV, &he);
@@ -692,6 +718,9 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
         A->nsnap_cap = nsnap;
     }
     std::memset(&A->stats, 0, sizeof(A->stats));
+    const int ntapes = tapes_for(A, nsteps);
+    if ((st = ensure_tapes(A, ntapes))) return st;
+    A->stats.tapes = ntapes;
     if ((st = reset_err(ws)) || (st = reset_err(A->wt)) || (st = reset_err(A->wa))) return st;
     HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), fs));
     HIPOK(hipMemsetAsync(ws->yu, 0, nU * sizeof(double), fs));
@@ -741,7 +770,7 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
     bool closing = false;
     long issued = 0;                     // jobs whose work the second thread has issued
     wedg_status worker_status = WEDG_OK;
-    std::vector<long> tape_job(kTapes, -1);   // the job that last used each tape
+    std::vector<long> tape_job(ntapes, -1);   // the job that last used each tape
     int device = 0;
     HIPOK(hipGetDevice(&device));
     auto issue = [&](const Job& jb) -> wedg_status {
@@ -777,7 +806,7 @@ extern "C" wedg_status wedg_adj_gradient(wedg_adj* A, double dt, long nsteps, in
     });
     long pushed = 0;
     auto leaf = [&](long n) -> wedg_status {
-        const int ti = (int)(n % kTapes);
+        const int ti = (int)(n % ntapes);
         AdjTape& tp = A->tape[ti];
         if (tape_job[ti] >= 0) {
             // the tape's last use must have been issued (its `used` recorded) before the wait

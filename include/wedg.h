@@ -343,7 +343,7 @@ typedef struct wedg_adj wedg_adj;
  * slots held at once. */
 typedef struct wedg_adj_stats {
     long forward_steps, taped_steps, adjoint_steps, snapshots_taken;
-    int snapshots_used, pad;
+    int snapshots_used, tapes;     /* tapes: steps the forward stream may run ahead of the adjoint */
     double enqueue_ms, total_ms;   /* host time to issue the run, and to its end */
 } wedg_adj_stats;
 

@@ -107,6 +107,7 @@ def test_gradient_against_oracle_and_checkpointing(q, mixed, N, ns):
     assert st.taped_steps == ns and st.adjoint_steps == ns
     assert st.forward_steps == _closed_form_advances(ns, 4)
     assert st.snapshots_used <= 4
+    assert st.tapes == min(32, ns)   # small tapes: one per step up to 32
     Jo, oc2, ob = A.gradient(pb, c2, beta, dt, ns, tgt)
     assert abs(J - Jo) <= 1e-9 * Jo
     assert rel(gc2, oc2) < 1e-9 and rel(gb, ob) < 1e-9
