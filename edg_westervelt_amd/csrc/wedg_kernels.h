@@ -41,7 +41,6 @@ namespace wedg {
 
 constexpr int kWave = 64;             // lanes per block: one wave
 constexpr int kMassLdsDoubles = 80;   // Chebyshev weights per lane kept in LDS: 4 waves fill 160 KB
-constexpr int kStaggerWaves = 512;    // grids of at least this many waves start in staggered groups
 
 // Coefficient layouts the element kernel is specialised for.
 enum {
@@ -1045,24 +1044,6 @@ __global__ __launch_bounds__(kWave) void k_stage(const std::conditional_t<BT, Ba
     const int lane = threadIdx.x;
     const int e = block_first_element<D::BT>() + lane;
     const int nw = (E + kWave - 1) / kWave;
-    // Staggered start on grids that fill the chip: with one wave per SIMD all
-    // waves would load (and store) at the same moments. The waves start in four
-    // groups, each one s_sleep 127 (64 x 127 clocks) later than the one before,
-    // which spreads the memory phases of the whole stage over time; waves with
-    // physical-boundary elements (which form their boundary slots first) take
-    // the two early groups. Measured on the 256 x 256, q = 5 instance: 5% less
-    // time per stage.
-    if (nw >= kStaggerWaves) {
-        int b = 0;
-        if (e < E) {
-            const int jj = e % N, kk = e / N;
-            b = (jj == 0 || jj == N - 1 || kk == 0 || kk == N - 1) ? 1 : 0;
-        }
-        for (int off = 32; off > 0; off >>= 1) b = max(b, __shfl_xor(b, off));
-        const int w = block_first_element<D::BT>() / kWave;
-        const int g = b ? (w & 1) : 2 + (w & 1);
-        for (int i = 0; i < g; ++i) __builtin_amdgcn_s_sleep(127);
-    }
     // the global count, the maximum over the waves' counts of this stage: the
     // slots are read (eight at a time) before the state loads are issued and
     // reduced after, so the two latencies overlap; lanes past E load element E - 1
