@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import gc
 import json
 import math
 import os
@@ -539,6 +540,12 @@ def main():
                  hbm_frac_model=bytes_step / 4 / stage_s / 1e9 / HBM_PEAK_GBS,
                  mean_cheb_steps=nc_timing, max_cheb_steps=info.cheb_steps)
 
+    # The other legs start from a process that holds no workspace of this one:
+    # a live workspace keeps its stream, and with the process's four hardware
+    # queues (GPU_MAX_HW_QUEUES) two of the gradient's three streams can then
+    # share a queue, whose event waits hold up the other stream's launches.
+    del ws
+    gc.collect()
     batched = bench_batched(cheb=args.cheb, lanes_per_element=blpe) if args.batched == "on" else None
     adjoint = bench_adjoint(lanes=args.adjoint_lanes) if args.adjoint == "on" else None
     batched_adjoint = (bench_batched_adjoint(T=args.batched_adjoint_T, lanes=args.adjoint_lanes)

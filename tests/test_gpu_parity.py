@@ -317,6 +317,41 @@ def test_full_size_rhs_and_step():
     assert rel(uo - u, ug - u) < 1e-11 and rel(vo - v, vg - v) < 1e-11
 
 
+def test_full_size_late_window():
+    """The 256 x 256, q = 5 benchmark instance late in its run, where the global
+    Chebyshev count is 15 to 18: 6,000 device RK4 steps (wedg_advance) from the
+    zero state, then at that state one right-hand side and one RK4 step against
+    the oracle (1e-12 relative; the step's increment 1e-11) and the global
+    count of the right-hand side equal to the oracle's."""
+    import ctypes as C
+    from edg_westervelt_amd._lib import lib
+    from edg_westervelt_amd.solver import _ptr, time_step
+    import bench
+    say("case full size late window q=5 N=256")
+    q, N, steps = 5, 256, 6000
+    pre, _ = bench.build_problem(q, N)
+    dt, _ = time_step(q, N, 0.5)
+    ws = w.Workspace(pre, N)
+    u = np.zeros((N, N, 6, 6), order="F")
+    v = np.zeros((N, N, 6, 6), order="F")
+    ws.dev.check(lib.wedg_set_state(ws.dev.h, _ptr(u), _ptr(v)))
+    tend = C.c_double(0.0)
+    ws.dev.check(lib.wedg_advance(ws.dev.h, 0.0, dt, steps, 0, C.byref(tend)))
+    ws.dev.check(lib.wedg_get_state(ws.dev.h, _ptr(u), _ptr(v)))
+    t = tend.value
+    assert abs(t - steps * dt) < 1e-12
+    op = ob.OraclePre(pre, N)
+    a = ob.rhs(op, u, v, t, nthreads=16)
+    b = w.edg_rhs_(ws, u, v, t, N, 1.0 / N, pre)
+    assert rel(a[0], b[0]) < TOL and rel(a[1], b[1]) < TOL
+    assert ws.cheb_steps == a[2].cheb_steps and ws.cheb_steps >= 15, (ws.cheb_steps, a[2].cheb_steps)
+    uo, vo, _, _ = ob.rk4_steps(op, u, v, t, dt, 1, nthreads=16)
+    ug = np.array(u, order="F", copy=True)
+    vg = np.array(v, order="F", copy=True)
+    w.rk4_step_(ug, vg, t, dt, N, 1.0 / N, pre, ws)
+    assert rel(uo - u, ug - u) < 1e-11 and rel(vo - v, vg - v) < 1e-11
+
+
 @pytest.mark.parametrize("q,N,mixed,mode,solver", [(3, 1, True, "mms", "chebyshev"), (2, 1, False, "source", "cg"),
                                                    (4, 2, True, "initial_value", "chebyshev"),
                                                    (5, 3, False, "mms", "cg")])
