@@ -523,6 +523,9 @@ def main():
         npk = (nu * nu * (nu * nu + 1)) // 2
         bytes_stage += 8 * E * (2 * nu * nu + npk + nh * nh)
         note += f" + the per-element records, {8 * (2 * nu * nu + npk + nh * nh)} B per element"
+    if not pre.mms_on and np.any(pre.PiCdiv):   # the static forcing of :initial_value mode, read every stage
+        bytes_stage += 8 * E * nv * nv
+        note += f" + the forcing projection PiCdiv, {8 * nv * nv} B per element"
     roof_hbm = dict(bound="hbm", achieved=bytes_stage / st_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=bytes_stage / st_s / 1e9 / HBM_PEAK_GBS, kernel=kname,
                     bytes_per_launch=bytes_stage, note=note)
