@@ -953,12 +953,16 @@ extern "C" wedg_status wedg_adj_rhs_tl(wedg_adj* A, const double* u, const doubl
 
 struct wedg_adj_batch {
     std::vector<wedg_adj*> inst;
-    hipStream_t stream = nullptr;
+    // Each degree group's rounds run on stream[g % kStreams]: the groups share
+    // no buffers, so one group's launches fill the SIMDs another's leave idle.
+    static constexpr int kStreams = 2;
+    hipStream_t stream[kStreams] = {};
     static constexpr int kRing = 8;
     char* hbuf[kRing] = {};
     char* dbuf[kRing] = {};
-    hipEvent_t done[kRing] = {};
+    hipEvent_t done[kRing][kStreams] = {};
     bool used[kRing] = {};
+    hipEvent_t join = nullptr;
     size_t cap = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double device_ms = 0.0, enqueue_ms = 0.0;
@@ -1122,7 +1126,7 @@ wedg_status batch_prepare_instance(wedg_adj* A, long nsteps, int nsnap, bool gra
 wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUnit>>& units, const double* dt,
                          bool forward_misfit) {
     const int n = (int)B->inst.size();
-    const hipStream_t s = B->stream;
+    constexpr int NS = wedg_adj_batch::kStreams;
     // groups
     std::vector<long> keys;
     for (wedg_adj* A : B->inst) keys.push_back(group_key(A));
@@ -1147,11 +1151,13 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
     auto state_v = [&](wedg_adj* A, int id) -> double* {
         return id < 0 ? A->ws->yv : A->snaps + (size_t)id * (A->nU() + A->nV()) + A->nU();
     };
-    HIPOK(hipEventRecord(B->t0, s));
+    HIPOK(hipEventRecord(B->t0, B->stream[0]));
+    for (int k = 1; k < NS; ++k) HIPOK(hipStreamWaitEvent(B->stream[k], B->t0, 0));
     const auto c0 = std::chrono::steady_clock::now();
     for (long r = 0; r < rounds; ++r) {
         const int ring = (int)(r % wedg_adj_batch::kRing);
-        if (B->used[ring]) HIPOK(hipEventSynchronize(B->done[ring]));
+        if (B->used[ring])
+            for (int k = 0; k < NS; ++k) HIPOK(hipEventSynchronize(B->done[ring][k]));
         char* hb = B->hbuf[ring];
         size_t off = 0;
         auto take = [&](size_t bytes) {
@@ -1165,6 +1171,11 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
             bool two;   // forward launches: the two-lane kernels
         };
         std::vector<Launch> launches;
+        struct Range {
+            size_t begin, end, first, count;   // the group's argument bytes and its launches
+            int stream;
+        };
+        std::vector<Range> ranges;
         for (size_t g = 0; g < gk.size(); ++g) {
             std::vector<int> F, R;
             for (int i : gmem[g])
@@ -1173,6 +1184,7 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                     if (units[i][r].rev) R.push_back(i);
                 }
             if (F.empty()) continue;
+            ranges.push_back({off, off, launches.size(), 0, (int)(g % NS)});
             wedg_adj* A0 = B->inst[F[0]];
             const int qu = A0->qu, nv = A0->nv, mode = A0->p->mode;
             // forward: block map and the four stages' arguments
@@ -1262,6 +1274,8 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                 }
                 launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf, false});
             }
+            ranges.back().end = off;
+            ranges.back().count = launches.size() - ranges.back().first;
             if (R.empty() || forward_misfit) continue;
             // the adjoint of the reversed steps
             int ablocks = 0;
@@ -1293,24 +1307,34 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
             launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af, false});
             for (int k = 1; k <= 3; ++k) launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af, false});
             launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af, false});
+            ranges.back().end = off;
+            ranges.back().count = launches.size() - ranges.back().first;
         }
         if (off > B->cap) return WEDG_ERR_ARGUMENT;
         char* db = B->dbuf[ring];
-        HIPOK(hipMemcpyAsync(db, hb, off, hipMemcpyHostToDevice, s));
-        for (const Launch& L : launches) {
-            BatchArgs ba;
-            ba.args = (const KArgs*)(const void*)(db + L.args);
-            ba.binst = (const int*)(db + L.binst);
-            ba.bfirst = (const int*)(db + L.bfirst);
-            if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, s));
-            else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, s, L.mode, L.two));
+        for (const Range& G : ranges) {
+            const hipStream_t gs = B->stream[G.stream];
+            HIPOK(hipMemcpyAsync(db + G.begin, hb + G.begin, G.end - G.begin, hipMemcpyHostToDevice, gs));
+            for (size_t x = G.first; x < G.first + G.count; ++x) {
+                const Launch& L = launches[x];
+                BatchArgs ba;
+                ba.args = (const KArgs*)(const void*)(db + L.args);
+                ba.binst = (const int*)(db + L.binst);
+                ba.bfirst = (const int*)(db + L.bfirst);
+                if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, gs));
+                else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, gs, L.mode, L.two));
+            }
         }
-        HIPOK(hipEventRecord(B->done[ring], s));
+        for (int k = 0; k < NS; ++k) HIPOK(hipEventRecord(B->done[ring][k], B->stream[k]));
         B->used[ring] = true;
     }
     const auto c1 = std::chrono::steady_clock::now();
-    HIPOK(hipEventRecord(B->t1, s));
-    HIPOK(hipStreamSynchronize(s));
+    for (int k = 1; k < NS; ++k) {
+        HIPOK(hipEventRecord(B->join, B->stream[k]));
+        HIPOK(hipStreamWaitEvent(B->stream[0], B->join, 0));
+    }
+    HIPOK(hipEventRecord(B->t1, B->stream[0]));
+    HIPOK(hipStreamSynchronize(B->stream[0]));
     float ms = 0.f;
     B->device_ms = hipEventElapsedTime(&ms, B->t0, B->t1) == hipSuccess ? ms : 0.0;
     B->enqueue_ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
@@ -1333,13 +1357,16 @@ extern "C" wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj
         wedg_adj_batch_destroy(B);
         return s;
     };
-    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return fail(WEDG_ERR_HIP);
+    for (auto& st : B->stream)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail(WEDG_ERR_HIP);
     if (hipEventCreate(&B->t0) != hipSuccess || hipEventCreate(&B->t1) != hipSuccess) return fail(WEDG_ERR_HIP);
+    if (hipEventCreateWithFlags(&B->join, hipEventDisableTiming) != hipSuccess) return fail(WEDG_ERR_HIP);
     B->cap = round_bytes(B);
     for (int k = 0; k < wedg_adj_batch::kRing; ++k) {
         if (hipHostMalloc((void**)&B->hbuf[k], B->cap, hipHostMallocDefault) != hipSuccess) return fail(WEDG_ERR_HIP);
         if (hipMalloc((void**)&B->dbuf[k], B->cap) != hipSuccess) return fail(WEDG_ERR_HIP);
-        if (hipEventCreateWithFlags(&B->done[k], hipEventDisableTiming) != hipSuccess) return fail(WEDG_ERR_HIP);
+        for (auto& ev : B->done[k])
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(WEDG_ERR_HIP);
     }
     *out = B;
     return WEDG_OK;
@@ -1347,15 +1374,19 @@ extern "C" wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj
 
 extern "C" wedg_status wedg_adj_batch_destroy(wedg_adj_batch* B) {
     if (!B) return WEDG_OK;
-    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    for (auto& st : B->stream)
+        if (st) (void)hipStreamSynchronize(st);
     for (int k = 0; k < wedg_adj_batch::kRing; ++k) {
         if (B->hbuf[k]) (void)hipHostFree(B->hbuf[k]);
         if (B->dbuf[k]) (void)hipFree(B->dbuf[k]);
-        if (B->done[k]) (void)hipEventDestroy(B->done[k]);
+        for (auto& ev : B->done[k])
+            if (ev) (void)hipEventDestroy(ev);
     }
     if (B->t0) (void)hipEventDestroy(B->t0);
     if (B->t1) (void)hipEventDestroy(B->t1);
-    if (B->stream) (void)hipStreamDestroy(B->stream);
+    if (B->join) (void)hipEventDestroy(B->join);
+    for (auto& st : B->stream)
+        if (st) (void)hipStreamDestroy(st);
     delete B;
     return WEDG_OK;
 }
