@@ -117,7 +117,7 @@ def steepening_series(case):
 
 
 def steepening_recv(case):
-    """(step, t, v_1, v_2, v_3) rows: every 64th step of data/steepening/<case>/recv.csv."""
+    """(step, t, v_1, v_2, v_3) rows: every 8th step of data/steepening/<case>/recv.csv."""
     return _rows(f"steepening_{case}_recv.csv")
 
 

@@ -9,8 +9,8 @@ fixtures (inputs and expected outputs only):
 * steepening_<case>_series.csv <- data/steepening/<case>/series.csv (t, energy,
   max|v_h|, min(1 - 2 beta v_h) every 32 steps; experiments/steepening/run.jl,
   cases nl, lin, nl_dbc), whole
-* steepening_<case>_recv.csv   <- data/steepening/<case>/recv.csv, every 64th
-  step (the rows after steps 64, 128, ..., 18432), prefixed by the step number
+* steepening_<case>_recv.csv   <- data/steepening/<case>/recv.csv, every 8th
+  step (the rows after steps 8, 16, ..., 18432), prefixed by the step number
 * steepening_nl_grid.csv       <- data/steepening/nl/grid.csv, whole
 * steepening_nl_snap_0.80.csv  <- data/steepening/nl/snap_0.80.csv, rows and
   columns 3, 11, 19, ... (every 8th, from the 4th), with the 0-based row index
@@ -36,7 +36,7 @@ for case in ("nl", "lin", "nl_dbc"):
     with open(os.path.join(REF, STEEP, case, "recv.csv")) as f:
         rows = f.read().splitlines()
     with open(os.path.join(HERE, f"steepening_{case}_recv.csv"), "w") as f:
-        for i in range(63, len(rows), 64):
+        for i in range(7, len(rows), 8):
             f.write(f"{i + 1},{rows[i]}\n")
     print("wrote", f"steepening_{case}_recv.csv")
 shutil.copyfile(os.path.join(REF, STEEP, "nl", "grid.csv"), os.path.join(HERE, "steepening_nl_grid.csv"))

@@ -126,7 +126,7 @@ def steepening_kwargs(case):
 def check_steepening(case, H):
     """H against data/steepening/<case>: the time axis exactly; the energy, max
     |v_h| and min (1 - 2 beta v_h) of all 576 records, 1e-12 relative; v_h at
-    the three receivers every 64th step, 1e-12 of the series' largest |v_h|;
+    the three receivers every 8th step, 1e-12 of the series' largest |v_h|;
     for nl the snapshot at t = 0.80 (every 8th point), 1e-12 of its largest
     |v_h|, and the grid coordinates."""
     ser = np.array(steepening_series(case))
