@@ -15,6 +15,18 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WEDG_LIBRARY") or os.path.join(_HERE, "libwedg.so")
 
+# Hardware queues of the process. HIP maps streams onto GPU_MAX_HW_QUEUES
+# queues (4 by default), and streams that share a queue run one after the other
+# (an event wait on one holds up the other). The batched mode runs a stream per
+# kernel group and a gradient three streams that wait on each other, so the
+# package raises the setting to 16 when it is lower (WEDG_KEEP_HW_QUEUES=1 keeps
+# it as it is). It takes effect when the HIP runtime is loaded, i.e. if nothing
+# in the process has loaded it yet.
+MIN_HW_QUEUES = 16
+_hwq = os.environ.get("GPU_MAX_HW_QUEUES", "")
+if not os.environ.get("WEDG_KEEP_HW_QUEUES") and not (_hwq.isdigit() and int(_hwq) >= MIN_HW_QUEUES):
+    os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
+
 KINDS = {"absorbing_2beta": 0, "absorbing": 1, "dirichlet": 2, "wall": 3}
 SIDES = ("bot", "top", "lft", "rgt")
 

@@ -130,7 +130,7 @@ class Batch:
         lib.wedg_batch_config(self.h, C.byref(lanes), C.byref(chunk), C.byref(sched))
         return lanes.value, chunk.value, sched.value
 
-    def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8), chunks=(0, 4), repeats: int = 2,
+    def tune(self, t0, dt, probe_steps: int = 32, lanes=(1, 2, 3, 4, 6, 8, 12, 16), chunks=(0, 4), repeats: int = 2,
              lanes_per_element=None):
         """Autotuner of the launch parameters: times ``probe_steps`` steps of the
         batch (device time between the batch's events, best of ``repeats``) for
