@@ -97,6 +97,25 @@ __global__ void k_copy_state(const double* __restrict__ su, double* __restrict__
     }
 }
 
+// States copied for several instances in one launch: job blockIdx.y copies
+// (su, sv) into (du, dv).
+struct CopyJob {
+    const double* su;
+    double* du;
+    const double* sv;
+    double* dv;
+    size_t nu, nv;
+};
+
+__global__ void k_copy_states(const CopyJob* __restrict__ jobs) {
+    const CopyJob J = jobs[blockIdx.y];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < J.nu + J.nv; i += stride) {
+        if (i < J.nu) J.du[i] = J.su[i];
+        else J.dv[i - J.nu] = J.sv[i - J.nu];
+    }
+}
+
 hipError_t copy_state(const double* su, double* du, size_t nu, const double* sv, double* dv, size_t nv,
                       hipStream_t s) {
     const size_t n = nu + nv;
@@ -955,13 +974,20 @@ struct wedg_adj_batch {
     std::vector<wedg_adj*> inst;
     // Each degree group's rounds run on stream[g % kStreams]: the groups share
     // no buffers, so one group's launches fill the SIMDs another's leave idle.
+    // In a gradient the reversed steps' adjoint launches of group g run on
+    // astream[g % kStreams], one round behind its forward launches: a reversal
+    // tapes into the tape of its round's parity and copies its start state into
+    // it, so the next round's forward launches touch nothing it reads.
     static constexpr int kStreams = 2;
     hipStream_t stream[kStreams] = {};
+    hipStream_t astream[kStreams] = {};
     static constexpr int kRing = 8;
     char* hbuf[kRing] = {};
     char* dbuf[kRing] = {};
-    hipEvent_t done[kRing][kStreams] = {};
+    hipEvent_t done[kRing][2 * kStreams] = {};   // a round's launches on each of the streams
     bool used[kRing] = {};
+    hipEvent_t fdone[kStreams] = {};             // a group's forward launches of the current round
+    hipEvent_t adone[2][kStreams] = {};          // the adjoint launches of the last round of each parity
     hipEvent_t join = nullptr;
     size_t cap = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -1078,7 +1104,7 @@ size_t round_bytes(const wedg_adj_batch* b) {
     const size_t per_group_arrays = 4 + 6 + 6;
     return 4 * align256(n * sizeof(KArgs)) + 6 * align256(2 * n * sizeof(AdjArgs)) +
            2 * align256(blocks * sizeof(int)) + align256(mblocks * sizeof(int)) + 3 * align256(n * sizeof(int)) +
-           b->inst.size() * per_group_arrays * 256 + 4096;
+           align256(n * sizeof(CopyJob)) + b->inst.size() * (per_group_arrays + 1) * 256 + 4096;
 }
 
 // the group key of an instance: kernels are instantiated per (q_u, q_v, layout)
@@ -1106,6 +1132,7 @@ wedg_status batch_prepare_instance(wedg_adj* A, long nsteps, int nsnap, bool gra
         A->nsnap_cap = nsnap;
     }
     std::memset(&A->stats, 0, sizeof(A->stats));
+    if (gradient && (st = ensure_tapes(A, 2))) return st;   // a tape per round parity
     if ((st = reset_err(ws))) return st;
     HIPOK(hipMemsetAsync(A->fail, 0, sizeof(int), s));
     HIPOK(hipMemsetAsync(ws->yu, 0, nU * sizeof(double), s));
@@ -1152,12 +1179,18 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
         return id < 0 ? A->ws->yv : A->snaps + (size_t)id * (A->nU() + A->nV()) + A->nU();
     };
     HIPOK(hipEventRecord(B->t0, B->stream[0]));
-    for (int k = 1; k < NS; ++k) HIPOK(hipStreamWaitEvent(B->stream[k], B->t0, 0));
+    for (int k = 0; k < NS; ++k) {
+        if (k > 0) HIPOK(hipStreamWaitEvent(B->stream[k], B->t0, 0));
+        HIPOK(hipStreamWaitEvent(B->astream[k], B->t0, 0));
+        HIPOK(hipEventRecord(B->adone[0][k], B->astream[k]));
+        HIPOK(hipEventRecord(B->adone[1][k], B->astream[k]));
+    }
     const auto c0 = std::chrono::steady_clock::now();
     for (long r = 0; r < rounds; ++r) {
         const int ring = (int)(r % wedg_adj_batch::kRing);
+        const int par = (int)(r & 1);   // the tape of this round's reversals
         if (B->used[ring])
-            for (int k = 0; k < NS; ++k) HIPOK(hipEventSynchronize(B->done[ring][k]));
+            for (int k = 0; k < 2 * NS; ++k) HIPOK(hipEventSynchronize(B->done[ring][k]));
         char* hb = B->hbuf[ring];
         size_t off = 0;
         auto take = [&](size_t bytes) {
@@ -1166,9 +1199,10 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
             return o;
         };
         struct Launch {
-            int qu, nv, mode, which, grid, adj;
+            int qu, nv, mode, which, grid, adj;   // adj: 1 adjoint kernel, 2 the start-state copies
             size_t args, binst, bfirst;
-            bool two;   // forward launches: the two-lane kernels
+            bool two;    // forward launches: the two-lane kernels
+            bool back;   // on the group's adjoint stream (the gradient's misfit and adjoint kernels)
         };
         std::vector<Launch> launches;
         struct Range {
@@ -1221,7 +1255,7 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                     a[2] = stage_args(ws, 3, ws->ou, ws->ov, iu, iv, ws->su, ws->sv, th, d, step, true, tf);
                     a[3] = stage_args(ws, 4, ws->su, ws->sv, iu, iv, ou, ov, tf, d / 6.0, step, false, tf);
                 } else {
-                    AdjTape& tp = A->tape[0];
+                    AdjTape& tp = A->tape[par];
                     const double ts[4] = {t, th, th, tf}, cs[4] = {hd, hd, d, d / 6.0}, tn[4] = {th, th, tf, tf};
                     for (int sg = 0; sg < 4; ++sg) {
                         const double* yu = sg == 0 ? iu : tp.Yu[sg];
@@ -1237,9 +1271,21 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                 if (u.rev) ++A->stats.taped_steps;
                 else ++A->stats.forward_steps;
             }
-            launches.push_back({qu, nv, mode, 0, fblocks, 0, o_st[0], o_bi, o_bf, two});   // k_prep
+            if (!forward_misfit && !R.empty()) {
+                // the reversed steps' start states into their tapes, for the adjoint kernels
+                const size_t o_cj = take(R.size() * sizeof(CopyJob));
+                CopyJob* cj = (CopyJob*)(hb + o_cj);
+                for (size_t x = 0; x < R.size(); ++x) {
+                    wedg_adj* A = B->inst[R[x]];
+                    const AdjUnit& u = units[R[x]][r];
+                    AdjTape& tp = A->tape[par];
+                    cj[x] = CopyJob{state_u(A, u.in), tp.Yu[0], state_v(A, u.in), tp.Yv[0], A->nU(), A->nV()};
+                }
+                launches.push_back({qu, nv, mode, 0, (int)R.size(), 2, o_cj, 0, 0, false, false});
+            }
+            launches.push_back({qu, nv, mode, 0, fblocks, 0, o_st[0], o_bi, o_bf, two, false});   // k_prep
             for (int sg = 0; sg < 4; ++sg)
-                launches.push_back({qu, nv, mode, 1, fblocks, 0, o_st[sg], o_bi, o_bf, two});
+                launches.push_back({qu, nv, mode, 1, fblocks, 0, o_st[sg], o_bi, o_bf, two, false});
             // misfit of the forward steps' end states (forward runs)
             std::vector<int> M = forward_misfit ? F : R;
             if (!M.empty()) {
@@ -1267,12 +1313,13 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                         mis.ybu = nullptr;
                     } else {
                         AdjArgs st5[5];
-                        adjoint_step_args(A, A->tape[0], u.n, dt[M[x]], state_u(A, u.in), state_v(A, u.in), mis, st5);
+                        const AdjTape& tp = A->tape[par];
+                        adjoint_step_args(A, tp, u.n, dt[M[x]], tp.Yu[0], tp.Yv[0], mis, st5);
                     }
                     std::memcpy(hb + o_ma + (2 * x) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
                     std::memcpy(hb + o_ma + (2 * x + 1) * sizeof(AdjArgs), &mis, sizeof(AdjArgs));
                 }
-                launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf, false});
+                launches.push_back({qu, nv, mode, 3, mblocks, 1, o_ma, o_mi, o_mf, false, !forward_misfit});
             }
             ranges.back().end = off;
             ranges.back().count = launches.size() - ranges.back().first;
@@ -1297,40 +1344,66 @@ wedg_status batch_rounds(wedg_adj_batch* B, const std::vector<std::vector<AdjUni
                 wedg_adj* A = B->inst[R[x]];
                 const AdjUnit& u = units[R[x]][r];
                 AdjArgs mis, st5[5];
-                adjoint_step_args(A, A->tape[0], u.n, dt[R[x]], state_u(A, u.in), state_v(A, u.in), mis, st5);
+                const AdjTape& tp = A->tape[par];
+                adjoint_step_args(A, tp, u.n, dt[R[x]], tp.Yu[0], tp.Yv[0], mis, st5);
                 const AdjArgs pairs[5][2] = {{st5[4], st5[4]}, {st5[4], st5[3]}, {st5[3], st5[2]}, {st5[2], st5[1]},
                                              {st5[1], st5[1]}};
                 for (int k = 0; k < 5; ++k)
                     std::memcpy(hb + o_ad[k] + 2 * x * sizeof(AdjArgs), pairs[k], 2 * sizeof(AdjArgs));
                 ++A->stats.adjoint_steps;
             }
-            launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af, false});
-            for (int k = 1; k <= 3; ++k) launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af, false});
-            launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af, false});
+            launches.push_back({qu, nv, mode, 0, ablocks, 1, o_ad[0], o_ai, o_af, false, true});
+            for (int k = 1; k <= 3; ++k)
+                launches.push_back({qu, nv, mode, 2, ablocks, 1, o_ad[k], o_ai, o_af, false, true});
+            launches.push_back({qu, nv, mode, 1, ablocks, 1, o_ad[4], o_ai, o_af, false, true});
             ranges.back().end = off;
             ranges.back().count = launches.size() - ranges.back().first;
         }
         if (off > B->cap) return WEDG_ERR_ARGUMENT;
         char* db = B->dbuf[ring];
+        // a reversal tapes into the tape of this round's parity: the forward
+        // stream waits for the adjoint launches of two rounds ago, which read it
+        if (!forward_misfit)
+            for (int k = 0; k < NS; ++k) HIPOK(hipStreamWaitEvent(B->stream[k], B->adone[par][k], 0));
         for (const Range& G : ranges) {
-            const hipStream_t gs = B->stream[G.stream];
+            const hipStream_t gs = B->stream[G.stream], as = B->astream[G.stream];
             HIPOK(hipMemcpyAsync(db + G.begin, hb + G.begin, G.end - G.begin, hipMemcpyHostToDevice, gs));
+            bool joined = false;
             for (size_t x = G.first; x < G.first + G.count; ++x) {
                 const Launch& L = launches[x];
+                if (L.back && !joined) {   // the group's forward launches of this round come first
+                    HIPOK(hipEventRecord(B->fdone[G.stream], gs));
+                    HIPOK(hipStreamWaitEvent(as, B->fdone[G.stream], 0));
+                    joined = true;
+                }
+                const hipStream_t ls = L.back ? as : gs;
+                if (L.adj == 2) {
+                    k_copy_states<<<dim3(256, L.grid), 256, 0, ls>>>((const CopyJob*)(const void*)(db + L.args));
+                    HIPOK(hipGetLastError());
+                    continue;
+                }
                 BatchArgs ba;
                 ba.args = (const KArgs*)(const void*)(db + L.args);
                 ba.binst = (const int*)(db + L.binst);
                 ba.bfirst = (const int*)(db + L.bfirst);
-                if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, gs));
-                else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, gs, L.mode, L.two));
+                if (L.adj) HIPOK(launch_adj_batch(ba, L.grid, L.qu, L.nv, L.which, ls));
+                else HIPOK(launch_fwd_batch(ba, L.grid, L.qu, L.nv, L.which, ls, L.mode, L.two));
             }
         }
-        for (int k = 0; k < NS; ++k) HIPOK(hipEventRecord(B->done[ring][k], B->stream[k]));
+        for (int k = 0; k < NS; ++k) {
+            HIPOK(hipEventRecord(B->done[ring][k], B->stream[k]));
+            HIPOK(hipEventRecord(B->done[ring][NS + k], B->astream[k]));
+            if (!forward_misfit) HIPOK(hipEventRecord(B->adone[par][k], B->astream[k]));
+        }
         B->used[ring] = true;
     }
     const auto c1 = std::chrono::steady_clock::now();
-    for (int k = 1; k < NS; ++k) {
-        HIPOK(hipEventRecord(B->join, B->stream[k]));
+    for (int k = 0; k < NS; ++k) {
+        if (k > 0) {
+            HIPOK(hipEventRecord(B->join, B->stream[k]));
+            HIPOK(hipStreamWaitEvent(B->stream[0], B->join, 0));
+        }
+        HIPOK(hipEventRecord(B->join, B->astream[k]));
         HIPOK(hipStreamWaitEvent(B->stream[0], B->join, 0));
     }
     HIPOK(hipEventRecord(B->t1, B->stream[0]));
@@ -1357,8 +1430,14 @@ extern "C" wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj
         wedg_adj_batch_destroy(B);
         return s;
     };
-    for (auto& st : B->stream)
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail(WEDG_ERR_HIP);
+    for (int k = 0; k < wedg_adj_batch::kStreams; ++k) {
+        if (hipStreamCreateWithFlags(&B->stream[k], hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&B->astream[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&B->fdone[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&B->adone[0][k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&B->adone[1][k], hipEventDisableTiming) != hipSuccess)
+            return fail(WEDG_ERR_HIP);
+    }
     if (hipEventCreate(&B->t0) != hipSuccess || hipEventCreate(&B->t1) != hipSuccess) return fail(WEDG_ERR_HIP);
     if (hipEventCreateWithFlags(&B->join, hipEventDisableTiming) != hipSuccess) return fail(WEDG_ERR_HIP);
     B->cap = round_bytes(B);
@@ -1376,6 +1455,8 @@ extern "C" wedg_status wedg_adj_batch_destroy(wedg_adj_batch* B) {
     if (!B) return WEDG_OK;
     for (auto& st : B->stream)
         if (st) (void)hipStreamSynchronize(st);
+    for (auto& st : B->astream)
+        if (st) (void)hipStreamSynchronize(st);
     for (int k = 0; k < wedg_adj_batch::kRing; ++k) {
         if (B->hbuf[k]) (void)hipHostFree(B->hbuf[k]);
         if (B->dbuf[k]) (void)hipFree(B->dbuf[k]);
@@ -1385,7 +1466,14 @@ extern "C" wedg_status wedg_adj_batch_destroy(wedg_adj_batch* B) {
     if (B->t0) (void)hipEventDestroy(B->t0);
     if (B->t1) (void)hipEventDestroy(B->t1);
     if (B->join) (void)hipEventDestroy(B->join);
+    for (int k = 0; k < wedg_adj_batch::kStreams; ++k) {
+        if (B->fdone[k]) (void)hipEventDestroy(B->fdone[k]);
+        if (B->adone[0][k]) (void)hipEventDestroy(B->adone[0][k]);
+        if (B->adone[1][k]) (void)hipEventDestroy(B->adone[1][k]);
+    }
     for (auto& st : B->stream)
+        if (st) (void)hipStreamDestroy(st);
+    for (auto& st : B->astream)
         if (st) (void)hipStreamDestroy(st);
     delete B;
     return WEDG_OK;
