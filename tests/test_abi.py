@@ -70,3 +70,24 @@ def test_workspace_rejects_unknown_lane_counts():
     for bad in (0, 3, "two"):
         with pytest.raises(ValueError, match="lanes"):
             w.Workspace(pre, 2, lanes=bad)
+
+
+@pytest.mark.parametrize("given,keep,expect", [(None, None, "16"), ("4", None, "16"), ("32", None, "32"),
+                                               ("4", "1", "4")])
+def test_hardware_queue_setting(given, keep, expect):
+    """Importing the binding raises GPU_MAX_HW_QUEUES to 16 when it is unset or
+    lower (the batched mode and the gradient run concurrent streams, which
+    share hardware queues otherwise), keeps a higher value, and keeps any value
+    with WEDG_KEEP_HW_QUEUES=1. A fresh interpreter each time: the setting is
+    made at import."""
+    import subprocess
+    import sys
+    env = {k: v for k, v in os.environ.items() if k not in ("GPU_MAX_HW_QUEUES", "WEDG_KEEP_HW_QUEUES")}
+    if given is not None:
+        env["GPU_MAX_HW_QUEUES"] = given
+    if keep is not None:
+        env["WEDG_KEEP_HW_QUEUES"] = keep
+    out = subprocess.run([sys.executable, "-c", "import os, edg_westervelt_amd._lib; "
+                          "print(os.environ['GPU_MAX_HW_QUEUES'])"],
+                         cwd=ROOT, env=env, capture_output=True, text=True, check=True)
+    assert out.stdout.strip() == expect
