@@ -359,7 +359,12 @@ wedg_status wedg_adj_set_target(wedg_adj* a, long nsteps, const double* target);
  * nsteps rows; null: not formed) and the bottom traces u_h after every step
  * into traces ([n][k][i], or null). */
 wedg_status wedg_adj_forward(wedg_adj* a, double dt, long nsteps, double* J, double* traces);
-/* J and its gradient, dJ_dc2 and dJ_dbeta (N, N) each. */
+/* J and its gradient, dJ_dc2 and dJ_dbeta (N, N) each, by binomial
+ * checkpointing with `snapshots` state slots. Three streams (forward, tape,
+ * adjoint) run concurrently; the forward stream runs ahead of the adjoint by
+ * as many reversed steps as there are tapes (as many as fit in 2 GiB, 4 to 32,
+ * allocated on first use). The streams should have hardware queues of their
+ * own (GPU_MAX_HW_QUEUES >= 4 plus the process's other live streams). */
 wedg_status wedg_adj_gradient(wedg_adj* a, double dt, long nsteps, int snapshots, double* J, double* dJ_dc2,
                               double* dJ_dbeta);
 wedg_status wedg_adj_last_stats(const wedg_adj* a, wedg_adj_stats* st);
@@ -394,7 +399,9 @@ wedg_status wedg_adj_batch_create(wedg_adj* const* a, int n, wedg_adj_batch** ou
 wedg_status wedg_adj_batch_destroy(wedg_adj_batch* b);
 /* J[i] after nsteps[i] steps of dt[i] from the zero state (a target of at least nsteps[i] rows each). */
 wedg_status wedg_adj_batch_forward(wedg_adj_batch* b, const double* dt, const long* nsteps, double* J, int* status);
-/* J[i] and dJ_dc2[i], dJ_dbeta[i] ((N_i, N_i) host arrays) with `snapshots` slots per instance. */
+/* J[i] and dJ_dc2[i], dJ_dbeta[i] ((N_i, N_i) host arrays) with `snapshots` slots per instance.
+ * Each kernel group runs its forward and its adjoint launches on two streams
+ * of its own, the adjoint one round behind (two tapes per instance). */
 wedg_status wedg_adj_batch_gradient(wedg_adj_batch* b, const double* dt, const long* nsteps, int snapshots, double* J,
                                     double* const* dJ_dc2, double* const* dJ_dbeta, int* status);
 /* Device time of the last batch call (stream events) and the host time to issue it. */
