@@ -214,7 +214,13 @@ wedg_status ensure_tapes(wedg_adj* A, int n) {
         if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.copied, hipEventDisableTiming);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.taped, hipEventDisableTiming);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&tp.used, hipEventDisableTiming);
-        if (he != hipSuccess) return WEDG_ERR_HIP;
+        if (he != hipSuccess) {
+            // a tape that is not whole is not kept (its buffers stay in allocs until destroy)
+            for (hipEvent_t ev : {tp.copied, tp.taped, tp.used})
+                if (ev) (void)hipEventDestroy(ev);
+            A->tape.pop_back();
+            return WEDG_ERR_HIP;
+        }
     }
     return WEDG_OK;
 }
